@@ -1,0 +1,204 @@
+/*
+ * proteus_amd.h -- C ABI of the MI355X (gfx950) join / aggregation hot path.
+ *
+ * Plain pointers and sizes only.  Device pointers ("d_") live in HBM of the
+ * context's device; host pointers ("h_") are ordinary or pinned host memory.
+ * Every entry point returns a pj_status; on failure pj_last_error() returns a
+ * message for the calling thread.  Work is issued on the context's stream and
+ * each entry point returns after its results are available unless its comment
+ * says otherwise.  One context per device; calls on one context are
+ * serialized by the caller.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the reference repository).
+ */
+#ifndef PROTEUS_AMD_H
+#define PROTEUS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PJ_OK = 0,
+  PJ_E_CAPACITY = 1,    /* build/group input exceeded maxBuildInputSize / maxInputSize */
+  PJ_E_OOM = 2,         /* device allocation failed or HBM budget exceeded   */
+  PJ_E_UNSUPPORTED = 3, /* operator / expression / type not on this path     */
+  PJ_E_HIP = 4,         /* HIP runtime error                                  */
+  PJ_E_ARG = 5,         /* invalid argument                                   */
+  PJ_E_PARSE = 6,       /* malformed plan JSON / unknown operator             */
+  PJ_E_CANCELLED = 7    /* request cancelled before completion                */
+} pj_status;
+
+typedef struct pj_ctx pj_ctx;
+
+/* ---- context, memory, errors --------------------------------------------
+ * Replaces MemoryManager::mallocGpu/freeGpu/mallocPinned/freePinned
+ * (core/platform/include/platform/memory/memory-manager.hpp) and the
+ * gpu_run / stream helpers of core/platform/include/platform/common/gpu/
+ * gpu-common.hpp:93,383-398 with status codes instead of aborts. */
+const char *pj_last_error(void);
+int pj_ctx_create(int device, uint64_t hbm_budget_bytes, pj_ctx **out);
+int pj_ctx_destroy(pj_ctx *ctx);
+int pj_ctx_sync(pj_ctx *ctx);
+int pj_ctx_device_info(pj_ctx *ctx, int *num_cus, uint64_t *hbm_bytes,
+                       uint64_t *lds_bytes_per_cu);
+int pj_malloc(pj_ctx *ctx, uint64_t bytes, void **d_ptr);
+int pj_free(pj_ctx *ctx, void *d_ptr);
+int pj_malloc_pinned(pj_ctx *ctx, uint64_t bytes, void **h_ptr);
+int pj_free_pinned(pj_ctx *ctx, void *h_ptr);
+int pj_memcpy_h2d(pj_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+int pj_memcpy_d2h(pj_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+int pj_memset(pj_ctx *ctx, void *d_dst, int byte, uint64_t bytes);
+int pj_mem_in_use(pj_ctx *ctx, uint64_t *bytes);
+
+/* ---- per-kernel timing ---------------------------------------------------
+ * Replaces the time_block / nvtx ranges around pipelines
+ * (core/platform/include/platform/util/timing.hpp:98-162,
+ * core/olap/lib/plan/prepared-statement.cpp:117): HIP events recorded on the
+ * context stream around every launch while enabled. */
+int pj_prof_enable(pj_ctx *ctx, int on);
+int pj_prof_reset(pj_ctx *ctx);
+/* Total milliseconds and launch count of kernels whose name starts with
+ * `prefix` since the last reset (synchronizes the context). */
+int pj_prof_get(pj_ctx *ctx, const char *prefix, double *total_ms,
+                uint64_t *launches);
+
+/* ---- seeded counter-based generators (device) ----------------------------
+ * The synthetic-data counterpart of tools/prepare_data.py:389-431: columns
+ * generated directly in HBM.  Output = f(seed, stream, row) only. */
+int pj_gen_perm_i32(pj_ctx *ctx, int32_t *d_out, uint64_t n, uint64_t seed);
+int pj_gen_uniform_i32(pj_ctx *ctx, int32_t *d_out, uint64_t n, uint64_t seed,
+                       uint64_t stream, int64_t lo, uint64_t range);
+int pj_gen_uniform_i64(pj_ctx *ctx, int64_t *d_out, uint64_t n, uint64_t seed,
+                       uint64_t stream, int64_t lo, uint64_t range);
+int pj_gen_unit_f32(pj_ctx *ctx, float *d_out, uint64_t n, uint64_t seed,
+                    uint64_t stream);
+int pj_gen_zipf_fk_i32(pj_ctx *ctx, int32_t *d_out, uint64_t n, uint64_t seed,
+                       uint64_t stream, uint64_t nbuild, uint64_t range,
+                       double theta, uint64_t perm_seed);
+int pj_gen_zipf_i32(pj_ctx *ctx, int32_t *d_out, uint64_t n, uint64_t seed,
+                    uint64_t stream, uint64_t g, double theta,
+                    uint64_t perm_seed);
+
+/* ---- join -> reduce --------------------------------------------------------
+ * Aggregates of the join output that a parent reduce consumes
+ * (opt::GpuReduce, core/olap/lib/operators/gpu/gpu-reduce.cpp:45-202, fed by
+ * the probe loop's getParent()->consume(), hash-join-chained.cpp:557-559):
+ * COUNT(*), SUM/MIN/MAX of the int32 build value and of the int32 probe value
+ * over all matches, in exact int64.  Empty result: sums 0, min INT64_MAX,
+ * max INT64_MIN (the monoid identities of reduce-opt.cpp:374-430). */
+typedef struct {
+  int64_t count;
+  int64_t sum_b;
+  int64_t sum_p;
+  int64_t min_b, max_b, min_p, max_p;
+} pj_join_agg;
+
+/* hashjoin-chained -> reduce.  Replaces the JIT'd GpuHashJoinChained
+ * build/probe (core/olap/lib/operators/hash-join-chained.cpp:262-566,
+ * gpu/gpu-hash-join-chained.cpp:31-73) behind the plan operator
+ * "hashjoin-chained" (core/olap/lib/plan/plan-parser.cpp:977-1006).
+ * hash_bits and max_build are the operator's fields; nb > max_build returns
+ * PJ_E_CAPACITY (the reference asserts, hash-join-chained.cpp:599-608).
+ * d_bv / d_pv may be NULL (value 0). */
+int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
+                           const int32_t *d_bv, uint64_t nb,
+                           const int32_t *d_pk, const int32_t *d_pv,
+                           uint64_t np, int hash_bits, uint64_t max_build,
+                           pj_join_agg *out);
+
+/* hashjoin-chained with materialized output: every (build row, probe row)
+ * pair the reference's probe loop hands to its parent
+ * (hash-join-chained.cpp:321-566).  Writes up to `cap` pairs to d_out_b /
+ * d_out_p (device, uint32 row ids), *n_out = total matches; if the total
+ * exceeds cap the first cap are written and PJ_E_CAPACITY is returned. */
+int pj_join_chained_pairs(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
+                          const int32_t *d_pk, uint64_t np, int hash_bits,
+                          uint64_t max_build, uint32_t *d_out_b,
+                          uint32_t *d_out_p, uint64_t cap, uint64_t *n_out);
+
+/* Radix-partitioned join tuning knobs; zero fields pick the autotuned value. */
+typedef struct {
+  int bits1;           /* pass-1 radix bits (fan-out 2^bits1)                */
+  int bits2;           /* pass-2 radix bits                                  */
+  int lds_log;         /* log2 of the LDS table's head array                 */
+  uint32_t probe_unit; /* max probe tuples per join work unit                */
+  int grid_mul;        /* persistent workgroups per CU                       */
+} pj_radix_cfg;
+
+/* partitioned-hashjoin-chained -> reduce.  Replaces HashPartitioner
+ * (core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained.cpp:47-404
+ * with the kernels of gpu-partitioned-hash-join-chained-helpers.cpp:63-481)
+ * and GpuPartitionedHashJoinChained's join loop (:612-1586) behind the plan
+ * operator "partitioned-hashjoin-chained" (plan-parser.cpp:851-976).
+ * nb > max_build or np > max_probe returns PJ_E_CAPACITY.  cfg may be NULL. */
+int pj_join_partitioned_reduce(pj_ctx *ctx, const int32_t *d_bk,
+                               const int32_t *d_bv, uint64_t nb,
+                               const int32_t *d_pk, const int32_t *d_pv,
+                               uint64_t np, uint64_t max_build,
+                               uint64_t max_probe, const pj_radix_cfg *cfg,
+                               pj_join_agg *out);
+
+/* The configuration pj_join_partitioned_reduce picks for (nb, np) when cfg
+ * is NULL (the autotuner's choice; runs a timed sweep the first time a size
+ * class is seen if tuning is enabled with pj_autotune_enable). */
+int pj_join_partitioned_config(pj_ctx *ctx, uint64_t nb, uint64_t np,
+                               pj_radix_cfg *cfg);
+int pj_autotune_enable(pj_ctx *ctx, int on);
+
+/* ---- hash group-by ---------------------------------------------------------
+ * groupby / hashgroupby-chained (plan-parser.cpp:479-530) replacing the JIT'd
+ * GpuHashGroupByChained build + scan (gpu/gpu-hash-group-by-chained.cpp:
+ * 77-556, hash-group-by-chained.cpp:638-853).  One int32 key; per group
+ * COUNT, SUM(v), MIN(v), MAX(v) in exact int64 and SUM((double) w) in FP64.
+ * d_v / d_w may be NULL.  Output columns are device arrays of capacity
+ * max_groups, compacted, in unspecified group order; *n_groups is set.
+ * More than max_groups distinct keys returns PJ_E_CAPACITY (the reference
+ * asserts, hash-group-by-chained.cpp:915-919).  hash_bits is accepted for
+ * API parity; the table geometry is chosen from max_groups. */
+int pj_groupby_i32(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v,
+                   const float *d_w, uint64_t n, int hash_bits,
+                   uint64_t max_groups, int32_t *d_out_k, int64_t *d_cnt,
+                   int64_t *d_sum, int64_t *d_min, int64_t *d_max,
+                   double *d_fsum, uint64_t *n_groups);
+
+/* ---- reduce ----------------------------------------------------------------
+ * reduce (plan-parser.cpp:248-286) over one int64 column: opt::GpuReduce's
+ * SUM / MIN / MAX monoids (gpu/gmonoids.cpp:122-244, gpu-reduce.cpp:155-202). */
+int pj_reduce_i64(pj_ctx *ctx, const int64_t *d_v, uint64_t n, int64_t *sum,
+                  int64_t *min, int64_t *max);
+
+/* ---- hash-rearrange --------------------------------------------------------
+ * hash-rearrange / hash-pack (plan-parser.cpp:1424-1457) replacing the JIT'd
+ * GpuHashRearrange (gpu/gpu-hash-rearrange.cpp:87-1049): bucket =
+ * hashInt32(key) URem buckets (:338-348).  Output rows are grouped by bucket:
+ * bucket b occupies [h_offsets[b], h_offsets[b+1]) of d_out_k / d_out_v
+ * (d_v / d_out_v may be NULL).  Order inside a bucket is unspecified. */
+int pj_hash_rearrange_i32(pj_ctx *ctx, const int32_t *d_k, const int32_t *d_v,
+                          uint64_t n, uint32_t buckets, int32_t *d_out_k,
+                          int32_t *d_out_v, uint64_t *h_offsets);
+
+/* ---- plan API ----------------------------------------------------------------
+ * PlanExecutor::parseOperator (core/olap/lib/plan/plan-parser.cpp:238-1675)
+ * + PreparedStatement::execute (prepared-statement.cpp:41-140): parse a
+ * Proteus JSON physical plan and run it.  Relations named by "scan" operators
+ * resolve to columns registered with pj_register_column (device or host
+ * memory; a host column is staged to HBM on first use) or to CSV files
+ * (plugin type "csv"/"pm-csv").  The result rows are written as text in the
+ * reference's print format (one row per line, fields comma-separated) into
+ * out (NUL-terminated, truncated to out_cap). */
+int pj_register_column(pj_ctx *ctx, const char *rel, const char *attr,
+                       const char *type, const void *ptr, uint64_t n,
+                       int on_device);
+int pj_clear_catalog(pj_ctx *ctx);
+int pj_plan_execute(pj_ctx *ctx, const char *plan_json, const char *base_dir,
+                    char *out, uint64_t out_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
