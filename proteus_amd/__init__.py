@@ -1,0 +1,339 @@
+"""proteus_amd -- MI355X (gfx950) join / aggregation hot path of Proteus.
+
+A thin ctypes binding of the C ABI in include/proteus_amd.h, implemented by
+libproteus_amd.so (hand-written HIP kernels + the C++ plan engine).  There is
+no CPU fallback: importing this package raises if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libproteus_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "proteus_amd: %s is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `make -C proteus_amd`" % LIB_PATH)
+
+_lib = C.CDLL(LIB_PATH)
+
+PJ_OK = 0
+PJ_E_CAPACITY = 1
+PJ_E_OOM = 2
+PJ_E_UNSUPPORTED = 3
+PJ_E_HIP = 4
+PJ_E_ARG = 5
+PJ_E_PARSE = 6
+PJ_E_CANCELLED = 7
+
+_STATUS = {1: "CAPACITY", 2: "OOM", 3: "UNSUPPORTED", 4: "HIP", 5: "ARG", 6: "PARSE",
+           7: "CANCELLED"}
+
+
+class PjError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("PJ_E_%s: %s" % (_STATUS.get(code, str(code)), msg))
+        self.code = code
+
+
+class JoinAgg(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in
+                ("count", "sum_b", "sum_p", "min_b", "max_b", "min_p", "max_p")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class RadixCfg(C.Structure):
+    _fields_ = [("bits1", C.c_int), ("bits2", C.c_int), ("lds_log", C.c_int),
+                ("probe_unit", C.c_uint32), ("grid_mul", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+vp, u64, i64, i32, u32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_uint32
+_SIGS = {
+    "pj_last_error": ([], C.c_char_p),
+    "pj_ctx_create": ([i32, u64, C.POINTER(vp)], i32),
+    "pj_ctx_destroy": ([vp], i32),
+    "pj_ctx_sync": ([vp], i32),
+    "pj_ctx_device_info": ([vp, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(u64)], i32),
+    "pj_malloc": ([vp, u64, C.POINTER(vp)], i32),
+    "pj_free": ([vp, vp], i32),
+    "pj_malloc_pinned": ([vp, u64, C.POINTER(vp)], i32),
+    "pj_free_pinned": ([vp, vp], i32),
+    "pj_memcpy_h2d": ([vp, vp, vp, u64], i32),
+    "pj_memcpy_d2h": ([vp, vp, vp, u64], i32),
+    "pj_memset": ([vp, vp, i32, u64], i32),
+    "pj_mem_in_use": ([vp, C.POINTER(u64)], i32),
+    "pj_prof_enable": ([vp, i32], i32),
+    "pj_prof_reset": ([vp], i32),
+    "pj_prof_get": ([vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], i32),
+    "pj_gen_perm_i32": ([vp, vp, u64, u64], i32),
+    "pj_gen_uniform_i32": ([vp, vp, u64, u64, u64, i64, u64], i32),
+    "pj_gen_uniform_i64": ([vp, vp, u64, u64, u64, i64, u64], i32),
+    "pj_gen_unit_f32": ([vp, vp, u64, u64, u64], i32),
+    "pj_gen_zipf_fk_i32": ([vp, vp, u64, u64, u64, u64, u64, C.c_double, u64], i32),
+    "pj_gen_zipf_i32": ([vp, vp, u64, u64, u64, u64, C.c_double, u64], i32),
+    "pj_join_chained_reduce": ([vp, vp, vp, u64, vp, vp, u64, i32, u64, C.POINTER(JoinAgg)],
+                               i32),
+    "pj_join_chained_pairs": ([vp, vp, u64, vp, u64, i32, u64, vp, vp, u64, C.POINTER(u64)],
+                              i32),
+    "pj_join_partitioned_reduce": ([vp, vp, vp, u64, vp, vp, u64, u64, u64, C.POINTER(RadixCfg),
+                                    C.POINTER(JoinAgg)], i32),
+    "pj_join_partitioned_config": ([vp, u64, u64, C.POINTER(RadixCfg)], i32),
+    "pj_autotune_enable": ([vp, i32], i32),
+    "pj_groupby_i32": ([vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, C.POINTER(u64)],
+                       i32),
+    "pj_reduce_i64": ([vp, vp, u64, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)], i32),
+    "pj_hash_rearrange_i32": ([vp, vp, vp, u64, u32, vp, vp, C.POINTER(u64)], i32),
+    "pj_register_column": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, i32], i32),
+    "pj_clear_catalog": ([vp], i32),
+    "pj_plan_execute": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, u64], i32),
+}
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+def lib():
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(rc):
+    if rc != PJ_OK:
+        raise PjError(rc, _lib.pj_last_error().decode(errors="replace"))
+
+
+_DT = {np.dtype(np.int32): "int32", np.dtype(np.int64): "int64",
+       np.dtype(np.float32): "float32", np.dtype(np.float64): "float64",
+       np.dtype(np.uint32): "uint32", np.dtype(np.uint8): "uint8"}
+
+
+class DeviceArray:
+    """A column in HBM owned by a Context (freed with free() or the context)."""
+
+    def __init__(self, ctx, ptr, n, dtype):
+        self.ctx, self.ptr, self.n, self.dtype = ctx, ptr, n, np.dtype(dtype)
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            _check(_lib.pj_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(_lib.pj_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __len__(self):
+        return self.n
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return int(a)
+
+
+class Context:
+    """One device context: stream, HBM pool (optional byte budget), profiler."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = vp()
+        _check(_lib.pj_ctx_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(_lib.pj_ctx_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        n, hbm, lds = C.c_int(), u64(), u64()
+        _check(_lib.pj_ctx_device_info(self.h, C.byref(n), C.byref(hbm), C.byref(lds)))
+        return {"num_cus": n.value, "hbm_bytes": hbm.value, "lds_per_cu": lds.value}
+
+    def sync(self):
+        _check(_lib.pj_ctx_sync(self.h))
+
+    # ---- memory ----------------------------------------------------------
+    def empty(self, n, dtype):
+        dtype = np.dtype(dtype)
+        p = vp()
+        _check(_lib.pj_malloc(self.h, max(n * dtype.itemsize, 1), C.byref(p)))
+        return DeviceArray(self, p.value, n, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = self.empty(len(a), a.dtype)
+        if len(a):
+            _check(_lib.pj_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def mem_in_use(self):
+        b = u64()
+        _check(_lib.pj_mem_in_use(self.h, C.byref(b)))
+        return b.value
+
+    # ---- profiling -------------------------------------------------------
+    def prof_enable(self, on=True):
+        _check(_lib.pj_prof_enable(self.h, 1 if on else 0))
+
+    def prof_reset(self):
+        _check(_lib.pj_prof_reset(self.h))
+
+    def prof_get(self, prefix):
+        t, n = C.c_double(), u64()
+        _check(_lib.pj_prof_get(self.h, prefix.encode(), C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    # ---- generators ------------------------------------------------------
+    def gen_perm_i32(self, n, seed):
+        d = self.empty(n, np.int32)
+        _check(_lib.pj_gen_perm_i32(self.h, d.ptr, n, seed))
+        return d
+
+    def gen_uniform_i32(self, n, seed, stream, lo, rng):
+        d = self.empty(n, np.int32)
+        _check(_lib.pj_gen_uniform_i32(self.h, d.ptr, n, seed, stream, lo, rng))
+        return d
+
+    def gen_uniform_i64(self, n, seed, stream, lo, rng):
+        d = self.empty(n, np.int64)
+        _check(_lib.pj_gen_uniform_i64(self.h, d.ptr, n, seed, stream, lo, rng))
+        return d
+
+    def gen_unit_f32(self, n, seed, stream):
+        d = self.empty(n, np.float32)
+        _check(_lib.pj_gen_unit_f32(self.h, d.ptr, n, seed, stream))
+        return d
+
+    def gen_zipf_fk_i32(self, n, seed, stream, nbuild, rng, theta, perm_seed):
+        d = self.empty(n, np.int32)
+        _check(_lib.pj_gen_zipf_fk_i32(self.h, d.ptr, n, seed, stream, nbuild, rng, theta,
+                                       perm_seed))
+        return d
+
+    def gen_zipf_i32(self, n, seed, stream, g, theta, perm_seed):
+        d = self.empty(n, np.int32)
+        _check(_lib.pj_gen_zipf_i32(self.h, d.ptr, n, seed, stream, g, theta, perm_seed))
+        return d
+
+    # ---- operators -------------------------------------------------------
+    def join_chained_reduce(self, bk, bv, pk, pv, hash_bits, max_build=None):
+        out = JoinAgg()
+        nb, np_ = len(bk), len(pk)
+        _check(_lib.pj_join_chained_reduce(self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk), _ptr(pv),
+                                           np_, hash_bits,
+                                           nb if max_build is None else max_build,
+                                           C.byref(out)))
+        return out.as_dict()
+
+    def join_chained_pairs(self, bk, pk, hash_bits, cap, max_build=None):
+        ob = self.empty(max(cap, 1), np.uint32)
+        op = self.empty(max(cap, 1), np.uint32)
+        n = u64()
+        rc = _lib.pj_join_chained_pairs(self.h, _ptr(bk), len(bk), _ptr(pk), len(pk), hash_bits,
+                                        len(bk) if max_build is None else max_build, ob.ptr,
+                                        op.ptr, cap, C.byref(n))
+        if rc not in (PJ_OK, PJ_E_CAPACITY):
+            ob.free()
+            op.free()
+            _check(rc)
+        return rc, n.value, ob, op
+
+    def join_partitioned_reduce(self, bk, bv, pk, pv, max_build=None, max_probe=None, cfg=None):
+        out = JoinAgg()
+        c = None
+        if cfg is not None:
+            c = RadixCfg(**cfg) if isinstance(cfg, dict) else cfg
+        nb, np_ = len(bk), len(pk)
+        _check(_lib.pj_join_partitioned_reduce(
+            self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk), _ptr(pv), np_,
+            nb if max_build is None else max_build, np_ if max_probe is None else max_probe,
+            C.byref(c) if c is not None else None, C.byref(out)))
+        return out.as_dict()
+
+    def join_partitioned_config(self, nb, np_):
+        c = RadixCfg()
+        _check(_lib.pj_join_partitioned_config(self.h, nb, np_, C.byref(c)))
+        return c.as_dict()
+
+    def autotune_enable(self, on=True):
+        _check(_lib.pj_autotune_enable(self.h, 1 if on else 0))
+
+    def groupby_i32(self, k, v, w, max_groups, hash_bits=10):
+        ok = self.empty(max(max_groups, 1), np.int32)
+        cols = [self.empty(max(max_groups, 1), t) for t in
+                (np.int64, np.int64, np.int64, np.int64, np.float64)]
+        ng = u64()
+        try:
+            _check(_lib.pj_groupby_i32(self.h, _ptr(k), _ptr(v), _ptr(w), len(k), hash_bits,
+                                       max_groups, ok.ptr, *[c.ptr for c in cols],
+                                       C.byref(ng)))
+            n = ng.value
+            res = {"keys": ok.numpy()[:n]}
+            for name, c in zip(("cnt", "sum", "min", "max", "fsum"), cols):
+                res[name] = c.numpy()[:n]
+            return res
+        finally:
+            ok.free()
+            for c in cols:
+                c.free()
+
+    def reduce_i64(self, v):
+        s, mn, mx = i64(), i64(), i64()
+        _check(_lib.pj_reduce_i64(self.h, _ptr(v), len(v), C.byref(s), C.byref(mn),
+                                  C.byref(mx)))
+        return s.value, mn.value, mx.value
+
+    def hash_rearrange_i32(self, k, v, buckets):
+        ok = self.empty(len(k), np.int32)
+        ov = self.empty(len(k), np.int32) if v is not None else None
+        offs = (u64 * (buckets + 1))()
+        _check(_lib.pj_hash_rearrange_i32(self.h, _ptr(k), _ptr(v), len(k), buckets, ok.ptr,
+                                          None if ov is None else ov.ptr, offs))
+        return ok, ov, np.array(list(offs), np.uint64)
+
+    # ---- plan API --------------------------------------------------------
+    def register_column(self, rel, attr, type_, data):
+        if isinstance(data, DeviceArray):
+            _check(_lib.pj_register_column(self.h, rel.encode(), attr.encode(), type_.encode(),
+                                           data.ptr, data.n, 1))
+        else:
+            a = np.ascontiguousarray(data)
+            self._keep = getattr(self, "_keep", [])
+            self._keep.append(a)
+            _check(_lib.pj_register_column(self.h, rel.encode(), attr.encode(), type_.encode(),
+                                           a.ctypes.data, len(a), 0))
+
+    def clear_catalog(self):
+        _check(_lib.pj_clear_catalog(self.h))
+        self._keep = []
+
+    def execute_plan(self, plan_json, base_dir=".", out_cap=1 << 20):
+        buf = C.create_string_buffer(out_cap)
+        _check(_lib.pj_plan_execute(self.h, plan_json.encode(), base_dir.encode(), buf, out_cap))
+        return buf.value.decode()

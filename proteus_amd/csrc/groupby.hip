@@ -1,0 +1,544 @@
+// groupby / hashgroupby-chained and reduce on one MI355X.
+//
+// Reference: GpuHashGroupByChained (core/olap/lib/operators/gpu/
+// gpu-hash-group-by-chained.cpp:77-556) and HashGroupByChained::generate_scan
+// (core/olap/lib/operators/hash-group-by-chained.cpp:638-853): one global
+// chained table in HBM, CAS insert + chain walk + one global atomic per
+// aggregate per row, then a scan kernel that emits the live entries.
+//
+// Global atomics on MI355X sustain only ~2e10/s (~6e9/s on 1024 hot
+// addresses), so one atomic per aggregate per row would take >100 ms for
+// 2^29 rows.  Here every row is aggregated in LDS:
+//   * a workgroup owns an LDS open-addressing table of SLOTS slots:
+//       slot = 1 << 32 | key (u64, 0 = empty) claimed with one LDS CAS,
+//       COUNT/SUM/MIN/MAX (int64) and FSUM (double) updated with LDS atomics;
+//   * a "unit" (a row range of one partition) is folded into the table, and
+//     the table's live entries are written out compactly (the scan);
+//   * when maxInputSize exceeds what one table holds, the rows are first
+//     radix-partitioned on the top bits of fmix32(key) (the join's scatter
+//     kernel, 16-byte rows), so each partition's groups fit one table;
+//   * units of the same partition are merged by the same kernel (entries in,
+//     entries out) until one unit per partition remains -- no global atomics.
+// Exact int64 arithmetic for COUNT/SUM/MIN/MAX; FSUM accumulates in FP64
+// (Proteus float is FP64, core/olap/lib/values/expressionTypes.cpp:87-89).
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "radix.hpp"
+
+namespace pj {
+
+constexpr int kGbSlots = 2048;   // LDS table slots (96 KiB)
+constexpr int kGbCap = 1024;     // groups per table the planner aims for
+constexpr int kGbBlock = 512;
+constexpr int kGbMergeFanIn = 64;
+
+struct GbEntries {  // SoA entry arrays of one level
+  int32_t *k;
+  int64_t *cnt, *sum, *mn, *mx;
+  double *fs;
+};
+
+struct GbUnit {
+  unsigned long long begin;  // first row (level 0) or first source unit (merge)
+  uint32_t len;              // rows (level 0) or source units (merge)
+  uint32_t part;
+};
+
+__device__ __forceinline__ bool gb_insert(unsigned long long *tk, int32_t key,
+                                          uint32_t &slot) {
+  const unsigned long long mine = (1ULL << 32) | (uint32_t)key;
+  uint32_t h = fmix32((uint32_t)key) & (kGbSlots - 1);
+  for (int probe = 0; probe < kGbSlots; ++probe) {
+    unsigned long long w = tk[h];
+    if (w == mine) {
+      slot = h;
+      return true;
+    }
+    if (w == 0ULL) {
+      unsigned long long old = atomicCAS(&tk[h], 0ULL, mine);
+      if (old == 0ULL || old == mine) {
+        slot = h;
+        return true;
+      }
+    }
+    h = (h + 1) & (kGbSlots - 1);
+  }
+  return false;
+}
+
+// One unit per workgroup iteration.  MERGE = inputs are entries of the
+// previous level's units; otherwise raw rows (k, v?, w?).
+template <bool MERGE>
+__global__ __launch_bounds__(kGbBlock) void k_gb_agg(
+    const GbUnit *__restrict__ units, uint32_t nunits,
+    const int32_t *__restrict__ rk, const int64_t *__restrict__ rv,
+    const float *__restrict__ rw, GbEntries in, const uint32_t *__restrict__ in_count,
+    GbEntries out, uint32_t *__restrict__ out_count, uint32_t *__restrict__ overflow) {
+  __shared__ unsigned long long tk[kGbSlots];
+  __shared__ unsigned long long tcnt[kGbSlots];
+  __shared__ long long tsum[kGbSlots], tmin[kGbSlots], tmax[kGbSlots];
+  __shared__ double tfs[kGbSlots];
+  __shared__ uint32_t wave_off[kGbBlock / 64 + 1];
+  const int tid = threadIdx.x;
+  for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    __syncthreads();
+    for (int i = tid; i < kGbSlots; i += kGbBlock) {
+      tk[i] = 0ULL;
+      tcnt[i] = 0ULL;
+      tsum[i] = 0;
+      tmin[i] = INT64_MAX;
+      tmax[i] = INT64_MIN;
+      tfs[i] = 0.0;
+    }
+    __syncthreads();
+    const GbUnit U = units[u];
+    bool ovf = false;
+    if (!MERGE) {
+      const uint64_t b = U.begin;
+      const uint32_t n = U.len;
+      for (uint32_t j = tid; j < n; j += kGbBlock) {
+        int32_t key = rk[b + j];
+        int64_t v = rv ? rv[b + j] : 0;
+        double w = rw ? (double)rw[b + j] : 0.0;
+        uint32_t s;
+        if (!gb_insert(tk, key, s)) {
+          ovf = true;
+          continue;
+        }
+        atomicAdd(&tcnt[s], 1ULL);
+        if (rv) {
+          atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)v);
+          atomicMin(&tmin[s], (long long)v);
+          atomicMax(&tmax[s], (long long)v);
+        }
+        if (rw) atomicAdd(&tfs[s], w);
+      }
+    } else {
+      for (uint32_t q = 0; q < U.len; ++q) {
+        const uint64_t src = U.begin + q;
+        const uint32_t n = in_count[src];
+        const uint64_t b = src * kGbSlots;
+        for (uint32_t j = tid; j < n; j += kGbBlock) {
+          uint32_t s;
+          if (!gb_insert(tk, in.k[b + j], s)) {
+            ovf = true;
+            continue;
+          }
+          atomicAdd(&tcnt[s], (unsigned long long)in.cnt[b + j]);
+          atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)in.sum[b + j]);
+          atomicMin(&tmin[s], (long long)in.mn[b + j]);
+          atomicMax(&tmax[s], (long long)in.mx[b + j]);
+          atomicAdd(&tfs[s], in.fs[b + j]);
+        }
+      }
+    }
+    if (ovf) atomicOr(overflow, 1u);
+    __syncthreads();
+    // scan: compact live slots into this unit's output area (wave ballots)
+    const int lane = tid & 63, wid = tid >> 6;
+    uint32_t running = 0;
+    for (int base = 0; base < kGbSlots; base += kGbBlock) {
+      int i = base + tid;
+      bool live = tk[i] != 0ULL;
+      unsigned long long bal = __ballot(live);
+      uint32_t before = __popcll(bal & ((1ULL << lane) - 1ULL));
+      if (lane == 0) wave_off[wid] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+      for (int x = 0; x < kGbBlock / 64; ++x) {
+        woff += x < wid ? wave_off[x] : 0;
+        tot += wave_off[x];
+      }
+      if (live) {
+        uint64_t o = (uint64_t)u * kGbSlots + running + woff + before;
+        out.k[o] = (int32_t)(uint32_t)tk[i];
+        out.cnt[o] = (int64_t)tcnt[i];
+        out.sum[o] = tsum[i];
+        out.mn[o] = tmin[i];
+        out.mx[o] = tmax[i];
+        out.fs[o] = tfs[i];
+      }
+      running += tot;
+      __syncthreads();
+    }
+    if (tid == 0) out_count[u] = running;
+  }
+}
+
+// Final compaction: unit u's entries go to [dst[u], dst[u] + count[u]).
+__global__ void k_gb_compact(GbEntries in, const uint32_t *__restrict__ count,
+                             const unsigned long long *__restrict__ dst, uint32_t nunits,
+                             int32_t *ok, int64_t *ocnt, int64_t *osum, int64_t *omin,
+                             int64_t *omax, double *ofs) {
+  for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    uint32_t n = count[u];
+    uint64_t s = (uint64_t)u * kGbSlots, d = dst[u];
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+      ok[d + j] = in.k[s + j];
+      if (ocnt) ocnt[d + j] = in.cnt[s + j];
+      if (osum) osum[d + j] = in.sum[s + j];
+      if (omin) omin[d + j] = in.mn[s + j];
+      if (omax) omax[d + j] = in.mx[s + j];
+      if (ofs) ofs[d + j] = in.fs[s + j];
+    }
+  }
+}
+
+struct GbLevel {
+  Scratch k, cnt, sum, mn, mx, fs, count;
+  GbEntries e{};
+  int alloc(pj_ctx *ctx, uint64_t nunits) {
+    uint64_t n = nunits * (uint64_t)kGbSlots;
+    PJ_TRY(k.alloc(ctx, n * 4));
+    PJ_TRY(cnt.alloc(ctx, n * 8));
+    PJ_TRY(sum.alloc(ctx, n * 8));
+    PJ_TRY(mn.alloc(ctx, n * 8));
+    PJ_TRY(mx.alloc(ctx, n * 8));
+    PJ_TRY(fs.alloc(ctx, n * 8));
+    PJ_TRY(count.alloc(ctx, nunits * 4));
+    e = GbEntries{k.as<int32_t>(), cnt.as<int64_t>(), sum.as<int64_t>(), mn.as<int64_t>(),
+                  mx.as<int64_t>(), fs.as<double>()};
+    return PJ_OK;
+  }
+};
+
+constexpr int kGbTile = 2048;
+
+template <int F>
+static int gb_scatter_f(pj_ctx *ctx, const char *name, const int32_t *ik, const int64_t *iv,
+                        const float *iw, int32_t *ok, int64_t *ov, float *ow,
+                        const ScatterPlan &sp, uint32_t grid, int shift) {
+  DigitRadix dig{shift, (uint32_t)(F - 1)};
+  PJ_LAUNCH(ctx, name, (k_scatter<F, kGbTile, 256, DigitRadix, int64_t, float, false>), grid,
+            256, 0, ik, iv, iw, ok, ov, ow, sp, dig);
+  return PJ_OK;
+}
+
+static int gb_scatter(pj_ctx *ctx, const char *name, int fbits, const int32_t *ik,
+                      const int64_t *iv, const float *iw, int32_t *ok, int64_t *ov, float *ow,
+                      const ScatterPlan &sp, uint32_t grid, int shift) {
+  if (sp.ntiles == 0) return PJ_OK;
+  switch (fbits) {
+    case 1: return gb_scatter_f<2>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 2: return gb_scatter_f<4>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 3: return gb_scatter_f<8>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 4: return gb_scatter_f<16>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 5: return gb_scatter_f<32>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 6: return gb_scatter_f<64>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    case 7: return gb_scatter_f<128>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
+    default: return fail(PJ_E_ARG, "group-by radix bits %d outside [1, 7]", fbits);
+  }
+}
+
+struct GbPass {
+  std::vector<uint64_t> reg_start, reg_count;
+  std::vector<uint32_t> tile_begin;
+  std::vector<unsigned long long> cfull, ctail;
+  uint32_t ntiles = 0, tiles_per_wg = 0, grid = 0;
+};
+
+static void gb_plan_pass(GbPass &p, const std::vector<uint64_t> &rs,
+                         const std::vector<uint64_t> &rc, const std::vector<uint64_t> &os,
+                         const std::vector<uint64_t> &oc, int num_cus) {
+  p.reg_start = rs;
+  p.reg_count = rc;
+  p.tile_begin.assign(rs.size() + 1, 0);
+  for (size_t r = 0; r < rs.size(); ++r)
+    p.tile_begin[r + 1] = p.tile_begin[r] + (uint32_t)ceil_div(rc[r], kGbTile);
+  p.ntiles = p.tile_begin.back();
+  p.cfull.resize(os.size());
+  p.ctail.resize(os.size());
+  for (size_t i = 0; i < os.size(); ++i) {
+    p.cfull[i] = os[i];
+    p.ctail[i] = os[i] + oc[i];
+  }
+  uint64_t g = (uint64_t)num_cus * 2;
+  if (g > p.ntiles) g = p.ntiles ? p.ntiles : 1;
+  p.tiles_per_wg = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, g);
+  p.grid = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, p.tiles_per_wg);
+}
+
+template <class T>
+static size_t pk_add(std::vector<uint8_t> &buf, const std::vector<T> &v) {
+  size_t off = (buf.size() + 255) & ~(size_t)255;
+  buf.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+struct GbPassDev {
+  size_t rs, rc, tb, cf, ct;
+};
+static GbPassDev gb_pack(std::vector<uint8_t> &buf, const GbPass &p) {
+  return GbPassDev{pk_add(buf, p.reg_start), pk_add(buf, p.reg_count), pk_add(buf, p.tile_begin),
+                   pk_add(buf, p.cfull), pk_add(buf, p.ctail)};
+}
+static ScatterPlan gb_dev_plan(const GbPass &p, const GbPassDev &o, uint8_t *d) {
+  ScatterPlan s;
+  s.reg_start = reinterpret_cast<const uint64_t *>(d + o.rs);
+  s.reg_count = reinterpret_cast<const uint64_t *>(d + o.rc);
+  s.tile_begin = reinterpret_cast<const uint32_t *>(d + o.tb);
+  s.nreg = (uint32_t)p.reg_start.size();
+  s.ntiles = p.ntiles;
+  s.tiles_per_wg = p.tiles_per_wg;
+  s.cfull = reinterpret_cast<unsigned long long *>(d + o.cf);
+  s.ctail = reinterpret_cast<unsigned long long *>(d + o.ct);
+  return s;
+}
+
+static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, const float *d_w,
+                       uint64_t n, int pbits, uint64_t max_groups, int32_t *d_ok,
+                       int64_t *d_cnt, int64_t *d_sum, int64_t *d_min, int64_t *d_max,
+                       double *d_fs, uint64_t *n_groups, bool *overflowed) {
+  *overflowed = false;
+  *n_groups = 0;
+  // ---- partition (optional) ---------------------------------------------
+  const int32_t *pk = d_k;
+  const int64_t *pv = d_v;
+  const float *pw = d_w;
+  std::vector<uint64_t> part_start{0}, part_count{n};
+  Scratch k1, v1, w1, k2, v2, w2, dplan;
+  Scratch zv, zw;  // zero columns when v / w are absent but partitioning runs
+  if (pbits > 0 && n) {
+    if (!pv) {
+      PJ_TRY(zv.alloc(ctx, n * 8));
+      PJ_TRY(zero(ctx, zv.p, n * 8));
+      pv = zv.as<int64_t>();
+    }
+    if (!pw) {
+      PJ_TRY(zw.alloc(ctx, n * 4));
+      PJ_TRY(zero(ctx, zw.p, n * 4));
+      pw = zw.as<float>();
+    }
+    std::vector<uint32_t> hist;
+    PJ_TRY(radix_histogram(ctx, d_k, n, pbits, hist));
+    int b1 = pbits <= 7 ? pbits : (pbits + 1) / 2;
+    int b2 = pbits - b1;
+    const int F1 = 1 << b1, F2 = 1 << b2;
+    std::vector<uint64_t> r1s(F1), r1c(F1, 0), fs((size_t)F1 * F2), fc((size_t)F1 * F2);
+    uint64_t pos = 0;
+    for (int a = 0; a < F1; ++a) {
+      for (int b = 0; b < F2; ++b) r1c[a] += hist[(size_t)a * F2 + b];
+      pos = (pos + 31) & ~31ULL;
+      r1s[a] = pos;
+      pos += r1c[a];
+    }
+    uint64_t total1 = pos;
+    pos = 0;
+    for (size_t i = 0; i < (size_t)F1 * F2; ++i) {
+      pos = (pos + 31) & ~31ULL;
+      fs[i] = pos;
+      fc[i] = hist[i];
+      pos += hist[i];
+    }
+    uint64_t total2 = pos;
+    GbPass p1, p2;
+    gb_plan_pass(p1, {0}, {n}, r1s, r1c, ctx->num_cus);
+    if (b2) gb_plan_pass(p2, r1s, r1c, fs, fc, ctx->num_cus);
+    std::vector<uint8_t> buf;
+    GbPassDev o1 = gb_pack(buf, p1), o2{};
+    if (b2) o2 = gb_pack(buf, p2);
+    PJ_TRY(dplan.alloc(ctx, buf.size()));
+    PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
+    PJ_TRY(k1.alloc(ctx, total1 * 4 + 256));
+    PJ_TRY(v1.alloc(ctx, total1 * 8 + 256));
+    PJ_TRY(w1.alloc(ctx, total1 * 4 + 256));
+    PJ_TRY(gb_scatter(ctx, "gb_scatter1", b1, pk, pv, pw, k1.as<int32_t>(), v1.as<int64_t>(),
+                      w1.as<float>(), gb_dev_plan(p1, o1, dplan.as<uint8_t>()), p1.grid,
+                      32 - b1));
+    pk = k1.as<int32_t>();
+    pv = v1.as<int64_t>();
+    pw = w1.as<float>();
+    if (b2) {
+      PJ_TRY(k2.alloc(ctx, total2 * 4 + 256));
+      PJ_TRY(v2.alloc(ctx, total2 * 8 + 256));
+      PJ_TRY(w2.alloc(ctx, total2 * 4 + 256));
+      PJ_TRY(gb_scatter(ctx, "gb_scatter2", b2, pk, pv, pw, k2.as<int32_t>(), v2.as<int64_t>(),
+                        w2.as<float>(), gb_dev_plan(p2, o2, dplan.as<uint8_t>()), p2.grid,
+                        32 - pbits));
+      pk = k2.as<int32_t>();
+      pv = v2.as<int64_t>();
+      pw = w2.as<float>();
+      part_start = fs;
+      part_count = fc;
+    } else {
+      part_start = r1s;
+      part_count = r1c;
+    }
+    if (!d_v) pv = nullptr;
+    if (!d_w) pw = nullptr;
+  }
+  // ---- level 0 units -------------------------------------------------------
+  const uint64_t target_units = (uint64_t)ctx->num_cus * 2;
+  uint64_t unit_rows = std::max<uint64_t>(ceil_div(n ? n : 1, target_units), 16384);
+  std::vector<GbUnit> units;
+  std::vector<uint32_t> unit_part;
+  for (size_t p = 0; p < part_start.size(); ++p) {
+    for (uint64_t r = 0; r < part_count[p]; r += unit_rows) {
+      GbUnit u;
+      u.begin = part_start[p] + r;
+      u.len = (uint32_t)std::min<uint64_t>(unit_rows, part_count[p] - r);
+      u.part = (uint32_t)p;
+      units.push_back(u);
+    }
+  }
+  if (units.empty()) return PJ_OK;
+  Scratch ovf, dunits;
+  PJ_TRY(ovf.alloc(ctx, 4));
+  PJ_TRY(zero(ctx, ovf.p, 4));
+  auto level = std::make_unique<GbLevel>();
+  PJ_TRY(level->alloc(ctx, units.size()));
+  PJ_TRY(dunits.alloc(ctx, units.size() * sizeof(GbUnit)));
+  PJ_TRY(h2d(ctx, dunits.p, units.data(), units.size() * sizeof(GbUnit)));
+  {
+    uint32_t grid = (uint32_t)std::min<uint64_t>(units.size(), (uint64_t)ctx->num_cus * 2);
+    GbEntries none{};
+    PJ_LAUNCH(ctx, "gb_agg", k_gb_agg<false>, grid, kGbBlock, 0,
+              dunits.as<const GbUnit>(), (uint32_t)units.size(), pk, pv, pw, none,
+              (const uint32_t *)nullptr, level->e, level->count.as<uint32_t>(),
+              ovf.as<uint32_t>());
+  }
+  // ---- merge levels: fold units of the same partition ------------------
+  // Units are ordered by partition.  Each level merges up to kGbMergeFanIn
+  // units of one partition into one, until every partition has one unit.
+  while (true) {
+    std::vector<GbUnit> next;
+    bool single = true;
+    for (size_t i = 0; i < units.size();) {
+      size_t j = i;
+      while (j < units.size() && units[j].part == units[i].part) ++j;
+      if (j - i > 1) single = false;
+      for (size_t a = i; a < j; a += kGbMergeFanIn) {
+        GbUnit m;
+        m.begin = a;
+        m.len = (uint32_t)std::min<size_t>(kGbMergeFanIn, j - a);
+        m.part = units[i].part;
+        next.push_back(m);
+      }
+      i = j;
+    }
+    if (single) break;
+    auto nl = std::make_unique<GbLevel>();
+    PJ_TRY(nl->alloc(ctx, next.size()));
+    Scratch dn;
+    PJ_TRY(dn.alloc(ctx, next.size() * sizeof(GbUnit)));
+    PJ_TRY(h2d(ctx, dn.p, next.data(), next.size() * sizeof(GbUnit)));
+    uint32_t grid = (uint32_t)std::min<uint64_t>(next.size(), (uint64_t)ctx->num_cus * 2);
+    PJ_LAUNCH(ctx, "gb_merge", k_gb_agg<true>, grid, kGbBlock, 0, dn.as<const GbUnit>(),
+              (uint32_t)next.size(), (const int32_t *)nullptr, (const int64_t *)nullptr,
+              (const float *)nullptr, level->e, level->count.as<const uint32_t>(), nl->e,
+              nl->count.as<uint32_t>(), ovf.as<uint32_t>());
+    PJ_TRY(sync(ctx));
+    units.swap(next);
+    level.swap(nl);
+  }
+  uint32_t of = 0;
+  PJ_TRY(d2h(ctx, &of, ovf.p, 4));
+  if (of) {
+    *overflowed = true;
+    return PJ_OK;
+  }
+  std::vector<uint32_t> cnt(units.size());
+  PJ_TRY(d2h(ctx, cnt.data(), level->count.p, units.size() * 4));
+  std::vector<unsigned long long> dst(units.size());
+  uint64_t tot = 0;
+  for (size_t i = 0; i < units.size(); ++i) {
+    dst[i] = tot;
+    tot += cnt[i];
+  }
+  *n_groups = tot;
+  if (tot > max_groups)
+    return fail(PJ_E_CAPACITY, "group-by produced %llu groups > maxInputSize %llu",
+                (unsigned long long)tot, (unsigned long long)max_groups);
+  Scratch ddst;
+  PJ_TRY(ddst.alloc(ctx, dst.size() * 8));
+  PJ_TRY(h2d(ctx, ddst.p, dst.data(), dst.size() * 8));
+  uint32_t grid = (uint32_t)std::min<uint64_t>(units.size(), 4096);
+  PJ_LAUNCH(ctx, "gb_compact", k_gb_compact, grid, 256, 0, level->e,
+            level->count.as<const uint32_t>(), ddst.as<const unsigned long long>(),
+            (uint32_t)units.size(), d_ok, d_cnt, d_sum, d_min, d_max, d_fs);
+  return sync(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// reduce
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce_i64(const int64_t *__restrict__ v, uint64_t n,
+                                                    unsigned long long *__restrict__ acc) {
+  int64_t s = 0, mn = INT64_MAX, mx = INT64_MIN;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    int64_t x = v[i];
+    s += x;
+    mn = x < mn ? x : mn;
+    mx = x > mx ? x : mx;
+  }
+  s = wave_sum_i64(s);
+  mn = wave_min_i64(mn);
+  mx = wave_max_i64(mx);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&acc[0], (unsigned long long)s);
+    atomicMin((long long *)&acc[1], (long long)mn);
+    atomicMax((long long *)&acc[2], (long long)mx);
+  }
+}
+
+__global__ void k_reduce_init(unsigned long long *acc) {
+  if (threadIdx.x == 0) {
+    acc[0] = 0;
+    acc[1] = (unsigned long long)INT64_MAX;
+    acc[2] = (unsigned long long)INT64_MIN;
+  }
+}
+
+}  // namespace pj
+
+using namespace pj;
+
+
+extern "C" int pj_groupby_i32(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v,
+                              const float *d_w, uint64_t n, int hash_bits,
+                              uint64_t max_groups, int32_t *d_out_k, int64_t *d_cnt,
+                              int64_t *d_sum, int64_t *d_min, int64_t *d_max, double *d_fsum,
+                              uint64_t *n_groups) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  (void)hash_bits;
+  *n_groups = 0;
+  if (!d_out_k) return fail(PJ_E_ARG, "output key column is required");
+  if (n >= (1ULL << 40)) return fail(PJ_E_CAPACITY, "input above the supported row count");
+  int pbits = 0;
+  while (pbits < 14 && ((uint64_t)kGbCap << pbits) < max_groups) ++pbits;
+  while (true) {
+    bool ovf = false;
+    PJ_TRY(groupby_run(ctx, d_k, d_v, d_w, n, pbits, max_groups, d_out_k, d_cnt, d_sum, d_min,
+                       d_max, d_fsum, n_groups, &ovf));
+    if (!ovf) return PJ_OK;
+    if (pbits >= 14)
+      return fail(PJ_E_CAPACITY,
+                  "group-by input has more distinct keys than maxInputSize %llu allows",
+                  (unsigned long long)max_groups);
+    ++pbits;  // a table overflowed: more partitions
+  }
+}
+
+extern "C" int pj_reduce_i64(pj_ctx *ctx, const int64_t *d_v, uint64_t n, int64_t *sum,
+                             int64_t *min, int64_t *max) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Scratch acc;
+  PJ_TRY(acc.alloc(ctx, 64));
+  PJ_LAUNCH(ctx, "reduce_init", k_reduce_init, 1, 64, 0, acc.as<unsigned long long>());
+  if (n) {
+    uint64_t grid = std::min<uint64_t>(ceil_div(n, 256 * 8), (uint64_t)ctx->num_cus * 8);
+    PJ_LAUNCH(ctx, "reduce_i64", k_reduce_i64, grid ? grid : 1, 256, 0, d_v, n,
+              acc.as<unsigned long long>());
+  }
+  int64_t h[3];
+  PJ_TRY(d2h(ctx, h, acc.p, sizeof(h)));
+  if (sum) *sum = h[0];
+  if (min) *min = h[1];
+  if (max) *max = h[2];
+  return PJ_OK;
+}

@@ -1,0 +1,220 @@
+// hashjoin-chained on one MI355X: a global open-addressing table in HBM.
+//
+// Reference: HashJoinChained / GpuHashJoinChained
+// (core/olap/lib/operators/hash-join-chained.cpp:233-566,
+//  core/olap/lib/operators/gpu/gpu-hash-join-chained.cpp:31-73).
+// The reference keeps a chained table: head[2^hash_bits] plus packets
+// {next, key, payloads}, one atomicAdd on a single counter and one atomicXchg
+// per build row, and a dependent head -> packet walk per probe.  Here the
+// build rows go into a linear-probing table of 64-bit slots
+//     slot = key << 32 | (build_row + 1),   0 = empty,
+// claimed with one 64-bit CAS (no shared counter, no sentinel key: every
+// occupied slot has a non-zero low word).  A probe reads consecutive slots of
+// one or two cache lines until an empty slot, emitting every key match, so
+// duplicates on the build side behave as the reference's chain walk.  The
+// slot index is hashInt32(key) mod 2^bits exactly as the reference's bucket
+// (hash-join-chained.cpp:233-239); the table has max(2^hash_bits,
+// 2*next_pow2(n_build)) slots so the load factor stays <= 1/2.
+#include "pj_internal.hpp"
+
+namespace pj {
+
+__global__ __launch_bounds__(256) void k_chained_build(
+    const int32_t *__restrict__ bk, uint64_t nb,
+    unsigned long long *__restrict__ slots, uint64_t mask) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < nb; i += stride) {
+    int32_t k = bk[i];
+    unsigned long long w =
+        ((unsigned long long)(uint32_t)k << 32) | (unsigned long long)(i + 1);
+    uint64_t h = hash_i32(k) & mask;
+    while (true) {
+      unsigned long long old = atomicCAS(&slots[h], 0ULL, w);
+      if (old == 0ULL) break;
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+// Probe with the fused reduce.  Each thread handles 4 probe rows per step so
+// four independent slot loads are in flight.
+__global__ __launch_bounds__(256) void k_chained_probe_reduce(
+    const int32_t *__restrict__ pk, const int32_t *__restrict__ pv, uint64_t np,
+    const int32_t *__restrict__ bv,
+    const unsigned long long *__restrict__ slots, uint64_t mask,
+    unsigned long long *__restrict__ acc_out) {
+  JoinAcc acc;
+  acc.init();
+  uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = tid * 4; base < np; base += nthr * 4) {
+    int32_t k[4];
+    int64_t v[4];
+    uint64_t h[4];
+    bool live[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      uint64_t j = base + u;
+      live[u] = j < np;
+      k[u] = live[u] ? pk[j] : 0;
+      v[u] = (live[u] && pv) ? (int64_t)pv[j] : 0;
+      h[u] = hash_i32(k[u]) & mask;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!live[u]) continue;
+      int64_t m = 0, sb = 0, mnb = INT64_MAX, mxb = INT64_MIN;
+      uint64_t p = h[u];
+      while (true) {
+        unsigned long long w = slots[p];
+        if (w == 0ULL) break;
+        if ((int32_t)(uint32_t)(w >> 32) == k[u]) {
+          uint64_t row = (w & 0xffffffffULL) - 1;
+          int64_t b = bv ? (int64_t)bv[row] : 0;
+          m += 1;
+          sb += b;
+          mnb = b < mnb ? b : mnb;
+          mxb = b > mxb ? b : mxb;
+        }
+        p = (p + 1) & mask;
+      }
+      acc.add(m, sb, mnb, mxb, v[u]);
+    }
+  }
+  join_acc_flush(acc, acc_out);
+}
+
+// Materialising probe: count a lane's matches, reserve them with one atomic
+// per wave (wave64 prefix sum), then walk again and write the pairs.
+__global__ __launch_bounds__(256) void k_chained_probe_pairs(
+    const int32_t *__restrict__ pk, uint64_t np,
+    const unsigned long long *__restrict__ slots, uint64_t mask,
+    uint32_t *__restrict__ out_b, uint32_t *__restrict__ out_p, uint64_t cap,
+    unsigned long long *__restrict__ cursor) {
+  uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t rounds = ceil_div(np, nthr);
+  for (uint64_t r = 0; r < rounds; ++r) {
+    uint64_t j = r * nthr + tid;
+    bool live = j < np;
+    int32_t k = live ? pk[j] : 0;
+    uint64_t h0 = hash_i32(k) & mask;
+    uint32_t m = 0;
+    if (live) {
+      for (uint64_t p = h0;; p = (p + 1) & mask) {
+        unsigned long long w = slots[p];
+        if (w == 0ULL) break;
+        m += ((int32_t)(uint32_t)(w >> 32) == k);
+      }
+    }
+    // inclusive scan of m over the wave
+    uint32_t incl = m;
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t x = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += x;
+    }
+    uint32_t total = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 63 && total) base = atomicAdd(cursor, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+    uint64_t pos = base + incl - m;
+    if (live && m) {
+      for (uint64_t p = h0;; p = (p + 1) & mask) {
+        unsigned long long w = slots[p];
+        if (w == 0ULL) break;
+        if ((int32_t)(uint32_t)(w >> 32) == k) {
+          if (pos < cap) {
+            out_b[pos] = (uint32_t)((w & 0xffffffffULL) - 1);
+            out_p[pos] = (uint32_t)j;
+          }
+          ++pos;
+        }
+      }
+    }
+  }
+}
+
+static int chained_table(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
+                         int hash_bits, uint64_t max_build, Scratch &tab,
+                         uint64_t *mask) {
+  if (hash_bits < 0 || hash_bits > 32)
+    return fail(PJ_E_ARG, "hash_bits %d out of range [0, 32]", hash_bits);
+  if (nb > max_build)
+    return fail(PJ_E_CAPACITY,
+                "Build input size exceeded given parameter: %llu rows > "
+                "maxBuildInputSize %llu",
+                (unsigned long long)nb, (unsigned long long)max_build);
+  if (nb >= 0xffffffffULL) return fail(PJ_E_CAPACITY, "build side above 2^32-1 rows");
+  uint64_t slots = 1ULL << hash_bits;
+  uint64_t need = 2ULL << ilog2_ceil(nb ? nb : 1);
+  if (slots < need) slots = need;
+  if (slots < 64) slots = 64;
+  PJ_TRY(tab.alloc(ctx, slots * 8));
+  PJ_TRY(zero(ctx, tab.p, slots * 8));
+  *mask = slots - 1;
+  if (nb) {
+    uint64_t grid = ceil_div(nb, 256);
+    if (grid > 8192) grid = 8192;
+    PJ_LAUNCH(ctx, "join_chained_build", k_chained_build, grid, 256, 0, d_bk, nb,
+              tab.as<unsigned long long>(), *mask);
+  }
+  return PJ_OK;
+}
+
+static uint64_t probe_grid(pj_ctx *ctx, uint64_t rows_per_thread, uint64_t np) {
+  uint64_t g = ceil_div(np, 256 * rows_per_thread);
+  uint64_t cap = (uint64_t)ctx->num_cus * 16;
+  if (g > cap) g = cap;
+  return g ? g : 1;
+}
+
+}  // namespace pj
+
+using namespace pj;
+
+extern "C" int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
+                                      const int32_t *d_bv, uint64_t nb,
+                                      const int32_t *d_pk, const int32_t *d_pv,
+                                      uint64_t np, int hash_bits,
+                                      uint64_t max_build, pj_join_agg *out) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Scratch tab, acc;
+  uint64_t mask = 0;
+  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask));
+  PJ_TRY(acc.alloc(ctx, 64));
+  PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
+  if (np && nb)
+    PJ_LAUNCH(ctx, "join_chained_probe", k_chained_probe_reduce, probe_grid(ctx, 4, np), 256,
+              0, d_pk, d_pv, np, d_bv, tab.as<const unsigned long long>(), mask,
+              acc.as<unsigned long long>());
+  return join_acc_read(ctx, acc.as<unsigned long long>(), out);
+}
+
+extern "C" int pj_join_chained_pairs(pj_ctx *ctx, const int32_t *d_bk,
+                                     uint64_t nb, const int32_t *d_pk,
+                                     uint64_t np, int hash_bits,
+                                     uint64_t max_build, uint32_t *d_out_b,
+                                     uint32_t *d_out_p, uint64_t cap,
+                                     uint64_t *n_out) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  *n_out = 0;
+  Scratch tab, cur;
+  uint64_t mask = 0;
+  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask));
+  PJ_TRY(cur.alloc(ctx, 8));
+  PJ_TRY(zero(ctx, cur.p, 8));
+  if (np && nb)
+    PJ_LAUNCH(ctx, "join_chained_pairs", k_chained_probe_pairs, probe_grid(ctx, 1, np), 256, 0,
+              d_pk, np, tab.as<const unsigned long long>(), mask, d_out_b, d_out_p, cap,
+              cur.as<unsigned long long>());
+  uint64_t total = 0;
+  PJ_TRY(d2h(ctx, &total, cur.p, 8));
+  *n_out = total;
+  if (total > cap)
+    return fail(PJ_E_CAPACITY, "join produced %llu pairs, output capacity %llu",
+                (unsigned long long)total, (unsigned long long)cap);
+  return PJ_OK;
+}

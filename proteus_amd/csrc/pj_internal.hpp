@@ -1,0 +1,280 @@
+// Internal declarations shared by the HIP translation units of the
+// join / aggregation path (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/proteus_amd.h"
+
+namespace pj {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define PJ_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return ::pj::fail(PJ_E_HIP, "%s failed: %s (%s:%d)", #call,           \
+                        hipGetErrorString(e_), __FILE__, __LINE__);         \
+  } while (0)
+
+#define PJ_TRY(call)               \
+  do {                             \
+    int rc_ = (call);              \
+    if (rc_ != PJ_OK) return rc_;  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// hashing (device + host)
+// ---------------------------------------------------------------------------
+// hashInt32 of core/olap/lib/expressions/expressions-hasher.cpp:39-58:
+// sign-extend, murmur3 fmix with logical shifts, in 64-bit arithmetic.
+__host__ __device__ __forceinline__ uint64_t hash_i32(int32_t k) {
+  uint64_t h = (uint64_t)(int64_t)k;
+  h ^= h >> 16;
+  h *= 0x85ebca6bULL;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35ULL;
+  h ^= h >> 16;
+  return h;
+}
+
+// 32-bit murmur3 finaliser with logical shifts, the in-partition hash of
+// gpu-partitioned-hash-join-chained.cpp:595-610.  Radix digits take its top
+// bits, LDS buckets its low bits.
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bU;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35U;
+  h ^= h >> 16;
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// counter-based RNG (splitmix64 finaliser; Steele, Lea, Flood 2014)
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;
+__host__ __device__ __forceinline__ uint64_t rng_key(uint64_t seed,
+                                                      uint64_t stream) {
+  return mix64(seed + kGolden * (stream + 1));
+}
+__host__ __device__ __forceinline__ uint64_t rng_at(uint64_t key, uint64_t i) {
+  return mix64(key + kGolden * (i + 1));
+}
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// device memory: caching pool with an HBM budget
+// ---------------------------------------------------------------------------
+struct Pool {
+  uint64_t budget = 0;      // 0 = unlimited
+  uint64_t in_use = 0;      // bytes handed out (user + scratch)
+  uint64_t cached = 0;      // bytes held in free lists
+  std::multimap<uint64_t, void *> free_blocks;  // size -> ptr
+  std::unordered_map<void *, uint64_t> live;    // ptr -> size
+};
+
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct TuneKey {
+  int lb, lp;  // log2 buckets of build / probe sizes
+  bool operator<(const TuneKey &o) const {
+    return lb < o.lb || (lb == o.lb && lp < o.lp);
+  }
+};
+
+}  // namespace pj
+
+struct pj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  uint64_t hbm_bytes = 0;
+  uint64_t lds_per_cu = 0;
+  pj::Pool pool;
+  std::mutex mu;
+  // profiling
+  bool prof = false;
+  std::vector<pj::ProfRec> prof_pending;
+  std::vector<hipEvent_t> ev_free;
+  std::map<std::string, std::pair<double, uint64_t>> prof_acc;
+  // pinned staging for small results
+  void *pinned_small = nullptr;
+  // autotuner
+  bool autotune = false;
+  std::map<pj::TuneKey, pj_radix_cfg> tuned;
+};
+
+namespace pj {
+
+// scratch allocation from the context pool (rounded, cached, budgeted)
+int dalloc(pj_ctx *ctx, uint64_t bytes, void **p);
+int dfree(pj_ctx *ctx, void *p);
+
+// RAII scratch buffer
+struct Scratch {
+  pj_ctx *ctx = nullptr;
+  void *p = nullptr;
+  Scratch() = default;
+  Scratch(const Scratch &) = delete;
+  Scratch &operator=(const Scratch &) = delete;
+  ~Scratch() {
+    if (p) dfree(ctx, p);
+  }
+  int alloc(pj_ctx *c, uint64_t bytes) {
+    ctx = c;
+    return dalloc(c, bytes ? bytes : 256, &p);
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// profiling hooks around launches
+void prof_begin(pj_ctx *ctx, const char *name, hipEvent_t *a);
+void prof_end(pj_ctx *ctx, const char *name, hipEvent_t a);
+
+#define PJ_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                 \
+  do {                                                                       \
+    hipEvent_t ev_a_ = nullptr;                                              \
+    ::pj::prof_begin((ctx), (name), &ev_a_);                                 \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shmem),             \
+                       (ctx)->stream, __VA_ARGS__);                          \
+    hipError_t le_ = hipGetLastError();                                      \
+    if (le_ != hipSuccess)                                                   \
+      return ::pj::fail(PJ_E_HIP, "launch of %s failed: %s", (name),         \
+                        hipGetErrorString(le_));                             \
+    ::pj::prof_end((ctx), (name), ev_a_);                                    \
+  } while (0)
+
+int h2d(pj_ctx *ctx, void *d, const void *h, uint64_t bytes);
+int d2h(pj_ctx *ctx, void *h, const void *d, uint64_t bytes);
+int sync(pj_ctx *ctx);
+int zero(pj_ctx *ctx, void *d, uint64_t bytes);
+int fill_u32(pj_ctx *ctx, uint32_t *d, uint32_t v, uint64_t n);
+
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline int ilog2_ceil(uint64_t x) {
+  int k = 0;
+  while ((1ULL << k) < x && k < 63) ++k;
+  return k;
+}
+
+// streaming 16-byte load (global_load_dwordx4 ... nt)
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 ld_nt4(const void *p) {
+  v4i_t x = __builtin_nontemporal_load(reinterpret_cast<const v4i_t *>(p));
+  return make_int4(x.x, x.y, x.z, x.w);
+}
+
+// wave64 helpers --------------------------------------------------------------
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int64_t x = __shfl_xor(v, o, 64);
+    v = x < v ? x : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int64_t x = __shfl_xor(v, o, 64);
+    v = x > v ? x : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// join aggregate accumulator ---------------------------------------------------
+struct JoinAcc {
+  int64_t count, sum_b, sum_p, min_b, max_b, min_p, max_p;
+  __device__ __forceinline__ void init() {
+    count = 0;
+    sum_b = 0;
+    sum_p = 0;
+    min_b = INT64_MAX;
+    max_b = INT64_MIN;
+    min_p = INT64_MAX;
+    max_p = INT64_MIN;
+  }
+  // `m` matches of one probe row whose build values sum to sb and span
+  // [mnb, mxb].
+  __device__ __forceinline__ void add(int64_t m, int64_t sb, int64_t mnb,
+                                      int64_t mxb, int64_t pv) {
+    count += m;
+    sum_b += sb;
+    sum_p += m * pv;
+    if (m) {
+      min_b = mnb < min_b ? mnb : min_b;
+      max_b = mxb > max_b ? mxb : max_b;
+      min_p = pv < min_p ? pv : min_p;
+      max_p = pv > max_p ? pv : max_p;
+    }
+  }
+};
+
+// Wave-reduce a JoinAcc and fold lane 0's result into the 7-slot global
+// accumulator (count, sum_b, sum_p as u64 adds; min/max as signed atomics).
+__device__ __forceinline__ void join_acc_flush(JoinAcc a,
+                                               unsigned long long *g) {
+  int64_t c = wave_sum_i64(a.count);
+  int64_t sb = wave_sum_i64(a.sum_b);
+  int64_t sp = wave_sum_i64(a.sum_p);
+  int64_t mnb = wave_min_i64(a.min_b);
+  int64_t mxb = wave_max_i64(a.max_b);
+  int64_t mnp = wave_min_i64(a.min_p);
+  int64_t mxp = wave_max_i64(a.max_p);
+  if ((threadIdx.x & 63) == 0 && c != 0) {
+    atomicAdd(&g[0], (unsigned long long)c);
+    atomicAdd(&g[1], (unsigned long long)sb);
+    atomicAdd(&g[2], (unsigned long long)sp);
+    atomicMin((long long *)&g[3], (long long)mnb);
+    atomicMax((long long *)&g[4], (long long)mxb);
+    atomicMin((long long *)&g[5], (long long)mnp);
+    atomicMax((long long *)&g[6], (long long)mxp);
+  }
+}
+
+int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc);
+int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
+                  pj_join_agg *out);
+
+}  // namespace pj
