@@ -69,6 +69,7 @@ _SIGS = {
     "pj_memcpy_d2h": ([vp, vp, vp, u64], i32),
     "pj_memset": ([vp, vp, i32, u64], i32),
     "pj_mem_in_use": ([vp, C.POINTER(u64)], i32),
+    "pj_copy": ([vp, vp, vp, u64], i32),
     "pj_prof_enable": ([vp, i32], i32),
     "pj_prof_reset": ([vp], i32),
     "pj_prof_get": ([vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], i32),
@@ -191,6 +192,9 @@ class Context:
         if len(a):
             _check(_lib.pj_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
         return d
+
+    def copy(self, dst, src, nbytes):
+        _check(_lib.pj_copy(self.h, _ptr(dst), _ptr(src), nbytes))
 
     def mem_in_use(self):
         b = u64()

@@ -7,6 +7,7 @@
 // gpu-common.hpp:93,383-398).  On one MI355X all of it collapses to one
 // context: one stream, one caching HBM pool with a byte budget, pinned staging
 // for small results, and status codes instead of aborts.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -88,8 +89,9 @@ int dfree(pj_ctx *ctx, void *p) {
   pool.in_use -= sz;
   pool.free_blocks.emplace(sz, p);
   pool.cached += sz;
-  // keep the cache bounded: drop the largest blocks beyond 8 GiB cached
-  while (pool.cached > (8ULL << 30) && !pool.free_blocks.empty()) {
+  // keep the cache bounded: drop the largest blocks beyond a quarter of HBM
+  const uint64_t cap = std::max<uint64_t>(8ULL << 30, ctx->hbm_bytes / 4);
+  while (pool.cached > cap && !pool.free_blocks.empty()) {
     auto last = std::prev(pool.free_blocks.end());
     // all work on this context is stream-ordered; free after the stream drains
     (void)hipStreamSynchronize(ctx->stream);
@@ -193,6 +195,19 @@ static void prof_collect(pj_ctx *ctx) {
     ctx->ev_free.push_back(r.b);
   }
   ctx->prof_pending.clear();
+}
+
+// Streaming device copy: 16-byte loads and stores, grid-stride.
+__global__ __launch_bounds__(256) void k_copy16(const int4 *__restrict__ src,
+                                                int4 *__restrict__ dst, uint64_t n16) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t s = (uint64_t)gridDim.x * 256;
+  for (; i < n16; i += s) dst[i] = ld_nt4(src + i);
+}
+__global__ void k_copy1(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += s) dst[i] = src[i];
 }
 
 // join accumulator: 7 x u64 in device memory -------------------------------
@@ -493,6 +508,20 @@ int pj_memset(pj_ctx *ctx, void *d_dst, int byte, uint64_t bytes) {
       if (grid > 4096) grid = 4096;
       PJ_LAUNCH(ctx, "fill_u8", k_fill_u8, grid, 256, 0, (uint8_t *)d_dst, (uint8_t)b, bytes);
     }
+  }
+  return sync(ctx);
+}
+
+int pj_copy(pj_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes) {
+  if (!bytes) return PJ_OK;
+  if ((((uintptr_t)d_dst | (uintptr_t)d_src | bytes) & 15) == 0) {
+    const uint64_t n16 = bytes / 16;
+    uint64_t grid = std::min<uint64_t>(ceil_div(n16, 256), (uint64_t)ctx->num_cus * 16);
+    PJ_LAUNCH(ctx, "copy", k_copy16, grid, 256, 0, (const int4 *)d_src, (int4 *)d_dst, n16);
+  } else {
+    uint64_t grid = std::min<uint64_t>(ceil_div(bytes, 256), (uint64_t)ctx->num_cus * 16);
+    PJ_LAUNCH(ctx, "copy", k_copy1, grid, 256, 0, (const uint8_t *)d_src, (uint8_t *)d_dst,
+              bytes);
   }
   return sync(ctx);
 }

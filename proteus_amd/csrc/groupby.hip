@@ -204,88 +204,28 @@ struct GbLevel {
   }
 };
 
-constexpr int kGbTile = 2048;
-
 template <int F>
-static int gb_scatter_f(pj_ctx *ctx, const char *name, const int32_t *ik, const int64_t *iv,
-                        const float *iw, int32_t *ok, int64_t *ov, float *ow,
-                        const ScatterPlan &sp, uint32_t grid, int shift) {
-  DigitRadix dig{shift, (uint32_t)(F - 1)};
-  PJ_LAUNCH(ctx, name, (k_scatter<F, kGbTile, 256, DigitRadix, int64_t, float, false>), grid,
-            256, 0, ik, iv, iw, ok, ov, ow, sp, dig);
-  return PJ_OK;
+static int gb_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
+                   const int64_t *iv, const float *iw, int32_t *ok, int64_t *ov, float *ow,
+                   uint64_t n, const RadixPlan &p, JointRadix j, RadixHist &h) {
+  return radix_launch_pass<F, 2048, 256, JointRadix, int64_t, float, false>(
+      ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
 }
 
-static int gb_scatter(pj_ctx *ctx, const char *name, int fbits, const int32_t *ik,
-                      const int64_t *iv, const float *iw, int32_t *ok, int64_t *ov, float *ow,
-                      const ScatterPlan &sp, uint32_t grid, int shift) {
-  if (sp.ntiles == 0) return PJ_OK;
-  switch (fbits) {
-    case 1: return gb_scatter_f<2>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 2: return gb_scatter_f<4>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 3: return gb_scatter_f<8>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 4: return gb_scatter_f<16>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 5: return gb_scatter_f<32>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 6: return gb_scatter_f<64>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    case 7: return gb_scatter_f<128>(ctx, name, ik, iv, iw, ok, ov, ow, sp, grid, shift);
-    default: return fail(PJ_E_ARG, "group-by radix bits %d outside [1, 7]", fbits);
+static int gb_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
+                        const int32_t *ik, const int64_t *iv, const float *iw, int32_t *ok,
+                        int64_t *ov, float *ow, uint64_t n, const RadixPlan &p, JointRadix j,
+                        RadixHist &h) {
+  switch (bits) {
+    case 1: return gb_pass<2>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 2: return gb_pass<4>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 3: return gb_pass<8>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 4: return gb_pass<16>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 5: return gb_pass<32>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 6: return gb_pass<64>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    case 7: return gb_pass<128>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
+    default: return fail(PJ_E_ARG, "group-by radix bits %d outside [1, 7]", bits);
   }
-}
-
-struct GbPass {
-  std::vector<uint64_t> reg_start, reg_count;
-  std::vector<uint32_t> tile_begin;
-  std::vector<unsigned long long> cfull, ctail;
-  uint32_t ntiles = 0, tiles_per_wg = 0, grid = 0;
-};
-
-static void gb_plan_pass(GbPass &p, const std::vector<uint64_t> &rs,
-                         const std::vector<uint64_t> &rc, const std::vector<uint64_t> &os,
-                         const std::vector<uint64_t> &oc, int num_cus) {
-  p.reg_start = rs;
-  p.reg_count = rc;
-  p.tile_begin.assign(rs.size() + 1, 0);
-  for (size_t r = 0; r < rs.size(); ++r)
-    p.tile_begin[r + 1] = p.tile_begin[r] + (uint32_t)ceil_div(rc[r], kGbTile);
-  p.ntiles = p.tile_begin.back();
-  p.cfull.resize(os.size());
-  p.ctail.resize(os.size());
-  for (size_t i = 0; i < os.size(); ++i) {
-    p.cfull[i] = os[i];
-    p.ctail[i] = os[i] + oc[i];
-  }
-  uint64_t g = (uint64_t)num_cus * 2;
-  if (g > p.ntiles) g = p.ntiles ? p.ntiles : 1;
-  p.tiles_per_wg = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, g);
-  p.grid = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, p.tiles_per_wg);
-}
-
-template <class T>
-static size_t pk_add(std::vector<uint8_t> &buf, const std::vector<T> &v) {
-  size_t off = (buf.size() + 255) & ~(size_t)255;
-  buf.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
-  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
-  return off;
-}
-
-struct GbPassDev {
-  size_t rs, rc, tb, cf, ct;
-};
-static GbPassDev gb_pack(std::vector<uint8_t> &buf, const GbPass &p) {
-  return GbPassDev{pk_add(buf, p.reg_start), pk_add(buf, p.reg_count), pk_add(buf, p.tile_begin),
-                   pk_add(buf, p.cfull), pk_add(buf, p.ctail)};
-}
-static ScatterPlan gb_dev_plan(const GbPass &p, const GbPassDev &o, uint8_t *d) {
-  ScatterPlan s;
-  s.reg_start = reinterpret_cast<const uint64_t *>(d + o.rs);
-  s.reg_count = reinterpret_cast<const uint64_t *>(d + o.rc);
-  s.tile_begin = reinterpret_cast<const uint32_t *>(d + o.tb);
-  s.nreg = (uint32_t)p.reg_start.size();
-  s.ntiles = p.ntiles;
-  s.tiles_per_wg = p.tiles_per_wg;
-  s.cfull = reinterpret_cast<unsigned long long *>(d + o.cf);
-  s.ctail = reinterpret_cast<unsigned long long *>(d + o.ct);
-  return s;
 }
 
 static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, const float *d_w,
@@ -299,7 +239,8 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
   const int64_t *pv = d_v;
   const float *pw = d_w;
   std::vector<uint64_t> part_start{0}, part_count{n};
-  Scratch k1, v1, w1, k2, v2, w2, dplan;
+  Scratch k1, v1, w1, k2, v2, w2;
+  RadixHist rh;
   Scratch zv, zw;  // zero columns when v / w are absent but partitioning runs
   if (pbits > 0 && n) {
     if (!pv) {
@@ -312,61 +253,33 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
       PJ_TRY(zero(ctx, zw.p, n * 4));
       pw = zw.as<float>();
     }
-    std::vector<uint32_t> hist;
-    PJ_TRY(radix_histogram(ctx, d_k, n, pbits, hist));
-    int b1 = pbits <= 7 ? pbits : (pbits + 1) / 2;
-    int b2 = pbits - b1;
-    const int F1 = 1 << b1, F2 = 1 << b2;
-    std::vector<uint64_t> r1s(F1), r1c(F1, 0), fs((size_t)F1 * F2), fc((size_t)F1 * F2);
-    uint64_t pos = 0;
-    for (int a = 0; a < F1; ++a) {
-      for (int b = 0; b < F2; ++b) r1c[a] += hist[(size_t)a * F2 + b];
-      pos = (pos + 31) & ~31ULL;
-      r1s[a] = pos;
-      pos += r1c[a];
-    }
-    uint64_t total1 = pos;
-    pos = 0;
-    for (size_t i = 0; i < (size_t)F1 * F2; ++i) {
-      pos = (pos + 31) & ~31ULL;
-      fs[i] = pos;
-      fc[i] = hist[i];
-      pos += hist[i];
-    }
-    uint64_t total2 = pos;
-    GbPass p1, p2;
-    gb_plan_pass(p1, {0}, {n}, r1s, r1c, ctx->num_cus);
-    if (b2) gb_plan_pass(p2, r1s, r1c, fs, fc, ctx->num_cus);
-    std::vector<uint8_t> buf;
-    GbPassDev o1 = gb_pack(buf, p1), o2{};
-    if (b2) o2 = gb_pack(buf, p2);
-    PJ_TRY(dplan.alloc(ctx, buf.size()));
-    PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
-    PJ_TRY(k1.alloc(ctx, total1 * 4 + 256));
-    PJ_TRY(v1.alloc(ctx, total1 * 8 + 256));
-    PJ_TRY(w1.alloc(ctx, total1 * 4 + 256));
-    PJ_TRY(gb_scatter(ctx, "gb_scatter1", b1, pk, pv, pw, k1.as<int32_t>(), v1.as<int64_t>(),
-                      w1.as<float>(), gb_dev_plan(p1, o1, dplan.as<uint8_t>()), p1.grid,
-                      32 - b1));
+    const int b1 = pbits <= 7 ? pbits : (pbits + 1) / 2;
+    const int b2 = pbits - b1;
+    RadixPlan plan = radix_make_plan(ctx, n, b1, b2, 1 << pbits, 2);
+    PJ_TRY(rh.alloc(ctx, plan));
+    JointRadix j{32 - pbits};
+    PJ_TRY(radix_histograms(ctx, d_k, n, plan, j, rh, true));
+    PJ_TRY(k1.alloc(ctx, n * 4 + 256));
+    PJ_TRY(v1.alloc(ctx, n * 8 + 256));
+    PJ_TRY(w1.alloc(ctx, n * 4 + 256));
+    PJ_TRY(gb_pass_bits(ctx, "gb_scatter1", b1, false, pk, pv, pw, k1.as<int32_t>(),
+                        v1.as<int64_t>(), w1.as<float>(), n, plan, j, rh));
     pk = k1.as<int32_t>();
     pv = v1.as<int64_t>();
     pw = w1.as<float>();
     if (b2) {
-      PJ_TRY(k2.alloc(ctx, total2 * 4 + 256));
-      PJ_TRY(v2.alloc(ctx, total2 * 8 + 256));
-      PJ_TRY(w2.alloc(ctx, total2 * 4 + 256));
-      PJ_TRY(gb_scatter(ctx, "gb_scatter2", b2, pk, pv, pw, k2.as<int32_t>(), v2.as<int64_t>(),
-                        w2.as<float>(), gb_dev_plan(p2, o2, dplan.as<uint8_t>()), p2.grid,
-                        32 - pbits));
+      PJ_TRY(k2.alloc(ctx, n * 4 + 256));
+      PJ_TRY(v2.alloc(ctx, n * 8 + 256));
+      PJ_TRY(w2.alloc(ctx, n * 4 + 256));
+      PJ_TRY(gb_pass_bits(ctx, "gb_scatter2", b2, true, pk, pv, pw, k2.as<int32_t>(),
+                          v2.as<int64_t>(), w2.as<float>(), n, plan, j, rh));
       pk = k2.as<int32_t>();
       pv = v2.as<int64_t>();
       pw = w2.as<float>();
-      part_start = fs;
-      part_count = fc;
-    } else {
-      part_start = r1s;
-      part_count = r1c;
     }
+    const int NB = 1 << pbits;
+    part_start.assign(rh.hS.begin(), rh.hS.begin() + NB);
+    part_count.assign(rh.hT.begin(), rh.hT.end());
     if (!d_v) pv = nullptr;
     if (!d_w) pw = nullptr;
   }

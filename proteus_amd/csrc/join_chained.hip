@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_chained_probe_reduce(
   uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = tid * 4; base < np; base += nthr * 4) {
     int32_t k[4];
-    int64_t v[4];
+    int32_t v[4];
     uint64_t h[4];
     bool live[4];
 #pragma unroll
@@ -58,24 +58,25 @@ __global__ __launch_bounds__(256) void k_chained_probe_reduce(
       uint64_t j = base + u;
       live[u] = j < np;
       k[u] = live[u] ? pk[j] : 0;
-      v[u] = (live[u] && pv) ? (int64_t)pv[j] : 0;
+      v[u] = (live[u] && pv) ? pv[j] : 0;
       h[u] = hash_i32(k[u]) & mask;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (!live[u]) continue;
-      int64_t m = 0, sb = 0, mnb = INT64_MAX, mxb = INT64_MIN;
+      int64_t m = 0, sb = 0;
+      int32_t mnb = INT32_MAX, mxb = INT32_MIN;
       uint64_t p = h[u];
       while (true) {
         unsigned long long w = slots[p];
         if (w == 0ULL) break;
         if ((int32_t)(uint32_t)(w >> 32) == k[u]) {
           uint64_t row = (w & 0xffffffffULL) - 1;
-          int64_t b = bv ? (int64_t)bv[row] : 0;
+          int32_t b = bv ? bv[row] : 0;
           m += 1;
           sb += b;
-          mnb = b < mnb ? b : mnb;
-          mxb = b > mxb ? b : mxb;
+          mnb = min(mnb, b);
+          mxb = max(mxb, b);
         }
         p = (p + 1) & mask;
       }

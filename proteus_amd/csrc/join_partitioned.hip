@@ -29,45 +29,58 @@
 
 namespace pj {
 
-__global__ void k_hist_reduce(const uint32_t *__restrict__ partial, int rows,
-                              int nb, uint32_t *__restrict__ hist) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  uint32_t s = 0;
-  for (int r = 0; r < rows; ++r) s += partial[(uint64_t)r * nb + i];
-  hist[i] = s;
+__global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb,
+                          uint32_t *__restrict__ E, uint32_t *__restrict__ T) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nb) return;
+  uint32_t run = 0;
+  for (int c = 0; c < G; ++c) {
+    const uint64_t i = (uint64_t)c * nb + d;
+    const uint32_t h = H[i];
+    E[i] = run;
+    run += h;
+  }
+  T[d] = run;
 }
 
-// Dynamic-LDS histogram over 2^bits bins (bits <= 15).
-__global__ __launch_bounds__(1024) void k_hist_dyn(const int32_t *__restrict__ keys,
-                                                   uint64_t n, int bits,
-                                                   uint32_t *__restrict__ partial) {
-  extern __shared__ uint32_t hsh[];
-  const int nb = 1 << bits;
-  const int shift = 32 - bits;
-  for (int i = threadIdx.x; i < nb; i += 1024) hsh[i] = 0;
+__global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__ T, int nb,
+                                                    unsigned long long *__restrict__ S) {
+  __shared__ unsigned long long part[1024];
+  const int tid = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int lo = tid * per, hi = lo + per < nb ? lo + per : nb;
+  unsigned long long sum = 0;
+  for (int i = lo; i < hi; ++i) sum += T[i];
+  part[tid] = sum;
   __syncthreads();
-  uint64_t tid = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
-  uint64_t nthr = (uint64_t)gridDim.x * 1024;
-  if (((uintptr_t)keys & 15) == 0) {
-    const int4 *kv = reinterpret_cast<const int4 *>(keys);
-    uint64_t nvec = n / 4;
-    for (uint64_t i = tid; i < nvec; i += nthr) {
-      int4 k = ld_nt4(&kv[i]);
-      atomicAdd(&hsh[fmix32((uint32_t)k.x) >> shift], 1u);
-      atomicAdd(&hsh[fmix32((uint32_t)k.y) >> shift], 1u);
-      atomicAdd(&hsh[fmix32((uint32_t)k.z) >> shift], 1u);
-      atomicAdd(&hsh[fmix32((uint32_t)k.w) >> shift], 1u);
-    }
-    for (uint64_t i = nvec * 4 + tid; i < n; i += nthr)
-      atomicAdd(&hsh[fmix32((uint32_t)keys[i]) >> shift], 1u);
-  } else {
-    for (uint64_t i = tid; i < n; i += nthr)
-      atomicAdd(&hsh[fmix32((uint32_t)keys[i]) >> shift], 1u);
+  // Hillis-Steele scan of the 1024 partial sums
+  for (int o = 1; o < 1024; o <<= 1) {
+    unsigned long long x = tid >= o ? part[tid - o] : 0ULL;
+    __syncthreads();
+    part[tid] += x;
+    __syncthreads();
   }
-  __syncthreads();
-  uint32_t *row = partial + (uint64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += 1024) row[i] = hsh[i];
+  unsigned long long run = part[tid] - sum;
+  for (int i = lo; i < hi; ++i) {
+    S[i] = run;
+    run += T[i];
+  }
+  if (tid == 1023) S[nb] = part[1023];
+}
+
+RadixPlan radix_make_plan(pj_ctx *ctx, uint64_t n, int b1, int b2, int nb, int per_cu) {
+  RadixPlan p;
+  p.b1 = b1;
+  p.b2 = b2;
+  p.nb = nb;
+  uint64_t g = (uint64_t)ctx->num_cus * (per_cu > 0 ? per_cu : 1);
+  uint64_t min_chunk = 16384;
+  uint64_t gmax = ceil_div(n ? n : 1, min_chunk);
+  if (g > gmax) g = gmax;
+  if (g < 1) g = 1;
+  p.chunk = (ceil_div(n ? n : 1, g) + 3) & ~3ULL;
+  p.G = (int)ceil_div(n ? n : 1, p.chunk);
+  return p;
 }
 
 struct JoinUnit {
@@ -75,18 +88,30 @@ struct JoinUnit {
   uint32_t r_len, s_len;
 };
 
-// Per-unit LDS chained table + probe.  Units [wg_begin[b], wg_begin[b+1])
-// belong to workgroup b; consecutive units of one partition reuse the table.
+// Per-unit LDS table + probe.  Units [wg_begin[b], wg_begin[b+1]) belong to
+// workgroup b; consecutive units of one build partition reuse the table.
+//
+// The table is bucket-sorted ("CSR"): tab[b] packs the [start, end) range of
+// bucket b = fmix32(key) & (H-1) in ent[] (16 + 16 bits), entries are
+// (key, value) pairs grouped by bucket.  Building it is count -> scan ->
+// scatter with LDS atomics; probing is one LDS read for the range plus the
+// bucket's entries, which sit next to each other -- the reference's chained
+// LDS table (head / next, gpu-partitioned-hash-join-chained.cpp:832-1037)
+// needs one dependent read per chain hop instead.  Every entry whose key
+// equals the probe key is a match, so duplicate build keys emit every pair.
 template <int BLOCK, int H, int NCAP>
-__global__ __launch_bounds__(BLOCK) void k_pjoin(
+__global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
     const JoinUnit *__restrict__ units, const uint32_t *__restrict__ wg_begin,
     unsigned long long *__restrict__ acc_out) {
-  static_assert(NCAP < 65535, "next indices are 16-bit");
-  __shared__ uint32_t head[H];
+  static_assert(NCAP < 65536, "bucket ranges are 16-bit");
+  static_assert(NCAP % BLOCK == 0 && H % BLOCK == 0, "whole rows per thread");
+  constexpr int RPT = NCAP / BLOCK;  // build rows per thread
+  constexpr int BPT = H / BLOCK;     // buckets per thread in the scan
+  __shared__ uint32_t tab[H];
   __shared__ int2 ent[NCAP];
-  __shared__ uint16_t nxt[NCAP];
+  __shared__ uint32_t scan_tmp[BLOCK / 64];
   const int tid = threadIdx.x;
   JoinAcc acc;
   acc.init();
@@ -100,62 +125,134 @@ __global__ __launch_bounds__(BLOCK) void k_pjoin(
       const uint32_t rl = (U.r_len - rb) < (uint32_t)NCAP ? (U.r_len - rb) : (uint32_t)NCAP;
       const bool reuse = rounds == 1 && built_r == U.r_begin;
       if (!reuse) {
-        __syncthreads();
-        for (int i = tid; i < H; i += BLOCK) head[i] = 0xffffffffu;
-        __syncthreads();
+        __syncthreads();  // the previous table is no longer probed
+        for (int i = tid; i < H; i += BLOCK) tab[i] = 0;
+        int32_t bk[RPT], bv[RPT];
+        uint32_t bb[RPT];
         const int32_t *rkp = rk + U.r_begin + rb;
         const int32_t *rvp = rv ? rv + U.r_begin + rb : nullptr;
-        for (uint32_t i = tid; i < rl; i += BLOCK) {
-          int32_t k = rkp[i];
-          int32_t v = rvp ? rvp[i] : 0;
-          ent[i] = make_int2(k, v);
-          uint32_t old = atomicExch(&head[fmix32((uint32_t)k) & (H - 1)], i);
-          nxt[i] = (uint16_t)old;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const uint32_t i = (uint32_t)(q * BLOCK + tid);
+          bk[q] = i < rl ? rkp[i] : 0;
+          bv[q] = (i < rl && rvp) ? rvp[i] : 0;
+          bb[q] = fmix32((uint32_t)bk[q]) & (H - 1);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if ((uint32_t)(q * BLOCK + tid) < rl) atomicAdd(&tab[bb[q]], 1u);
+        __syncthreads();
+        {  // exclusive scan of the bucket counts -> bucket starts
+          uint32_t c[BPT], sum = 0;
+#pragma unroll
+          for (int x = 0; x < BPT; ++x) {
+            c[x] = tab[tid * BPT + x];
+            sum += c[x];
+          }
+          uint32_t tot;
+          uint32_t run = block_excl_scan<BLOCK>(sum, scan_tmp, &tot);
+#pragma unroll
+          for (int x = 0; x < BPT; ++x) {
+            tab[tid * BPT + x] = run;
+            run += c[x];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if ((uint32_t)(q * BLOCK + tid) < rl) {
+            const uint32_t pos = atomicAdd(&tab[bb[q]], 1u);
+            ent[pos] = make_int2(bk[q], bv[q]);
+          }
+        __syncthreads();
+        {  // tab[b] now holds the end of bucket b: pack (start, end)
+          uint32_t e[BPT];
+          const uint32_t prev = tid ? tab[tid * BPT - 1] : 0u;
+#pragma unroll
+          for (int x = 0; x < BPT; ++x) e[x] = tab[tid * BPT + x];
+          __syncthreads();
+          uint32_t st = prev;
+#pragma unroll
+          for (int x = 0; x < BPT; ++x) {
+            tab[tid * BPT + x] = st | (e[x] << 16);
+            st = e[x];
+          }
         }
         __syncthreads();
         built_r = rounds == 1 ? U.r_begin : ~0ULL;
       }
-      const int32_t *skp = sk + U.s_begin;
-      const int32_t *svp = sv ? sv + U.s_begin : nullptr;
-      const uint32_t sl = U.s_len;
-      const bool vec = (((uintptr_t)skp & 15) == 0) && (!svp || (((uintptr_t)svp & 15) == 0));
-      for (uint32_t j = tid * 4; j < sl; j += BLOCK * 4) {
-        int32_t k[4], v[4];
-        if (vec && j + 3 < sl) {
-          int4 kk = ld_nt4(skp + j);
-          k[0] = kk.x; k[1] = kk.y; k[2] = kk.z; k[3] = kk.w;
-          if (svp) {
-            int4 vv = ld_nt4(svp + j);
-            v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
-          } else {
-            v[0] = v[1] = v[2] = v[3] = 0;
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            bool ok = j + c < sl;
-            k[c] = ok ? skp[j + c] : 0;
-            v[c] = (ok && svp) ? svp[j + c] : 0;
+      // Probe tuples [s_begin, s_begin + s_len): the 16-byte-aligned interior
+      // is read as int4 vectors (4 tuples per thread per step, the next step
+      // prefetched); the <= 3 tuples before and after it one by one.
+      const unsigned long long s0 = U.s_begin, s1 = U.s_begin + U.s_len;
+      const unsigned long long a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
+      auto probe1 = [&](int32_t key, int32_t pv, uint32_t rgv) {
+        const uint32_t st = rgv & 0xffffu, en = rgv >> 16;
+        if (st == en) return;
+        int64_t m = 0, sb = 0;
+        int32_t mnb = INT32_MAX, mxb = INT32_MIN;
+        for (uint32_t i = st; i < en; ++i) {
+          const int2 kv = ent[i];
+          if (kv.x == key) {
+            m += 1;
+            sb += kv.y;
+            mnb = min(mnb, kv.y);
+            mxb = max(mxb, kv.y);
           }
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if (j + c >= sl) break;
-          int64_t m = 0, sb = 0, mnb = INT64_MAX, mxb = INT64_MIN;
-          uint32_t e = head[fmix32((uint32_t)k[c]) & (H - 1)];
-          while (e != 0xffffffffu) {
-            int2 kv = ent[e];
-            if (kv.x == k[c]) {
-              int64_t b = kv.y;
-              m += 1;
-              sb += b;
-              mnb = b < mnb ? b : mnb;
-              mxb = b > mxb ? b : mxb;
-            }
-            uint16_t nx = nxt[e];
-            e = nx == 0xffff ? 0xffffffffu : (uint32_t)nx;
+        acc.add(m, sb, mnb, mxb, pv);
+      };
+      if (a0 >= a1) {  // short unit: every tuple on its own
+        for (unsigned long long x = s0 + tid; x < s1; x += BLOCK) {
+          const int32_t key = sk[x];
+          probe1(key, sv ? sv[x] : 0, tab[fmix32((uint32_t)key) & (H - 1)]);
+        }
+      } else {
+        const int tail_lo = (int)(a0 - s0), tail_hi = (int)(s1 - a1);
+        if (tid < tail_lo + tail_hi) {
+          const unsigned long long x = tid < tail_lo ? s0 + tid : a1 + (tid - tail_lo);
+          const int32_t key = sk[x];
+          probe1(key, sv ? sv[x] : 0, tab[fmix32((uint32_t)key) & (H - 1)]);
+        }
+        const int4 *kq = reinterpret_cast<const int4 *>(sk + a0);
+        const int4 *vq = sv ? reinterpret_cast<const int4 *>(sv + a0) : nullptr;
+        const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);
+        uint32_t q = tid;
+        int4 k4 = make_int4(0, 0, 0, 0), v4 = make_int4(0, 0, 0, 0);
+        if (q < nvec) {
+          k4 = ld_nt4(kq + q);
+          if (vq) v4 = ld_nt4(vq + q);
+        }
+        for (; q < nvec; q += BLOCK) {
+          int4 kn = make_int4(0, 0, 0, 0), vn = make_int4(0, 0, 0, 0);
+          if (q + BLOCK < nvec) {
+            kn = ld_nt4(kq + q + BLOCK);
+            if (vq) vn = ld_nt4(vq + q + BLOCK);
           }
-          acc.add(m, sb, mnb, mxb, (int64_t)v[c]);
+          const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+          const int32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+          uint32_t rg[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) rg[c] = tab[fmix32((uint32_t)kk[c]) & (H - 1)];
+          // the first entry of every bucket, four independent LDS reads
+          int2 e0[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const uint32_t st = rg[c] & 0xffffu;  // == NCAP when the tail buckets are empty
+            e0[c] = ent[st < (uint32_t)NCAP ? st : 0u];
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+            if (en == st + 1) {  // one-entry bucket (the common case)
+              if (e0[c].x == kk[c]) acc.add1(e0[c].y, vv[c]);
+            } else if (en > st + 1) {
+              probe1(kk[c], vv[c], rg[c]);
+            }
+          }
+          k4 = kn;
+          v4 = vn;
         }
       }
     }
@@ -164,211 +261,86 @@ __global__ __launch_bounds__(BLOCK) void k_pjoin(
 }
 
 // ---------------------------------------------------------------------------
-// host planning
+// host side
 // ---------------------------------------------------------------------------
-struct Pack {  // host staging of plan arrays, uploaded with one copy
-  std::vector<uint8_t> buf;
-  template <class T>
-  size_t add(const std::vector<T> &v) {
-    size_t off = (buf.size() + 255) & ~(size_t)255;
-    buf.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
-    if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
-    return off;
-  }
-};
-
-struct PassPlan {
-  std::vector<uint64_t> reg_start, reg_count;  // input regions
-  std::vector<uint32_t> tile_begin;
-  std::vector<unsigned long long> cfull, ctail;  // output cursors
-  uint32_t ntiles = 0, tiles_per_wg = 0, grid = 0;
-  size_t o_rs = 0, o_rc = 0, o_tb = 0, o_cf = 0, o_ct = 0;
-};
-
-constexpr int kTile = 4096;
-constexpr int kScatterBlock = 256;
-
-static uint64_t align32(uint64_t x) { return (x + 31) & ~31ULL; }
-
-static int scatter_occupancy(int fbits) { return fbits <= 7 ? 2 : 1; }
-
-static void plan_pass(PassPlan &p, const std::vector<uint64_t> &rs,
-                      const std::vector<uint64_t> &rc,
-                      const std::vector<uint64_t> &out_start,
-                      const std::vector<uint64_t> &out_count, int num_cus,
-                      int fbits) {
-  p.reg_start = rs;
-  p.reg_count = rc;
-  p.tile_begin.assign(rs.size() + 1, 0);
-  for (size_t r = 0; r < rs.size(); ++r)
-    p.tile_begin[r + 1] = p.tile_begin[r] + (uint32_t)ceil_div(rc[r], kTile);
-  p.ntiles = p.tile_begin.back();
-  p.cfull.resize(out_start.size());
-  p.ctail.resize(out_start.size());
-  for (size_t i = 0; i < out_start.size(); ++i) {
-    p.cfull[i] = out_start[i];
-    p.ctail[i] = out_start[i] + out_count[i];
-  }
-  uint64_t g = (uint64_t)num_cus * scatter_occupancy(fbits);
-  if (g > p.ntiles) g = p.ntiles ? p.ntiles : 1;
-  p.tiles_per_wg = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, g);
-  p.grid = (uint32_t)ceil_div(p.ntiles ? p.ntiles : 1, p.tiles_per_wg);
+template <class T>
+static size_t pack_add(std::vector<uint8_t> &buf, const std::vector<T> &v) {
+  size_t off = (buf.size() + 255) & ~(size_t)255;
+  buf.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
 }
 
-static void pack_pass(Pack &pk, PassPlan &p) {
-  p.o_rs = pk.add(p.reg_start);
-  p.o_rc = pk.add(p.reg_count);
-  p.o_tb = pk.add(p.tile_begin);
-  p.o_cf = pk.add(p.cfull);
-  p.o_ct = pk.add(p.ctail);
-}
-
-static ScatterPlan device_plan(const PassPlan &p, uint8_t *dbase) {
-  ScatterPlan s;
-  s.reg_start = reinterpret_cast<const uint64_t *>(dbase + p.o_rs);
-  s.reg_count = reinterpret_cast<const uint64_t *>(dbase + p.o_rc);
-  s.tile_begin = reinterpret_cast<const uint32_t *>(dbase + p.o_tb);
-  s.nreg = (uint32_t)p.reg_start.size();
-  s.ntiles = p.ntiles;
-  s.tiles_per_wg = p.tiles_per_wg;
-  s.cfull = reinterpret_cast<unsigned long long *>(dbase + p.o_cf);
-  s.ctail = reinterpret_cast<unsigned long long *>(dbase + p.o_ct);
-  return s;
-}
-
+// Tile / block of each pass by fan-out: larger fan-outs need more LDS for
+// the write-combining lines, so they run one 512-thread workgroup per CU with
+// 8192-tuple tiles; smaller ones two 256-thread workgroups with 4096.
 template <int F>
-static int launch_scatter_f(pj_ctx *ctx, const char *name, const int32_t *ik,
-                            const int32_t *iv, int32_t *ok, int32_t *ov,
-                            const ScatterPlan &sp, uint32_t grid, int shift) {
-  DigitRadix dig{shift, (uint32_t)(F - 1)};
-  if (iv)
-    PJ_LAUNCH(ctx, name,
-              (k_scatter<F, kTile, kScatterBlock, DigitRadix, int32_t, NoVal, false>), grid,
-              kScatterBlock, 0, ik, iv, (const NoVal *)nullptr, ok, ov, (NoVal *)nullptr, sp,
-              dig);
-  else
-    PJ_LAUNCH(ctx, name,
-              (k_scatter<F, kTile, kScatterBlock, DigitRadix, int32_t, NoVal, true>), grid,
-              kScatterBlock, 0, ik, iv, (const NoVal *)nullptr, ok, ov, (NoVal *)nullptr, sp,
-              dig);
-  return PJ_OK;
+static int join_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
+                     const int32_t *iv, int32_t *ok, int32_t *ov, uint64_t n,
+                     const RadixPlan &p, JointRadix j, RadixHist &h) {
+  // fan-out 256: 16-tuple (64 B) lines keep two workgroups per CU
+  constexpr int T = 4096;
+  constexpr int BLOCK = 512;
+  constexpr int LINE = F >= 256 ? 16 : 32;
+  if (!pass2 && !iv)
+    return radix_launch_pass<F, T, BLOCK, JointRadix, int32_t, NoVal, true, LINE>(
+        ctx, name, false, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
+  return radix_launch_pass<F, T, BLOCK, JointRadix, int32_t, NoVal, false, LINE>(
+      ctx, name, pass2, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
 }
 
-static int launch_scatter(pj_ctx *ctx, const char *name, int fbits,
-                          const int32_t *ik, const int32_t *iv, int32_t *ok,
-                          int32_t *ov, const ScatterPlan &sp, uint32_t grid,
-                          int shift) {
-  if (sp.ntiles == 0) return PJ_OK;
-  switch (fbits) {
-    case 1: return launch_scatter_f<2>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 2: return launch_scatter_f<4>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 3: return launch_scatter_f<8>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 4: return launch_scatter_f<16>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 5: return launch_scatter_f<32>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 6: return launch_scatter_f<64>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 7: return launch_scatter_f<128>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    case 8: return launch_scatter_f<256>(ctx, name, ik, iv, ok, ov, sp, grid, shift);
-    default: return fail(PJ_E_ARG, "radix bits %d outside [1, 8]", fbits);
+static int join_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
+                          const int32_t *ik, const int32_t *iv, int32_t *ok, int32_t *ov,
+                          uint64_t n, const RadixPlan &p, JointRadix j, RadixHist &h) {
+  switch (bits) {
+    case 1: return join_pass<2>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 2: return join_pass<4>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 3: return join_pass<8>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 4: return join_pass<16>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 5: return join_pass<32>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 6: return join_pass<64>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 7: return join_pass<128>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    case 8: return join_pass<256>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
+    default: return fail(PJ_E_ARG, "radix bits %d outside [1, 8]", bits);
   }
 }
 
-int radix_histogram(pj_ctx *ctx, const int32_t *d_k, uint64_t n, int bits,
-                    std::vector<uint32_t> &h) {
-  const int nb = 1 << bits;
-  h.assign(nb, 0);
-  if (!n) return PJ_OK;
-  uint64_t grid = std::min<uint64_t>((uint64_t)ctx->num_cus, ceil_div(n, 1024 * 16));
-  if (grid < 1) grid = 1;
-  Scratch part, hist;
-  PJ_TRY(part.alloc(ctx, grid * nb * 4));
-  PJ_TRY(hist.alloc(ctx, (uint64_t)nb * 4));
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k_hist_dyn,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
-    attr_set = true;
-  }
-  PJ_LAUNCH(ctx, "radix_hist", k_hist_dyn, grid, 1024, (size_t)nb * 4, d_k, n, bits,
-            part.as<uint32_t>());
-  PJ_LAUNCH(ctx, "radix_hist_reduce", k_hist_reduce, ceil_div(nb, 256), 256, 0,
-            part.as<const uint32_t>(), (int)grid, nb, hist.as<uint32_t>());
-  return d2h(ctx, h.data(), hist.p, (uint64_t)nb * 4);
-}
-
-// Partitioned side: pass-1 regions, final regions, and both pass plans.
+// One side of the join, partitioned into 2^(b1+b2) dense partitions.
 struct Side {
-  uint64_t n = 0;
-  std::vector<uint32_t> hist;               // [2^B]
-  std::vector<uint64_t> r1_start, r1_count; // [2^bits1]
-  std::vector<uint64_t> f_start, f_count;   // [2^B] final partitions
-  uint64_t total1 = 0, total2 = 0;
-  PassPlan p1, p2;
+  RadixPlan plan;
+  RadixHist hist;
   Scratch k1, v1, k2, v2;
+  const int32_t *fk = nullptr, *fv = nullptr;  // final partitioned columns
 };
 
-static void plan_side(Side &s, int bits1, int bits2, int num_cus) {
-  const int F1 = 1 << bits1, F2 = 1 << bits2;
-  s.r1_start.assign(F1, 0);
-  s.r1_count.assign(F1, 0);
-  s.f_start.assign((size_t)F1 * F2, 0);
-  s.f_count.assign((size_t)F1 * F2, 0);
-  uint64_t pos = 0;
-  for (int d1 = 0; d1 < F1; ++d1) {
-    uint64_t c = 0;
-    for (int d2 = 0; d2 < F2; ++d2) c += s.hist[(size_t)d1 * F2 + d2];
-    pos = align32(pos);
-    s.r1_start[d1] = pos;
-    s.r1_count[d1] = c;
-    pos += c;
-  }
-  s.total1 = pos;
-  if (bits2 == 0) {
-    s.f_start = s.r1_start;
-    s.f_count = s.r1_count;
-    s.total2 = 0;
-  } else {
-    pos = 0;
-    for (size_t i = 0; i < (size_t)F1 * F2; ++i) {
-      pos = align32(pos);
-      s.f_start[i] = pos;
-      s.f_count[i] = s.hist[i];
-      pos += s.hist[i];
-    }
-    s.total2 = pos;
-  }
-  plan_pass(s.p1, std::vector<uint64_t>{0}, std::vector<uint64_t>{s.n}, s.r1_start,
-            s.r1_count, num_cus, bits1);
-  if (bits2)
-    plan_pass(s.p2, s.r1_start, s.r1_count, s.f_start, s.f_count, num_cus, bits2);
-}
-
-static int run_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
-                    int bits1, int bits2, uint8_t *dplan, const int32_t **fk,
-                    const int32_t **fv, const char *tag) {
-  const int B = bits1 + bits2;
-  PJ_TRY(s.k1.alloc(ctx, s.total1 * 4 + 256));
-  PJ_TRY(s.v1.alloc(ctx, s.total1 * 4 + 256));
+static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
+                          uint64_t n, int b1, int b2, const char *tag) {
+  const int B = b1 + b2;
+  s.plan = radix_make_plan(ctx, n, b1, b2, 1 << B, 2);
+  PJ_TRY(s.hist.alloc(ctx, s.plan));
+  JointRadix j{32 - B};
+  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, j, s.hist, true));
+  PJ_TRY(s.k1.alloc(ctx, n * 4 + 256));
+  PJ_TRY(s.v1.alloc(ctx, n * 4 + 256));
   char name[64];
   snprintf(name, sizeof(name), "radix_scatter1_%s", tag);
-  PJ_TRY(launch_scatter(ctx, name, bits1, d_k, d_v, s.k1.as<int32_t>(), s.v1.as<int32_t>(),
-                        device_plan(s.p1, dplan), s.p1.grid, 32 - bits1));
-  if (bits2) {
-    PJ_TRY(s.k2.alloc(ctx, s.total2 * 4 + 256));
-    PJ_TRY(s.v2.alloc(ctx, s.total2 * 4 + 256));
+  PJ_TRY(join_pass_bits(ctx, name, b1, false, d_k, d_v, s.k1.as<int32_t>(),
+                        s.v1.as<int32_t>(), n, s.plan, j, s.hist));
+  s.fk = s.k1.as<int32_t>();
+  s.fv = s.v1.as<int32_t>();
+  if (b2) {
+    PJ_TRY(s.k2.alloc(ctx, n * 4 + 256));
+    PJ_TRY(s.v2.alloc(ctx, n * 4 + 256));
     snprintf(name, sizeof(name), "radix_scatter2_%s", tag);
-    PJ_TRY(launch_scatter(ctx, name, bits2, s.k1.as<int32_t>(), s.v1.as<int32_t>(),
-                          s.k2.as<int32_t>(), s.v2.as<int32_t>(), device_plan(s.p2, dplan),
-                          s.p2.grid, 32 - B));
-    *fk = s.k2.as<int32_t>();
-    *fv = s.v2.as<int32_t>();
-  } else {
-    *fk = s.k1.as<int32_t>();
-    *fv = s.v1.as<int32_t>();
+    PJ_TRY(join_pass_bits(ctx, name, b2, true, s.fk, s.fv, s.k2.as<int32_t>(),
+                          s.v2.as<int32_t>(), n, s.plan, j, s.hist));
+    s.fk = s.k2.as<int32_t>();
+    s.fv = s.v2.as<int32_t>();
   }
   return PJ_OK;
 }
 
-// LDS table geometries.  NCAP rows, H heads; occupancy from the LDS bytes.
+// LDS table geometries: NCAP rows, H heads.
 struct Geometry {
   int lds_log;
   int ncap;
@@ -405,26 +377,6 @@ static int launch_pjoin(pj_ctx *ctx, int lds_log, const int32_t *rk, const int32
 }
 static int join_occupancy(int lds_log) { return lds_log >= 13 ? 1 : (lds_log == 12 ? 2 : 4); }
 
-// Default configuration: final partitions of ~2/3 of the LDS table capacity.
-static pj_radix_cfg default_cfg(uint64_t nb) {
-  pj_radix_cfg c{};
-  c.lds_log = 12;
-  c.probe_unit = 65536;
-  c.grid_mul = 0;
-  Geometry g = geometry_for(c.lds_log);
-  uint64_t target = (uint64_t)g.ncap * 2 / 3;
-  int B = 0;
-  while (B < 15 && (nb >> B) > target) ++B;
-  if (B <= 8) {
-    c.bits1 = B;
-    c.bits2 = 0;
-  } else {
-    c.bits1 = B / 2 + (B & 1);
-    c.bits2 = B - c.bits1;
-  }
-  return c;
-}
-
 static void balance_units(const std::vector<JoinUnit> &units, uint32_t grid,
                           std::vector<uint32_t> &wgb) {
   wgb.assign(grid + 1, (uint32_t)units.size());
@@ -448,12 +400,9 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
   Scratch acc;
   PJ_TRY(acc.alloc(ctx, 64));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
-  Geometry geo = geometry_for(cfg.lds_log);
+  const Geometry geo = geometry_for(cfg.lds_log);
   std::vector<JoinUnit> units;
   Side R, S;
-  R.n = nb;
-  S.n = np;
-  Pack pk;
   const int32_t *rk = d_bk, *rv = d_bv, *sk = d_pk, *sv = d_pv;
   if (nb && np) {
     if (B == 0) {
@@ -466,54 +415,39 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
         units.push_back(u);
       }
     } else {
-      PJ_TRY(radix_histogram(ctx, d_bk, nb, B, R.hist));
-      PJ_TRY(radix_histogram(ctx, d_pk, np, B, S.hist));
-      plan_side(R, cfg.bits1, cfg.bits2, ctx->num_cus);
-      plan_side(S, cfg.bits1, cfg.bits2, ctx->num_cus);
-      for (size_t q = 0; q < R.f_start.size(); ++q) {
-        uint64_t rl = R.f_count[q], sl = S.f_count[q];
+      PJ_TRY(partition_side(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build"));
+      PJ_TRY(partition_side(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe"));
+      rk = R.fk;
+      rv = d_bv ? R.fv : nullptr;
+      sk = S.fk;
+      sv = d_pv ? S.fv : nullptr;
+      const int NB = 1 << B;
+      for (int q = 0; q < NB; ++q) {
+        const uint64_t rl = R.hist.hT[q], sl = S.hist.hT[q];
         if (!rl || !sl) continue;
         for (uint64_t s0 = 0; s0 < sl; s0 += cfg.probe_unit) {
           JoinUnit u;
-          u.r_begin = R.f_start[q];
+          u.r_begin = R.hist.hS[q];
           u.r_len = (uint32_t)rl;
-          u.s_begin = S.f_start[q] + s0;
+          u.s_begin = S.hist.hS[q] + s0;
           u.s_len = (uint32_t)std::min<uint64_t>(cfg.probe_unit, sl - s0);
           units.push_back(u);
         }
       }
-      pack_pass(pk, R.p1);
-      pack_pass(pk, S.p1);
-      if (cfg.bits2) {
-        pack_pass(pk, R.p2);
-        pack_pass(pk, S.p2);
-      }
     }
   }
-  uint32_t grid = 1;
-  std::vector<uint32_t> wgb;
-  size_t o_units = 0, o_wgb = 0;
   if (!units.empty()) {
     uint64_t g = (uint64_t)ctx->num_cus * join_occupancy(geo.lds_log) *
                  (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
-    grid = (uint32_t)std::min<uint64_t>(g, units.size());
+    uint32_t grid = (uint32_t)std::min<uint64_t>(g, units.size());
+    std::vector<uint32_t> wgb;
     balance_units(units, grid, wgb);
-    o_units = pk.add(units);
-    o_wgb = pk.add(wgb);
-  }
-  Scratch dplan;
-  if (!pk.buf.empty()) {
-    PJ_TRY(dplan.alloc(ctx, pk.buf.size()));
-    PJ_TRY(h2d(ctx, dplan.p, pk.buf.data(), pk.buf.size()));
-  }
-  uint8_t *dp = dplan.as<uint8_t>();
-  if (nb && np && B > 0) {
-    PJ_TRY(run_side(ctx, R, d_bk, d_bv, cfg.bits1, cfg.bits2, dp, &rk, &rv, "build"));
-    PJ_TRY(run_side(ctx, S, d_pk, d_pv, cfg.bits1, cfg.bits2, dp, &sk, &sv, "probe"));
-    if (!d_bv) rv = nullptr;
-    if (!d_pv) sv = nullptr;
-  }
-  if (!units.empty()) {
+    std::vector<uint8_t> buf;
+    size_t o_units = pack_add(buf, units), o_wgb = pack_add(buf, wgb);
+    Scratch dplan;
+    PJ_TRY(dplan.alloc(ctx, buf.size()));
+    PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
+    uint8_t *dp = dplan.as<uint8_t>();
     PJ_TRY(launch_pjoin(ctx, geo.lds_log, rk, rv, sk, sv,
                         reinterpret_cast<const JoinUnit *>(dp + o_units),
                         reinterpret_cast<const uint32_t *>(dp + o_wgb), grid,
@@ -522,30 +456,30 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
   return join_acc_read(ctx, acc.as<unsigned long long>(), out);
 }
 
-int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
-  pj_radix_cfg d = default_cfg(nb);
-  if (c.lds_log == 0) c.lds_log = d.lds_log;
-  if (c.probe_unit == 0) c.probe_unit = d.probe_unit;
-  if (c.bits1 == 0 && c.bits2 == 0) {
-    // derive the bits for the chosen table size
-    Geometry g = geometry_for(c.lds_log);
-    uint64_t target = (uint64_t)g.ncap * 2 / 3;
-    int B = 0;
-    while (B < 15 && (nb >> B) > target) ++B;
-    if (B <= 8) {
-      c.bits1 = B;
-      c.bits2 = 0;
-    } else {
-      c.bits1 = B / 2 + (B & 1);
-      c.bits2 = B - c.bits1;
-    }
+static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
+  const Geometry g = geometry_for(c.lds_log);
+  const uint64_t target = (uint64_t)g.ncap * 2 / 3;
+  int B = 0;
+  while (B < 15 && (nb >> B) > target) ++B;
+  if (B <= 8) {
+    c.bits1 = B;
+    c.bits2 = 0;
+  } else {
+    c.bits1 = B / 2 + (B & 1);
+    c.bits2 = B - c.bits1;
   }
+}
+
+int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
+  if (c.lds_log == 0) c.lds_log = 12;
+  if (c.probe_unit == 0) c.probe_unit = 65536;
+  if (c.lds_log < 11 || c.lds_log > 13) return fail(PJ_E_ARG, "lds_log %d not in [11, 13]", c.lds_log);
+  if (c.bits1 == 0 && c.bits2 == 0) derive_bits(c, nb);
   if (c.bits1 < 0 || c.bits1 > 8 || c.bits2 < 0 || c.bits2 > 8 || (c.bits1 == 0 && c.bits2 > 0))
     return fail(PJ_E_ARG, "radix bits (%d, %d) invalid: each in [0, 8], bits2 needs bits1",
                 c.bits1, c.bits2);
   if (c.bits1 + c.bits2 > 15)
     return fail(PJ_E_ARG, "bits1 + bits2 = %d exceeds 15", c.bits1 + c.bits2);
-  if (c.lds_log < 11 || c.lds_log > 13) return fail(PJ_E_ARG, "lds_log %d not in [11, 13]", c.lds_log);
   if (c.probe_unit < 1024 || (c.probe_unit & 3))
     return fail(PJ_E_ARG, "probe_unit must be a multiple of 4 and >= 1024");
   if (c.grid_mul < 0 || c.grid_mul > 8) return fail(PJ_E_ARG, "grid_mul %d not in [0, 8]", c.grid_mul);
@@ -553,65 +487,30 @@ int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
 }
 
 // ---------------------------------------------------------------------------
-// hash-rearrange
+// hash-rearrange: one pass with bucket = hashInt32(key) URem buckets
 // ---------------------------------------------------------------------------
 template <int F>
 static int rearrange_f(pj_ctx *ctx, const int32_t *d_k, const int32_t *d_v, uint64_t n,
                        uint32_t buckets, int32_t *d_ok, int32_t *d_ov, uint64_t *h_off) {
-  DigitMod dig{buckets};
-  uint64_t grid = std::min<uint64_t>((uint64_t)ctx->num_cus * 2, ceil_div(n, 4096));
-  if (grid < 1) grid = 1;
-  Scratch part, hist;
-  PJ_TRY(part.alloc(ctx, grid * F * 4));
-  PJ_TRY(hist.alloc(ctx, F * 4));
-  PJ_LAUNCH(ctx, "rearrange_hist", (k_hist_digit<F, 256, DigitMod>), grid, 256, 0, d_k, n, dig,
-            part.as<uint32_t>());
-  PJ_LAUNCH(ctx, "radix_hist_reduce", k_hist_reduce, ceil_div(F, 256), 256, 0,
-            part.as<const uint32_t>(), (int)grid, F, hist.as<uint32_t>());
-  std::vector<uint32_t> hc(F);
-  PJ_TRY(d2h(ctx, hc.data(), hist.p, F * 4));
-  // Output regions must be 32-element aligned for full-line writes, so the
-  // scatter goes to an aligned staging buffer that is then compacted.
-  std::vector<uint64_t> st(F), cnt(F), dense(F + 1, 0);
-  uint64_t pos = 0;
-  for (int b = 0; b < F; ++b) {
-    pos = align32(pos);
-    st[b] = pos;
-    cnt[b] = hc[b];
-    pos += hc[b];
-    dense[b + 1] = dense[b] + hc[b];
+  RadixPlan p = radix_make_plan(ctx, n, 0, 0, (int)buckets, 2);
+  RadixHist h;
+  PJ_TRY(h.alloc(ctx, p));
+  JointMod j{buckets};
+  PJ_TRY(radix_histograms(ctx, d_k, n, p, j, h, true));
+  Scratch tmpv;
+  int32_t *ov = d_ov;
+  if (!ov) {  // the pass always moves a value column
+    PJ_TRY(tmpv.alloc(ctx, n * 4 + 256));
+    ov = tmpv.as<int32_t>();
   }
-  PassPlan p;
-  plan_pass(p, std::vector<uint64_t>{0}, std::vector<uint64_t>{n}, st, cnt, ctx->num_cus, 5);
-  Pack pk;
-  pack_pass(pk, p);
-  Scratch dplan, sk, sv;
-  PJ_TRY(dplan.alloc(ctx, pk.buf.size()));
-  PJ_TRY(h2d(ctx, dplan.p, pk.buf.data(), pk.buf.size()));
-  PJ_TRY(sk.alloc(ctx, pos * 4 + 256));
-  PJ_TRY(sv.alloc(ctx, pos * 4 + 256));
-  if (p.ntiles) {
-    if (d_v)
-      PJ_LAUNCH(ctx, "rearrange_scatter",
-                (k_scatter<F, kTile, kScatterBlock, DigitMod, int32_t, NoVal, false>), p.grid,
-                kScatterBlock, 0, d_k, d_v, (const NoVal *)nullptr, sk.as<int32_t>(),
-                sv.as<int32_t>(), (NoVal *)nullptr, device_plan(p, dplan.as<uint8_t>()), dig);
-    else
-      PJ_LAUNCH(ctx, "rearrange_scatter",
-                (k_scatter<F, kTile, kScatterBlock, DigitMod, int32_t, NoVal, true>), p.grid,
-                kScatterBlock, 0, d_k, d_v, (const NoVal *)nullptr, sk.as<int32_t>(),
-                sv.as<int32_t>(), (NoVal *)nullptr, device_plan(p, dplan.as<uint8_t>()), dig);
-  }
-  for (uint32_t b = 0; b < buckets; ++b) {
-    if (!cnt[b]) continue;
-    PJ_HIP(hipMemcpyAsync(d_ok + dense[b], sk.as<int32_t>() + st[b], cnt[b] * 4,
-                          hipMemcpyDeviceToDevice, ctx->stream));
-    if (d_ov)
-      PJ_HIP(hipMemcpyAsync(d_ov + dense[b], sv.as<int32_t>() + st[b], cnt[b] * 4,
-                            hipMemcpyDeviceToDevice, ctx->stream));
-  }
+  if (d_v)
+    PJ_TRY((radix_launch_pass<F, 4096, 256, JointMod, int32_t, NoVal, false>(
+        ctx, "rearrange_scatter", false, d_k, d_v, nullptr, d_ok, ov, nullptr, n, p, j, h)));
+  else
+    PJ_TRY((radix_launch_pass<F, 4096, 256, JointMod, int32_t, NoVal, true>(
+        ctx, "rearrange_scatter", false, d_k, d_v, nullptr, d_ok, ov, nullptr, n, p, j, h)));
   PJ_TRY(sync(ctx));
-  for (uint32_t b = 0; b <= buckets; ++b) h_off[b] = dense[b];
+  for (uint32_t b = 0; b <= buckets; ++b) h_off[b] = h.hS[b];
   return PJ_OK;
 }
 

@@ -224,44 +224,61 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 }
 
 // join aggregate accumulator ---------------------------------------------------
+// Build and probe values are int32, so a thread tracks MIN/MAX in 32 bits and
+// counts single matches in 32 bits; sums stay 64-bit.  Widened at the flush.
 struct JoinAcc {
-  int64_t count, sum_b, sum_p, min_b, max_b, min_p, max_p;
+  int64_t count, sum_b, sum_p;
+  uint32_t n1;  // single matches since the last fold into count
+  int32_t min_b, max_b, min_p, max_p;
   __device__ __forceinline__ void init() {
     count = 0;
     sum_b = 0;
     sum_p = 0;
-    min_b = INT64_MAX;
-    max_b = INT64_MIN;
-    min_p = INT64_MAX;
-    max_p = INT64_MIN;
+    n1 = 0;
+    min_b = INT32_MAX;
+    max_b = INT32_MIN;
+    min_p = INT32_MAX;
+    max_p = INT32_MIN;
+  }
+  // exactly one match
+  __device__ __forceinline__ void add1(int32_t b, int32_t pv) {
+    n1 += 1;
+    sum_b += b;
+    sum_p += pv;
+    min_b = min(min_b, b);
+    max_b = max(max_b, b);
+    min_p = min(min_p, pv);
+    max_p = max(max_p, pv);
   }
   // `m` matches of one probe row whose build values sum to sb and span
   // [mnb, mxb].
-  __device__ __forceinline__ void add(int64_t m, int64_t sb, int64_t mnb,
-                                      int64_t mxb, int64_t pv) {
-    count += m;
-    sum_b += sb;
-    sum_p += m * pv;
+  __device__ __forceinline__ void add(int64_t m, int64_t sb, int32_t mnb, int32_t mxb,
+                                      int32_t pv) {
     if (m) {
-      min_b = mnb < min_b ? mnb : min_b;
-      max_b = mxb > max_b ? mxb : max_b;
-      min_p = pv < min_p ? pv : min_p;
-      max_p = pv > max_p ? pv : max_p;
+      count += m;
+      sum_b += sb;
+      sum_p += m * (int64_t)pv;
+      min_b = min(min_b, mnb);
+      max_b = max(max_b, mxb);
+      min_p = min(min_p, pv);
+      max_p = max(max_p, pv);
     }
   }
 };
 
 // Wave-reduce a JoinAcc and fold lane 0's result into the 7-slot global
 // accumulator (count, sum_b, sum_p as u64 adds; min/max as signed atomics).
-__device__ __forceinline__ void join_acc_flush(JoinAcc a,
-                                               unsigned long long *g) {
-  int64_t c = wave_sum_i64(a.count);
+// With no match at all the identities are INT64_MAX / INT64_MIN.
+__device__ __forceinline__ void join_acc_flush(JoinAcc a, unsigned long long *g) {
+  const int64_t cnt = a.count + (int64_t)a.n1;
+  int64_t c = wave_sum_i64(cnt);
   int64_t sb = wave_sum_i64(a.sum_b);
   int64_t sp = wave_sum_i64(a.sum_p);
-  int64_t mnb = wave_min_i64(a.min_b);
-  int64_t mxb = wave_max_i64(a.max_b);
-  int64_t mnp = wave_min_i64(a.min_p);
-  int64_t mxp = wave_max_i64(a.max_p);
+  const bool any = cnt != 0;
+  int64_t mnb = wave_min_i64(any ? (int64_t)a.min_b : INT64_MAX);
+  int64_t mxb = wave_max_i64(any ? (int64_t)a.max_b : INT64_MIN);
+  int64_t mnp = wave_min_i64(any ? (int64_t)a.min_p : INT64_MAX);
+  int64_t mxp = wave_max_i64(any ? (int64_t)a.max_p : INT64_MIN);
   if ((threadIdx.x & 63) == 0 && c != 0) {
     atomicAdd(&g[0], (unsigned long long)c);
     atomicAdd(&g[1], (unsigned long long)sb);

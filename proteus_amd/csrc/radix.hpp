@@ -1,45 +1,55 @@
-// Radix partitioning kernels shared by the partitioned join, hash-rearrange
-// and the partitioned group-by.
+// Radix partitioning shared by the partitioned join, hash-rearrange and the
+// partitioned group-by.
 //
 // Reference: HashPartitioner::close and build_partitions
 // (core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained.cpp:249-404,
 //  gpu-partitioned-hash-join-chained-helpers.cpp:243-481).  The reference
-// chains 4096-tuple buckets per partition, reserving space with 64-bit
-// atomics plus a retry loop, after materialising the input with one global
-// atomicAdd per tuple.  Here:
-//   * one histogram kernel counts every (pass-1, pass-2) digit pair at once,
-//     so both passes write into exactly sized, 32-element-aligned regions;
-//   * each workgroup keeps, per output digit, a 32-element (128 B) LDS
-//     write-combining buffer; a tile is shuffled through LDS by digit and
-//     only FULL aligned 128 B lines are written, their space reserved with
-//     one atomic per (tile, digit);
-//   * the < 32-element remainders left when a workgroup finishes a region are
-//     written from the BACK of the region ("tails"), so full lines and tails
-//     never interleave and every region ends up dense.
+// materialises its input with one global atomicAdd per tuple, then chains
+// 4096-tuple buckets per partition, reserving space with 64-bit atomics and a
+// retry loop.  Here partitioning is deterministic and atomic-free:
+//   1. k_chunk_hist: the input is cut into G chunks (one per pass-1
+//      workgroup); every chunk's joint histogram over all NB = 2^(b1+b2)
+//      (pass-1 digit, pass-2 digit) bins is one row of H[G][NB];
+//   2. k_colscan + k_excl_scan: E[c][d] = sum of H[c'][d] over c' < c, the
+//      bin totals T[d] and the dense partition starts S[d] = sum T[d' < d];
+//   3. pass 1: chunk c writes its digit-d1 tuples to
+//         S[d1 << b2] + sum over d2 of E[c][d1][d2]     (exact, no atomics);
+//   4. pass 2: the tuples chunk c wrote for digit d1 form one contiguous
+//      segment whose pass-2 histogram is H[c][d1][*], so digit d2 goes to
+//         S[d1][d2] + E[c][d1][d2].
+// Every workgroup shuffles a tile through LDS by digit and keeps one 32-tuple
+// (128 B per column) write-combining buffer per digit; it only writes whole
+// aligned lines except the first and last line of each of its runs.  The next
+// tile is loaded into registers while the current one is shuffled.
+// The result: both sides laid out densely, final partition d occupying
+// [S[d], S[d] + T[d]).
 #pragma once
+
+#include <algorithm>
+#include <vector>
 
 #include "pj_internal.hpp"
 
 namespace pj {
 
-constexpr int kLine = 32;  // elements per 128 B line of int32
+constexpr int kLine = 32;  // tuples per write-combining line (128 B of int32)
 
-// Digit functors -----------------------------------------------------------
-struct DigitRadix {  // (fmix32(key) >> shift) & mask
-  int shift;
-  uint32_t mask;
+// Joint digit functors: bin in [0, NB) of a key.  d1 = bin >> b2,
+// d2 = bin & (2^b2 - 1).
+struct JointRadix {  // top B bits of fmix32(key)
+  int shift;         // 32 - B, B >= 1
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
-    return (fmix32((uint32_t)k) >> shift) & mask;
+    return fmix32((uint32_t)k) >> shift;
   }
 };
-struct DigitMod {  // hashInt32(key) URem buckets (gpu-hash-rearrange.cpp:338-348)
+struct JointMod {  // hashInt32(key) URem buckets (gpu-hash-rearrange.cpp:338-348)
   uint32_t buckets;
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
     return (uint32_t)(hash_i32(k) % (uint64_t)buckets);
   }
 };
 
-// Block-wide exclusive scan of one value per thread (threads >= count pass 0).
+// Block-wide exclusive scan of one value per thread.
 template <int BLOCK>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp,
                                                     uint32_t *total) {
@@ -65,95 +75,18 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp,
   return pre + incl - v;
 }
 
-// Histogram of (fmix32(key) >> shift) over 2^BITS bins: one LDS histogram per
-// workgroup, written out as a row of `partial` (no global atomics), summed by
-// k_hist_reduce.
-template <int BITS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_hist(const int32_t *__restrict__ keys,
-                                                uint64_t n, int shift,
-                                                uint32_t *__restrict__ partial) {
-  constexpr int NB = 1 << BITS;
-  __shared__ uint32_t h[NB];
-  for (int i = threadIdx.x; i < NB; i += BLOCK) h[i] = 0;
-  __syncthreads();
-  uint64_t nvec = n / 4;
-  const int4 *kv = reinterpret_cast<const int4 *>(keys);
-  bool vec = ((uintptr_t)keys & 15) == 0;
-  uint64_t tid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
-  if (vec) {
-    for (uint64_t i = tid; i < nvec; i += nthr) {
-      int4 k = ld_nt4(&kv[i]);
-      atomicAdd(&h[fmix32((uint32_t)k.x) >> shift], 1u);
-      atomicAdd(&h[fmix32((uint32_t)k.y) >> shift], 1u);
-      atomicAdd(&h[fmix32((uint32_t)k.z) >> shift], 1u);
-      atomicAdd(&h[fmix32((uint32_t)k.w) >> shift], 1u);
-    }
-    for (uint64_t i = nvec * 4 + tid; i < n; i += nthr)
-      atomicAdd(&h[fmix32((uint32_t)keys[i]) >> shift], 1u);
-  } else {
-    for (uint64_t i = tid; i < n; i += nthr)
-      atomicAdd(&h[fmix32((uint32_t)keys[i]) >> shift], 1u);
-  }
-  __syncthreads();
-  uint32_t *row = partial + (uint64_t)blockIdx.x * NB;
-  for (int i = threadIdx.x; i < NB; i += BLOCK) row[i] = h[i];
-}
-
-// Joint histogram of the top `bits` bits of fmix32(key) (bits <= 15), read
-// back to the host.
-int radix_histogram(pj_ctx *ctx, const int32_t *d_k, uint64_t n, int bits,
-                    std::vector<uint32_t> &h);
-
-__global__ void k_hist_reduce(const uint32_t *__restrict__ partial, int rows,
-                              int nb, uint32_t *__restrict__ hist);
-
-// Histogram of an arbitrary digit functor over `F` bins (hash-rearrange).
-template <int F, int BLOCK, class Digit>
-__global__ __launch_bounds__(BLOCK) void k_hist_digit(
-    const int32_t *__restrict__ keys, uint64_t n, Digit dig,
-    uint32_t *__restrict__ partial) {
-  __shared__ uint32_t h[F];
-  for (int i = threadIdx.x; i < F; i += BLOCK) h[i] = 0;
-  __syncthreads();
-  uint64_t tid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t i = tid; i < n; i += nthr) atomicAdd(&h[dig(keys[i])], 1u);
-  __syncthreads();
-  for (int i = threadIdx.x; i < F; i += BLOCK) partial[(uint64_t)blockIdx.x * F + i] = h[i];
-}
-
-// Reserve c elements at the back of a region: returns the first slot.
-__device__ __forceinline__ unsigned long long reserve_tail(unsigned long long *ct,
-                                                           uint32_t c) {
-  unsigned long long old = atomicAdd(ct, (unsigned long long)(-(long long)c));
-  return old - c;
-}
-
-// Scatter plan: input regions (one per parent digit), the flat tile list over
-// them, and the output cursors of every (region, digit) pair.
-struct ScatterPlan {
-  const uint64_t *reg_start;   // [nreg] first input element of each region
-  const uint64_t *reg_count;   // [nreg]
-  const uint32_t *tile_begin;  // [nreg + 1] prefix of tiles per region
-  uint32_t nreg;
-  uint32_t ntiles;
-  uint32_t tiles_per_wg;
-  unsigned long long *cfull;  // [nreg * F] next full-line slot (front)
-  unsigned long long *ctail;  // [nreg * F] end of the unreserved space (back)
-};
-
-// Value-column traits.  NoVal marks an absent second value column.
+// Value-column traits.  NoVal marks an absent column.
 struct NoVal {};
 template <class V>
 struct ValIO {
   static constexpr bool present = true;
-  // Load elements e..e+3 of a tile (vector loads when aligned and in range).
+  // Elements e..e+3 of a tile whose valid elements are [lo, len): one or two
+  // 16-byte loads when all four are valid and the address is aligned.
   __device__ static __forceinline__ void load4(const V *__restrict__ p, uint64_t base,
-                                               uint32_t e, uint32_t len, bool vec,
-                                               V out[4]) {
-    if (vec && e + 3 < len) {
-      const int4 *q = reinterpret_cast<const int4 *>(p + base + e);
+                                               uint32_t e, uint32_t lo, uint32_t len,
+                                               bool vec, V out[4]) {
+    if (vec && e >= lo && e + 3 < len) {
+      const char *q = reinterpret_cast<const char *>(p + base + e);
       if constexpr (sizeof(V) == 4) {
         int4 a = ld_nt4(q);
         out[0] = __builtin_bit_cast(V, a.x);
@@ -163,7 +96,7 @@ struct ValIO {
       } else {
         static_assert(sizeof(V) == 8, "4- or 8-byte values");
         int4 a = ld_nt4(q);
-        int4 b = ld_nt4(q + 1);
+        int4 b = ld_nt4(q + 16);
         out[0] = __builtin_bit_cast(V, make_int2(a.x, a.y));
         out[1] = __builtin_bit_cast(V, make_int2(a.z, a.w));
         out[2] = __builtin_bit_cast(V, make_int2(b.x, b.y));
@@ -171,7 +104,8 @@ struct ValIO {
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) out[c] = (e + c < len) ? p[base + e + c] : V{};
+      for (int c = 0; c < 4; ++c)
+        out[c] = (e + c >= lo && e + c < len) ? p[base + e + c] : V{};
     }
   }
   __device__ static __forceinline__ bool aligned(const V *p, uint64_t base) {
@@ -182,213 +116,425 @@ template <>
 struct ValIO<NoVal> {
   static constexpr bool present = false;
   __device__ static __forceinline__ void load4(const NoVal *, uint64_t, uint32_t, uint32_t,
-                                               bool, NoVal out[4]) {}
+                                               uint32_t, bool, NoVal *) {}
   __device__ static __forceinline__ bool aligned(const NoVal *, uint64_t) { return true; }
 };
 
-// One radix scatter pass over (key, V1[, V2]) rows.  Per tile of T rows:
-//   load -> digit + LDS rank -> scan -> reserve full lines (1 atomic/digit)
-//   -> shuffle into LDS by digit -> write full lines (32 rows; a half-wave
-//   per line) -> keep remainders in the per-digit LDS line buffers.
-// Buffers are flushed as tails when the workgroup leaves a region.
-// With ROWID the V1 (int32) column is the input row index instead of a load
-// (region 0 must start at row 0).
-template <int F, int T, int BLOCK, class Digit, class V1, class V2, bool ROWID>
-__global__ __launch_bounds__(BLOCK) void k_scatter(
+// Per-chunk joint histogram: chunk c = [c*chunk, min(n, (c+1)*chunk)) into
+// row c of H (nb u32 bins, dynamic LDS).  chunk is a multiple of 4.
+template <class Joint>
+__global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__ keys,
+                                                     uint64_t n, uint64_t chunk, int nb,
+                                                     Joint joint, uint32_t *__restrict__ H) {
+  extern __shared__ uint32_t hsh[];
+  for (int i = threadIdx.x; i < nb; i += 1024) hsh[i] = 0;
+  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+  if (lo < hi) {
+    const bool vec = ((uintptr_t)(keys + lo) & 15) == 0;
+    uint64_t i = lo + (uint64_t)threadIdx.x * 4;
+    if (vec) {
+      for (; i + 3 < hi; i += 4096) {
+        int4 k = ld_nt4(keys + i);
+        atomicAdd(&hsh[joint(k.x)], 1u);
+        atomicAdd(&hsh[joint(k.y)], 1u);
+        atomicAdd(&hsh[joint(k.z)], 1u);
+        atomicAdd(&hsh[joint(k.w)], 1u);
+      }
+    }
+    for (; i < hi; i += 4096)
+      for (uint64_t j = i; j < i + 4 && j < hi; ++j) atomicAdd(&hsh[joint(keys[j])], 1u);
+  }
+  __syncthreads();
+  uint32_t *row = H + (uint64_t)blockIdx.x * nb;
+  for (int i = threadIdx.x; i < nb; i += 1024) row[i] = hsh[i];
+}
+
+// Column-wise exclusive scan over chunks: E[c][d] = sum_{c'<c} H[c'][d];
+// T[d] = sum over all chunks.
+__global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb,
+                          uint32_t *__restrict__ E, uint32_t *__restrict__ T);
+// Exclusive scan of T[0..nb) into S[0..nb] (one workgroup of 1024 threads).
+__global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
+                            unsigned long long *__restrict__ S);
+
+// One radix pass.  PASS2 = false: workgroup c partitions input chunk c on the
+// pass-1 digit.  PASS2 = true: workgroups take (d1, group of cpg chunks) items
+// of the pass-1 output from a counter and partition them on the pass-2 digit.
+// ROWID: the V1 (int32) column is the input row index.
+template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, bool PASS2,
+          int LINE = kLine>
+__global__ __launch_bounds__(BLOCK) void k_radix_pass(
     const int32_t *__restrict__ in_k, const V1 *__restrict__ in_v,
-    const V2 *__restrict__ in_w, int32_t *__restrict__ out_k,
-    V1 *__restrict__ out_v, V2 *__restrict__ out_w, ScatterPlan plan, Digit dig) {
+    const V2 *__restrict__ in_w, int32_t *__restrict__ out_k, V1 *__restrict__ out_v,
+    V2 *__restrict__ out_w, uint64_t n, uint64_t chunk, int G, int nb, int b2, uint32_t cpg,
+    Joint joint,
+    const uint32_t *__restrict__ H, const uint32_t *__restrict__ E,
+    const unsigned long long *__restrict__ S, uint32_t *__restrict__ seg_counter) {
   constexpr int IPT = T / BLOCK;
   static_assert(IPT % 4 == 0, "tile must give each thread a multiple of 4 items");
   static_assert(F <= BLOCK, "one thread per digit");
   static_assert(!ROWID || sizeof(V1) == 4, "row ids are int32");
+  static_assert(!(ROWID && PASS2), "row ids only in pass 1");
   constexpr bool HW = ValIO<V2>::present;
-  constexpr int MAXLINES = T / kLine + F;
+  static_assert(LINE == 16 || LINE == 32, "16- or 32-tuple lines");
+  constexpr int LOGL = LINE == 32 ? 5 : 4;
+  constexpr int MAXLINES = T / LINE + 2 * F;
   constexpr int TW = HW ? T : 1;
   constexpr int FW = HW ? F : 1;
 
   __shared__ int32_t st_k[T];
   __shared__ V1 st_v[T];
   __shared__ V2 st_w[TW];
-  __shared__ int32_t lf_k[F][kLine];
-  __shared__ V1 lf_v[F][kLine];
-  __shared__ V2 lf_w[FW][kLine];
-  __shared__ uint32_t hist[F], off[F], lcnt[F], nfull[F], linepre[F];
-  __shared__ unsigned long long gbase[F];
+  __shared__ int32_t lf_k[F][LINE];
+  __shared__ V1 lf_v[F][LINE];
+  __shared__ V2 lf_w[FW][LINE];
+  // per-digit state, read with one or two wide LDS loads
+  struct Meta {
+    unsigned long long cursor;  // next output position of the digit
+    uint32_t lcnt;              // tuples waiting in the digit's line buffer
+    uint32_t hist;              // tuples of the digit in this tile
+    uint32_t nwrite;            // tuples written this tile
+    uint32_t off;               // start of the digit's run in the stage
+    uint32_t linepre;           // first line of the digit in this tile
+    uint32_t pad;
+  };
+  __shared__ Meta meta[F];
   __shared__ uint16_t line_digit[MAXLINES];
   __shared__ uint32_t scan_tmp[BLOCK / 64];
   __shared__ uint32_t s_total_lines;
+  __shared__ unsigned long long s_seg_lo, s_seg_hi;
+  __shared__ uint32_t s_seg;
 
   const int tid = threadIdx.x;
-  uint32_t t0 = blockIdx.x * plan.tiles_per_wg;
-  uint32_t t1 = t0 + plan.tiles_per_wg;
-  if (t1 > plan.ntiles) t1 = plan.ntiles;
-  if (t0 >= t1) return;
+  const uint32_t F2 = 1u << b2;
+  const uint32_t F1 = (uint32_t)nb >> b2;  // pass-1 digits
+  // number of digits of this pass (<= F)
+  const uint32_t ND = PASS2 ? F2 : F1;
 
-  uint32_t r = 0;
-  {
-    uint32_t lo = 0, hi = plan.nreg;  // last r with tile_begin[r] <= t0
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (plan.tile_begin[mid] <= t0) lo = mid; else hi = mid;
-    }
-    r = lo;
-  }
-  if (tid < F) lcnt[tid] = 0;
-
-  auto flush_tails = [&](uint32_t reg) {
-    __syncthreads();
-    if (tid < F) {
-      uint32_t c = lcnt[tid];
-      gbase[tid] = c ? reserve_tail(&plan.ctail[(uint64_t)reg * F + tid], c) : 0ULL;
-    }
-    __syncthreads();
-    for (int x = tid; x < F * kLine; x += BLOCK) {
-      int d = x / kLine, s = x % kLine;
-      if ((uint32_t)s < lcnt[d]) {
-        unsigned long long g = gbase[d] + s;
-        out_k[g] = lf_k[d][s];
-        out_v[g] = lf_v[d][s];
-        if constexpr (HW) out_w[g] = lf_w[d][s];
+  for (;;) {
+    // ---- pick the segment and set its cursors -------------------------------
+    if (!PASS2) {
+      const uint64_t c = blockIdx.x;
+      if (tid == 0) {
+        uint64_t a = c * chunk, b = a + chunk;
+        s_seg_lo = a < n ? a : n;
+        s_seg_hi = b < n ? b : n;
       }
-    }
-    __syncthreads();
-    if (tid < F) lcnt[tid] = 0;
-  };
-
-  for (uint32_t t = t0; t < t1; ++t) {
-    while (t >= plan.tile_begin[r + 1]) {
-      flush_tails(r);
-      ++r;
-    }
-    const uint64_t rstart = plan.reg_start[r];
-    const uint64_t rcount = plan.reg_count[r];
-    const uint64_t toff = (uint64_t)(t - plan.tile_begin[r]) * T;
-    const uint64_t base = rstart + toff;
-    const uint32_t len = (uint32_t)((rcount - toff) < (uint64_t)T ? (rcount - toff) : T);
-
-    // ---- load ----------------------------------------------------------
-    int32_t k[IPT];
-    V1 v[IPT];
-    V2 w[IPT];
-    uint32_t dg[IPT], rk[IPT];
-    const bool vec = (((uintptr_t)(in_k + base) & 15) == 0) &&
-                     (ROWID || ValIO<V1>::aligned(in_v, base)) &&
-                     ValIO<V2>::aligned(in_w, base);
-#pragma unroll
-    for (int q = 0; q < IPT / 4; ++q) {
-      uint32_t e = (uint32_t)(q * BLOCK + tid) * 4;
-      ValIO<int32_t>::load4(in_k, base, e, len, vec, &k[4 * q]);
-      if constexpr (ROWID) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[4 * q + c] = __builtin_bit_cast(V1, (int32_t)(base + e + c));
-      } else {
-        ValIO<V1>::load4(in_v, base, e, len, vec, &v[4 * q]);
+      // cursor[d1] = S[d1 << b2] + sum_{d2} E[c][d1][d2]
+      const uint64_t row = c * (uint64_t)nb;
+      for (int d = tid; d < F; d += BLOCK) {
+        unsigned long long cur = 0;
+        if ((uint32_t)d < F1) {
+          cur = S[(uint64_t)d << b2];
+          for (uint32_t d2 = 0; d2 < F2; ++d2) cur += E[row + ((uint64_t)d << b2) + d2];
+        }
+        meta[d].cursor = cur;
+        meta[d].lcnt = 0;
+        meta[d].hist = 0;
       }
-      ValIO<V2>::load4(in_w, base, e, len, vec, &w[4 * q]);
-    }
-    if (tid < F) hist[tid] = 0;
-    __syncthreads();
-    // ---- digits and LDS ranks ------------------------------------------
-#pragma unroll
-    for (int q = 0; q < IPT / 4; ++q) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        uint32_t e = (uint32_t)(q * BLOCK + tid) * 4 + c;
-        int i = 4 * q + c;
-        if (e < len) {
-          dg[i] = dig(k[i]);
-          rk[i] = atomicAdd(&hist[dg[i]], 1u);
-        } else {
-          dg[i] = 0xffffffffu;
-          rk[i] = 0;
+    } else {
+      // work item = (d1, group of `cpg` consecutive chunks): the runs those
+      // chunks wrote for d1 are adjacent, so the item is one contiguous range
+      if (tid == 0) s_seg = atomicAdd(seg_counter, 1u);
+      __syncthreads();
+      const uint32_t seg = s_seg;
+      const uint32_t ngrp = ((uint32_t)G + cpg - 1) / cpg;
+      if (seg >= ngrp * F1) break;
+      const uint32_t d1 = seg / ngrp, g = seg % ngrp;
+      const uint32_t c0 = g * cpg, c1 = c0 + cpg < (uint32_t)G ? c0 + cpg : (uint32_t)G;
+      const uint64_t row0 = (uint64_t)c0 * (uint64_t)nb + ((uint64_t)d1 << b2);
+      const uint64_t row1 = (uint64_t)c1 * (uint64_t)nb + ((uint64_t)d1 << b2);
+      if (tid < 64) {
+        unsigned long long pre0 = 0, pre1 = 0;
+        for (uint32_t d2 = tid; d2 < F2; d2 += 64) {
+          pre0 += E[row0 + d2];
+          if (c1 < (uint32_t)G) pre1 += E[row1 + d2];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          pre0 += __shfl_xor(pre0, o, 64);
+          pre1 += __shfl_xor(pre1, o, 64);
+        }
+        if (tid == 0) {
+          const unsigned long long r0 = S[(uint64_t)d1 << b2];
+          s_seg_lo = r0 + pre0;
+          s_seg_hi = c1 < (uint32_t)G ? r0 + pre1 : S[(uint64_t)(d1 + 1) << b2];
         }
       }
+      for (uint32_t d = tid; d < (uint32_t)F; d += BLOCK) {
+        meta[d].cursor = d < F2 ? S[((uint64_t)d1 << b2) + d] + E[row0 + d] : 0ULL;
+        meta[d].lcnt = 0;
+        meta[d].hist = 0;
+      }
     }
     __syncthreads();
-    // ---- scan, full-line reservation ------------------------------------
-    {
-      uint32_t hv = tid < F ? hist[tid] : 0;
-      uint32_t tot;
-      uint32_t o = block_excl_scan<BLOCK>(hv, scan_tmp, &tot);
-      uint32_t n = tid < F ? lcnt[tid] + hv : 0;
-      uint32_t full = n & ~(uint32_t)(kLine - 1);
-      uint32_t ltot;
-      uint32_t lp = block_excl_scan<BLOCK>(full / kLine, scan_tmp, &ltot);
+    const uint64_t seg_lo = s_seg_lo, seg_hi = s_seg_hi;
+    // tiles cover [base0, seg_hi) with base0 = seg_lo rounded down to 4
+    const uint64_t base0 = seg_lo & ~3ULL;
+    const uint32_t lo0 = (uint32_t)(seg_lo - base0);
+    const uint64_t span = seg_hi > base0 ? seg_hi - base0 : 0;
+    const uint32_t ntiles = (uint32_t)((span + T - 1) / T);
+
+    int32_t k[IPT], kn[IPT];
+    V1 v[IPT], vn[IPT];
+    V2 w[IPT], wn[IPT];
+    auto load_tile = [&](uint32_t t, int32_t *kk, V1 *vv, V2 *ww) {
+      const uint64_t base = base0 + (uint64_t)t * T;
+      const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
+      const uint32_t lo = t == 0 ? lo0 : 0;
+      const bool vec = (((uintptr_t)(in_k + base) & 15) == 0) &&
+                       (ROWID || ValIO<V1>::aligned(in_v, base)) &&
+                       ValIO<V2>::aligned(in_w, base);
+#pragma unroll
+      for (int q = 0; q < IPT / 4; ++q) {
+        uint32_t e = (uint32_t)(q * BLOCK + tid) * 4;
+        ValIO<int32_t>::load4(in_k, base, e, lo, len, vec, &kk[4 * q]);
+        if constexpr (ROWID) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            vv[4 * q + c] = __builtin_bit_cast(V1, (int32_t)(base + e + c));
+        } else {
+          ValIO<V1>::load4(in_v, base, e, lo, len, vec, &vv[4 * q]);
+        }
+        ValIO<V2>::load4(in_w, base, e, lo, len, vec, &ww[4 * q]);
+      }
+    };
+    if (ntiles) load_tile(0, k, v, w);
+
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      // the next tile streams in while this one is shuffled and written
+      if (t + 1 < ntiles) load_tile(t + 1, kn, vn, wn);
+      const uint64_t base = base0 + (uint64_t)t * T;
+      const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
+      const uint32_t lo = t == 0 ? lo0 : 0;
+      // ---- digits and LDS ranks (hist was cleared after the last tile) ------
+      uint32_t dg[IPT], rk[IPT];
+#pragma unroll
+      for (int q = 0; q < IPT / 4; ++q) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          uint32_t e = (uint32_t)(q * BLOCK + tid) * 4 + c;
+          int i = 4 * q + c;
+          if (e >= lo && e < len) {
+            uint32_t j = joint(k[i]);
+            dg[i] = PASS2 ? (j & (F2 - 1)) : (j >> b2);
+            rk[i] = atomicAdd(&meta[dg[i]].hist, 1u);
+          } else {
+            dg[i] = 0xffffffffu;
+            rk[i] = 0;
+          }
+        }
+      }
+      __syncthreads();
+      // ---- one scan of (run length, line count) packed as 16 + 16 bits -----
+      {
+        uint32_t hv = 0, nl = 0, nw = 0;
+        if ((uint32_t)tid < ND) {
+          const Meta m = meta[tid];
+          hv = m.hist;
+          const unsigned long long end = m.cursor + m.lcnt + hv;
+          const unsigned long long aend = end & ~(unsigned long long)(LINE - 1);
+          if (aend > m.cursor) {
+            nw = (uint32_t)(aend - m.cursor);
+            nl = (uint32_t)(((aend - 1) >> LOGL) - (m.cursor >> LOGL) + 1);
+          }
+        }
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan<BLOCK>(hv | (nl << 16), scan_tmp, &tot);
+        if (tid < F) {
+          meta[tid].off = pre & 0xffffu;
+          meta[tid].nwrite = nw;
+          meta[tid].linepre = pre >> 16;
+        }
+        if (tid == 0) s_total_lines = tot >> 16;
+      }
+      __syncthreads();
+      // ---- shuffle into LDS by digit; line -> digit map --------------------
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        if (dg[i] != 0xffffffffu) {
+          uint32_t p = meta[dg[i]].off + rk[i];
+          st_k[p] = k[i];
+          st_v[p] = v[i];
+          if constexpr (HW) st_w[p] = w[i];
+        }
+      }
       if (tid < F) {
-        off[tid] = o;
-        nfull[tid] = full;
-        linepre[tid] = lp;
-        gbase[tid] = full ? atomicAdd(&plan.cfull[(uint64_t)r * F + tid],
-                                      (unsigned long long)full)
-                          : 0ULL;
-      }
-      if (tid == 0) s_total_lines = ltot;
-    }
-    __syncthreads();
-    // ---- shuffle into LDS by digit; line -> digit map --------------------
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      if (dg[i] != 0xffffffffu) {
-        uint32_t p = off[dg[i]] + rk[i];
-        st_k[p] = k[i];
-        st_v[p] = v[i];
-        if constexpr (HW) st_w[p] = w[i];
-      }
-    }
-    if (tid < F) {
-      uint32_t nl = nfull[tid] / kLine, lp = linepre[tid];
-      for (uint32_t l = 0; l < nl; ++l) line_digit[lp + l] = (uint16_t)tid;
-    }
-    __syncthreads();
-    // ---- write full lines: one half-wave per 32-row line ----------------
-    {
-      const uint32_t nlines = s_total_lines;
-      const int hw = tid >> 5, l32 = tid & 31;
-      for (uint32_t L = hw; L < nlines; L += BLOCK / 32) {
-        uint32_t d = line_digit[L];
-        uint32_t j = (L - linepre[d]) * kLine + l32;
-        uint32_t lc = lcnt[d];
-        int32_t ok;
-        V1 ov;
-        V2 ow;
-        if (j < lc) {
-          ok = lf_k[d][j];
-          ov = lf_v[d][j];
-          if constexpr (HW) ow = lf_w[d][j];
-        } else {
-          uint32_t src = off[d] + j - lc;
-          ok = st_k[src];
-          ov = st_v[src];
-          if constexpr (HW) ow = st_w[src];
+        const Meta m = meta[tid];
+        if (m.nwrite) {
+          const uint32_t nl =
+              (uint32_t)(((m.cursor + m.nwrite - 1) >> LOGL) - (m.cursor >> LOGL) + 1);
+          for (uint32_t l = 0; l < nl; ++l) line_digit[m.linepre + l] = (uint16_t)tid;
         }
-        unsigned long long g = gbase[d] + j;
-        __builtin_nontemporal_store(ok, out_k + g);
-        __builtin_nontemporal_store(ov, out_v + g);
-        if constexpr (HW) __builtin_nontemporal_store(ow, out_w + g);
+      }
+      __syncthreads();
+      // ---- write lines: LINE lanes per line ----------------------------------
+      {
+        const uint32_t nlines = s_total_lines;
+        const int hw = tid / LINE, l32 = tid % LINE;
+        for (uint32_t L = hw; L < nlines; L += BLOCK / LINE) {
+          const uint32_t d = line_digit[L];
+          const Meta m = meta[d];
+          const unsigned long long pos =
+              (((m.cursor >> LOGL) + (L - m.linepre)) << LOGL) + (unsigned long long)l32;
+          if (pos >= m.cursor && pos < m.cursor + m.nwrite) {
+            const uint32_t j = (uint32_t)(pos - m.cursor);
+            int32_t ok;
+            V1 ov;
+            V2 ow;
+            if (j < m.lcnt) {
+              ok = lf_k[d][j];
+              ov = lf_v[d][j];
+              if constexpr (HW) ow = lf_w[d][j];
+            } else {
+              const uint32_t src = m.off + j - m.lcnt;
+              ok = st_k[src];
+              ov = st_v[src];
+              if constexpr (HW) ow = st_w[src];
+            }
+            __builtin_nontemporal_store(ok, out_k + pos);
+            __builtin_nontemporal_store(ov, out_v + pos);
+            if constexpr (HW) __builtin_nontemporal_store(ow, out_w + pos);
+          }
+        }
+      }
+      __syncthreads();
+      // ---- keep the remainders (< LINE per digit), LINE lanes per digit -----
+      // A digit's first write completes its first (partial) line and later
+      // writes are whole lines, so nwrite > lcnt whenever nwrite > 0.
+      for (int x = tid; x < F * LINE; x += BLOCK) {
+        const int d = x / LINE, s = x % LINE;
+        const Meta m = meta[d];
+        const uint32_t nn = m.lcnt + m.hist;
+        int64_t src = -1;
+        if (m.nwrite) {
+          if ((uint32_t)s < nn - m.nwrite) src = (int64_t)m.off + m.nwrite + s - m.lcnt;
+        } else if ((uint32_t)s >= m.lcnt && (uint32_t)s < nn) {
+          src = (int64_t)m.off + s - m.lcnt;
+        }
+        if (src >= 0) {
+          lf_k[d][s] = st_k[src];
+          lf_v[d][s] = st_v[src];
+          if constexpr (HW) lf_w[d][s] = st_w[src];
+        }
+      }
+      __syncthreads();
+      if (tid < F) {
+        Meta &m = meta[tid];
+        m.cursor += m.nwrite;
+        m.lcnt = m.lcnt + m.hist - m.nwrite;
+        m.hist = 0;
+      }
+      __syncthreads();
+      if (t + 1 < ntiles) {
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+          k[i] = kn[i];
+          v[i] = vn[i];
+          if constexpr (HW) w[i] = wn[i];
+        }
+      }
+    }
+    // ---- flush the remainders of this segment --------------------------------
+    for (int x = tid; x < F * LINE; x += BLOCK) {
+      const int d = x / LINE, s = x % LINE;
+      if ((uint32_t)s < meta[d].lcnt) {
+        const unsigned long long pos = meta[d].cursor + s;
+        out_k[pos] = lf_k[d][s];
+        out_v[pos] = lf_v[d][s];
+        if constexpr (HW) out_w[pos] = lf_w[d][s];
       }
     }
     __syncthreads();
-    // ---- keep remainders ---------------------------------------------------
-    for (int x = tid; x < F * kLine; x += BLOCK) {
-      int d = x / kLine, s = x % kLine;
-      uint32_t lc = lcnt[d], h = hist[d], full = nfull[d], n = lc + h;
-      int64_t src = -1;
-      if (full) {
-        if ((uint32_t)s < n - full) src = (int64_t)off[d] + full + s - lc;
-      } else if ((uint32_t)s >= lc && (uint32_t)s < n) {
-        src = (int64_t)off[d] + s - lc;
-      }
-      if (src >= 0) {
-        lf_k[d][s] = st_k[src];
-        lf_v[d][s] = st_v[src];
-        if constexpr (HW) lf_w[d][s] = st_w[src];
-      }
-    }
-    __syncthreads();
-    if (tid < F) lcnt[tid] = lcnt[tid] + hist[tid] - nfull[tid];
+    if (!PASS2) break;
   }
-  flush_tails(r);
+}
+
+// Host side ------------------------------------------------------------------
+struct RadixPlan {
+  int b1 = 0, b2 = 0;  // fan-out bits of pass 1 / pass 2 (b2 = 0: one pass)
+  int nb = 0;          // joint bins (2^(b1+b2), or the bucket count)
+  int G = 0;           // pass-1 chunks (workgroups)
+  uint64_t chunk = 0;  // tuples per chunk, a multiple of 4
+};
+
+// Chunking of n tuples for pass 1: about `per_cu` chunks per CU.
+RadixPlan radix_make_plan(pj_ctx *ctx, uint64_t n, int b1, int b2, int nb, int per_cu);
+
+// Device scratch of one side's histograms.
+struct RadixHist {
+  Scratch H, E, T, S, ctr;
+  std::vector<uint32_t> hT;            // partition sizes (host)
+  std::vector<unsigned long long> hS;  // partition starts (host), nb + 1
+  int alloc(pj_ctx *ctx, const RadixPlan &p) {
+    PJ_TRY(H.alloc(ctx, (uint64_t)p.G * p.nb * 4));
+    PJ_TRY(E.alloc(ctx, (uint64_t)p.G * p.nb * 4));
+    PJ_TRY(T.alloc(ctx, (uint64_t)p.nb * 4));
+    PJ_TRY(S.alloc(ctx, (uint64_t)(p.nb + 1) * 8));
+    PJ_TRY(ctr.alloc(ctx, 256));
+    return PJ_OK;
+  }
+};
+
+// Histogram + scans for one side into h (device), optionally reading T and S
+// back to the host.
+template <class Joint>
+int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPlan &p,
+                     Joint joint, RadixHist &h, bool readback) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)k_chunk_hist<Joint>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
+    attr_set = true;
+  }
+  PJ_LAUNCH(ctx, "radix_hist", k_chunk_hist<Joint>, p.G, 1024, (size_t)p.nb * 4, d_k, n,
+            p.chunk, p.nb, joint, h.H.as<uint32_t>());
+  PJ_LAUNCH(ctx, "radix_colscan", k_colscan, ceil_div(p.nb, 256), 256, 0,
+            h.H.as<const uint32_t>(), p.G, p.nb, h.E.as<uint32_t>(), h.T.as<uint32_t>());
+  PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, 1024, 0, h.T.as<const uint32_t>(), p.nb,
+            h.S.as<unsigned long long>());
+  PJ_TRY(fill_u32(ctx, h.ctr.as<uint32_t>(), 0, 1));
+  if (readback) {
+    h.hT.resize(p.nb);
+    h.hS.resize(p.nb + 1);
+    PJ_TRY(d2h(ctx, h.hT.data(), h.T.p, (uint64_t)p.nb * 4));
+    PJ_TRY(d2h(ctx, h.hS.data(), h.S.p, (uint64_t)(p.nb + 1) * 8));
+  }
+  return PJ_OK;
+}
+
+// One pass launch (pass2 selects the segment-walking variant).
+template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE = kLine>
+int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
+                      const V1 *iv, const V2 *iw, int32_t *ok, V1 *ov, V2 *ow, uint64_t n,
+                      const RadixPlan &p, Joint joint, RadixHist &h) {
+  if (!pass2) {
+    PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE>),
+              p.G, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, 1u, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
+              h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
+  } else {
+    if constexpr (!ROWID) {
+      const uint32_t F1 = 1u << p.b1;
+      constexpr uint64_t lds = (uint64_t)T * (4 + sizeof(V1) + (ValIO<V2>::present ? sizeof(V2) : 0)) +
+                               (uint64_t)F * LINE * (4 + sizeof(V1) + (ValIO<V2>::present ? sizeof(V2) : 0));
+      const uint64_t per_cu = lds > 80000 ? 1 : 2;
+      uint64_t grid = (uint64_t)ctx->num_cus * per_cu;
+      // about 4 items per workgroup, each a group of consecutive chunks
+      uint64_t want = grid * 4;
+      uint32_t ngrp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(p.G, ceil_div(want, F1)));
+      uint32_t cpg = (uint32_t)ceil_div(p.G, ngrp);
+      ngrp = (uint32_t)ceil_div(p.G, cpg);
+      uint64_t nseg = (uint64_t)ngrp * F1;
+      if (grid > nseg) grid = nseg;
+      PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE>),
+                grid, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, cpg, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
+                h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
+    } else {
+      return fail(PJ_E_ARG, "row-id values only in pass 1");
+    }
+  }
+  return PJ_OK;
 }
 
 }  // namespace pj
