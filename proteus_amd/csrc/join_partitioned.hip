@@ -92,15 +92,18 @@ struct JoinUnit {
 // workgroup b; consecutive units of one build partition reuse the table.
 //
 // The table is bucket-sorted ("CSR"): tab[b] packs the [start, end) range of
-// bucket b = fmix32(key) & (H-1) in ent[] (16 + 16 bits), entries are
-// (key, value) pairs grouped by bucket.  Building it is count -> scan ->
-// scatter with LDS atomics; probing is one LDS read for the range plus the
-// bucket's entries, which sit next to each other -- the reference's chained
-// LDS table (head / next, gpu-partitioned-hash-join-chained.cpp:832-1037)
-// needs one dependent read per chain hop instead.  Every entry whose key
-// equals the probe key is a match, so duplicate build keys emit every pair.
-template <int BLOCK, int H, int NCAP>
-__global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
+// bucket b in ent[] (16 + 16 bits) and entries are (key, value) pairs grouped
+// by bucket.  Building it is count -> scan -> scatter with LDS atomics;
+// probing is one LDS read for the range plus the bucket's entries, which sit
+// next to each other -- the reference's chained LDS table (head / next,
+// gpu-partitioned-hash-join-chained.cpp:832-1037) needs one dependent read per
+// chain hop instead.  Every entry whose key equals the probe key is a match,
+// so duplicate build keys emit every pair.
+// HASHED: keys are fmix32(original key) (written by the radix passes); the
+// bucket is their low bits and equality of hashes is equality of keys.
+// Otherwise keys are raw and hashed here.
+template <int BLOCK, int H, int NCAP, bool HASHED>
+__global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
     const JoinUnit *__restrict__ units, const uint32_t *__restrict__ wg_begin,
@@ -112,6 +115,9 @@ __global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
   __shared__ uint32_t tab[H];
   __shared__ int2 ent[NCAP];
   __shared__ uint32_t scan_tmp[BLOCK / 64];
+  auto bucket = [](int32_t k) -> uint32_t {
+    return (HASHED ? (uint32_t)k : fmix32((uint32_t)k)) & (H - 1);
+  };
   const int tid = threadIdx.x;
   JoinAcc acc;
   acc.init();
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
           const uint32_t i = (uint32_t)(q * BLOCK + tid);
           bk[q] = i < rl ? rkp[i] : 0;
           bv[q] = (i < rl && rvp) ? rvp[i] : 0;
-          bb[q] = fmix32((uint32_t)bk[q]) & (H - 1);
+          bb[q] = bucket(bk[q]);
         }
         __syncthreads();
 #pragma unroll
@@ -187,9 +193,8 @@ __global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
       // prefetched); the <= 3 tuples before and after it one by one.
       const unsigned long long s0 = U.s_begin, s1 = U.s_begin + U.s_len;
       const unsigned long long a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
-      auto probe1 = [&](int32_t key, int32_t pv, uint32_t rgv) {
-        const uint32_t st = rgv & 0xffffu, en = rgv >> 16;
-        if (st == en) return;
+      // all entries of a bucket of more than one entry
+      auto probe_multi = [&](int32_t key, int32_t pv, uint32_t st, uint32_t en) {
         int64_t m = 0, sb = 0;
         int32_t mnb = INT32_MAX, mxb = INT32_MIN;
         for (uint32_t i = st; i < en; ++i) {
@@ -203,17 +208,18 @@ __global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
         }
         acc.add(m, sb, mnb, mxb, pv);
       };
+      auto probe1 = [&](int32_t key, int32_t pv) {
+        const uint32_t rg = tab[bucket(key)];
+        const uint32_t st = rg & 0xffffu, en = rg >> 16;
+        if (st < en) probe_multi(key, pv, st, en);
+      };
       if (a0 >= a1) {  // short unit: every tuple on its own
-        for (unsigned long long x = s0 + tid; x < s1; x += BLOCK) {
-          const int32_t key = sk[x];
-          probe1(key, sv ? sv[x] : 0, tab[fmix32((uint32_t)key) & (H - 1)]);
-        }
+        for (unsigned long long x = s0 + tid; x < s1; x += BLOCK) probe1(sk[x], sv ? sv[x] : 0);
       } else {
         const int tail_lo = (int)(a0 - s0), tail_hi = (int)(s1 - a1);
         if (tid < tail_lo + tail_hi) {
           const unsigned long long x = tid < tail_lo ? s0 + tid : a1 + (tid - tail_lo);
-          const int32_t key = sk[x];
-          probe1(key, sv ? sv[x] : 0, tab[fmix32((uint32_t)key) & (H - 1)]);
+          probe1(sk[x], sv ? sv[x] : 0);
         }
         const int4 *kq = reinterpret_cast<const int4 *>(sk + a0);
         const int4 *vq = sv ? reinterpret_cast<const int4 *>(sv + a0) : nullptr;
@@ -234,21 +240,27 @@ __global__ __launch_bounds__(BLOCK, (NCAP <= 6144 ? 4 : 2)) void k_pjoin(
           const int32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
           uint32_t rg[4];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) rg[c] = tab[fmix32((uint32_t)kk[c]) & (H - 1)];
-          // the first entry of every bucket, four independent LDS reads
+          for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
           int2 e0[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu;  // == NCAP when the tail buckets are empty
             e0[c] = ent[st < (uint32_t)NCAP ? st : 0u];
           }
+          // one-entry buckets without branches; longer ones in probe_multi
+          bool multi = false;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
-            if (en == st + 1) {  // one-entry bucket (the common case)
-              if (e0[c].x == kk[c]) acc.add1(e0[c].y, vv[c]);
-            } else if (en > st + 1) {
-              probe1(kk[c], vv[c], rg[c]);
+            const bool hit = (en == st + 1) & (e0[c].x == kk[c]);
+            acc.add1_if(hit, e0[c].y, vv[c]);
+            multi |= en > st + 1;
+          }
+          if (multi) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+              if (en > st + 1) probe_multi(kk[c], vv[c], st, en);
             }
           }
           k4 = kn;
@@ -277,21 +289,27 @@ static size_t pack_add(std::vector<uint8_t> &buf, const std::vector<T> &v) {
 template <int F>
 static int join_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
                      const int32_t *iv, int32_t *ok, int32_t *ov, uint64_t n,
-                     const RadixPlan &p, JointRadix j, RadixHist &h) {
-  // fan-out 256: 16-tuple (64 B) lines keep two workgroups per CU
+                     const RadixPlan &p, JointHashed j, RadixHist &h) {
+  // pass 1 replaces keys by fmix32(key); pass 2 and the join read digits,
+  // buckets and equality off the hashes
+  // LDS write-combining; fan-out 256 uses 16-tuple (64 B) lines to keep two
+  // workgroups per CU
   constexpr int T = 4096;
   constexpr int BLOCK = 512;
   constexpr int LINE = F >= 256 ? 16 : 32;
-  if (!pass2 && !iv)
-    return radix_launch_pass<F, T, BLOCK, JointRadix, int32_t, NoVal, true, LINE>(
+  if (pass2)
+    return radix_launch_pass<F, T, BLOCK, JointHashed, int32_t, NoVal, false, LINE, false>(
+        ctx, name, true, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
+  if (!iv)
+    return radix_launch_pass<F, T, BLOCK, JointHashed, int32_t, NoVal, true, LINE, true>(
         ctx, name, false, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
-  return radix_launch_pass<F, T, BLOCK, JointRadix, int32_t, NoVal, false, LINE>(
-      ctx, name, pass2, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
+  return radix_launch_pass<F, T, BLOCK, JointHashed, int32_t, NoVal, false, LINE, true>(
+      ctx, name, false, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
 }
 
 static int join_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
                           const int32_t *ik, const int32_t *iv, int32_t *ok, int32_t *ov,
-                          uint64_t n, const RadixPlan &p, JointRadix j, RadixHist &h) {
+                          uint64_t n, const RadixPlan &p, JointHashed j, RadixHist &h) {
   switch (bits) {
     case 1: return join_pass<2>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
     case 2: return join_pass<4>(ctx, name, pass2, ik, iv, ok, ov, n, p, j, h);
@@ -318,8 +336,8 @@ static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_
   const int B = b1 + b2;
   s.plan = radix_make_plan(ctx, n, b1, b2, 1 << B, 2);
   PJ_TRY(s.hist.alloc(ctx, s.plan));
-  JointRadix j{32 - B};
-  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, j, s.hist, true));
+  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, true));
+  JointHashed j{32 - B};
   PJ_TRY(s.k1.alloc(ctx, n * 4 + 256));
   PJ_TRY(s.v1.alloc(ctx, n * 4 + 256));
   char name[64];
@@ -340,42 +358,53 @@ static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_
   return PJ_OK;
 }
 
-// LDS table geometries: NCAP rows, H heads.
+// LDS table geometries (bucket count H = 2^lds_log, capacity NCAP rows).
 struct Geometry {
   int lds_log;
   int ncap;
+  int per_cu;  // workgroups per CU the LDS allows
 };
 static Geometry geometry_for(int lds_log) {
   switch (lds_log) {
-    case 11: return {11, 3072};
-    case 12: return {12, 6144};
-    default: return {13, 12288};
+    case 11: return {11, 3072, 4};
+    case 12: return {12, 6144, 2};
+    case 14: return {14, 11264, 1};
+    default: return {13, 5632, 2};
   }
 }
 
 constexpr int kJoinBlock = 512;
 
-static int launch_pjoin(pj_ctx *ctx, int lds_log, const int32_t *rk, const int32_t *rv,
-                        const int32_t *sk, const int32_t *sv, const JoinUnit *units,
-                        const uint32_t *wgb, uint32_t grid, unsigned long long *acc) {
+template <int H, int NCAP>
+static int launch_pjoin_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
+                          const int32_t *sk, const int32_t *sv, const JoinUnit *units,
+                          const uint32_t *wgb, uint32_t grid, unsigned long long *acc) {
+  if (hashed)
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, true>), grid, kJoinBlock, 0,
+              rk, rv, sk, sv, units, wgb, acc);
+  else
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, false>), grid, kJoinBlock, 0,
+              rk, rv, sk, sv, units, wgb, acc);
+  return PJ_OK;
+}
+
+static int launch_pjoin(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk,
+                        const int32_t *rv, const int32_t *sk, const int32_t *sv,
+                        const JoinUnit *units, const uint32_t *wgb, uint32_t grid,
+                        unsigned long long *acc) {
   switch (lds_log) {
     case 11:
-      PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, 2048, 3072>), grid, kJoinBlock, 0, rk,
-                rv, sk, sv, units, wgb, acc);
-      return PJ_OK;
+      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
     case 12:
-      PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, 4096, 6144>), grid, kJoinBlock, 0, rk,
-                rv, sk, sv, units, wgb, acc);
-      return PJ_OK;
+      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
     case 13:
-      PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, 8192, 12288>), grid, kJoinBlock, 0,
-                rk, rv, sk, sv, units, wgb, acc);
-      return PJ_OK;
+      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
+    case 14:
+      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
     default:
-      return fail(PJ_E_ARG, "lds_log %d not in {11, 12, 13}", lds_log);
+      return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", lds_log);
   }
 }
-static int join_occupancy(int lds_log) { return lds_log >= 13 ? 1 : (lds_log == 12 ? 2 : 4); }
 
 static void balance_units(const std::vector<JoinUnit> &units, uint32_t grid,
                           std::vector<uint32_t> &wgb) {
@@ -437,7 +466,7 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
     }
   }
   if (!units.empty()) {
-    uint64_t g = (uint64_t)ctx->num_cus * join_occupancy(geo.lds_log) *
+    uint64_t g = (uint64_t)ctx->num_cus * geo.per_cu *
                  (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
     uint32_t grid = (uint32_t)std::min<uint64_t>(g, units.size());
     std::vector<uint32_t> wgb;
@@ -448,7 +477,7 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
     PJ_TRY(dplan.alloc(ctx, buf.size()));
     PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
     uint8_t *dp = dplan.as<uint8_t>();
-    PJ_TRY(launch_pjoin(ctx, geo.lds_log, rk, rv, sk, sv,
+    PJ_TRY(launch_pjoin(ctx, geo.lds_log, B > 0, rk, rv, sk, sv,
                         reinterpret_cast<const JoinUnit *>(dp + o_units),
                         reinterpret_cast<const uint32_t *>(dp + o_wgb), grid,
                         acc.as<unsigned long long>()));
@@ -458,7 +487,7 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
 
 static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
   const Geometry g = geometry_for(c.lds_log);
-  const uint64_t target = (uint64_t)g.ncap * 2 / 3;
+  const uint64_t target = (uint64_t)g.ncap * 3 / 4;
   int B = 0;
   while (B < 15 && (nb >> B) > target) ++B;
   if (B <= 8) {
@@ -471,9 +500,9 @@ static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
 }
 
 int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
-  if (c.lds_log == 0) c.lds_log = 12;
+  if (c.lds_log == 0) c.lds_log = 13;
   if (c.probe_unit == 0) c.probe_unit = 65536;
-  if (c.lds_log < 11 || c.lds_log > 13) return fail(PJ_E_ARG, "lds_log %d not in [11, 13]", c.lds_log);
+  if (c.lds_log < 11 || c.lds_log > 14) return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", c.lds_log);
   if (c.bits1 == 0 && c.bits2 == 0) derive_bits(c, nb);
   if (c.bits1 < 0 || c.bits1 > 8 || c.bits2 < 0 || c.bits2 > 8 || (c.bits1 == 0 && c.bits2 > 0))
     return fail(PJ_E_ARG, "radix bits (%d, %d) invalid: each in [0, 8], bits2 needs bits1",
@@ -573,7 +602,7 @@ extern "C" int pj_join_partitioned_reduce(pj_ctx *ctx, const int32_t *d_bk,
 int pj_autotune_join(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv, uint64_t nb,
                      const int32_t *d_pk, const int32_t *d_pv, uint64_t np) {
   std::vector<pj_radix_cfg> cands;
-  for (int lds : {12, 13}) {
+  for (int lds : {12, 13, 14}) {
     pj_radix_cfg base{};
     base.lds_log = lds;
     if (validate_cfg(base, nb) != PJ_OK) continue;

@@ -250,6 +250,16 @@ struct JoinAcc {
     min_p = min(min_p, pv);
     max_p = max(max_p, pv);
   }
+  // one match if `hit`, without branching
+  __device__ __forceinline__ void add1_if(bool hit, int32_t b, int32_t pv) {
+    n1 += hit ? 1u : 0u;
+    sum_b += hit ? (int64_t)b : 0;
+    sum_p += hit ? (int64_t)pv : 0;
+    min_b = hit ? min(min_b, b) : min_b;
+    max_b = hit ? max(max_b, b) : max_b;
+    min_p = hit ? min(min_p, pv) : min_p;
+    max_p = hit ? max(max_p, pv) : max_p;
+  }
   // `m` matches of one probe row whose build values sum to sb and span
   // [mnb, mxb].
   __device__ __forceinline__ void add(int64_t m, int64_t sb, int32_t mnb, int32_t mxb,

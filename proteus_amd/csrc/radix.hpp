@@ -42,6 +42,10 @@ struct JointRadix {  // top B bits of fmix32(key)
     return fmix32((uint32_t)k) >> shift;
   }
 };
+struct JointHashed {  // top B bits of a key that already is fmix32(original key)
+  int shift;
+  __device__ __forceinline__ uint32_t operator()(int32_t h) const { return (uint32_t)h >> shift; }
+};
 struct JointMod {  // hashInt32(key) URem buckets (gpu-hash-rearrange.cpp:338-348)
   uint32_t buckets;
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
@@ -164,7 +168,7 @@ __global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
 // of the pass-1 output from a counter and partition them on the pass-2 digit.
 // ROWID: the V1 (int32) column is the input row index.
 template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, bool PASS2,
-          int LINE = kLine>
+          int LINE = kLine, bool HASHKEYS = false>
 __global__ __launch_bounds__(BLOCK) void k_radix_pass(
     const int32_t *__restrict__ in_k, const V1 *__restrict__ in_v,
     const V2 *__restrict__ in_w, int32_t *__restrict__ out_k, V1 *__restrict__ out_v,
@@ -190,17 +194,18 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
   __shared__ int32_t lf_k[F][LINE];
   __shared__ V1 lf_v[F][LINE];
   __shared__ V2 lf_w[FW][LINE];
-  // per-digit state, read with one or two wide LDS loads
+  // per-digit state: hist / off are dense (rank atomics and per-tuple reads
+  // spread over all banks), the rest is read per line with one wide load
   struct Meta {
     unsigned long long cursor;  // next output position of the digit
     uint32_t lcnt;              // tuples waiting in the digit's line buffer
-    uint32_t hist;              // tuples of the digit in this tile
     uint32_t nwrite;            // tuples written this tile
-    uint32_t off;               // start of the digit's run in the stage
     uint32_t linepre;           // first line of the digit in this tile
     uint32_t pad;
   };
   __shared__ Meta meta[F];
+  __shared__ uint32_t hist[F];  // tuples of the digit in this tile
+  __shared__ uint32_t off[F];   // start of the digit's run in the stage
   __shared__ uint16_t line_digit[MAXLINES];
   __shared__ uint32_t scan_tmp[BLOCK / 64];
   __shared__ uint32_t s_total_lines;
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
         }
         meta[d].cursor = cur;
         meta[d].lcnt = 0;
-        meta[d].hist = 0;
+        hist[d] = 0;
       }
     } else {
       // work item = (d1, group of `cpg` consecutive chunks): the runs those
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
       for (uint32_t d = tid; d < (uint32_t)F; d += BLOCK) {
         meta[d].cursor = d < F2 ? S[((uint64_t)d1 << b2) + d] + E[row0 + d] : 0ULL;
         meta[d].lcnt = 0;
-        meta[d].hist = 0;
+        hist[d] = 0;
       }
     }
     __syncthreads();
@@ -309,6 +314,12 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
       const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
       const uint32_t lo = t == 0 ? lo0 : 0;
       // ---- digits and LDS ranks (hist was cleared after the last tile) ------
+      // HASHKEYS: the output carries fmix32(key), a bijection, so later
+      // passes and the join take digits / buckets / equality from it directly
+      if constexpr (HASHKEYS) {
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) k[i] = (int32_t)fmix32((uint32_t)k[i]);
+      }
       uint32_t dg[IPT], rk[IPT];
 #pragma unroll
       for (int q = 0; q < IPT / 4; ++q) {
@@ -319,7 +330,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
           if (e >= lo && e < len) {
             uint32_t j = joint(k[i]);
             dg[i] = PASS2 ? (j & (F2 - 1)) : (j >> b2);
-            rk[i] = atomicAdd(&meta[dg[i]].hist, 1u);
+            rk[i] = atomicAdd(&hist[dg[i]], 1u);
           } else {
             dg[i] = 0xffffffffu;
             rk[i] = 0;
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
         uint32_t hv = 0, nl = 0, nw = 0;
         if ((uint32_t)tid < ND) {
           const Meta m = meta[tid];
-          hv = m.hist;
+          hv = hist[tid];
           const unsigned long long end = m.cursor + m.lcnt + hv;
           const unsigned long long aend = end & ~(unsigned long long)(LINE - 1);
           if (aend > m.cursor) {
@@ -343,7 +354,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
         uint32_t tot;
         const uint32_t pre = block_excl_scan<BLOCK>(hv | (nl << 16), scan_tmp, &tot);
         if (tid < F) {
-          meta[tid].off = pre & 0xffffu;
+          off[tid] = pre & 0xffffu;
           meta[tid].nwrite = nw;
           meta[tid].linepre = pre >> 16;
         }
@@ -354,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         if (dg[i] != 0xffffffffu) {
-          uint32_t p = meta[dg[i]].off + rk[i];
+          uint32_t p = off[dg[i]] + rk[i];
           st_k[p] = k[i];
           st_v[p] = v[i];
           if constexpr (HW) st_w[p] = w[i];
@@ -388,7 +399,7 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
               ov = lf_v[d][j];
               if constexpr (HW) ow = lf_w[d][j];
             } else {
-              const uint32_t src = m.off + j - m.lcnt;
+              const uint32_t src = off[d] + j - m.lcnt;
               ok = st_k[src];
               ov = st_v[src];
               if constexpr (HW) ow = st_w[src];
@@ -406,12 +417,12 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
       for (int x = tid; x < F * LINE; x += BLOCK) {
         const int d = x / LINE, s = x % LINE;
         const Meta m = meta[d];
-        const uint32_t nn = m.lcnt + m.hist;
+        const uint32_t nn = m.lcnt + hist[d], o = off[d];
         int64_t src = -1;
         if (m.nwrite) {
-          if ((uint32_t)s < nn - m.nwrite) src = (int64_t)m.off + m.nwrite + s - m.lcnt;
+          if ((uint32_t)s < nn - m.nwrite) src = (int64_t)o + m.nwrite + s - m.lcnt;
         } else if ((uint32_t)s >= m.lcnt && (uint32_t)s < nn) {
-          src = (int64_t)m.off + s - m.lcnt;
+          src = (int64_t)o + s - m.lcnt;
         }
         if (src >= 0) {
           lf_k[d][s] = st_k[src];
@@ -423,8 +434,8 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
       if (tid < F) {
         Meta &m = meta[tid];
         m.cursor += m.nwrite;
-        m.lcnt = m.lcnt + m.hist - m.nwrite;
-        m.hist = 0;
+        m.lcnt = m.lcnt + hist[tid] - m.nwrite;
+        hist[tid] = 0;
       }
       __syncthreads();
       if (t + 1 < ntiles) {
@@ -505,12 +516,13 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
 }
 
 // One pass launch (pass2 selects the segment-walking variant).
-template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE = kLine>
+template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE = kLine,
+          bool HASHKEYS = false>
 int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
                       const V1 *iv, const V2 *iw, int32_t *ok, V1 *ov, V2 *ow, uint64_t n,
                       const RadixPlan &p, Joint joint, RadixHist &h) {
   if (!pass2) {
-    PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE>),
+    PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE, HASHKEYS>),
               p.G, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, 1u, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
               h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
   } else {
@@ -527,7 +539,7 @@ int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *
       ngrp = (uint32_t)ceil_div(p.G, cpg);
       uint64_t nseg = (uint64_t)ngrp * F1;
       if (grid > nseg) grid = nseg;
-      PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE>),
+      PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE, HASHKEYS>),
                 grid, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, cpg, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
                 h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
     } else {
