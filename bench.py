@@ -28,11 +28,15 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", choices=["uniform", "zipf"], default="uniform")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the extra Zipf / configs[0] / group-by lines")
+                    help="skip the secondary measurements (Zipf fk variant)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy 1 GiB once with the library's copy "
+                         "kernel (the known-size launch that scales rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE readings)")
     ap.add_argument("--verify", action="store_true",
                     help="check COUNT/checksum against the dense oracle (host, full size)")
     return ap.parse_args()
@@ -69,20 +73,8 @@ def main():
 
     for _ in range(a.warmup):
         res = ctx.join_partitioned_reduce(rk, rv, sk, sv)
-    ctx.prof_enable(True)
-    ctx.prof_reset()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        res = ctx.join_partitioned_reduce(rk, rv, sk, sv)
-    ctx.sync()
-    t1 = time.perf_counter()
-    ctx.prof_enable(False)
-    sec = (t1 - t0) / a.steps
-    names = ["radix_hist", "radix_hist_reduce", "radix_scatter1_build", "radix_scatter1_probe",
-             "radix_scatter2_build", "radix_scatter2_probe", "pjoin_probe", "join_acc_init"]
-    kt = kernel_table(ctx, names)
-    # per-launch algorithmic bytes (HBM traffic an ideal kernel must move)
+    sec, kt, res = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
+    # per-launch algorithmic bytes: what an ideal kernel must move through HBM
     algo = {
         "radix_scatter1_probe": 16.0 * NS, "radix_scatter2_probe": 16.0 * NS,
         "radix_scatter1_build": 16.0 * NR, "radix_scatter2_build": 16.0 * NR,
@@ -104,15 +96,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 keys/payloads, int64 accumulators",
+        "dtype": "int32",
         "data": "synthetic (seeded in HBM: R keys = permutation of 0..2^27-1, S fk uniform "
                 "over [0, 1.33*2^27), payloads uniform [0, 2^31))",
-        "config": {"workload": "partitioned-hashjoin-chained -> reduce, |R|=2^27, |S|=2^30, "
-                               "%s fk, 8-byte columnar tuples" % a.variant,
+        "config": {"workload": "partitioned-hashjoin-chained -> reduce(COUNT, SUM), "
+                               "|R|=2^27, |S|=2^30, %s fk, 8-byte columnar tuples, "
+                               "int64 accumulators" % a.variant,
                    "radix": cfg},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": pmc_traffic(dom, a.variant),
+                     "algorithmic_bytes": algo[dom], "ms_per_launch": dom_ms},
         "e2e_roofline": {"algorithmic_bytes": 40.0 * (NR + NS), "achieved_gbs": e2e_gbs,
                          "frac": e2e_gbs / HBM_PEAK_GBS},
         "input_gtuples_per_s": (NR + NS) / sec / 1e9,
@@ -124,18 +118,65 @@ def main():
         from oracle import pyoracle as O
         rc, want = O.join_dense(rk.numpy(), rv.numpy(), sk.numpy(), sv.numpy())
         line["verified"] = (rc == 0 and want == res)
+    if a.copy_calibration:
+        src = ctx.empty(1 << 28, "int32")
+        dst = ctx.empty(1 << 28, "int32")
+        ctx.copy(dst, src, 1 << 30)
+        src.free()
+        dst.free()
+    if not a.no_extra and a.variant == "uniform":
+        # the Zipf(theta=1) fk variant on the same R (same miss rate)
+        sk.free()
+        sk = ctx.gen_zipf_fk_i32(NS, SEED, 2, NR, RANGE, 1.0, SEED)
+        ctx.join_partitioned_reduce(rk, rv, sk, sv)
+        zsec, _, zres = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
+        line["variants"] = {"zipf_theta1": {"ms_per_step": zsec * 1e3,
+                                            "value": NS / zsec / 1e9, "result": zres}}
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.variant)
     print(json.dumps(line), flush=True)
     ctx.close()
 
 
+def timed(ctx, steps, fn):
+    """Runs fn `steps` times between two device synchronisations; returns the
+    mean seconds per step, the per-kernel (ms, launches) recorded with HIP
+    events on the context's stream, and the last result."""
+    names = ["radix_hist", "radix_colscan", "radix_scan", "radix_scatter1_build",
+             "radix_scatter1_probe", "radix_scatter2_build", "radix_scatter2_probe",
+             "pjoin_probe", "join_acc_init"]
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = fn()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.prof_enable(False)
+    return (t1 - t0) / steps, kernel_table(ctx, names), res
+
+
+def pmc_traffic(kernel, variant):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by the reading of a
+    1 GiB copy), or None when that summary does not cover the kernel."""
+    path = os.path.join(ROOT, "profiles", "join_2p27x2p30_pmc_traffic.txt")
+    if variant != "uniform" or not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = dict(kv.split("=", 1) for kv in ln.split() if "=" in kv)
+        if f.get("kernel") == kernel and "traffic_bytes" in f:
+            return float(f["traffic_bytes"])
+    return None
+
+
 def cpu_baseline(variant):
     """The oracle's morsel-parallel chained join (restating the reference's CPU
-    HashJoinChainedMorsel) on a 1/8-scale sample of the same workload."""
+    HashJoinChainedMorsel) on a 1/4-scale sample of the same workload."""
     from oracle import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
-    nr, ns = NR >> 3, NS >> 3
+    nr, ns = NR >> 2, NS >> 2
     rng = int(1.33 * nr)
     rk = O.gen_perm_i32(nr, SEED)
     rv = O.gen_uniform_i32(nr, SEED, 1, 0, 1 << 31)
@@ -144,13 +185,13 @@ def cpu_baseline(variant):
     else:
         sk = O.gen_zipf_fk_i32(ns, SEED, 2, nr, rng, 1.0, SEED)
     sv = O.gen_uniform_i32(ns, SEED, 3, 0, 1 << 31)
-    hb = 25  # ceil(log2(2^24)) + 1, the RelBuilder default
+    hb = 26  # ceil(log2(2^25)) + 1, the RelBuilder default
     t0 = time.perf_counter()
     rc, out = O.join_chained(rk, rv, sk, sv, hb, max_build=nr, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": ns / dt / 1e9, "unit": "Gtuples/s", "cores": threads, "kind": "port",
-            "sample": "|R|=2^24, |S|=2^27 (1/8 scale, same distributions), chained hash join "
-                      "(morsel-parallel shared table, hash_bits=25), build + probe + reduce, "
+            "sample": "|R|=2^25, |S|=2^28 (1/4 scale, same distributions), chained hash join "
+                      "(morsel-parallel shared table, hash_bits=26), build + probe + reduce, "
                       "%.2f s" % dt,
             "result_count": out["count"]}
 
