@@ -338,6 +338,17 @@ class Context:
         self._keep = []
 
     def execute_plan(self, plan_json, base_dir=".", out_cap=1 << 20):
-        buf = C.create_string_buffer(out_cap)
-        _check(_lib.pj_plan_execute(self.h, plan_json.encode(), base_dir.encode(), buf, out_cap))
-        return buf.value.decode()
+        """Runs a JSON physical plan; returns what its print operator wrote."""
+        if not isinstance(plan_json, str):
+            import json
+            plan_json = json.dumps(plan_json)
+        while True:
+            buf = C.create_string_buffer(out_cap)
+            rc = _lib.pj_plan_execute(self.h, plan_json.encode(), base_dir.encode(), buf, out_cap)
+            if rc == PJ_E_CAPACITY:
+                msg = _lib.pj_last_error().decode(errors="replace")
+                if msg.startswith("plan output needs "):
+                    out_cap = int(msg.split()[3]) + 1
+                    continue
+            _check(rc)
+            return buf.value.decode()

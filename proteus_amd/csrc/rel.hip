@@ -1,0 +1,688 @@
+// Device building blocks of the plan executor (declared in rel_api.hpp).
+//
+// Expressions run on a small stack VM: the program (postfix, at most 64
+// instructions over at most 16 columns) travels in the kernel arguments, so
+// every lane of a wave executes the same instruction and dispatch is a scalar
+// branch; each lane keeps its stack in LDS (slot d of lane t at st[d][t],
+// conflict-free).  This plays the role of the reference's expression code
+// generation (core/olap/lib/expressions/expressions-generator.cpp) without a
+// JIT.
+#include "pj_internal.hpp"
+#include "rel_api.hpp"
+
+namespace pj {
+namespace rel {
+
+__device__ __forceinline__ int64_t ld_slot(const void *c, int t, uint64_t i) {
+  switch (t) {
+    case I32: return ((const int32_t *)c)[i];
+    case I64: return ((const int64_t *)c)[i];
+    case F64: return ((const int64_t *)c)[i];
+    case BOOL: return ((const uint8_t *)c)[i] != 0;
+    default: return __builtin_bit_cast(int64_t, (double)((const float *)c)[i]);
+  }
+}
+
+__device__ __forceinline__ void st_slot(void *o, int t, uint64_t i, int64_t v) {
+  switch (t) {
+    case I32: ((int32_t *)o)[i] = (int32_t)v; break;
+    case I64:
+    case F64: ((int64_t *)o)[i] = v; break;
+    case BOOL: ((uint8_t *)o)[i] = v != 0; break;
+    default: ((float *)o)[i] = (float)__builtin_bit_cast(double, v); break;
+  }
+}
+
+__device__ __forceinline__ double as_f(int64_t v) { return __builtin_bit_cast(double, v); }
+__device__ __forceinline__ int64_t of_f(double v) { return __builtin_bit_cast(int64_t, v); }
+__device__ __forceinline__ int64_t wrap32(int64_t v) { return (int64_t)(int32_t)(uint32_t)v; }
+
+// Integer arithmetic in two's complement (wrapping as LLVM's add/sub/mul on
+// i32/i64); division by zero yields 0 and INT_MIN / -1 wraps.
+__device__ __forceinline__ int64_t arith(int op, int t, int64_t a, int64_t b) {
+  if (t == F64) {
+    double x = as_f(a), y = as_f(b), r;
+    switch (op) {
+      case OP_ADD: r = x + y; break;
+      case OP_SUB: r = x - y; break;
+      case OP_MUL: r = x * y; break;
+      case OP_DIV: r = x / y; break;
+      default: r = fmod(x, y); break;
+    }
+    return of_f(r);
+  }
+  uint64_t x = (uint64_t)a, y = (uint64_t)b, r;
+  switch (op) {
+    case OP_ADD: r = x + y; break;
+    case OP_SUB: r = x - y; break;
+    case OP_MUL: r = x * y; break;
+    case OP_DIV:
+      if (b == 0) r = 0;
+      else if (b == -1) r = 0 - x;
+      else r = (uint64_t)(a / b);
+      break;
+    default:
+      if (b == 0 || b == -1) r = 0;
+      else r = (uint64_t)(a % b);
+      break;
+  }
+  return t == I32 ? wrap32((int64_t)r) : (int64_t)r;
+}
+
+__device__ __forceinline__ bool cmp(int op, int t, int64_t a, int64_t b) {
+  if (t == F64) {
+    double x = as_f(a), y = as_f(b);
+    switch (op) {
+      case OP_EQ: return x == y;
+      case OP_NE: return x != y;
+      case OP_LT: return x < y;
+      case OP_LE: return x <= y;
+      case OP_GT: return x > y;
+      default: return x >= y;
+    }
+  }
+  switch (op) {
+    case OP_EQ: return a == b;
+    case OP_NE: return a != b;
+    case OP_LT: return a < b;
+    case OP_LE: return a <= b;
+    case OP_GT: return a > b;
+    default: return a >= b;
+  }
+}
+
+__device__ __forceinline__ int64_t cast(int from, int to, int64_t v) {
+  if (from == F64 || from == F32) {
+    double x = as_f(v);
+    if (to == F64 || to == F32) return v;
+    if (to == BOOL) return x != 0.0;
+    // fptosi: truncation toward zero (out-of-range values saturate here)
+    if (x != x) return 0;
+    if (x >= 9.2233720368547758e18) return INT64_MAX;
+    if (x <= -9.2233720368547758e18) return INT64_MIN;
+    int64_t r = (int64_t)x;
+    return to == I32 ? wrap32(r) : r;
+  }
+  switch (to) {
+    case F64:
+    case F32: return of_f((double)v);
+    case BOOL: return v != 0;
+    case I32: return wrap32(v);
+    default: return v;
+  }
+}
+
+// One row of the program; st points at this lane's slot 0, stride BLOCK.
+template <int BLOCK>
+__device__ __forceinline__ int64_t run_prog(const Prog &p, uint64_t row, int64_t *st) {
+  int sp = 0;
+  for (int k = 0; k < p.n; ++k) {
+    const Instr in = p.ins[k];
+    switch (in.op) {
+      case OP_COL: st[sp++ * BLOCK] = ld_slot(p.cols[in.a], p.ctype[in.a], row); break;
+      case OP_CONST: st[sp++ * BLOCK] = in.imm; break;
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD: {
+        int64_t b = st[--sp * BLOCK], a = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] = arith(in.op, in.t, a, b);
+        break;
+      }
+      case OP_NEG: {
+        int64_t a = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] =
+            in.t == F64 ? of_f(-as_f(a)) : (in.t == I32 ? wrap32(0 - a) : (int64_t)(0 - (uint64_t)a));
+        break;
+      }
+      case OP_EQ: case OP_NE: case OP_LT: case OP_LE: case OP_GT: case OP_GE: {
+        int64_t b = st[--sp * BLOCK], a = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] = cmp(in.op, in.t, a, b);
+        break;
+      }
+      case OP_AND: {
+        int64_t b = st[--sp * BLOCK], a = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] = (a != 0) & (b != 0);
+        break;
+      }
+      case OP_OR: {
+        int64_t b = st[--sp * BLOCK], a = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] = (a != 0) | (b != 0);
+        break;
+      }
+      case OP_NOT: st[(sp - 1) * BLOCK] = st[(sp - 1) * BLOCK] == 0; break;
+      case OP_CAST: st[(sp - 1) * BLOCK] = cast(in.a, in.t, st[(sp - 1) * BLOCK]); break;
+      case OP_IF: {
+        int64_t e = st[--sp * BLOCK], t = st[--sp * BLOCK], c = st[(sp - 1) * BLOCK];
+        st[(sp - 1) * BLOCK] = c ? t : e;
+        break;
+      }
+      case OP_MIN: case OP_MAX: {
+        int64_t b = st[--sp * BLOCK], a = st[(sp - 1) * BLOCK];
+        bool lt = in.t == F64 ? as_f(a) < as_f(b) : a < b;
+        st[(sp - 1) * BLOCK] = (in.op == OP_MIN) == lt ? a : b;
+        break;
+      }
+      default: break;
+    }
+  }
+  return st[0];
+}
+
+constexpr int kVmBlock = 256;
+
+__global__ __launch_bounds__(kVmBlock) void k_vm_eval(Prog p, uint64_t n, void *out, int out_t) {
+  extern __shared__ int64_t vm_st[];
+  int64_t *st = vm_st + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * kVmBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kVmBlock + threadIdx.x; i < n; i += stride)
+    st_slot(out, out_t, i, run_prog<kVmBlock>(p, i, st));
+}
+
+// Ordered compaction.  Tiles of kSelTile rows: pass 1 evaluates the
+// predicate into a byte mask and counts each tile, a one-workgroup scan turns
+// counts into offsets, pass 2 writes the row ids of each tile in order.
+constexpr int kSelTile = 4096;
+
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t *tmp, uint32_t *tot) {
+  constexpr int NW = kVmBlock / 64;
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t x = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += x;
+  }
+  if (lane == 63) tmp[wid] = incl;
+  __syncthreads();
+  uint32_t pre = 0, t = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    uint32_t x = tmp[w];
+    pre += w < wid ? x : 0;
+    t += x;
+  }
+  *tot = t;
+  __syncthreads();
+  return pre + incl - v;
+}
+
+__global__ __launch_bounds__(kVmBlock) void k_sel_mask(Prog p, uint64_t n, uint8_t *mask,
+                                                        uint32_t *tile_cnt) {
+  extern __shared__ int64_t vm_st[];
+  __shared__ uint32_t red[kVmBlock / 64];
+  int64_t *st = vm_st + threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kSelTile;
+  uint32_t c = 0;
+  for (int j = threadIdx.x; j < kSelTile; j += kVmBlock) {
+    uint64_t i = base + j;
+    if (i < n) {
+      bool keep = run_prog<kVmBlock>(p, i, st) != 0;
+      mask[i] = keep;
+      c += keep;
+    }
+  }
+  c = (uint32_t)wave_sum_i64(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kVmBlock / 64; ++w) t += red[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// Mask of non-zero u32 entries (occupied hash slots) with tile counts.
+__global__ __launch_bounds__(kVmBlock) void k_nz_mask(const uint32_t *v, uint64_t n,
+                                                       uint8_t *mask, uint32_t *tile_cnt) {
+  __shared__ uint32_t red[kVmBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kSelTile;
+  uint32_t c = 0;
+  for (int j = threadIdx.x; j < kSelTile; j += kVmBlock) {
+    uint64_t i = base + j;
+    if (i < n) {
+      bool keep = v[i] != 0;
+      mask[i] = keep;
+      c += keep;
+    }
+  }
+  c = (uint32_t)wave_sum_i64(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kVmBlock / 64; ++w) t += red[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of m u32 counts into u64 offsets (off[m] = total), one
+// workgroup: each thread scans a contiguous run.
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *cnt, uint64_t m,
+                                                       unsigned long long *off) {
+  __shared__ unsigned long long part[1024];
+  const uint64_t per = (m + 1023) / 1024;
+  const uint64_t a = threadIdx.x * per, b = a + per < m ? a + per : m;
+  unsigned long long s = 0;
+  for (uint64_t i = a; i < b; ++i) s += cnt[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    unsigned long long x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned long long run = part[threadIdx.x] - s;
+  for (uint64_t i = a; i < b; ++i) {
+    off[i] = run;
+    run += cnt[i];
+  }
+  if (threadIdx.x == 1023) off[m] = part[1023];
+}
+
+__global__ __launch_bounds__(kVmBlock) void k_sel_write(const uint8_t *mask, uint64_t n,
+                                                         const unsigned long long *off,
+                                                         uint32_t *out) {
+  __shared__ uint32_t tmp[kVmBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kSelTile;
+  unsigned long long pos = off[blockIdx.x];
+  constexpr int IPT = kSelTile / kVmBlock;  // 16 consecutive rows per thread
+  const uint64_t r0 = base + (uint64_t)threadIdx.x * IPT;
+  uint32_t bits = 0, c = 0;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint64_t i = r0 + j;
+    bool k = i < n && mask[i];
+    bits |= (uint32_t)k << j;
+    c += k;
+  }
+  uint32_t tot;
+  uint32_t pre = block_scan_excl(c, tmp, &tot);
+  unsigned long long p = pos + pre;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (bits >> j & 1) out[p++] = (uint32_t)(r0 + j);
+}
+
+__global__ void k_gather(const void *src, int esize, const uint32_t *idx, uint64_t n, void *dst) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t j = idx[i];
+    if (esize == 4) ((uint32_t *)dst)[i] = ((const uint32_t *)src)[j];
+    else if (esize == 8) ((uint64_t *)dst)[i] = ((const uint64_t *)src)[j];
+    else ((uint8_t *)dst)[i] = ((const uint8_t *)src)[j];
+  }
+}
+
+__global__ void k_iota(uint32_t *d, uint64_t n) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    d[i] = (uint32_t)i;
+}
+
+// Reductions ---------------------------------------------------------------
+__device__ __forceinline__ int64_t red_ident(int m, bool f) {
+  switch (m) {
+    case M_MIN: return f ? of_f(__builtin_inf()) : INT64_MAX;
+    case M_MAX: return f ? of_f(-__builtin_inf()) : INT64_MIN;
+    case M_AND: return 1;
+    default: return f ? of_f(0.0) : 0;
+  }
+}
+
+__device__ __forceinline__ int64_t red_op(int m, bool f, int64_t a, int64_t b) {
+  if (f) {
+    double x = as_f(a), y = as_f(b);
+    switch (m) {
+      case M_MIN: return of_f(x < y ? x : y);
+      case M_MAX: return of_f(x > y ? x : y);
+      default: return of_f(x + y);
+    }
+  }
+  switch (m) {
+    case M_MIN: return a < b ? a : b;
+    case M_MAX: return a > b ? a : b;
+    case M_AND: return (a != 0) & (b != 0);
+    case M_OR: return (a != 0) | (b != 0);
+    default: return (int64_t)((uint64_t)a + (uint64_t)b);
+  }
+}
+
+__device__ __forceinline__ int64_t block_red(int m, bool f, int64_t v) {
+  __shared__ int64_t tmp[1024 / 64];
+  for (int o = 32; o > 0; o >>= 1) v = red_op(m, f, v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int64_t r = red_ident(m, f);
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = red_op(m, f, r, tmp[w]);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_part(const void *col, int t, uint64_t n, int m,
+                                                      int64_t *part) {
+  const bool f = t == F64 || t == F32;
+  int64_t acc = red_ident(m, f);
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc = red_op(m, f, acc, ld_slot(col, t, i));
+  acc = block_red(m, f, acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_final(const int64_t *part, int np, int t, int m,
+                                                        int64_t *out) {
+  const bool f = t == F64 || t == F32;
+  int64_t acc = red_ident(m, f);
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc = red_op(m, f, acc, part[i]);
+  acc = block_red(m, f, acc);
+  if (threadIdx.x == 0) *out = acc;
+}
+
+// General group-by ----------------------------------------------------------
+struct GbArgs {
+  const void *kc[kGbMaxKeys];
+  int32_t kt[kGbMaxKeys];
+  const void *ac[kGbMaxAggs];
+  int32_t at[kGbMaxAggs];
+  int32_t am[kGbMaxAggs];
+  int32_t nk, na;
+};
+
+__device__ __forceinline__ uint64_t key_hash(const GbArgs &a, uint64_t row) {
+  uint64_t h = 0x243f6a8885a308d3ULL;
+  for (int k = 0; k < a.nk; ++k) {
+    uint64_t v = (uint64_t)ld_slot(a.kc[k], a.kt[k], row);
+    h = mix64(h ^ (v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2)));
+  }
+  return h;
+}
+
+__device__ __forceinline__ bool key_eq(const GbArgs &a, uint64_t r1, uint64_t r2) {
+  for (int k = 0; k < a.nk; ++k)
+    if (ld_slot(a.kc[k], a.kt[k], r1) != ld_slot(a.kc[k], a.kt[k], r2)) return false;
+  return true;
+}
+
+__device__ __forceinline__ void atomic_min_f64(double *p, double v) {
+  unsigned long long *q = (unsigned long long *)p;
+  unsigned long long old = *q;
+  while (v < __builtin_bit_cast(double, old)) {
+    unsigned long long prev = atomicCAS(q, old, __builtin_bit_cast(unsigned long long, v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+__device__ __forceinline__ void atomic_max_f64(double *p, double v) {
+  unsigned long long *q = (unsigned long long *)p;
+  unsigned long long old = *q;
+  while (v > __builtin_bit_cast(double, old)) {
+    unsigned long long prev = atomicCAS(q, old, __builtin_bit_cast(unsigned long long, v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// Insert every row's key (slot = representative row + 1, linear probing,
+// at most S probes: a full table raises the overflow flag) and apply its
+// aggregates to the slot's accumulators with atomics.
+__global__ __launch_bounds__(256) void k_gb_generic(GbArgs a, uint64_t n, uint32_t *slots,
+                                                     uint64_t mask, int64_t *acc,
+                                                     unsigned int *overflow) {
+  const uint64_t S = mask + 1;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
+    uint64_t s = key_hash(a, row) & mask;
+    bool found = false;
+    for (uint64_t probes = 0; probes < S; ++probes) {
+      uint32_t cur = __hip_atomic_load(&slots[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == 0) {
+        uint32_t prev = atomicCAS(&slots[s], 0u, (uint32_t)(row + 1));
+        if (prev == 0) {
+          found = true;
+          break;
+        }
+        cur = prev;
+      }
+      if (key_eq(a, row, cur - 1)) {
+        found = true;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+    if (!found) {
+      atomicOr(overflow, 1u);
+      continue;
+    }
+    for (int j = 0; j < a.na; ++j) {
+      int64_t *slot = acc + (uint64_t)j * S + s;
+      const int m = a.am[j];
+      if (m == M_CNT) {
+        atomicAdd((unsigned long long *)slot, 1ULL);
+        continue;
+      }
+      const int t = a.at[j];
+      int64_t v = ld_slot(a.ac[j], t, row);
+      if (t == F64 || t == F32) {
+        double x = as_f(v);
+        if (m == M_SUM) unsafeAtomicAdd((double *)slot, x);
+        else if (m == M_MIN) atomic_min_f64((double *)slot, x);
+        else atomic_max_f64((double *)slot, x);
+      } else {
+        switch (m) {
+          case M_SUM: atomicAdd((unsigned long long *)slot, (unsigned long long)v); break;
+          case M_MIN: atomicMin((long long *)slot, (long long)v); break;
+          case M_MAX: atomicMax((long long *)slot, (long long)v); break;
+          case M_AND: atomicAnd((unsigned long long *)slot, (unsigned long long)(v != 0)); break;
+          default: atomicOr((unsigned long long *)slot, (unsigned long long)(v != 0)); break;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_fill_i64(int64_t *d, int64_t v, uint64_t n) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = v;
+}
+
+// Group g of the output = the g-th occupied slot (slot order).
+__global__ void k_gb_emit(GbArgs a, const uint32_t *slots, const uint32_t *occ, uint64_t ng,
+                          const int64_t *acc, uint64_t S, GbOut out) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += stride) {
+    const uint32_t s = occ[g];
+    const uint64_t rep = slots[s] - 1;
+    for (int k = 0; k < a.nk; ++k) st_slot(out.keys[k], a.kt[k], g, ld_slot(a.kc[k], a.kt[k], rep));
+    for (int j = 0; j < a.na; ++j) ((int64_t *)out.aggs[j])[g] = acc[(uint64_t)j * S + s];
+  }
+}
+
+__global__ void k_hash_buckets(const int32_t *k, uint64_t n, uint32_t buckets, uint32_t *out) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (uint32_t)(hash_i32(k[i]) % buckets);
+}
+
+__global__ void k_hash_values(const int32_t *k, uint64_t n, int64_t *out) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (int64_t)hash_i32(k[i]);
+}
+
+// Host side -----------------------------------------------------------------
+static uint64_t grid_for(pj_ctx *ctx, uint64_t n, int block) {
+  uint64_t g = ceil_div(n, (uint64_t)block);
+  uint64_t cap = (uint64_t)ctx->num_cus * 8;
+  if (g > cap) g = cap;
+  return g ? g : 1;
+}
+
+static int check_prog(const Prog &p) {
+  if (p.n <= 0 || p.n > kMaxInstr) return fail(PJ_E_ARG, "expression program of %d instructions", p.n);
+  if (p.depth <= 0 || p.depth > kMaxDepth)
+    return fail(PJ_E_UNSUPPORTED, "expression needs %d stack slots (limit %d)", p.depth, kMaxDepth);
+  for (int k = 0; k < p.n; ++k)
+    if (p.ins[k].op == OP_COL && (p.ins[k].a < 0 || p.ins[k].a >= p.ncols || !p.cols[p.ins[k].a]))
+      return fail(PJ_E_ARG, "expression reads an unbound column");
+  return PJ_OK;
+}
+
+int eval(pj_ctx *ctx, const Prog &p, uint64_t n, void *out, int out_t) {
+  PJ_TRY(check_prog(p));
+  if (!n) return PJ_OK;
+  size_t lds = (size_t)p.depth * kVmBlock * 8;
+  PJ_LAUNCH(ctx, "rel_eval", k_vm_eval, grid_for(ctx, n, kVmBlock), kVmBlock, lds, p, n, out, out_t);
+  return PJ_OK;
+}
+
+static int compact_mask(pj_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t tiles,
+                        uint32_t *tile_cnt, uint32_t *out_idx, uint64_t *n_out) {
+  Scratch off;
+  PJ_TRY(off.alloc(ctx, (tiles + 1) * 8));
+  PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, tile_cnt, tiles,
+            off.as<unsigned long long>());
+  PJ_LAUNCH(ctx, "rel_compact", k_sel_write, tiles, kVmBlock, 0, mask, n,
+            off.as<const unsigned long long>(), out_idx);
+  unsigned long long total = 0;
+  PJ_TRY(d2h(ctx, &total, off.as<unsigned long long>() + tiles, 8));
+  *n_out = total;
+  return PJ_OK;
+}
+
+int select(pj_ctx *ctx, const Prog &pred, uint64_t n, uint32_t *out_idx, uint64_t *n_out) {
+  PJ_TRY(check_prog(pred));
+  *n_out = 0;
+  if (!n) return PJ_OK;
+  if (n > 0xffffffffULL) return fail(PJ_E_CAPACITY, "select over more than 2^32-1 rows");
+  const uint64_t tiles = ceil_div(n, kSelTile);
+  Scratch mask, cnt;
+  PJ_TRY(mask.alloc(ctx, n));
+  PJ_TRY(cnt.alloc(ctx, tiles * 4));
+  size_t lds = (size_t)pred.depth * kVmBlock * 8;
+  PJ_LAUNCH(ctx, "rel_select", k_sel_mask, tiles, kVmBlock, lds, pred, n, mask.as<uint8_t>(),
+            cnt.as<uint32_t>());
+  return compact_mask(ctx, mask.as<uint8_t>(), n, tiles, cnt.as<uint32_t>(), out_idx, n_out);
+}
+
+int gather(pj_ctx *ctx, const void *src, int esize, const uint32_t *idx, uint64_t n, void *dst) {
+  if (!n) return PJ_OK;
+  if (esize != 1 && esize != 4 && esize != 8) return fail(PJ_E_ARG, "gather of %d-byte elements", esize);
+  PJ_LAUNCH(ctx, "rel_gather", k_gather, grid_for(ctx, n, 256), 256, 0, src, esize, idx, n, dst);
+  return PJ_OK;
+}
+
+int iota(pj_ctx *ctx, uint32_t *d, uint64_t n) {
+  if (!n) return PJ_OK;
+  PJ_LAUNCH(ctx, "rel_iota", k_iota, grid_for(ctx, n, 256), 256, 0, d, n);
+  return PJ_OK;
+}
+
+int reduce(pj_ctx *ctx, const void *col, int t, uint64_t n, int monoid, int64_t *out_i,
+           double *out_f) {
+  const bool f = t == F64 || t == F32;
+  if (monoid == M_CNT) {
+    if (out_i) *out_i = (int64_t)n;
+    if (out_f) *out_f = (double)n;
+    return PJ_OK;
+  }
+  if (f && (monoid == M_AND || monoid == M_OR))
+    return fail(PJ_E_UNSUPPORTED, "and/or accumulator over a float expression");
+  Scratch part, res;
+  const int np = (int)std::min<uint64_t>(1024, grid_for(ctx, n ? n : 1, 256));
+  PJ_TRY(part.alloc(ctx, (uint64_t)np * 8));
+  PJ_TRY(res.alloc(ctx, 8));
+  PJ_LAUNCH(ctx, "rel_reduce", k_reduce_part, np, 256, 0, col, t, n, monoid, part.as<int64_t>());
+  PJ_LAUNCH(ctx, "rel_reduce_final", k_reduce_final, 1, 1024, 0, part.as<const int64_t>(), np, t,
+            monoid, res.as<int64_t>());
+  int64_t v = 0;
+  PJ_TRY(d2h(ctx, &v, res.p, 8));
+  if (f) {
+    double d = __builtin_bit_cast(double, v);
+    if (out_f) *out_f = d;
+    if (out_i) *out_i = (int64_t)d;
+  } else {
+    if (out_i) *out_i = v;
+    if (out_f) *out_f = (double)v;
+  }
+  return PJ_OK;
+}
+
+int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, uint64_t n,
+            uint64_t max_groups, const GbOut &out, uint64_t *n_groups) {
+  *n_groups = 0;
+  if (nk < 1 || nk > kGbMaxKeys) return fail(PJ_E_UNSUPPORTED, "group-by over %d keys (1..%d)", nk, kGbMaxKeys);
+  if (na < 0 || na > kGbMaxAggs) return fail(PJ_E_UNSUPPORTED, "group-by with %d aggregates (max %d)", na, kGbMaxAggs);
+  if (n > 0xfffffffeULL) return fail(PJ_E_CAPACITY, "group-by input above 2^32-2 rows");
+  if (!n) return PJ_OK;
+  if (max_groups < 1) max_groups = 1;
+  GbArgs a{};
+  a.nk = nk;
+  a.na = na;
+  for (int k = 0; k < nk; ++k) {
+    a.kc[k] = keys[k].col;
+    a.kt[k] = keys[k].t;
+  }
+  for (int j = 0; j < na; ++j) {
+    a.ac[j] = aggs[j].col;
+    a.at[j] = aggs[j].t;
+    a.am[j] = aggs[j].monoid;
+    if (aggs[j].monoid != M_CNT && !aggs[j].col) return fail(PJ_E_ARG, "aggregate %d has no column", j);
+  }
+  uint64_t S = 2ULL << ilog2_ceil(std::min<uint64_t>(max_groups, n));
+  if (S < 64) S = 64;
+  Scratch slots, acc, flag;
+  PJ_TRY(slots.alloc(ctx, S * 4));
+  PJ_TRY(zero(ctx, slots.p, S * 4));
+  PJ_TRY(acc.alloc(ctx, std::max<uint64_t>(1, (uint64_t)na) * S * 8));
+  PJ_TRY(flag.alloc(ctx, 4));
+  PJ_TRY(zero(ctx, flag.p, 4));
+  for (int j = 0; j < na; ++j) {
+    const bool f = aggs[j].monoid != M_CNT && (aggs[j].t == F64 || aggs[j].t == F32);
+    int64_t ident;
+    switch (aggs[j].monoid) {
+      case M_MIN: ident = f ? __builtin_bit_cast(int64_t, __builtin_inf()) : INT64_MAX; break;
+      case M_MAX: ident = f ? __builtin_bit_cast(int64_t, -__builtin_inf()) : INT64_MIN; break;
+      case M_AND: ident = 1; break;
+      default: ident = 0; break;
+    }
+    PJ_LAUNCH(ctx, "rel_fill", k_fill_i64, grid_for(ctx, S, 256), 256, 0, acc.as<int64_t>() + (uint64_t)j * S,
+              ident, S);
+  }
+  PJ_LAUNCH(ctx, "rel_groupby", k_gb_generic, grid_for(ctx, n, 256), 256, 0, a, n,
+            slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>());
+  unsigned int of = 0;
+  PJ_TRY(d2h(ctx, &of, flag.p, 4));
+  if (of) return fail(PJ_E_CAPACITY, "group-by: more groups than the table of %llu slots", (unsigned long long)S);
+  const uint64_t tiles = ceil_div(S, kSelTile);
+  Scratch mask, cnt, occ;
+  PJ_TRY(mask.alloc(ctx, S));
+  PJ_TRY(cnt.alloc(ctx, tiles * 4));
+  PJ_TRY(occ.alloc(ctx, S * 4));
+  PJ_LAUNCH(ctx, "rel_groupby_occ", k_nz_mask, tiles, kVmBlock, 0, slots.as<const uint32_t>(), S,
+            mask.as<uint8_t>(), cnt.as<uint32_t>());
+  uint64_t ng = 0;
+  PJ_TRY(compact_mask(ctx, mask.as<uint8_t>(), S, tiles, cnt.as<uint32_t>(), occ.as<uint32_t>(), &ng));
+  if (ng > max_groups)
+    return fail(PJ_E_CAPACITY, "group-by produced %llu groups, maxInputSize %llu",
+                (unsigned long long)ng, (unsigned long long)max_groups);
+  if (ng)
+    PJ_LAUNCH(ctx, "rel_groupby_emit", k_gb_emit, grid_for(ctx, ng, 256), 256, 0, a,
+              slots.as<const uint32_t>(), occ.as<const uint32_t>(), ng, acc.as<const int64_t>(), S, out);
+  PJ_TRY(sync(ctx));
+  *n_groups = ng;
+  return PJ_OK;
+}
+
+int hash_buckets(pj_ctx *ctx, const int32_t *k, uint64_t n, uint32_t buckets, uint32_t *out) {
+  if (!buckets) return fail(PJ_E_ARG, "hash-rearrange with 0 buckets");
+  if (!n) return PJ_OK;
+  PJ_LAUNCH(ctx, "rel_hash_buckets", k_hash_buckets, grid_for(ctx, n, 256), 256, 0, k, n, buckets, out);
+  return PJ_OK;
+}
+
+int hash_values(pj_ctx *ctx, const int32_t *k, uint64_t n, int64_t *out) {
+  if (!n) return PJ_OK;
+  PJ_LAUNCH(ctx, "rel_hash_values", k_hash_values, grid_for(ctx, n, 256), 256, 0, k, n, out);
+  return PJ_OK;
+}
+
+}  // namespace rel
+}  // namespace pj

@@ -1,0 +1,90 @@
+// Device building blocks of the plan executor: an expression VM over
+// columns, ordered compaction, gathers, typed reductions and a general hash
+// group-by.  Plain C++ types only, so the host engine (built with g++) can
+// call them; the kernels live in rel.hip.
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/proteus_amd.h"
+
+namespace pj {
+namespace rel {
+
+// Column types.  The plan language's "int" is int32, "int64"/"date" int64,
+// "float" FP64 (core/olap/lib/values/expressionTypes.cpp:87-97); bool is one
+// byte; F32 is a storage type for registered float32 columns, widened to FP64
+// when read.
+enum ColT : int32_t { I32 = 0, I64 = 1, F64 = 2, BOOL = 3, F32 = 4 };
+inline int col_size(int t) { return t == I32 || t == F32 ? 4 : t == BOOL ? 1 : 8; }
+
+// Postfix program of one expression.  Every value on the stack is a 64-bit
+// slot: integers and bools sign-extended, FP64 by bit pattern.
+enum OpCode : int32_t {
+  OP_COL,    // push column a (typed by ctype[a])
+  OP_CONST,  // push imm
+  OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD,  // t = operand/result type (I32 wraps)
+  OP_NEG,
+  OP_EQ, OP_NE, OP_LT, OP_LE, OP_GT, OP_GE,  // t = operand type, result bool
+  OP_AND, OP_OR, OP_NOT,
+  OP_CAST,   // a = from type, t = to type
+  OP_IF,     // pops else, then, cond
+  OP_MIN, OP_MAX,  // t = operand type
+};
+
+struct Instr {
+  int32_t op;
+  int32_t t;
+  int32_t a;
+  int32_t pad;
+  int64_t imm;
+};
+
+constexpr int kMaxInstr = 64;
+constexpr int kMaxCols = 16;
+constexpr int kMaxDepth = 16;
+
+struct Prog {
+  Instr ins[kMaxInstr];
+  const void *cols[kMaxCols];
+  int32_t ctype[kMaxCols];
+  int32_t n = 0;      // instructions
+  int32_t ncols = 0;
+  int32_t depth = 0;  // stack slots needed
+};
+
+// out[i] = prog(row i), stored as out_t, i < n.
+int eval(pj_ctx *ctx, const Prog &p, uint64_t n, void *out, int out_t);
+// Row ids (ascending) of the rows whose predicate holds; *n_out set.
+// out_idx must hold n entries.
+int select(pj_ctx *ctx, const Prog &pred, uint64_t n, uint32_t *out_idx, uint64_t *n_out);
+// dst[i] = src[idx[i]] for elements of esize bytes (1, 4 or 8).
+int gather(pj_ctx *ctx, const void *src, int esize, const uint32_t *idx, uint64_t n, void *dst);
+int iota(pj_ctx *ctx, uint32_t *d, uint64_t n);
+
+enum Monoid : int32_t { M_SUM = 0, M_MIN = 1, M_MAX = 2, M_AND = 3, M_OR = 4, M_CNT = 5 };
+// Reduction of a column: integer types and bool accumulate in int64, F64/F32
+// in FP64 (fixed order: per-workgroup partials, then one workgroup).
+int reduce(pj_ctx *ctx, const void *col, int t, uint64_t n, int monoid, int64_t *out_i,
+           double *out_f);
+
+// General hash group-by over up to 4 key columns and up to 16 aggregates.
+struct GbKey { const void *col; int32_t t; };
+struct GbAgg { const void *col; int32_t t; int32_t monoid; };  // col may be null for M_CNT
+constexpr int kGbMaxKeys = 4;
+constexpr int kGbMaxAggs = 16;
+struct GbOut {
+  void *keys[kGbMaxKeys];  // device, capacity max_groups each, key types
+  void *aggs[kGbMaxAggs];  // device, capacity max_groups: int64 (integer/bool/count) or FP64
+};
+int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, uint64_t n,
+            uint64_t max_groups, const GbOut &out, uint64_t *n_groups);
+
+// bucket[i] = hashInt32(k[i]) mod buckets (the reference's hash-rearrange
+// target, gpu-hash-rearrange.cpp).
+int hash_buckets(pj_ctx *ctx, const int32_t *k, uint64_t n, uint32_t buckets, uint32_t *out);
+// hashInt32(k[i]) as int64
+int hash_values(pj_ctx *ctx, const int32_t *k, uint64_t n, int64_t *out);
+
+}  // namespace rel
+}  // namespace pj

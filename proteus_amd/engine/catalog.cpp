@@ -28,6 +28,8 @@ bool parse_type(const char *s, Type *t) {
     *t = Type::F64;
   } else if (!strcmp(s, "bool")) {
     *t = Type::BOOL;
+  } else if (!strcmp(s, "float32")) {
+    *t = Type::F32;
   } else {
     return false;
   }
@@ -40,6 +42,7 @@ int type_size(Type t) {
     case Type::I64: return 8;
     case Type::F64: return 8;
     case Type::BOOL: return 1;
+    case Type::F32: return 4;
   }
   return 0;
 }

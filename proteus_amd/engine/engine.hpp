@@ -20,8 +20,9 @@ namespace engine {
 
 // Column types of the plan language (core/olap/lib/values/expressionTypes.cpp):
 // "int" = int32, "int64", "date" = int64, "float" = FP64, "bool",
-// "dstring" = dictionary-encoded int32.
-enum class Type { I32, I64, F64, BOOL };
+// "dstring" = dictionary-encoded int32; "float32" is a storage type for
+// single-precision columns (read as FP64).
+enum class Type { I32, I64, F64, BOOL, F32 };
 bool parse_type(const char *s, Type *t);
 int type_size(Type t);
 

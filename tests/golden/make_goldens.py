@@ -1,6 +1,8 @@
-"""Copies the reference's own test inputs and expected outputs for this path
-into tests/golden/ as fixtures (data only).  Run in a container where the
-reference checkout is mounted; the GPU tests only read the copies."""
+"""Copies the reference's own test inputs, physical plans and expected
+outputs for this path into tests/golden/ as fixtures (data only), keeping the
+reference's "inputs/..." layout so the plans run unchanged with
+tests/golden as their base directory.  Run in a container where the reference
+checkout is mounted; the tests only read the copies."""
 import json
 import os
 import shutil
@@ -9,7 +11,18 @@ import sys
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-INPUTS = ["tests/inputs/sailors.csv", "tests/inputs/reserves.csv", "tests/inputs/boats.csv"]
+INPUTS = ["tests/inputs/sailors.csv", "tests/inputs/reserves.csv", "tests/inputs/boats.csv",
+          "tests/inputs/A.csv", "tests/inputs/B.csv",
+          "tests/inputs/parser/A.csv", "tests/inputs/parser/B.csv", "tests/inputs/parser/catalog.json",
+          "tests/inputs/plans/catalog.json",
+          "tests/inputs/synthetic/100-30cols-fixed.field1",
+          "tests/inputs/synthetic/100-30cols-fixed.field4",
+          "tests/inputs/tpch/part_0.001.tbl", "tests/inputs/tpch/partsupp_0.001.tbl"]
+# physical plans of tests/olap/tests-plan-parsing.cpp and the GPU plan fixtures
+PLANS = ["reduce-scan", "reduce-twofields-scan", "reduce-join", "reduce-select",
+         "reduce-join-record", "reduce-join-record-nonselective", "reduce-scan-bin",
+         "reduce-scan-gpu", "gpu-group-by", "reduce-join-parts", "reduce-scan-part",
+         "reduce-scan-partsupp"]
 OUTPUTS = {
     # RelBuilder operator tests, core/olap/tests/operator-tests/tests-join-sailors.cpp
     "sailorsScan": "tests/olap/outputs/tests-sailors/sailorsScan.csv",
@@ -23,12 +36,22 @@ OUTPUTS = {
     "reduce-scan": "tests/olap/outputs/tests-plan-parsing/reduce-scan-log.json",
     "reduce-select": "tests/olap/outputs/tests-plan-parsing/reduce-select-log.json",
     "reduce-twofields-scan": "tests/olap/outputs/tests-plan-parsing/reduce-twofields-scan-log.json",
+    "reduce-join-record": "tests/olap/outputs/tests-plan-parsing/reduce-join-record-log.json",
+    "reduce-join-record-nonselective":
+        "tests/olap/outputs/tests-plan-parsing/reduce-join-record-nonselective-log.json",
+    "reduce-scan-bin": "tests/olap/outputs/tests-plan-parsing/reduce-scan-bin-log.json",
 }
 
 
 def main():
     for f in INPUTS:
-        shutil.copy(os.path.join(REF, f), os.path.join(HERE, os.path.basename(f)))
+        dst = os.path.join(HERE, f[len("tests/"):])
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        shutil.copy(os.path.join(REF, f), dst)
+    for pl in PLANS:
+        dst = os.path.join(HERE, "inputs", "plans", pl + ".json")
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        shutil.copy(os.path.join(REF, "tests/inputs/plans", pl + ".json"), dst)
     gold = {}
     for name, f in OUTPUTS.items():
         with open(os.path.join(REF, f)) as fh:

@@ -9,7 +9,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def load_csv(name, cols, dtype=np.int64):
     rows = []
-    with open(os.path.join(GOLDEN, name)) as f:
+    with open(os.path.join(GOLDEN, "inputs", name)) as f:
         for line in f:
             parts = line.strip().split(";")
             if parts and parts[0]:

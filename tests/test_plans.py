@@ -1,0 +1,246 @@
+"""Plans in the reference's JSON plan language, run unchanged through
+pj_plan_execute: the reference's own plan fixtures against their golden
+outputs (tests/olap/tests-plan-parsing.cpp, tests/olap/outputs/), and
+synthetic plans built with the RelBuilder against the oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import proteus_amd as pa
+from proteus_amd import relbuilder as rb
+from oracle import pyoracle as O
+from tests import helpers as H
+
+GOLD = json.load(open(os.path.join(H.GOLDEN, "reference_outputs.json")))
+PLANS = os.path.join(H.GOLDEN, "inputs", "plans")
+
+
+def plan_text(name):
+    with open(os.path.join(PLANS, name + ".json")) as f:
+        return f.read()
+
+
+def run(ctx, name):
+    return ctx.execute_plan(plan_text(name), H.GOLDEN).strip()
+
+
+# ---- CPU: plan construction ---------------------------------------------
+def test_relbuilder_builds_reference_shapes():
+    r = rb.RelBuilder()
+    s = r.scan("S", ["k", "v"]).to_gpu().unpack()
+    b = r.scan("R", ["k", "v"]).to_gpu().unpack()
+    q = s.join(b, rb.col("R", "k"), rb.col("S", "k"), hash_bits=10, max_build=1024) \
+         .reduce([rb.as_(rb.int64(1), "agg", "cnt")], ["sum"]).to_cpu().print()
+    p = q.plan()
+    assert p["operator"] == "print"
+    j = p["input"]["input"]["input"]
+    assert j["operator"] == "hashjoin-chained"
+    assert set(j) >= {"build_input", "probe_input", "build_k", "probe_k", "hash_bits",
+                      "maxBuildInputSize"}
+    assert j["build_input"]["input"]["input"]["plugin"]["name"] == "R"
+    json.dumps(p)
+
+
+def test_reference_plan_fixtures_parse_as_json():
+    for f in os.listdir(PLANS):
+        with open(os.path.join(PLANS, f)) as fh:
+            json.load(fh)
+
+
+# ---- GPU: the reference's plan fixtures -----------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["reduce-scan", "reduce-twofields-scan", "reduce-join",
+                                  "reduce-select", "reduce-scan-bin"])
+def test_reference_plan_golden(ctx, name):
+    assert run(ctx, name) == GOLD[name]["expected"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["reduce-join-record", "reduce-join-record-nonselective"])
+def test_reference_join_record_plans(ctx, name):
+    # same rows as the golden; the row order of a parallel join is not fixed
+    got = json.loads(run(ctx, name))
+    want = json.loads(GOLD[name]["expected"])
+    key = lambda r: (r["A3"], r["B3"])
+    assert sorted(got, key=key) == sorted(want, key=key)
+
+
+def _tbl(name, cols):
+    rows = [ln.rstrip("\n").split("|") for ln in open(os.path.join(H.GOLDEN, "inputs", "tpch", name))
+            if ln.strip()]
+    return [np.array([int(r[c]) for r in rows]) for c in cols]
+
+
+@pytest.mark.gpu
+def test_reference_tpch_plans(ctx):
+    (pk,) = _tbl("part_0.001.tbl", [0])
+    (ps,) = _tbl("partsupp_0.001.tbl", [0])
+    assert run(ctx, "reduce-scan-part") == str(len(pk))
+    assert run(ctx, "reduce-scan-partsupp") == str(len(ps))
+    assert run(ctx, "reduce-join-parts") == str(int(np.isin(ps, pk).sum()))
+
+
+def _register_sailors(ctx):
+    s = H.sailors()
+    ctx.clear_catalog()
+    ctx.register_column("inputs/sailors.csv", "sid", "int", s["sid"])
+    ctx.register_column("inputs/sailors.csv", "rating", "int", s["rating"])
+    ctx.register_column("inputs/sailors.csv", "age", "float", s["age"].astype(np.float64))
+    return s
+
+
+@pytest.mark.gpu
+def test_reference_gpu_plans_on_registered_columns(ctx):
+    s = _register_sailors(ctx)
+    # reduce-scan-gpu: sum(1), max(sid) -> sailorsScan's count and max
+    cnt, mx = run(ctx, "reduce-scan-gpu").split(",")
+    want = GOLD["sailorsScan"]["expected"].split(",")
+    assert (mx, cnt) == (want[0], want[2])
+    # gpu-group-by: sum(sid) group by rating
+    rows = [ln.split(",") for ln in run(ctx, "gpu-group-by").splitlines()]
+    got = {int(r[0]): int(r[1]) for r in rows}
+    want = {}
+    for k, v in zip(s["rating"], s["sid"]):
+        want[int(k)] = want.get(int(k), 0) + int(v)
+    assert got == want
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_sailors_relbuilder_golden(ctx):
+    """tests-join-sailors.cpp JoinLeft3 and the scans, built with RelBuilder."""
+    _register_sailors(ctx)
+    r = H.reserves()
+    ctx.register_column("inputs/reserves.csv", "sid", "int", r["sid"])
+    ctx.register_column("inputs/reserves.csv", "bid", "int", r["bid"])
+    b = rb.RelBuilder(ctx, H.GOLDEN)
+    sail = b.scan("inputs/sailors.csv", ["sid", "rating", "age"]).to_gpu().unpack()
+    res = b.scan("inputs/reserves.csv", ["sid", "bid"]).to_gpu().unpack()
+    # scan: max(sid), sum(sid), count
+    q = sail.reduce([rb.col("inputs/sailors.csv", "sid"), rb.col("inputs/sailors.csv", "sid"),
+                     rb.lit(1)], ["max", "sum", "sum"]).to_cpu()
+    assert str(q.prepare().execute()).strip() == GOLD["sailorsScan"]["expected"]
+    # select age > 40 (sailorsSel)
+    q = sail.filter(rb.gt(rb.col("inputs/sailors.csv", "age"), rb.lit(40.0))) \
+            .reduce([rb.col("inputs/sailors.csv", "sid"), rb.col("inputs/sailors.csv", "sid"),
+                     rb.lit(1)], ["max", "sum", "sum"])
+    assert str(q.prepare().execute()).strip() == GOLD["sailorsSel"]["expected"]
+    # JoinLeft3: reserves JOIN sailors, max(sid), sum(sid), count
+    for fused in (True, False):
+        j = res.join(sail, rb.col("inputs/sailors.csv", "sid"), rb.col("inputs/reserves.csv", "sid"),
+                     hash_bits=10, max_build=16)
+        e = rb.col("inputs/sailors.csv", "sid")
+        # a projection between join and reduce turns the fused path off
+        if not fused:
+            j = j.project([rb.as_(e, "p", "sid")], "p")
+            e = rb.col("p", "sid")
+        q = j.reduce([e, e, rb.lit(1)], ["max", "sum", "sum"])
+        assert str(q.prepare().execute()).strip() == GOLD["sailorsJoinLeft3"]["expected"], fused
+    ctx.clear_catalog()
+
+
+# ---- GPU: synthetic plans vs the oracle ---------------------------------
+def _synthetic(ctx, nr, ns, seed):
+    rk = O.gen_perm_i32(nr, seed)
+    rv = O.gen_uniform_i32(nr, seed, 1, 0, 1 << 31)
+    sk = O.gen_uniform_i32(ns, seed, 2, 0, int(1.33 * nr))
+    sv = O.gen_uniform_i32(ns, seed, 3, 0, 1 << 31)
+    ctx.clear_catalog()
+    for rel, k, v in (("R", rk, rv), ("S", sk, sv)):
+        ctx.register_column(rel, "k", "int", ctx.to_device(k))
+        ctx.register_column(rel, "v", "int", ctx.to_device(v))
+    return rk, rv, sk, sv
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["hashjoin-chained", "partitioned-hashjoin-chained"])
+def test_join_reduce_plan_matches_oracle(ctx, kind):
+    nr, ns = 1 << 16, 1 << 20
+    rk, rv, sk, sv = _synthetic(ctx, nr, ns, 5)
+    rc, want = O.join_dense(rk, rv, sk, sv)
+    b = rb.RelBuilder(ctx)
+    R = b.scan("R", ["k", "v"]).to_gpu().unpack()
+    S = b.scan("S", ["k", "v"]).to_gpu().unpack()
+    if kind == "hashjoin-chained":
+        j = S.join(R, rb.col("R", "k"), rb.col("S", "k"), hash_bits=17, max_build=nr)
+    else:
+        j = S.partitioned_join(R, rb.col("R", "k"), rb.col("S", "k"), nr, ns)
+    tot = rb.add(rb.cast(rb.col("R", "v"), "int64"), rb.cast(rb.col("S", "v"), "int64"))
+    q = j.reduce([rb.int64(1), tot, rb.col("R", "v"), rb.col("S", "v")],
+                 ["sum", "sum", "max", "min"])
+    got = [int(x) for x in str(q.prepare().execute()).strip().split(",")]
+    assert got == [want["count"], want["sum_b"] + want["sum_p"], want["max_b"], want["min_p"]]
+    # int32 sum wraps like the reference's i32 accumulator
+    q32 = j.reduce([rb.add(rb.col("R", "v"), rb.col("S", "v"))], ["sum"])
+    w32 = (want["sum_b"] + want["sum_p"]) & 0xffffffff
+    w32 = w32 - (1 << 32) if w32 >= 1 << 31 else w32
+    assert int(str(q32.prepare().execute()).strip()) == w32
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_materialized_join_filter_groupby_plan(ctx):
+    """join -> project -> select -> groupby -> sort, the non-fused path."""
+    nr, ns = 1 << 12, 1 << 16
+    rk, rv, sk, sv = _synthetic(ctx, nr, ns, 9)
+    b = rb.RelBuilder(ctx)
+    R = b.scan("R", ["k", "v"]).to_gpu().unpack()
+    S = b.scan("S", ["k", "v"]).to_gpu().unpack()
+    g = rb.mod(rb.col("R", "k"), rb.lit(13))
+    q = S.join(R, rb.col("R", "k"), rb.col("S", "k"), hash_bits=13, max_build=nr) \
+         .project([rb.as_(g, "p", "g"), rb.as_(rb.col("S", "v"), "p", "sv"),
+                   rb.as_(rb.col("R", "v"), "p", "rv")], "p") \
+         .filter(rb.gt(rb.col("p", "sv"), rb.lit(1 << 30))) \
+         .groupby([rb.col("p", "g")],
+                  [(rb.as_(rb.int64(1), "g", "cnt"), "sum"),
+                   (rb.as_(rb.cast(rb.col("p", "rv"), "int64"), "g", "s"), "sum"),
+                   (rb.as_(rb.col("p", "sv"), "g", "mx"), "max")], 4, 64) \
+         .sort([rb.col("p", "g"), rb.col("g", "cnt"), rb.col("g", "s"), rb.col("g", "mx")],
+               ["ASC", "NONE", "NONE", "NONE"])
+    rows = [[int(x) for x in r] for r in q.prepare().execute().rows()]
+    # expected with numpy
+    pos = np.full(nr, -1, np.int64)
+    pos[rk] = np.arange(nr)
+    hit = (sk < nr)
+    bi = pos[sk[hit]]
+    svh = sv[hit].astype(np.int64)
+    keep = svh > (1 << 30)
+    gk = rk[bi][keep] % 13
+    rvv = rv[bi][keep].astype(np.int64)
+    want = []
+    for k in range(13):
+        m = gk == k
+        if m.any():
+            want.append([k, int(m.sum()), int(rvv[m].sum()), int(svh[keep][m].max())])
+    assert rows == want
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_hash_rearrange_plan_keeps_rows(ctx):
+    nr, ns = 1 << 10, 1 << 14
+    rk, rv, sk, sv = _synthetic(ctx, nr, ns, 3)
+    b = rb.RelBuilder(ctx)
+    S = b.scan("S", ["k", "v"]).to_gpu().unpack()
+    q = S.pack([rb.col("S", "k"), rb.col("S", "v")], rb.col("S", "k"), 7) \
+         .reduce([rb.int64(1), rb.cast(rb.col("S", "v"), "int64"), rb.col("S", "k")],
+                 ["sum", "sum", "max"])
+    got = [int(x) for x in str(q.prepare().execute()).strip().split(",")]
+    assert got == [ns, int(sv.astype(np.int64).sum()), int(sk.max())]
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_plan_errors(ctx):
+    with pytest.raises(pa.PjError) as e:
+        ctx.execute_plan("{not json", ".")
+    assert e.value.code == pa.PJ_E_PARSE
+    with pytest.raises(pa.PjError) as e:
+        ctx.execute_plan(json.dumps({"operator": "nest", "input": {}}), ".")
+    assert e.value.code == pa.PJ_E_UNSUPPORTED
+    with pytest.raises(pa.PjError) as e:
+        ctx.execute_plan(json.dumps({"operator": "scan", "plugin": {"name": "nope", "type": "block",
+                                                                     "projections": []}}), ".")
+    assert e.value.code == pa.PJ_E_PARSE
