@@ -185,6 +185,80 @@ int pj_hash_rearrange_i32(pj_ctx *ctx, const int32_t *d_k, const int32_t *d_v,
                           uint64_t n, uint32_t buckets, int32_t *d_out_k,
                           int32_t *d_out_v, uint64_t *h_offsets);
 
+/* ---- resident dimensions, batched queries, serving ---------------------------
+ * A query is the plan hashjoin-chained -> groupby over a resident build
+ * relation ("dimension"): SELECT S.grp, COUNT(*), SUM(R.value + S.val)
+ * FROM S JOIN R ON S.fk = R.key GROUP BY S.grp, with S.grp dense in
+ * [0, groups).  The reference runs one JIT'd pipeline per query
+ * (PreparedStatement::execute, prepared-statement.cpp:41-140, build side
+ * rebuilt in every open, gpu-hash-join-chained.cpp:31-73); here dimension
+ * tables stay resident in HBM across queries and many queries share one
+ * persistent launch over a device work queue of (query, tile) items,
+ * replayed from a hipGraph per stream slot. */
+typedef struct pj_dim pj_dim;
+/* Resident HBM table of (key, value); keys unique and != INT32_MIN
+ * (PJ_E_ARG otherwise).  d_vals may be NULL (value 0).  The input columns
+ * are not kept. */
+int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
+                  uint64_t n, pj_dim **out);
+int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim);
+int pj_dim_info(pj_ctx *ctx, const pj_dim *dim, uint64_t *rows,
+                uint64_t *table_bytes);
+
+typedef struct {
+  const pj_dim *dim;       /* build side                                     */
+  const int32_t *d_fk;     /* probe columns (device); with gen_seed != 0 the  */
+  const int32_t *d_val;    /* batch first fills them from the seed:           */
+  const int32_t *d_grp;    /*   fk uniform [0, fk_range), val [0, 2^31),      */
+  uint64_t n;              /*   grp [0, groups) (pj_gen_uniform_i32 streams   */
+  uint32_t groups;         /*   1, 2, 3 of gen_seed)                          */
+  uint32_t flags;          /* 0                                               */
+  uint64_t gen_seed;
+  uint64_t fk_range;
+  int64_t *d_cnt;          /* outputs, `groups` entries each                  */
+  int64_t *d_sum;
+} pj_query;
+
+typedef struct {
+  uint32_t tile_rows;      /* probe rows per work item (0: 65536)            */
+  int wg_per_cu;           /* persistent workgroups per CU (0: 2)            */
+} pj_batch_cfg;
+
+/* One query, one launch sequence, synchronous (the plain path). */
+int pj_query_run(pj_ctx *ctx, const pj_query *q, const pj_batch_cfg *cfg);
+/* Creates n (1..8) stream slots for pj_batch_launch (default priority). */
+int pj_batch_slots(pj_ctx *ctx, int n);
+/* Starts a batch of 1..4096 queries on a slot and returns; the slot's
+ * graph is captured on first use and replayed afterwards. */
+int pj_batch_launch(pj_ctx *ctx, int slot, const pj_query *qs, uint32_t nq,
+                    const pj_batch_cfg *cfg);
+int pj_batch_poll(pj_ctx *ctx, int slot, int *done);
+int pj_batch_wait(pj_ctx *ctx, int slot);
+
+/* Serving loop: queries arrive at arrival_s[i] seconds after the call
+ * starts; mode 0 admits every waiting query whose probe inputs fit the HBM
+ * budget into the next free slot (continuous batching: batches of later
+ * arrivals overlap the ones already on the device); mode 1 runs one query
+ * at a time with pj_query_run in arrival order.  Probe inputs of queries
+ * with gen_seed != 0 and d_fk == NULL are allocated at admission and freed
+ * at completion.  done_s[i] = completion time of query i (seconds after
+ * the start). */
+typedef struct {
+  int mode;                /* 0 batched, 1 one query per launch              */
+  int slots;               /* streams in flight (mode 0), 1..8               */
+  uint32_t max_batch;      /* queries per batch (<= 4096)                    */
+  uint64_t hbm_budget;     /* bytes of probe inputs in flight (0: no limit)  */
+  pj_batch_cfg batch;
+} pj_serve_cfg;
+typedef struct {
+  uint64_t batches;
+  uint64_t max_in_flight_bytes;
+  double wall_s;
+} pj_serve_stats;
+int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s,
+                 uint32_t nq, const pj_serve_cfg *cfg, double *done_s,
+                 pj_serve_stats *stats);
+
 /* ---- plan API ----------------------------------------------------------------
  * PlanExecutor::parseOperator (core/olap/lib/plan/plan-parser.cpp:238-1675)
  * + PreparedStatement::execute (prepared-statement.cpp:41-140): parse a

@@ -54,6 +54,28 @@ class RadixCfg(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Query(C.Structure):
+    """pj_query: one join -> group-by over a resident dimension."""
+    _fields_ = [("dim", C.c_void_p), ("d_fk", C.c_void_p), ("d_val", C.c_void_p),
+                ("d_grp", C.c_void_p), ("n", C.c_uint64), ("groups", C.c_uint32),
+                ("flags", C.c_uint32), ("gen_seed", C.c_uint64), ("fk_range", C.c_uint64),
+                ("d_cnt", C.c_void_p), ("d_sum", C.c_void_p)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("tile_rows", C.c_uint32), ("wg_per_cu", C.c_int)]
+
+
+class ServeCfg(C.Structure):
+    _fields_ = [("mode", C.c_int), ("slots", C.c_int), ("max_batch", C.c_uint32),
+                ("hbm_budget", C.c_uint64), ("batch", BatchCfg)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("max_in_flight_bytes", C.c_uint64),
+                ("wall_s", C.c_double)]
+
+
 vp, u64, i64, i32, u32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_uint32
 _SIGS = {
     "pj_last_error": ([], C.c_char_p),
@@ -91,6 +113,16 @@ _SIGS = {
                        i32),
     "pj_reduce_i64": ([vp, vp, u64, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)], i32),
     "pj_hash_rearrange_i32": ([vp, vp, vp, u64, u32, vp, vp, C.POINTER(u64)], i32),
+    "pj_dim_create": ([vp, vp, vp, u64, C.POINTER(vp)], i32),
+    "pj_dim_destroy": ([vp, vp], i32),
+    "pj_dim_info": ([vp, vp, C.POINTER(u64), C.POINTER(u64)], i32),
+    "pj_query_run": ([vp, C.POINTER(Query), C.POINTER(BatchCfg)], i32),
+    "pj_batch_slots": ([vp, i32], i32),
+    "pj_batch_launch": ([vp, i32, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
+    "pj_batch_poll": ([vp, i32, C.POINTER(C.c_int)], i32),
+    "pj_batch_wait": ([vp, i32], i32),
+    "pj_serve_run": ([vp, C.POINTER(Query), C.POINTER(C.c_double), u32, C.POINTER(ServeCfg),
+                      C.POINTER(C.c_double), C.POINTER(ServeStats)], i32),
     "pj_register_column": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, i32], i32),
     "pj_clear_catalog": ([vp], i32),
     "pj_plan_execute": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, u64], i32),
@@ -332,6 +364,46 @@ class Context:
             self._keep.append(a)
             _check(_lib.pj_register_column(self.h, rel.encode(), attr.encode(), type_.encode(),
                                            a.ctypes.data, len(a), 0))
+
+    # ---- resident dimensions, batched queries, serving --------------------
+    def dim_create(self, keys, vals=None):
+        h = vp()
+        _check(_lib.pj_dim_create(self.h, _ptr(keys), _ptr(vals), len(keys), C.byref(h)))
+        return h.value
+
+    def dim_destroy(self, dim):
+        _check(_lib.pj_dim_destroy(self.h, dim))
+
+    def dim_info(self, dim):
+        rows, b = u64(), u64()
+        _check(_lib.pj_dim_info(self.h, dim, C.byref(rows), C.byref(b)))
+        return rows.value, b.value
+
+    def query_run(self, q, cfg=None):
+        _check(_lib.pj_query_run(self.h, C.byref(q), C.byref(cfg) if cfg else None))
+
+    def batch_slots(self, n):
+        _check(_lib.pj_batch_slots(self.h, n))
+
+    def batch_launch(self, slot, queries, cfg=None):
+        arr = (Query * len(queries))(*queries)
+        _check(_lib.pj_batch_launch(self.h, slot, arr, len(queries), C.byref(cfg) if cfg else None))
+
+    def batch_wait(self, slot):
+        _check(_lib.pj_batch_wait(self.h, slot))
+
+    def serve_run(self, queries, arrivals, mode=0, slots=2, max_batch=4096, hbm_budget=0,
+                  tile_rows=0, wg_per_cu=0):
+        """Runs the serving loop; returns (completion seconds per query, stats)."""
+        n = len(queries)
+        arr = (Query * n)(*queries)
+        arrv = (C.c_double * n)(*arrivals)
+        done = (C.c_double * n)()
+        cfg = ServeCfg(mode, slots, max_batch, hbm_budget, BatchCfg(tile_rows, wg_per_cu))
+        st = ServeStats()
+        _check(_lib.pj_serve_run(self.h, arr, arrv, n, C.byref(cfg), done, C.byref(st)))
+        return list(done), {"batches": st.batches, "max_in_flight_bytes": st.max_in_flight_bytes,
+                            "wall_s": st.wall_s}
 
     def clear_catalog(self):
         _check(_lib.pj_clear_catalog(self.h))

@@ -1,0 +1,510 @@
+// Resident dimension tables and batched join -> group-by queries.
+//
+// A query joins a probe relation S(fk, val, grp) with a resident dimension
+// R(key, value) on S.fk = R.key and groups the matches by S.grp:
+//     SELECT grp, COUNT(*), SUM(R.value + S.val) GROUP BY grp
+// (the plan hashjoin-chained -> groupby of the reference; group keys are
+// dense in [0, groups)).
+//
+// Dimension tables stay in HBM across queries: open addressing over 64-bit
+// slots key << 32 | value (empty = key INT32_MIN), load factor <= 1/2.  A
+// 2^24-row dimension is a 256 MiB table, the size of MI355X's Infinity
+// Cache, so probes of resident dimensions are served mostly from MALL.
+// Dimensions of <= kLdsDim rows are copied into LDS once per work tile.
+//
+// One batch = one persistent launch over a device work queue of
+// (query, tile) items: a workgroup takes the next item from a counter,
+// binary-searches its query, probes and aggregates the tile (group counts
+// and sums in LDS when the query has <= kLdsGroups groups, else with HBM
+// atomics) and flushes the LDS partials with one atomic per group.  The
+// batch sequence (descriptor upload, output clearing, input generation,
+// probe) is captured once per stream slot as a hipGraph and replayed for
+// every batch; the plain path issues the same kernels once per query.
+#include <algorithm>
+#include <vector>
+
+#include "pj_internal.hpp"
+
+struct pj_dim {
+  uint64_t n = 0;
+  uint64_t slots = 0;  // power of two
+  unsigned long long *table = nullptr;
+};
+
+namespace pj {
+
+constexpr unsigned long long kEmptySlot = 0x8000000000000000ULL;  // key INT32_MIN
+constexpr uint32_t kLdsDim = 2048;      // dims copied into LDS (<= 4096 slots)
+constexpr uint32_t kLdsSlots = 4096;
+constexpr uint32_t kLdsGroups = 2048;   // groups aggregated in LDS
+constexpr int kBatchBlock = 512;
+constexpr uint32_t kMaxBatch = 4096;    // queries per batch
+
+struct QDesc {
+  const unsigned long long *table;
+  uint64_t mask;
+  const int32_t *fk;
+  const int32_t *val;
+  const int32_t *grp;
+  int32_t *gfk, *gval, *ggrp;  // generator targets (same buffers)
+  int64_t *cnt, *sum;
+  uint64_t n;
+  uint64_t tile0;      // first work item of this query
+  uint64_t gen_seed;   // 0: inputs given
+  uint64_t fk_range;
+  uint32_t dn;         // dimension rows
+  uint32_t groups;
+};
+
+struct BatchHdr {
+  uint32_t nq;
+  uint32_t tile_rows;
+  uint64_t total_tiles;
+  uint32_t gen_ctr, probe_ctr;
+  uint32_t bad_group;  // rows whose group key is outside [0, groups)
+  uint32_t pad;
+};
+
+__global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    t[i] = kEmptySlot;
+}
+
+// flags: bit 0 duplicate key, bit 1 reserved key INT32_MIN
+__global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__restrict__ v, uint64_t n,
+                            unsigned long long *t, uint64_t mask, unsigned int *flags) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t key = k[i];
+    if (key == INT32_MIN) {
+      atomicOr(flags, 2u);
+      continue;
+    }
+    const unsigned long long w = (unsigned long long)(uint32_t)key << 32 | (uint32_t)(v ? v[i] : 0);
+    uint64_t h = hash_i32(key) & mask;
+    for (;;) {
+      unsigned long long old = atomicCAS(&t[h], kEmptySlot, w);
+      if (old == kEmptySlot) break;
+      if ((int32_t)(old >> 32) == key) {
+        atomicOr(flags, 1u);
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t find_query(const QDesc *d, uint32_t nq, uint64_t item) {
+  uint32_t lo = 0, hi = nq;  // last q with tile0 <= item
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (d[mid].tile0 <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_batch_clear(BatchHdr *hdr, const QDesc *d) {
+  const uint32_t nq = hdr->nq;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    const uint32_t g = d[q].groups;
+    for (uint32_t i = threadIdx.x; i < g; i += blockDim.x) {
+      d[q].cnt[i] = 0;
+      d[q].sum[i] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hdr->gen_ctr = 0;
+    hdr->probe_ctr = 0;
+    hdr->bad_group = 0;
+  }
+}
+
+// Seeded probe inputs: fk uniform [0, fk_range), val uniform [0, 2^31),
+// grp uniform [0, groups) -- streams 1, 2, 3 of the query's seed, the same
+// formula as pj_gen_uniform_i32.
+__global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d) {
+  __shared__ uint64_t s_item;
+  const uint32_t nq = hdr->nq;
+  const uint64_t total = hdr->total_tiles;
+  const uint32_t T = hdr->tile_rows;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&hdr->gen_ctr, 1u);
+    __syncthreads();
+    const uint64_t item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const QDesc &q = d[find_query(d, nq, item)];
+    if (!q.gen_seed) continue;
+    const uint64_t r0 = (item - q.tile0) * T;
+    const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
+    const uint64_t k1 = rng_key(q.gen_seed, 1), k2 = rng_key(q.gen_seed, 2), k3 = rng_key(q.gen_seed, 3);
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      q.gfk[r] = (int32_t)mulhi64(rng_at(k1, r), q.fk_range);
+      q.gval[r] = (int32_t)mulhi64(rng_at(k2, r), 1ULL << 31);
+      q.ggrp[r] = (int32_t)mulhi64(rng_at(k3, r), q.groups);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, const QDesc *d) {
+  __shared__ unsigned long long ltab[kLdsSlots];
+  __shared__ uint32_t gcnt[kLdsGroups];
+  __shared__ unsigned long long gsum[kLdsGroups];
+  __shared__ uint64_t s_item;
+  __shared__ uint32_t s_bad;
+  const uint32_t nq = hdr->nq;
+  const uint64_t total = hdr->total_tiles;
+  const uint32_t T = hdr->tile_rows;
+  const int tid = threadIdx.x;
+  uint32_t last_q = 0xffffffffu;
+  for (;;) {
+    if (tid == 0) {
+      s_item = atomicAdd(&hdr->probe_ctr, 1u);
+      s_bad = 0;
+    }
+    __syncthreads();
+    const uint64_t item = s_item;
+    if (item >= total) break;
+    const uint32_t qi = find_query(d, nq, item);
+    const QDesc &q = d[qi];
+    const bool lds_tab = q.dn <= kLdsDim;
+    const bool lds_grp = q.groups <= kLdsGroups;
+    const uint64_t mask = q.mask;
+    if (lds_tab && qi != last_q)
+      for (uint32_t i = tid; i <= (uint32_t)mask; i += kBatchBlock) ltab[i] = q.table[i];
+    last_q = lds_tab ? qi : 0xffffffffu;
+    if (lds_grp)
+      for (uint32_t i = tid; i < q.groups; i += kBatchBlock) {
+        gcnt[i] = 0;
+        gsum[i] = 0;
+      }
+    __syncthreads();
+    const uint64_t r0 = (item - q.tile0) * T;
+    const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
+    const uint32_t G = q.groups;
+    uint32_t bad = 0;
+    for (uint64_t base = r0 + (uint64_t)tid * 4; base < r1; base += (uint64_t)kBatchBlock * 4) {
+      int32_t k[4], v[4], g[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t r = base + u;
+        const bool live = r < r1;
+        k[u] = live ? q.fk[r] : INT32_MIN;
+        v[u] = live ? q.val[r] : 0;
+        g[u] = live ? q.grp[r] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k[u] == INT32_MIN) continue;
+        uint64_t h = hash_i32(k[u]) & mask;
+        int32_t b = 0;
+        bool hit = false;
+        for (;;) {
+          const unsigned long long s = lds_tab ? ltab[h] : q.table[h];
+          const int32_t sk = (int32_t)(s >> 32);
+          if (sk == k[u]) {
+            hit = true;
+            b = (int32_t)(uint32_t)s;
+            break;
+          }
+          if (s == kEmptySlot) break;
+          h = (h + 1) & mask;
+        }
+        if (!hit) continue;
+        if ((uint32_t)g[u] >= G) {
+          ++bad;
+          continue;
+        }
+        const long long sv = (long long)b + (long long)v[u];
+        if (lds_grp) {
+          atomicAdd(&gcnt[g[u]], 1u);
+          atomicAdd(&gsum[g[u]], (unsigned long long)sv);
+        } else {
+          atomicAdd((unsigned long long *)&q.cnt[g[u]], 1ULL);
+          atomicAdd((unsigned long long *)&q.sum[g[u]], (unsigned long long)sv);
+        }
+      }
+    }
+    if (bad) atomicAdd(&s_bad, bad);
+    __syncthreads();
+    if (lds_grp)
+      for (uint32_t i = tid; i < G; i += kBatchBlock)
+        if (gcnt[i]) {
+          atomicAdd((unsigned long long *)&q.cnt[i], (unsigned long long)gcnt[i]);
+          atomicAdd((unsigned long long *)&q.sum[i], gsum[i]);
+        }
+    if (tid == 0 && s_bad) atomicAdd(&hdr->bad_group, s_bad);
+    __syncthreads();
+  }
+}
+
+// One stream slot: pinned + device descriptor buffers, its graph.
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  void *h_buf = nullptr;  // pinned: BatchHdr then QDesc[kMaxBatch]
+  void *d_buf = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_grid = 0;
+  bool busy = false;
+};
+
+struct Batches {
+  std::vector<Slot> slots;
+};
+
+constexpr size_t kBufBytes = 256 + sizeof(QDesc) * kMaxBatch;
+
+static Batches &batches(pj_ctx *ctx) {
+  if (!ctx->batch) ctx->batch = new Batches();
+  return *(Batches *)ctx->batch;
+}
+
+static int fill_desc(const pj_query &q, QDesc *d) {
+  if (!q.dim) return fail(PJ_E_ARG, "query without a dimension");
+  if (!q.d_cnt || !q.d_sum) return fail(PJ_E_ARG, "query without output arrays");
+  if (!q.groups) return fail(PJ_E_ARG, "query with 0 groups");
+  if (q.n && (!q.d_fk || !q.d_val || !q.d_grp)) return fail(PJ_E_ARG, "query without probe columns");
+  if (q.gen_seed && !q.fk_range) return fail(PJ_E_ARG, "generated query needs fk_range");
+  if (q.n >= (1ULL << 40)) return fail(PJ_E_ARG, "query probe side too large");
+  d->table = q.dim->table;
+  d->mask = q.dim->slots - 1;
+  d->dn = (uint32_t)std::min<uint64_t>(q.dim->n, 0xffffffffULL);
+  d->fk = q.d_fk;
+  d->val = q.d_val;
+  d->grp = q.d_grp;
+  d->gfk = const_cast<int32_t *>(q.d_fk);
+  d->gval = const_cast<int32_t *>(q.d_val);
+  d->ggrp = const_cast<int32_t *>(q.d_grp);
+  d->cnt = q.d_cnt;
+  d->sum = q.d_sum;
+  d->n = q.n;
+  d->gen_seed = q.gen_seed;
+  d->fk_range = q.fk_range;
+  d->groups = q.groups;
+  return PJ_OK;
+}
+
+static uint32_t tile_rows_of(const pj_batch_cfg *cfg) {
+  return cfg && cfg->tile_rows ? cfg->tile_rows : 65536u;
+}
+static uint32_t grid_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
+  int per = cfg && cfg->wg_per_cu > 0 ? cfg->wg_per_cu : 2;
+  return (uint32_t)(ctx->num_cus * per);
+}
+
+// Writes header + descriptors into a host buffer; returns total tiles.
+static int pack(const pj_query *qs, uint32_t nq, uint32_t T, void *h_buf) {
+  BatchHdr *hdr = (BatchHdr *)h_buf;
+  QDesc *d = (QDesc *)((char *)h_buf + 256);
+  uint64_t tiles = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    PJ_TRY(fill_desc(qs[i], &d[i]));
+    d[i].tile0 = tiles;
+    tiles += ceil_div(qs[i].n, T);
+  }
+  hdr->nq = nq;
+  hdr->tile_rows = T;
+  hdr->total_tiles = tiles;
+  hdr->gen_ctr = hdr->probe_ctr = hdr->bad_group = 0;
+  return PJ_OK;
+}
+
+// The batch sequence on `s` (graph-capturable: kernels and one copy).
+static int enqueue(pj_ctx *ctx, hipStream_t s, void *h_buf, void *d_buf, uint32_t grid,
+                   size_t bytes) {
+  PJ_HIP(hipMemcpyAsync(d_buf, h_buf, bytes, hipMemcpyHostToDevice, s));
+  BatchHdr *hdr = (BatchHdr *)d_buf;
+  const QDesc *d = (const QDesc *)((char *)d_buf + 256);
+  hipLaunchKernelGGL(k_batch_clear, dim3(std::min<uint32_t>(grid, 1024)), dim3(256), 0, s, hdr, d);
+  hipLaunchKernelGGL(k_batch_gen, dim3(grid * 2), dim3(256), 0, s, hdr, d);
+  hipLaunchKernelGGL(k_batch_probe, dim3(grid), dim3(kBatchBlock), 0, s, hdr, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PJ_E_HIP, "batch launch: %s", hipGetErrorString(e));
+  return PJ_OK;
+}
+
+static int slot_init(pj_ctx *ctx, Slot &sl) {
+  PJ_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+  PJ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  PJ_HIP(hipHostMalloc(&sl.h_buf, kBufBytes, hipHostMallocDefault));
+  PJ_TRY(dalloc(ctx, kBufBytes, &sl.d_buf));
+  return PJ_OK;
+}
+
+static void slot_free(pj_ctx *ctx, Slot &sl) {
+  if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+  if (sl.graph) (void)hipGraphDestroy(sl.graph);
+  if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.h_buf) (void)hipHostFree(sl.h_buf);
+  if (sl.d_buf) (void)dfree(ctx, sl.d_buf);
+  if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  sl = Slot();
+}
+
+static int check_bad(const void *h_buf) {
+  const BatchHdr *hdr = (const BatchHdr *)h_buf;
+  if (hdr->bad_group)
+    return fail(PJ_E_ARG, "%u probe rows had a group key outside [0, groups)", hdr->bad_group);
+  return PJ_OK;
+}
+
+}  // namespace pj
+
+using namespace pj;
+
+extern "C" void pj_batch_release(pj_ctx *ctx) {
+  if (!ctx->batch) return;
+  Batches *b = (Batches *)ctx->batch;
+  for (auto &sl : b->slots) slot_free(ctx, sl);
+  delete b;
+  ctx->batch = nullptr;
+}
+
+extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
+                             pj_dim **out) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  *out = nullptr;
+  if (n && !d_keys) return fail(PJ_E_ARG, "dimension without keys");
+  if (n >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
+  pj_dim *dm = new pj_dim();
+  dm->n = n;
+  dm->slots = std::max<uint64_t>(64, 2ULL << ilog2_ceil(n ? n : 1));
+  int rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table);
+  if (rc != PJ_OK) {
+    delete dm;
+    return rc;
+  }
+  Scratch flags;
+  auto fin = [&](int r) {
+    if (r != PJ_OK) {
+      dfree(ctx, dm->table);
+      delete dm;
+    }
+    return r;
+  };
+  if ((rc = flags.alloc(ctx, 4)) != PJ_OK) return fin(rc);
+  if ((rc = zero(ctx, flags.p, 4)) != PJ_OK) return fin(rc);
+  const uint64_t gs = std::min<uint64_t>(ceil_div(dm->slots, 256), (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
+  if (n)
+    hipLaunchKernelGGL(k_dim_build, dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
+                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->table, dm->slots - 1,
+                       flags.as<unsigned int>());
+  if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
+  unsigned int f = 0;
+  if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
+  if (f & 2) return fin(fail(PJ_E_ARG, "dimension key INT32_MIN is reserved"));
+  if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
+  *out = dm;
+  return PJ_OK;
+}
+
+extern "C" int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim) {
+  if (!dim) return PJ_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  PJ_TRY(sync(ctx));
+  dfree(ctx, dim->table);
+  delete dim;
+  return PJ_OK;
+}
+
+extern "C" int pj_dim_info(pj_ctx *ctx, const pj_dim *dim, uint64_t *rows, uint64_t *table_bytes) {
+  (void)ctx;
+  if (!dim) return fail(PJ_E_ARG, "null dimension");
+  if (rows) *rows = dim->n;
+  if (table_bytes) *table_bytes = dim->slots * 8;
+  return PJ_OK;
+}
+
+extern "C" int pj_query_run(pj_ctx *ctx, const pj_query *q, const pj_batch_cfg *cfg) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  const uint32_t T = tile_rows_of(cfg);
+  char *h = (char *)ctx->pinned_small;
+  static_assert(256 + sizeof(QDesc) <= (1 << 20), "pinned staging too small");
+  PJ_TRY(pack(q, 1, T, h));
+  Scratch dbuf;
+  PJ_TRY(dbuf.alloc(ctx, 256 + sizeof(QDesc)));
+  PJ_TRY(enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, cfg), 256 + sizeof(QDesc)));
+  PJ_HIP(hipMemcpyAsync(h, dbuf.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, ctx->stream));
+  PJ_TRY(sync(ctx));
+  return check_bad(h);
+}
+
+extern "C" int pj_batch_slots(pj_ctx *ctx, int n) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (n < 1 || n > 8) return fail(PJ_E_ARG, "batch slots %d not in [1, 8]", n);
+  Batches &b = batches(ctx);
+  for (auto &sl : b.slots) slot_free(ctx, sl);
+  b.slots.assign(n, Slot());
+  for (auto &sl : b.slots) PJ_TRY(slot_init(ctx, sl));
+  return PJ_OK;
+}
+
+extern "C" int pj_batch_launch(pj_ctx *ctx, int slot, const pj_query *qs, uint32_t nq,
+                               const pj_batch_cfg *cfg) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Batches &b = batches(ctx);
+  if (slot < 0 || slot >= (int)b.slots.size()) return fail(PJ_E_ARG, "batch slot %d not created", slot);
+  if (nq < 1 || nq > kMaxBatch) return fail(PJ_E_ARG, "batch of %u queries not in [1, %u]", nq, kMaxBatch);
+  Slot &sl = b.slots[slot];
+  if (sl.busy) return fail(PJ_E_ARG, "batch slot %d is busy", slot);
+  const uint32_t T = tile_rows_of(cfg);
+  PJ_TRY(pack(qs, nq, T, sl.h_buf));
+  const uint32_t grid = grid_of(ctx, cfg);
+  if (!sl.exec || sl.graph_grid != grid) {
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    sl.exec = nullptr;
+    sl.graph = nullptr;
+    PJ_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(ctx, sl.stream, sl.h_buf, sl.d_buf, grid, kBufBytes);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(sl.stream, &gr);
+    if (rc != PJ_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(PJ_E_HIP, "batch graph capture: %s", hipGetErrorString(e));
+    sl.graph = gr;
+    PJ_HIP(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    sl.graph_grid = grid;
+  }
+  PJ_HIP(hipGraphLaunch(sl.exec, sl.stream));
+  PJ_HIP(hipMemcpyAsync(sl.h_buf, sl.d_buf, sizeof(BatchHdr), hipMemcpyDeviceToHost, sl.stream));
+  PJ_HIP(hipEventRecord(sl.done, sl.stream));
+  sl.busy = true;
+  return PJ_OK;
+}
+
+extern "C" int pj_batch_poll(pj_ctx *ctx, int slot, int *done) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Batches &b = batches(ctx);
+  if (slot < 0 || slot >= (int)b.slots.size()) return fail(PJ_E_ARG, "batch slot %d not created", slot);
+  Slot &sl = b.slots[slot];
+  *done = 1;
+  if (!sl.busy) return PJ_OK;
+  hipError_t e = hipEventQuery(sl.done);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return PJ_OK;
+  }
+  if (e != hipSuccess) return fail(PJ_E_HIP, "batch: %s", hipGetErrorString(e));
+  sl.busy = false;
+  return check_bad(sl.h_buf);
+}
+
+extern "C" int pj_batch_wait(pj_ctx *ctx, int slot) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Batches &b = batches(ctx);
+  if (slot < 0 || slot >= (int)b.slots.size()) return fail(PJ_E_ARG, "batch slot %d not created", slot);
+  Slot &sl = b.slots[slot];
+  if (!sl.busy) return PJ_OK;
+  PJ_HIP(hipEventSynchronize(sl.done));
+  sl.busy = false;
+  return check_bad(sl.h_buf);
+}

@@ -130,6 +130,8 @@ struct pj_ctx {
   // autotuner
   bool autotune = false;
   std::map<pj::TuneKey, pj_radix_cfg> tuned;
+  // batched-query stream slots (owned by batch.hip)
+  void *batch = nullptr;
 };
 
 namespace pj {

@@ -1,0 +1,136 @@
+"""Resident dimensions, batched join -> group-by queries and the serving
+loop: every query's result is the same alone, in a batch and in the served
+mix, and equals the oracle's."""
+import numpy as np
+import pytest
+
+import proteus_amd as pa
+from proteus_amd import workload
+from oracle import pyoracle as O
+
+
+def expected(dk, dv, n, groups, seed, fk_range):
+    fk = O.gen_uniform_i32(n, seed, 1, 0, fk_range).astype(np.int64)
+    val = O.gen_uniform_i32(n, seed, 2, 0, 1 << 31).astype(np.int64)
+    grp = O.gen_uniform_i32(n, seed, 3, 0, groups).astype(np.int64)
+    pos = np.full(len(dk), -1, np.int64)
+    pos[dk] = np.arange(len(dk))
+    hit = fk < len(dk)
+    b = dv.astype(np.int64)[pos[fk[hit]]]
+    cnt = np.zeros(groups, np.int64)
+    s = np.zeros(groups, np.int64)
+    np.add.at(cnt, grp[hit], 1)
+    np.add.at(s, grp[hit], b + val[hit])
+    return cnt, s
+
+
+def _inputs(ctx, n):
+    return [ctx.empty(n, np.int32) for _ in range(3)]
+
+
+@pytest.mark.gpu
+def test_dimension_errors(ctx):
+    k = ctx.to_device(np.array([1, 2, 2, 5], np.int32))
+    with pytest.raises(pa.PjError) as e:
+        ctx.dim_create(k)
+    assert e.value.code == pa.PJ_E_ARG
+    k2 = ctx.to_device(np.array([1, np.iinfo(np.int32).min], np.int32))
+    with pytest.raises(pa.PjError) as e:
+        ctx.dim_create(k2)
+    assert e.value.code == pa.PJ_E_ARG
+    d = ctx.dim_create(ctx.to_device(np.arange(100, dtype=np.int32)))
+    assert ctx.dim_info(d) == (100, 256 * 8)
+    ctx.dim_destroy(d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("u,groups", [(10, 16), (10, 1 << 13), (13, 16), (13, 1 << 13),
+                                      (11, 2048), (12, 2049)])
+def test_query_matches_oracle(ctx, u, groups):
+    nb = 1 << u
+    dk = ctx.gen_perm_i32(nb, 77 + u)
+    dv = ctx.gen_uniform_i32(nb, 77 + u, 7, 0, 1 << 31)
+    dim = ctx.dim_create(dk, dv)
+    n = 37 * nb + 5
+    seed, fk_range = 1234 + u, int(1.33 * nb)
+    fk, val, grp = _inputs(ctx, n)
+    cnt, sm = ctx.empty(groups, np.int64), ctx.empty(groups, np.int64)
+    q = pa.Query(dim, fk.ptr, val.ptr, grp.ptr, n, groups, 0, seed, fk_range, cnt.ptr, sm.ptr)
+    for tile in (0, 4096):
+        ctx.query_run(q, pa.BatchCfg(tile, 0))
+        wc, ws = expected(dk.numpy(), dv.numpy(), n, groups, seed, fk_range)
+        assert np.array_equal(cnt.numpy(), wc)
+        assert np.array_equal(sm.numpy(), ws)
+    # the same columns again, given (no generation)
+    q.gen_seed = 0
+    ctx.query_run(q)
+    assert np.array_equal(cnt.numpy(), wc) and np.array_equal(sm.numpy(), ws)
+    for a in (fk, val, grp, cnt, sm):
+        a.free()
+    ctx.dim_destroy(dim)
+
+
+@pytest.mark.gpu
+def test_bad_group_key_is_an_error(ctx):
+    dim = ctx.dim_create(ctx.to_device(np.arange(64, dtype=np.int32)))
+    fk = ctx.to_device(np.arange(10, dtype=np.int32))
+    val = ctx.to_device(np.zeros(10, np.int32))
+    grp = ctx.to_device(np.array([0, 1, 2, 3, 9, 0, 1, 2, 3, 0], np.int32))
+    cnt, sm = ctx.empty(4, np.int64), ctx.empty(4, np.int64)
+    q = pa.Query(dim, fk.ptr, val.ptr, grp.ptr, 10, 4, 0, 0, 0, cnt.ptr, sm.ptr)
+    with pytest.raises(pa.PjError) as e:
+        ctx.query_run(q)
+    assert e.value.code == pa.PJ_E_ARG
+    ctx.dim_destroy(dim)
+
+
+def _alloc_inputs(ctx, mix):
+    bufs, qs = [], mix.queries()
+    for q in qs:
+        b = _inputs(ctx, q.n)
+        bufs += b
+        q.d_fk, q.d_val, q.d_grp = b[0].ptr, b[1].ptr, b[2].ptr
+    return qs, bufs
+
+
+@pytest.mark.gpu
+def test_batch_and_serving_equal_alone(ctx):
+    mix = workload.Mix(ctx, 48, seed=11, u_lo=10, u_hi=15, qps=4000)
+    qs, bufs = _alloc_inputs(ctx, mix)
+    alone = []
+    for q in qs:
+        ctx.query_run(q)
+    alone = mix.results()
+    # spot-check three queries against the oracle
+    for i in (0, 17, 47):
+        u = int(mix.u[i])
+        dk, dv = (c.numpy() for c in mix.dim_cols[u])
+        wc, ws = expected(dk, dv, int(mix.n[i]), int(mix.groups[i]), int(mix.gen_seed[i]),
+                          int(mix.fk_range[i]))
+        assert np.array_equal(alone[i][0], wc) and np.array_equal(alone[i][1], ws)
+    # two batches on two slots, graph replays included
+    ctx.batch_slots(2)
+    for rep in range(3):
+        ctx.batch_launch(0, qs[:20])
+        ctx.batch_launch(1, qs[20:])
+        ctx.batch_wait(0)
+        ctx.batch_wait(1)
+        got = mix.results()
+        for i in range(mix.nq):
+            assert np.array_equal(got[i][0], alone[i][0]), (rep, i)
+            assert np.array_equal(got[i][1], alone[i][1]), (rep, i)
+        ctx.sync()  # eager work between replays
+        _ = ctx.gen_uniform_i32(1 << 16, rep, 1, 0, 100)
+    for b in bufs:
+        b.free()
+    # served mix (inputs allocated by the server), batched and one-by-one
+    for mode in (0, 1):
+        done, st = ctx.serve_run(mix.queries(), mix.arrivals, mode=mode, slots=3,
+                                 hbm_budget=64 << 20)
+        assert all(d >= a for d, a in zip(done, mix.arrivals))
+        assert st["max_in_flight_bytes"] <= 64 << 20
+        got = mix.results()
+        for i in range(mix.nq):
+            assert np.array_equal(got[i][0], alone[i][0]), (mode, i)
+            assert np.array_equal(got[i][1], alone[i][1]), (mode, i)
+    mix.close()
