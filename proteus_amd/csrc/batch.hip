@@ -60,9 +60,9 @@ struct BatchHdr {
   uint32_t nq;
   uint32_t tile_rows;
   uint64_t total_tiles;
-  uint32_t gen_ctr, probe_ctr;
+  uint64_t lds_tiles;  // items [0, lds_tiles) belong to LDS-table queries
+  uint32_t gen_ctr, probe_ctr_lds, probe_ctr_hbm;
   uint32_t bad_group;  // rows whose group key is outside [0, groups)
-  uint32_t pad;
 };
 
 __global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
@@ -105,25 +105,11 @@ __device__ __forceinline__ uint32_t find_query(const QDesc *d, uint32_t nq, uint
   return lo;
 }
 
-__global__ void k_batch_clear(BatchHdr *hdr, const QDesc *d) {
-  const uint32_t nq = hdr->nq;
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    const uint32_t g = d[q].groups;
-    for (uint32_t i = threadIdx.x; i < g; i += blockDim.x) {
-      d[q].cnt[i] = 0;
-      d[q].sum[i] = 0;
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    hdr->gen_ctr = 0;
-    hdr->probe_ctr = 0;
-    hdr->bad_group = 0;
-  }
-}
-
 // Seeded probe inputs: fk uniform [0, fk_range), val uniform [0, 2^31),
 // grp uniform [0, groups) -- streams 1, 2, 3 of the query's seed, the same
-// formula as pj_gen_uniform_i32.
+// formula as pj_gen_uniform_i32.  The workgroup that takes a query's first
+// item also clears its outputs (every query owns at least one item); the
+// probe kernel starts after this kernel ends.
 __global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d) {
   __shared__ uint64_t s_item;
   const uint32_t nq = hdr->nq;
@@ -136,6 +122,11 @@ __global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d
     __syncthreads();
     if (item >= total) break;
     const QDesc &q = d[find_query(d, nq, item)];
+    if (item == q.tile0)
+      for (uint32_t i = threadIdx.x; i < q.groups; i += blockDim.x) {
+        q.cnt[i] = 0;
+        q.sum[i] = 0;
+      }
     if (!q.gen_seed) continue;
     const uint64_t r0 = (item - q.tile0) * T;
     const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
@@ -148,33 +139,100 @@ __global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d
   }
 }
 
+// One probe row: table lookup, then the group update.
+template <bool LDS_TAB, bool LDS_GRP>
+__device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long long *ltab,
+                                          uint32_t *gcnt, unsigned long long *gsum, int32_t k,
+                                          int32_t v, int32_t g, uint32_t &bad) {
+  uint64_t h = hash_i32(k) & q.mask;
+  int32_t b = 0;
+  for (;;) {
+    const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];
+    if ((int32_t)(s >> 32) == k) {
+      b = (int32_t)(uint32_t)s;
+      break;
+    }
+    if (s == kEmptySlot) return;
+    h = (h + 1) & q.mask;
+  }
+  if ((uint32_t)g >= q.groups) {
+    ++bad;
+    return;
+  }
+  const long long sv = (long long)b + (long long)v;
+  if (LDS_GRP) {
+    atomicAdd(&gcnt[g], 1u);
+    atomicAdd(&gsum[g], (unsigned long long)sv);
+  } else {
+    atomicAdd((unsigned long long *)&q.cnt[g], 1ULL);
+    atomicAdd((unsigned long long *)&q.sum[g], (unsigned long long)sv);
+  }
+}
+
+// Rows [r0, r1) of one item: 8 rows per lane per step, 16-byte loads where
+// the columns allow.
+template <bool LDS_TAB, bool LDS_GRP>
+__device__ __forceinline__ void probe_rows(const QDesc &q, uint64_t r0, uint64_t r1,
+                                           const unsigned long long *ltab, uint32_t *gcnt,
+                                           unsigned long long *gsum, uint32_t &bad) {
+  const int tid = threadIdx.x;
+  const bool vec = ((((uintptr_t)q.fk | (uintptr_t)q.val | (uintptr_t)q.grp) & 15) == 0) && (r0 % 4 == 0);
+  uint64_t r = r0;
+  if (vec) {
+    const uint64_t full = r0 + (r1 - r0) / 8 * 8;
+    for (uint64_t base = r0 + (uint64_t)tid * 8; base < full; base += (uint64_t)kBatchBlock * 8) {
+      int4 k[2], v[2], g[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        k[h] = ld_nt4(q.fk + base + 4 * h);
+        v[h] = ld_nt4(q.val + base + 4 * h);
+        g[h] = ld_nt4(q.grp + base + 4 * h);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].x, v[h].x, g[h].x, bad);
+        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].y, v[h].y, g[h].y, bad);
+        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].z, v[h].z, g[h].z, bad);
+        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].w, v[h].w, g[h].w, bad);
+      }
+    }
+    r = full;
+  }
+  for (uint64_t i = r + tid; i < r1; i += kBatchBlock)
+    probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, q.fk[i], q.val[i], q.grp[i], bad);
+}
+
+// Persistent probe over items [first, last): LDS_TAB copies the query's
+// table into LDS (dimensions of <= kLdsDim rows); otherwise the table is
+// read from HBM / Infinity Cache and the LDS holds only group partials.
+template <bool LDS_TAB>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, const QDesc *d) {
-  __shared__ unsigned long long ltab[kLdsSlots];
+  __shared__ unsigned long long ltab[LDS_TAB ? kLdsSlots : 1];
   __shared__ uint32_t gcnt[kLdsGroups];
   __shared__ unsigned long long gsum[kLdsGroups];
   __shared__ uint64_t s_item;
   __shared__ uint32_t s_bad;
   const uint32_t nq = hdr->nq;
-  const uint64_t total = hdr->total_tiles;
+  const uint64_t first = LDS_TAB ? 0 : hdr->lds_tiles;
+  const uint64_t last = LDS_TAB ? hdr->lds_tiles : hdr->total_tiles;
+  uint32_t *ctr = LDS_TAB ? &hdr->probe_ctr_lds : &hdr->probe_ctr_hbm;
   const uint32_t T = hdr->tile_rows;
   const int tid = threadIdx.x;
   uint32_t last_q = 0xffffffffu;
   for (;;) {
     if (tid == 0) {
-      s_item = atomicAdd(&hdr->probe_ctr, 1u);
+      s_item = first + atomicAdd(ctr, 1u);
       s_bad = 0;
     }
     __syncthreads();
     const uint64_t item = s_item;
-    if (item >= total) break;
+    if (item >= last) break;
     const uint32_t qi = find_query(d, nq, item);
     const QDesc &q = d[qi];
-    const bool lds_tab = q.dn <= kLdsDim;
     const bool lds_grp = q.groups <= kLdsGroups;
-    const uint64_t mask = q.mask;
-    if (lds_tab && qi != last_q)
-      for (uint32_t i = tid; i <= (uint32_t)mask; i += kBatchBlock) ltab[i] = q.table[i];
-    last_q = lds_tab ? qi : 0xffffffffu;
+    if (LDS_TAB && qi != last_q)
+      for (uint32_t i = tid; i <= (uint32_t)q.mask; i += kBatchBlock) ltab[i] = q.table[i];
+    last_q = qi;
     if (lds_grp)
       for (uint32_t i = tid; i < q.groups; i += kBatchBlock) {
         gcnt[i] = 0;
@@ -183,54 +241,15 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, cons
     __syncthreads();
     const uint64_t r0 = (item - q.tile0) * T;
     const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
-    const uint32_t G = q.groups;
     uint32_t bad = 0;
-    for (uint64_t base = r0 + (uint64_t)tid * 4; base < r1; base += (uint64_t)kBatchBlock * 4) {
-      int32_t k[4], v[4], g[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint64_t r = base + u;
-        const bool live = r < r1;
-        k[u] = live ? q.fk[r] : INT32_MIN;
-        v[u] = live ? q.val[r] : 0;
-        g[u] = live ? q.grp[r] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (k[u] == INT32_MIN) continue;
-        uint64_t h = hash_i32(k[u]) & mask;
-        int32_t b = 0;
-        bool hit = false;
-        for (;;) {
-          const unsigned long long s = lds_tab ? ltab[h] : q.table[h];
-          const int32_t sk = (int32_t)(s >> 32);
-          if (sk == k[u]) {
-            hit = true;
-            b = (int32_t)(uint32_t)s;
-            break;
-          }
-          if (s == kEmptySlot) break;
-          h = (h + 1) & mask;
-        }
-        if (!hit) continue;
-        if ((uint32_t)g[u] >= G) {
-          ++bad;
-          continue;
-        }
-        const long long sv = (long long)b + (long long)v[u];
-        if (lds_grp) {
-          atomicAdd(&gcnt[g[u]], 1u);
-          atomicAdd(&gsum[g[u]], (unsigned long long)sv);
-        } else {
-          atomicAdd((unsigned long long *)&q.cnt[g[u]], 1ULL);
-          atomicAdd((unsigned long long *)&q.sum[g[u]], (unsigned long long)sv);
-        }
-      }
+    if (r0 < r1) {
+      if (lds_grp) probe_rows<LDS_TAB, true>(q, r0, r1, ltab, gcnt, gsum, bad);
+      else probe_rows<LDS_TAB, false>(q, r0, r1, ltab, gcnt, gsum, bad);
     }
     if (bad) atomicAdd(&s_bad, bad);
     __syncthreads();
     if (lds_grp)
-      for (uint32_t i = tid; i < G; i += kBatchBlock)
+      for (uint32_t i = tid; i < q.groups; i += kBatchBlock)
         if (gcnt[i]) {
           atomicAdd((unsigned long long *)&q.cnt[i], (unsigned long long)gcnt[i]);
           atomicAdd((unsigned long long *)&q.sum[i], gsum[i]);
@@ -296,20 +315,30 @@ static uint32_t grid_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
   return (uint32_t)(ctx->num_cus * per);
 }
 
-// Writes header + descriptors into a host buffer; returns total tiles.
+// Writes header + descriptors into a host buffer.  Queries whose dimension
+// fits LDS come first (their items run in the LDS-table kernel); every query
+// owns at least one item (its first item clears its outputs).
 static int pack(const pj_query *qs, uint32_t nq, uint32_t T, void *h_buf) {
   BatchHdr *hdr = (BatchHdr *)h_buf;
   QDesc *d = (QDesc *)((char *)h_buf + 256);
-  uint64_t tiles = 0;
-  for (uint32_t i = 0; i < nq; ++i) {
-    PJ_TRY(fill_desc(qs[i], &d[i]));
-    d[i].tile0 = tiles;
-    tiles += ceil_div(qs[i].n, T);
-  }
+  uint64_t tiles = 0, lds_tiles = 0;
+  uint32_t j = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t i = 0; i < nq; ++i) {
+      if (!qs[i].dim) return fail(PJ_E_ARG, "query without a dimension");
+      const bool lds = qs[i].dim->n <= kLdsDim;
+      if (lds != (pass == 0)) continue;
+      PJ_TRY(fill_desc(qs[i], &d[j]));
+      d[j].tile0 = tiles;
+      tiles += std::max<uint64_t>(1, ceil_div(qs[i].n, T));
+      if (lds) lds_tiles = tiles;
+      ++j;
+    }
   hdr->nq = nq;
   hdr->tile_rows = T;
   hdr->total_tiles = tiles;
-  hdr->gen_ctr = hdr->probe_ctr = hdr->bad_group = 0;
+  hdr->lds_tiles = lds_tiles;
+  hdr->gen_ctr = hdr->probe_ctr_lds = hdr->probe_ctr_hbm = hdr->bad_group = 0;
   return PJ_OK;
 }
 
@@ -319,9 +348,9 @@ static int enqueue(pj_ctx *ctx, hipStream_t s, void *h_buf, void *d_buf, uint32_
   PJ_HIP(hipMemcpyAsync(d_buf, h_buf, bytes, hipMemcpyHostToDevice, s));
   BatchHdr *hdr = (BatchHdr *)d_buf;
   const QDesc *d = (const QDesc *)((char *)d_buf + 256);
-  hipLaunchKernelGGL(k_batch_clear, dim3(std::min<uint32_t>(grid, 1024)), dim3(256), 0, s, hdr, d);
   hipLaunchKernelGGL(k_batch_gen, dim3(grid * 2), dim3(256), 0, s, hdr, d);
-  hipLaunchKernelGGL(k_batch_probe, dim3(grid), dim3(kBatchBlock), 0, s, hdr, d);
+  hipLaunchKernelGGL(k_batch_probe<true>, dim3(grid), dim3(kBatchBlock), 0, s, hdr, d);
+  hipLaunchKernelGGL(k_batch_probe<false>, dim3(grid * 2), dim3(kBatchBlock), 0, s, hdr, d);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PJ_E_HIP, "batch launch: %s", hipGetErrorString(e));
   return PJ_OK;

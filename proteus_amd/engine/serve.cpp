@@ -15,8 +15,7 @@ using Clock = std::chrono::steady_clock;
 
 struct InFlight {
   std::vector<uint32_t> qs;
-  std::vector<void *> bufs;
-  uint64_t bytes = 0;
+  uint64_t bytes = 0;  // used bytes of the slot's arena
   bool busy = false;
 };
 
@@ -33,53 +32,76 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   for (uint32_t i = 1; i < nq; ++i)
     if (arrival_s[i] < arrival_s[i - 1]) return pj_fail(PJ_E_ARG, "arrivals must be sorted");
   const uint64_t budget = cfg->hbm_budget ? cfg->hbm_budget : ~0ULL;
-  for (uint32_t i = 0; i < nq; ++i)
-    if (input_bytes(qs[i]) > budget)
-      return pj_fail(PJ_E_CAPACITY, "query %u needs %llu bytes of inputs, budget %llu", i,
-                     (unsigned long long)input_bytes(qs[i]), (unsigned long long)budget);
   pj_serve_stats st{};
   const auto t0 = Clock::now();
   auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
-  auto alloc_inputs = [&](pj_query &q, std::vector<void *> &bufs) -> int {
-    if (!input_bytes(q)) return PJ_OK;
-    void *p = nullptr;
-    int rc = pj_malloc(ctx, q.n * 12, &p);
-    if (rc != PJ_OK) return rc;
-    bufs.push_back(p);
-    q.d_fk = (const int32_t *)p;
-    q.d_val = (const int32_t *)((char *)p + q.n * 4);
-    q.d_grp = (const int32_t *)((char *)p + q.n * 8);
-    return PJ_OK;
+  // Probe inputs are carved from preallocated arenas (one per slot), so
+  // admission never calls the allocator: 16-byte aligned columns.
+  auto place = [&](pj_query &q, char *base) {
+    const uint64_t c = (q.n * 4 + 15) & ~15ULL;
+    q.d_fk = (const int32_t *)base;
+    q.d_val = (const int32_t *)(base + c);
+    q.d_grp = (const int32_t *)(base + 2 * c);
   };
-  auto release = [&](std::vector<void *> &bufs, const std::vector<uint32_t> &ids) {
-    for (void *p : bufs) pj_free(ctx, p);
-    bufs.clear();
+  auto padded = [&](const pj_query &q) -> uint64_t {
+    return input_bytes(q) ? 3 * ((q.n * 4 + 15) & ~15ULL) : 0;
+  };
+  auto release = [&](const std::vector<uint32_t> &ids) {
     for (uint32_t i : ids)
       if (qs[i].gen_seed) qs[i].d_fk = qs[i].d_val = qs[i].d_grp = nullptr;
   };
+  uint64_t arena_bytes = 0;
+  for (uint32_t i = 0; i < nq; ++i) arena_bytes = std::max(arena_bytes, padded(qs[i]));
+  if (arena_bytes > budget)
+    return pj_fail(PJ_E_CAPACITY, "a query needs %llu bytes of inputs, budget %llu",
+                   (unsigned long long)arena_bytes, (unsigned long long)budget);
 
   if (cfg->mode == 1) {
+    void *arena = nullptr;
+    if (arena_bytes) {
+      int rc = pj_malloc(ctx, arena_bytes, &arena);
+      if (rc != PJ_OK) return rc;
+    }
     for (uint32_t i = 0; i < nq; ++i) {
       double wait = arrival_s[i] - now();
       if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-      std::vector<void *> bufs;
-      int rc = alloc_inputs(qs[i], bufs);
-      if (rc == PJ_OK) rc = pj_query_run(ctx, &qs[i], &cfg->batch);
-      release(bufs, {i});
-      if (rc != PJ_OK) return rc;
+      if (input_bytes(qs[i])) place(qs[i], (char *)arena);
+      int rc = pj_query_run(ctx, &qs[i], &cfg->batch);
+      release({i});
+      if (rc != PJ_OK) {
+        if (arena) pj_free(ctx, arena);
+        return rc;
+      }
       done_s[i] = now();
       st.batches++;
       st.max_in_flight_bytes = std::max(st.max_in_flight_bytes, input_bytes(qs[i]));
     }
     st.wall_s = now();
+    if (arena) pj_free(ctx, arena);
     if (stats) *stats = st;
     return PJ_OK;
   }
 
-  const int nslots = cfg->slots > 0 ? cfg->slots : 2;
+  // the budget splits evenly over the slots; every slot's arena must hold
+  // the largest query, which bounds the number of slots
+  int nslots = cfg->slots > 0 ? std::min(cfg->slots, 8) : 2;
+  if (cfg->hbm_budget && arena_bytes)
+    nslots = (int)std::max<uint64_t>(1, std::min<uint64_t>(nslots, budget / arena_bytes));
+  const uint64_t slot_bytes = cfg->hbm_budget ? budget / nslots : arena_bytes;
   const uint32_t max_batch = cfg->max_batch ? std::min<uint32_t>(cfg->max_batch, 4096) : 4096;
   int rc = pj_batch_slots(ctx, nslots);
   if (rc != PJ_OK) return rc;
+  std::vector<void *> arenas(nslots, nullptr);
+  auto free_arenas = [&]() {
+    for (void *a : arenas)
+      if (a) pj_free(ctx, a);
+  };
+  if (arena_bytes)
+    for (int s = 0; s < nslots; ++s)
+      if ((rc = pj_malloc(ctx, slot_bytes, &arenas[s])) != PJ_OK) {
+        free_arenas();
+        return rc;
+      }
   std::vector<InFlight> fl(nslots);
   std::deque<uint32_t> waiting;
   uint32_t next = 0, completed = 0;
@@ -99,7 +121,7 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
       for (uint32_t i : fl[s].qs) done_s[i] = tc;
       completed += (uint32_t)fl[s].qs.size();
       in_flight -= fl[s].bytes;
-      release(fl[s].bufs, fl[s].qs);
+      release(fl[s].qs);
       fl[s] = InFlight();
       progress = true;
     }
@@ -110,16 +132,15 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
       batch.clear();
       while (!waiting.empty() && batch.size() < max_batch) {
         pj_query &q = qs[waiting.front()];
-        const uint64_t b = input_bytes(q);
-        if (in_flight + f.bytes + b > budget) break;
-        if ((err = alloc_inputs(q, f.bufs)) != PJ_OK) break;
+        const uint64_t b = padded(q);
+        if (f.bytes + b > slot_bytes) break;
+        if (b) place(q, (char *)arenas[s] + f.bytes);
         f.bytes += b;
         f.qs.push_back(waiting.front());
         batch.push_back(q);
         waiting.pop_front();
       }
-      if (err != PJ_OK) break;
-      if (batch.empty()) break;  // budget full: wait for a completion
+      if (batch.empty()) break;
       if ((err = pj_batch_launch(ctx, s, batch.data(), (uint32_t)batch.size(), &cfg->batch)) != PJ_OK)
         break;
       f.busy = true;
@@ -144,8 +165,9 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   for (int s = 0; s < nslots; ++s)
     if (fl[s].busy) {
       pj_batch_wait(ctx, s);
-      release(fl[s].bufs, fl[s].qs);
+      release(fl[s].qs);
     }
+  free_arenas();
   if (err != PJ_OK) return err;
   st.wall_s = now();
   if (stats) *stats = st;
