@@ -33,6 +33,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary measurements (Zipf fk variant)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched / served query mix")
+    ap.add_argument("--batched-queries", type=int, default=4096)
+    ap.add_argument("--qps", type=float, default=200.0,
+                    help="offered load of the Poisson arrivals for the latency run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB once with the library's copy "
                          "kernel (the known-size launch that scales rocprofv3 FETCH_SIZE / "
@@ -132,10 +137,53 @@ def main():
         zsec, _, zres = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
         line["variants"] = {"zipf_theta1": {"ms_per_step": zsec * 1e3,
                                             "value": NS / zsec / 1e9, "result": zres}}
+    if not a.no_batched:
+        for x in (rk, rv, sk, sv):
+            x.free()
+        line["batched"] = batched_mix(ctx, a.batched_queries, a.qps)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.variant)
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def batched_mix(ctx, nq, qps):
+    """The served mix: nq seeded join -> group-by queries over resident
+    dimensions (build 2^10..2^24 log-uniform, probe 8-64x build, 2^4..2^16
+    groups).  Closed loop (all queries waiting at t=0) for queries/s of the
+    batched path and of the one-launch-per-query path, results compared
+    bit for bit; then Poisson arrivals at `qps` for p50/p99 latency
+    (arrival -> result) of both paths.  Probe inputs are generated on the
+    device inside each query's service time."""
+    import numpy as np
+    from proteus_amd import workload
+    budget = 48 << 30
+    out = {"queries": nq, "offered_qps": qps, "hbm_budget_bytes": budget}
+    mix = workload.Mix(ctx, nq, seed=SEED, qps=qps)
+    out["probe_rows"] = mix.probe_rows()
+    zeros = np.zeros(nq)
+    res = {}
+    for mode, name in ((0, "batched"), (1, "plain")):
+        done, st = ctx.serve_run(mix.queries(), zeros, mode=mode, slots=3, hbm_budget=budget)
+        res[mode] = mix.results()
+        out[name + "_qps"] = nq / st["wall_s"]
+        out[name + "_probe_rows_per_s"] = mix.probe_rows() / st["wall_s"]
+        if mode == 0:
+            out["batches"] = st["batches"]
+    out["speedup"] = out["batched_qps"] / out["plain_qps"]
+    out["identical"] = all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+                           for x, y in zip(res[0], res[1]))
+    arr = np.asarray(mix.arrivals)
+    for mode, name in ((0, "batched"), (1, "plain")):
+        done, st = ctx.serve_run(mix.queries(), arr, mode=mode, slots=3, hbm_budget=budget)
+        lat = (np.asarray(done) - arr) * 1e3
+        out[name + "_p50_ms"] = float(np.percentile(lat, 50))
+        out[name + "_p99_ms"] = float(np.percentile(lat, 99))
+        got = mix.results()
+        out["identical"] = out["identical"] and all(
+            np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(got, res[1]))
+    mix.close()
+    return out
 
 
 def timed(ctx, steps, fn):

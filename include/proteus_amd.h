@@ -238,16 +238,19 @@ int pj_batch_wait(pj_ctx *ctx, int slot);
 /* Serving loop: queries arrive at arrival_s[i] seconds after the call
  * starts; mode 0 admits every waiting query whose probe inputs fit the HBM
  * budget into the next free slot (continuous batching: batches of later
- * arrivals overlap the ones already on the device); mode 1 runs one query
- * at a time with pj_query_run in arrival order.  Probe inputs of queries
- * with gen_seed != 0 and d_fk == NULL are allocated at admission and freed
- * at completion.  done_s[i] = completion time of query i (seconds after
- * the start). */
+ * arrivals overlap the ones already on the device; with >= 2 slots, slot 0
+ * takes only light queries so they never wait behind a heavy batch); mode
+ * 1 runs one query at a time with pj_query_run in arrival order.  Probe
+ * inputs of queries with gen_seed != 0 and d_fk == NULL are placed in
+ * arenas allocated once per call.  done_s[i] = completion time of query i
+ * (seconds after the start). */
 typedef struct {
   int mode;                /* 0 batched, 1 one query per launch              */
   int slots;               /* streams in flight (mode 0), 1..8               */
   uint32_t max_batch;      /* queries per batch (<= 4096)                    */
   uint64_t hbm_budget;     /* bytes of probe inputs in flight (0: no limit)  */
+  uint64_t heavy_rows;     /* with >= 2 slots, slot 0 serves only queries of */
+                           /* fewer probe rows (0: 2^22)                     */
   pj_batch_cfg batch;
 } pj_serve_cfg;
 typedef struct {

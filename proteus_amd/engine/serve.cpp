@@ -82,12 +82,34 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
     return PJ_OK;
   }
 
-  // the budget splits evenly over the slots; every slot's arena must hold
-  // the largest query, which bounds the number of slots
+  // Size-class lanes: slot 0 serves only light queries (probe rows <
+  // heavy_rows), so they never queue behind a heavy batch and their kernels
+  // run beside it on the device; the other slots take queries in arrival
+  // order.  Every slot's arena must hold the largest query it may take.
   int nslots = cfg->slots > 0 ? std::min(cfg->slots, 8) : 2;
-  if (cfg->hbm_budget && arena_bytes)
-    nslots = (int)std::max<uint64_t>(1, std::min<uint64_t>(nslots, budget / arena_bytes));
-  const uint64_t slot_bytes = cfg->hbm_budget ? budget / nslots : arena_bytes;
+  const uint64_t heavy_rows = cfg->heavy_rows ? cfg->heavy_rows : (1ULL << 22);
+  uint64_t light_max = 0, any_max = arena_bytes;
+  for (uint32_t i = 0; i < nq; ++i)
+    if (qs[i].n < heavy_rows) light_max = std::max(light_max, padded(qs[i]));
+  std::vector<uint64_t> slot_bytes;
+  if (nslots >= 2) {
+    uint64_t light_arena = cfg->hbm_budget ? std::max(light_max, budget / (4 * (uint64_t)nslots)) : light_max;
+    uint64_t rest = cfg->hbm_budget ? budget - std::min(budget, light_arena) : 0;
+    int heavy = nslots - 1;
+    if (cfg->hbm_budget && any_max)
+      heavy = (int)std::min<uint64_t>(heavy, rest / any_max);
+    if (heavy < 1) {  // budget too small for lanes: one general slot
+      nslots = 1;
+      slot_bytes.push_back(cfg->hbm_budget ? budget : any_max);
+    } else {
+      nslots = heavy + 1;
+      slot_bytes.push_back(light_arena);
+      for (int i = 0; i < heavy; ++i) slot_bytes.push_back(cfg->hbm_budget ? rest / heavy : any_max);
+    }
+  } else {
+    slot_bytes.push_back(cfg->hbm_budget ? budget : any_max);
+  }
+  const bool lanes = nslots >= 2;
   const uint32_t max_batch = cfg->max_batch ? std::min<uint32_t>(cfg->max_batch, 4096) : 4096;
   int rc = pj_batch_slots(ctx, nslots);
   if (rc != PJ_OK) return rc;
@@ -98,20 +120,24 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   };
   if (arena_bytes)
     for (int s = 0; s < nslots; ++s)
-      if ((rc = pj_malloc(ctx, slot_bytes, &arenas[s])) != PJ_OK) {
+      if (slot_bytes[s] && (rc = pj_malloc(ctx, slot_bytes[s], &arenas[s])) != PJ_OK) {
         free_arenas();
         return rc;
       }
   std::vector<InFlight> fl(nslots);
-  std::deque<uint32_t> waiting;
+  std::deque<uint32_t> light, heavy;  // waiting, arrival order each
   uint32_t next = 0, completed = 0;
   uint64_t in_flight = 0;
   std::vector<pj_query> batch;
   int err = PJ_OK;
+  auto is_light = [&](uint32_t i) { return lanes && qs[i].n < heavy_rows; };
   while (completed < nq && err == PJ_OK) {
     bool progress = false;
     const double t = now();
-    while (next < nq && arrival_s[next] <= t) waiting.push_back(next++);
+    while (next < nq && arrival_s[next] <= t) {
+      (is_light(next) ? light : heavy).push_back(next);
+      ++next;
+    }
     for (int s = 0; s < nslots; ++s) {
       if (!fl[s].busy) continue;
       int done = 0;
@@ -126,21 +152,32 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
       progress = true;
     }
     if (err != PJ_OK) break;
-    for (int s = 0; s < nslots && !waiting.empty(); ++s) {
+    for (int s = 0; s < nslots; ++s) {
       if (fl[s].busy) continue;
       InFlight &f = fl[s];
       batch.clear();
-      while (!waiting.empty() && batch.size() < max_batch) {
-        pj_query &q = qs[waiting.front()];
+      while (batch.size() < max_batch) {
+        // slot 0 (with lanes) takes light queries only; the others take the
+        // earliest waiting query of either class
+        std::deque<uint32_t> *src = nullptr;
+        if (lanes && s == 0) {
+          src = light.empty() ? nullptr : &light;
+        } else if (!light.empty() && (heavy.empty() || light.front() < heavy.front())) {
+          src = &light;
+        } else if (!heavy.empty()) {
+          src = &heavy;
+        }
+        if (!src) break;
+        pj_query &q = qs[src->front()];
         const uint64_t b = padded(q);
-        if (f.bytes + b > slot_bytes) break;
+        if (f.bytes + b > slot_bytes[s]) break;
         if (b) place(q, (char *)arenas[s] + f.bytes);
         f.bytes += b;
-        f.qs.push_back(waiting.front());
+        f.qs.push_back(src->front());
         batch.push_back(q);
-        waiting.pop_front();
+        src->pop_front();
       }
-      if (batch.empty()) break;
+      if (batch.empty()) continue;
       if ((err = pj_batch_launch(ctx, s, batch.data(), (uint32_t)batch.size(), &cfg->batch)) != PJ_OK)
         break;
       f.busy = true;
@@ -153,7 +190,7 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
     if (!progress) {
       bool any_busy = false;
       for (auto &f : fl) any_busy |= f.busy;
-      if (!any_busy && waiting.empty() && next < nq) {
+      if (!any_busy && light.empty() && heavy.empty() && next < nq) {
         double wait = arrival_s[next] - now();
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
       } else {
