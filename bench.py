@@ -137,6 +137,8 @@ def main():
         zsec, _, zres = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
         line["variants"] = {"zipf_theta1": {"ms_per_step": zsec * 1e3,
                                             "value": NS / zsec / 1e9, "result": zres}}
+    if not a.no_extra:
+        line["groupby"] = groupby_c3(ctx, a.steps)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
             x.free()
@@ -145,6 +147,60 @@ def main():
         line["cpu_baseline"] = cpu_baseline(a.variant)
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def groupby_c3(ctx, steps):
+    """Hash group-by of 2^29 rows (int32 key, int64 v uniform [0, 2^20),
+    float32 w = u/2^24): COUNT, SUM, MIN, MAX of v and FP64 SUM of w per
+    group, for 2^10 / 2^16 / 2^22 groups, uniform and Zipf(0.8) keys.
+    Algorithmic bytes: 16 per row read + 44 per group written.  Checked by
+    conservation: the counts sum to the row count and the SUMs to SUM(v)."""
+    import ctypes as C
+    import numpy as np
+    import proteus_amd as pa
+    lib = pa.lib()
+    n = 1 << 29
+    v = ctx.gen_uniform_i64(n, SEED, 5, 0, 1 << 20)
+    w = ctx.gen_unit_f32(n, SEED, 6)
+    vsum = ctx.reduce_i64(v)[0]
+    out = {"rows": n, "configs": []}
+    for g in (1 << 10, 1 << 16, 1 << 22):
+        outs = [ctx.empty(g, t) for t in (np.int32, np.int64, np.int64, np.int64, np.int64,
+                                          np.float64)]
+        for dist in ("uniform", "zipf0.8"):
+            if dist == "uniform":
+                k = ctx.gen_uniform_i32(n, SEED, 4, 0, g)
+            else:
+                k = ctx.gen_zipf_i32(n, SEED, 4, g, 0.8, SEED)
+            ng = C.c_uint64()
+            hb = max(1, (g - 1).bit_length())
+
+            def run():
+                pa._check(lib.pj_groupby_i32(ctx.h, k.ptr, v.ptr, w.ptr, n, hb, g, outs[0].ptr,
+                                             outs[1].ptr, outs[2].ptr, outs[3].ptr, outs[4].ptr,
+                                             outs[5].ptr, C.byref(ng)))
+            run()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                run()
+            ctx.sync()
+            sec = (time.perf_counter() - t0) / steps
+            m = ng.value
+            cnt = outs[1].numpy()[:m]
+            sums = outs[2].numpy()[:m]
+            ok = int(cnt.sum()) == n and int(sums.sum()) == vsum
+            gbs = (16.0 * n + 44.0 * m) / sec / 1e9
+            out["configs"].append({"groups": g, "keys": dist, "ms": sec * 1e3,
+                                   "rows_per_s": n / sec, "groups_out": m,
+                                   "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                                   "conservation_ok": ok})
+            k.free()
+        for o in outs:
+            o.free()
+    v.free()
+    w.free()
+    return out
 
 
 def batched_mix(ctx, nq, qps):
