@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "pj_internal.hpp"
+#include "rel_api.hpp"
 
 struct pj_dim {
   uint64_t n = 0;
@@ -383,6 +384,13 @@ static int check_bad(const void *h_buf) {
 }
 
 }  // namespace pj
+
+int pj::rel::dim_table(const pj_dim *dim, PipeTable *t) {
+  if (!dim) return fail(PJ_E_ARG, "null dimension");
+  t->table = dim->table;
+  t->mask = dim->slots - 1;
+  return PJ_OK;
+}
 
 using namespace pj;
 

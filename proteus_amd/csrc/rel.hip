@@ -7,6 +7,8 @@
 // conflict-free).  This plays the role of the reference's expression code
 // generation (core/olap/lib/expressions/expressions-generator.cpp) without a
 // JIT.
+#include <cstring>
+
 #include "pj_internal.hpp"
 #include "rel_api.hpp"
 
@@ -507,6 +509,135 @@ __global__ void k_hash_values(const int32_t *k, uint64_t n, int64_t *out) {
     out[i] = (int64_t)hash_i32(k[i]);
 }
 
+// Fused probe pipeline ------------------------------------------------------
+struct PipeArgs {
+  const Instr *ins;
+  const void *const *cols;
+  const int32_t *ctype;
+  const int32_t *clevel;
+  void *const *outs;
+  const int32_t *out_t;
+  PipeTable tables[kPipeMaxLevels];
+  uint64_t n;
+  unsigned long long *cursor;
+  int32_t nins, nout, depth, nlev;
+};
+
+constexpr unsigned long long kPipeEmpty = 0x8000000000000000ULL;  // key INT32_MIN
+
+// One row per lane; the program is uniform, so every lane of a wave walks
+// the same instruction.  A wave whose lanes have all ended skips the rest
+// of the program.  Survivors take consecutive output positions from one
+// atomic per wave.
+__global__ __launch_bounds__(kVmBlock) void k_pipe(PipeArgs a) {
+  extern __shared__ int64_t sh[];
+  int64_t *st = sh + threadIdx.x;
+  int64_t *rid = sh + (size_t)a.depth * kVmBlock + threadIdx.x;
+  int64_t *ov = sh + (size_t)(a.depth + a.nlev) * kVmBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t base = (uint64_t)blockIdx.x * kVmBlock; base < a.n;
+       base += (uint64_t)gridDim.x * kVmBlock) {
+    const uint64_t row = base + threadIdx.x;
+    bool alive = row < a.n;
+    rid[0] = (int64_t)row;
+    int sp = 0;
+    for (int k = 0; k < a.nins; ++k) {
+      const Instr in = a.ins[k];
+      bool check = false;
+      switch (in.op) {
+        case OP_COL: {
+          int64_t v = 0;
+          if (alive) v = ld_slot(a.cols[in.a], a.ctype[in.a], (uint64_t)rid[a.clevel[in.a] * kVmBlock]);
+          st[sp++ * kVmBlock] = v;
+          break;
+        }
+        case OP_CONST: st[sp++ * kVmBlock] = in.imm; break;
+        case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD: {
+          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] = arith(in.op, in.t, x, b);
+          break;
+        }
+        case OP_NEG: {
+          int64_t x = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] =
+              in.t == F64 ? of_f(-as_f(x)) : (in.t == I32 ? wrap32(0 - x) : (int64_t)(0 - (uint64_t)x));
+          break;
+        }
+        case OP_EQ: case OP_NE: case OP_LT: case OP_LE: case OP_GT: case OP_GE: {
+          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] = cmp(in.op, in.t, x, b);
+          break;
+        }
+        case OP_AND: {
+          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] = (x != 0) & (b != 0);
+          break;
+        }
+        case OP_OR: {
+          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] = (x != 0) | (b != 0);
+          break;
+        }
+        case OP_NOT: st[(sp - 1) * kVmBlock] = st[(sp - 1) * kVmBlock] == 0; break;
+        case OP_CAST: st[(sp - 1) * kVmBlock] = cast(in.a, in.t, st[(sp - 1) * kVmBlock]); break;
+        case OP_IF: {
+          int64_t e = st[--sp * kVmBlock], t = st[--sp * kVmBlock], c = st[(sp - 1) * kVmBlock];
+          st[(sp - 1) * kVmBlock] = c ? t : e;
+          break;
+        }
+        case OP_MIN: case OP_MAX: {
+          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
+          bool lt = in.t == F64 ? as_f(x) < as_f(b) : x < b;
+          st[(sp - 1) * kVmBlock] = (in.op == OP_MIN) == lt ? x : b;
+          break;
+        }
+        case OP_PROBE: {
+          const int32_t key = (int32_t)st[--sp * kVmBlock];
+          if (alive) {
+            const PipeTable tb = a.tables[in.a];
+            uint64_t h = hash_i32(key) & tb.mask;
+            bool hit = false;
+            for (;;) {
+              const unsigned long long s = tb.table[h];
+              if ((int32_t)(s >> 32) == key && s != kPipeEmpty) {
+                rid[(in.a + 1) * kVmBlock] = (int64_t)(uint32_t)s;
+                hit = true;
+                break;
+              }
+              if (s == kPipeEmpty) break;
+              h = (h + 1) & tb.mask;
+            }
+            alive = hit;
+          }
+          check = true;
+          break;
+        }
+        case OP_FILTER: {
+          const bool c = st[--sp * kVmBlock] != 0;
+          alive = alive && c;
+          check = true;
+          break;
+        }
+        case OP_OUT: ov[in.a * kVmBlock] = st[--sp * kVmBlock]; break;
+        default: break;
+      }
+      if (check && __ballot(alive) == 0) break;
+    }
+    const unsigned long long m = __ballot(alive);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      const unsigned long long below = lane ? (m & ((1ULL << lane) - 1)) : 0ULL;
+      unsigned long long b0 = 0;
+      if (lane == leader) b0 = atomicAdd(a.cursor, (unsigned long long)__popcll(m));
+      b0 = __shfl(b0, leader, 64);
+      if (alive) {
+        const uint64_t pos = b0 + __popcll(below);
+        for (int j = 0; j < a.nout; ++j) st_slot(a.outs[j], a.out_t[j], pos, ov[j * kVmBlock]);
+      }
+    }
+  }
+}
+
 // Host side -----------------------------------------------------------------
 static uint64_t grid_for(pj_ctx *ctx, uint64_t n, int block) {
   uint64_t g = ceil_div(n, (uint64_t)block);
@@ -668,6 +799,65 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
               slots.as<const uint32_t>(), occ.as<const uint32_t>(), ng, acc.as<const int64_t>(), S, out);
   PJ_TRY(sync(ctx));
   *n_groups = ng;
+  return PJ_OK;
+}
+
+int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint64_t *n_out) {
+  *n_out = 0;
+  const int nlev = (int)p.tables.size() + 1;
+  if (nlev > kPipeMaxLevels) return fail(PJ_E_UNSUPPORTED, "pipeline of more than %d joins", kPipeMaxLevels - 1);
+  if ((int)p.out_t.size() > kPipeMaxOut) return fail(PJ_E_UNSUPPORTED, "pipeline with more than %d outputs", kPipeMaxOut);
+  if (p.depth < 1 || p.depth > kMaxDepth || p.ins.empty()) return fail(PJ_E_ARG, "bad pipeline program");
+  for (auto &i : p.ins) {
+    if (i.op == OP_COL && (i.a < 0 || i.a >= (int)p.cols.size() || p.clevel[i.a] < 0 || p.clevel[i.a] >= nlev))
+      return fail(PJ_E_ARG, "pipeline column out of range");
+    if (i.op == OP_PROBE && (i.a < 0 || i.a >= (int)p.tables.size())) return fail(PJ_E_ARG, "pipeline table out of range");
+    if (i.op == OP_OUT && (i.a < 0 || i.a >= (int)p.out_t.size())) return fail(PJ_E_ARG, "pipeline output out of range");
+  }
+  if (!n) return PJ_OK;
+  // program, column tables and output tables in one device block
+  const size_t ni = p.ins.size() * sizeof(Instr), nc = p.cols.size(), no = p.out_t.size();
+  const size_t bytes = ni + nc * 8 + nc * 4 + nc * 4 + no * 8 + no * 4 + 64;
+  std::vector<char> h(bytes, 0);
+  size_t off = 0;
+  auto put = [&](const void *src, size_t len) {
+    size_t at = off;
+    memcpy(h.data() + off, src, len);
+    off = (off + len + 7) & ~(size_t)7;
+    return at;
+  };
+  const size_t o_ins = put(p.ins.data(), ni);
+  const size_t o_cols = put(p.cols.data(), nc * 8);
+  const size_t o_ct = put(p.ctype.data(), nc * 4);
+  const size_t o_cl = put(p.clevel.data(), nc * 4);
+  const size_t o_outs = put(outs, no * 8);
+  const size_t o_ot = put(p.out_t.data(), no * 4);
+  Scratch dev, cur;
+  PJ_TRY(dev.alloc(ctx, off));
+  PJ_TRY(h2d(ctx, dev.p, h.data(), off));
+  PJ_TRY(cur.alloc(ctx, 8));
+  PJ_TRY(zero(ctx, cur.p, 8));
+  PipeArgs a{};
+  char *d = (char *)dev.p;
+  a.ins = (const Instr *)(d + o_ins);
+  a.cols = (const void *const *)(d + o_cols);
+  a.ctype = (const int32_t *)(d + o_ct);
+  a.clevel = (const int32_t *)(d + o_cl);
+  a.outs = (void *const *)(d + o_outs);
+  a.out_t = (const int32_t *)(d + o_ot);
+  for (size_t j = 0; j < p.tables.size(); ++j) a.tables[j] = p.tables[j];
+  a.n = n;
+  a.cursor = cur.as<unsigned long long>();
+  a.nins = (int32_t)p.ins.size();
+  a.nout = (int32_t)no;
+  a.depth = p.depth;
+  a.nlev = nlev;
+  const size_t lds = (size_t)(p.depth + nlev + no) * kVmBlock * 8;
+  if (lds > 64 * 1024) return fail(PJ_E_UNSUPPORTED, "pipeline needs %zu bytes of LDS", lds);
+  PJ_LAUNCH(ctx, "rel_pipeline", k_pipe, grid_for(ctx, n, kVmBlock), kVmBlock, lds, a);
+  unsigned long long m = 0;
+  PJ_TRY(d2h(ctx, &m, cur.p, 8));
+  *n_out = m;
   return PJ_OK;
 }
 

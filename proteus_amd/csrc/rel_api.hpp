@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/proteus_amd.h"
 
@@ -30,6 +31,11 @@ enum OpCode : int32_t {
   OP_CAST,   // a = from type, t = to type
   OP_IF,     // pops else, then, cond
   OP_MIN, OP_MAX,  // t = operand type
+  // pipeline programs only
+  OP_PROBE,   // pops an int32 key, looks it up in table a; a miss ends the row,
+              // a hit sets the row id of level a + 1
+  OP_FILTER,  // pops a bool; false ends the row
+  OP_OUT,     // pops into output column a
 };
 
 struct Instr {
@@ -79,6 +85,29 @@ struct GbOut {
 };
 int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, uint64_t n,
             uint64_t max_groups, const GbOut &out, uint64_t *n_groups);
+
+// Fused probe pipeline.  Columns have a level: 0 = the source row,
+// j + 1 = the build row matched by table j.  Tables are resident unique-key
+// dimension tables whose value is the build row id.  Surviving rows are
+// compacted (order unspecified) into the output columns, which must hold n
+// rows.
+struct PipeTable {
+  const unsigned long long *table;
+  uint64_t mask;
+};
+constexpr int kPipeMaxLevels = 8;  // source + 7 joins
+constexpr int kPipeMaxOut = 24;
+struct PipeProg {
+  std::vector<Instr> ins;
+  std::vector<const void *> cols;
+  std::vector<int32_t> ctype, clevel;
+  std::vector<PipeTable> tables;
+  std::vector<int32_t> out_t;
+  int32_t depth = 0;
+};
+int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint64_t *n_out);
+// The table of a dimension built with pj_dim_create.
+int dim_table(const pj_dim *dim, PipeTable *t);
 
 // bucket[i] = hashInt32(k[i]) mod buckets (the reference's hash-rearrange
 // target, gpu-hash-rearrange.cpp).

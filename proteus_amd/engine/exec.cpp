@@ -16,6 +16,7 @@
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -165,7 +166,12 @@ static std::string fmt_double(double v) {
 
 class Exec {
  public:
-  Exec(pj_ctx *ctx, std::string base) : ctx_(ctx), base_(std::move(base)) {}
+  // PROTEUS_AMD_NO_PIPELINE=1 runs every join operator-at-a-time (the
+  // fused probe pipelines off), for cross-checking the two executions.
+  Exec(pj_ctx *ctx, std::string base) : ctx_(ctx), base_(std::move(base)) {
+    const char *e = getenv("PROTEUS_AMD_NO_PIPELINE");
+    no_pipeline_ = e && *e && strcmp(e, "0") != 0;
+  }
 
   std::string run(const Json &plan) {
     Need top;
@@ -482,12 +488,18 @@ class Exec {
     const std::string &k = op["operator"].as_string();
     if (is_pass_through(k)) return exec(op["input"], need);
     if (k == "scan") return scan(op);
-    if (k == "select") return select(op, need);
-    if (k == "project") return project(op, need);
+    if (k == "select" || k == "project") {
+      Rel r;
+      if (!no_pipeline_ && pipeline(op, need, &r)) return r;
+      return k == "select" ? select(op, need) : project(op, need);
+    }
     if (k == "reduce") return reduce(op, need);
     if (k == "groupby" || k == "hashgroupby-chained") return groupby(op, need);
-    if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join")
+    if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join") {
+      Rel r;
+      if (!no_pipeline_ && pipeline(op, need, &r)) return r;
       return join(op, need);
+    }
     if (k == "hash-rearrange" || k == "hash-pack") return rearrange(op, need);
     if (k == "sort") return sort(op, need);
     if (k == "print") return print(op, need);
@@ -1102,6 +1114,373 @@ class Exec {
     for (auto &c : pr.cols) out.add(c.first, c.second);
     out.n = m;
     return out;
+  }
+
+  // ---- fused probe pipelines -----------------------------------------
+  // A chain of joins along the probe side, with select / project /
+  // placement operators between them, runs as one kernel: the source rows
+  // stream once; each join probes a unique-key table of its build side
+  // (key -> build row id) and later expressions read build columns through
+  // the matched row ids; only the attributes read above the chain are
+  // written out.  Builds with duplicate keys (or the reserved key INT32_MIN)
+  // fall back to operator-at-a-time execution.
+  struct Binding {
+    int level = 0;
+    Col col;
+    const Json *expr = nullptr;  // project output: expression in `scope`
+    int scope = 0;
+    int type = R::I32;
+  };
+  struct Scope {
+    std::vector<std::pair<Attr, Binding>> attrs;
+    const Binding *find(const std::string &r, const std::string &a) const {
+      for (auto &x : attrs)
+        if (x.first.rel == r && x.first.name == a) return &x.second;
+      for (auto &x : attrs)
+        if (x.first.name == a) return &x.second;
+      return nullptr;
+    }
+  };
+
+  struct PipeCompiler {
+    Exec &ex;
+    std::vector<Scope> &scopes;
+    R::PipeProg p;
+    int sp = 0;
+    std::vector<std::pair<const void *, int>> bound;  // (column, level) -> index
+
+    const Binding &resolve(const Json &e, int sc) {
+      const Json &a = e["attribute"];
+      std::string rel, name;
+      if (a.is_object()) {
+        rel = a["relName"].is_string() ? a["relName"].as_string() : "";
+        name = a["attrName"].as_string();
+      } else if (a.is_string()) {
+        name = a.as_string();
+      } else {
+        raise(PJ_E_PARSE, "recordProjection without attribute");
+      }
+      const Binding *b = scopes[sc].find(rel, name);
+      if (!b) raise(PJ_E_PARSE, "attribute " + rel + "." + name + " not found in the pipeline");
+      return *b;
+    }
+
+    int infer(const Json &e, int sc) {
+      const std::string &k = e["expression"].as_string();
+      if (k == "recordProjection") {
+        const Binding &b = resolve(e, sc);
+        if (b.expr) return infer(*b.expr, b.scope);
+        return b.col.t == R::F32 ? R::F64 : b.col.t;
+      }
+      if (k == "int" || k == "dstring" || k == "string") return R::I32;
+      if (k == "int64" || k == "date" || k == "datetime") return R::I64;
+      if (k == "float") return R::F64;
+      if (k == "bool") return R::BOOL;
+      if (k == "cast") {
+        bool s;
+        return parse_col_type(e["type"], &s);
+      }
+      if (k == "add" || k == "sub" || k == "multiply" || k == "div" || k == "mod")
+        return promote(infer(e["left"], sc), infer(e["right"], sc));
+      if (k == "neg") return promote(infer(e["e"], sc), R::I32);
+      if (k == "eq" || k == "neq" || k == "lt" || k == "le" || k == "gt" || k == "ge" || k == "and" ||
+          k == "or")
+        return R::BOOL;
+      if (k == "if") return promote(infer(e["then"], sc), infer(e["else"], sc));
+      raise(PJ_E_UNSUPPORTED, "expression '" + k + "' is not supported on the device path");
+    }
+
+    void push(int op, int t, int a, int64_t imm) {
+      if (p.ins.size() >= 256) raise(PJ_E_UNSUPPORTED, "pipeline program longer than 256 instructions");
+      R::Instr i{op, t, a, 0, imm};
+      p.ins.push_back(i);
+      if (op == R::OP_COL || op == R::OP_CONST) {
+        if (++sp > p.depth) p.depth = sp;
+      } else if (op == R::OP_IF) {
+        sp -= 2;
+      } else if (op == R::OP_PROBE || op == R::OP_FILTER || op == R::OP_OUT) {
+        --sp;
+      } else if (op != R::OP_NEG && op != R::OP_NOT && op != R::OP_CAST) {
+        --sp;
+      }
+    }
+
+    int bind(const Col &c, int level) {
+      for (size_t i = 0; i < bound.size(); ++i)
+        if (bound[i].first == c.dev && bound[i].second == level) return (int)i;
+      bound.emplace_back(c.dev, level);
+      p.cols.push_back(c.dev);
+      p.ctype.push_back(c.t);
+      p.clevel.push_back(level);
+      return (int)bound.size() - 1;
+    }
+
+    const std::shared_ptr<Dict> *dict_of(const Json &e, int sc) {
+      if (e["expression"].as_string() != "recordProjection") return nullptr;
+      const Binding &b = resolve(e, sc);
+      if (b.expr) return dict_of(*b.expr, b.scope);
+      return b.col.dict ? &b.col.dict : nullptr;
+    }
+
+    void emit(const Json &e, int sc, int want, const Json *peer = nullptr) {
+      int t = emit_natural(e, sc, peer);
+      if (t != want) push(R::OP_CAST, want, t, 0);
+    }
+
+    int emit_natural(const Json &e, int sc, const Json *peer) {
+      const std::string &k = e["expression"].as_string();
+      if (k == "recordProjection") {
+        const Binding &b = resolve(e, sc);
+        if (b.expr) return emit_natural(*b.expr, b.scope, peer);
+        push(R::OP_COL, b.col.t, bind(b.col, b.level), 0);
+        return b.col.t == R::F32 ? R::F64 : b.col.t;
+      }
+      if (k == "int") {
+        push(R::OP_CONST, R::I32, 0, (int32_t)e["v"].as_int());
+        return R::I32;
+      }
+      if (k == "int64" || k == "date" || k == "datetime") {
+        push(R::OP_CONST, R::I64, 0, e["v"].as_int());
+        return R::I64;
+      }
+      if (k == "float") {
+        double d = e["v"].as_double();
+        int64_t v;
+        memcpy(&v, &d, 8);
+        push(R::OP_CONST, R::F64, 0, v);
+        return R::F64;
+      }
+      if (k == "bool") {
+        push(R::OP_CONST, R::BOOL, 0, e["v"].as_bool() ? 1 : 0);
+        return R::BOOL;
+      }
+      if (k == "dstring") {
+        int64_t code = e["v"].is_number() ? e["v"].as_int() : ex.dict_lookup(e["v"].as_string(), e["dict"]);
+        push(R::OP_CONST, R::I32, 0, code);
+        return R::I32;
+      }
+      if (k == "string") {
+        const std::shared_ptr<Dict> *d = peer ? dict_of(*peer, sc) : nullptr;
+        if (!d) raise(PJ_E_UNSUPPORTED, "string constant outside a comparison with a string column");
+        int64_t code = -1;
+        for (size_t i = 0; i < (*d)->size(); ++i)
+          if ((**d)[i] == e["v"].as_string()) code = (int64_t)i;
+        push(R::OP_CONST, R::I32, 0, code);
+        return R::I32;
+      }
+      if (k == "cast") {
+        bool s;
+        int to = parse_col_type(e["type"], &s);
+        int from = emit_natural(e["e"], sc, nullptr);
+        if (from != to) push(R::OP_CAST, to, from, 0);
+        return to;
+      }
+      static const struct { const char *n; int op; } bin[] = {
+          {"add", R::OP_ADD}, {"sub", R::OP_SUB}, {"multiply", R::OP_MUL},
+          {"div", R::OP_DIV}, {"mod", R::OP_MOD}};
+      for (auto &b : bin)
+        if (k == b.n) {
+          int t = promote(infer(e["left"], sc), infer(e["right"], sc));
+          emit(e["left"], sc, t);
+          emit(e["right"], sc, t);
+          push(b.op, t, 0, 0);
+          return t;
+        }
+      static const struct { const char *n; int op; } cmp[] = {
+          {"eq", R::OP_EQ}, {"neq", R::OP_NE}, {"lt", R::OP_LT},
+          {"le", R::OP_LE}, {"gt", R::OP_GT}, {"ge", R::OP_GE}};
+      for (auto &c : cmp)
+        if (k == c.n) {
+          const Json &l = e["left"], &r = e["right"];
+          int t = promote(infer(l, sc), infer(r, sc));
+          emit(l, sc, t, &r);
+          emit(r, sc, t, &l);
+          push(c.op, t, 0, 0);
+          return R::BOOL;
+        }
+      if (k == "and" || k == "or") {
+        emit(e["left"], sc, R::BOOL);
+        emit(e["right"], sc, R::BOOL);
+        push(k == "and" ? R::OP_AND : R::OP_OR, R::BOOL, 0, 0);
+        return R::BOOL;
+      }
+      if (k == "neg") {
+        int t = promote(infer(e["e"], sc), R::I32);
+        emit(e["e"], sc, t);
+        push(R::OP_NEG, t, 0, 0);
+        return t;
+      }
+      if (k == "if") {
+        int t = promote(infer(e["then"], sc), infer(e["else"], sc));
+        emit(e["cond"], sc, R::BOOL);
+        emit(e["then"], sc, t);
+        emit(e["else"], sc, t);
+        push(R::OP_IF, t, 0, 0);
+        return t;
+      }
+      raise(PJ_E_UNSUPPORTED, "expression '" + k + "' is not supported on the device path");
+    }
+  };
+
+  bool no_pipeline_ = false;
+
+  bool pipeline(const Json &top, const Need &need, Rel *out) {
+    // stages top -> bottom along the probe side
+    std::vector<const Json *> stages;
+    const Json *node = &top;
+    int joins = 0;
+    for (;;) {
+      const std::string k = (*node)["operator"].is_string() ? (*node)["operator"].as_string() : "";
+      if (is_pass_through(k)) {
+        node = &(*node)["input"];
+        continue;
+      }
+      if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join") {
+        stages.push_back(node);
+        ++joins;
+        node = join_sides(*node).probe_in;
+        continue;
+      }
+      if (k == "select" || k == "project") {
+        stages.push_back(node);
+        node = &(*node)["input"];
+        continue;
+      }
+      break;
+    }
+    if (!joins || joins >= R::kPipeMaxLevels) return false;
+    Need na = need;
+    for (const Json *st : stages) na = need_below(*st, na);
+    // execute the source and every build side
+    Rel src = exec(*node, na);
+    std::vector<Rel> builds;
+    std::vector<pj_dim *> dims;
+    auto drop = [&]() {
+      for (pj_dim *d : dims) pj_dim_destroy(ctx_, d);
+      dims.clear();
+    };
+    std::vector<Scope> scopes(1);
+    for (auto &c : src.cols) {
+      Binding b;
+      b.level = 0;
+      b.col = c.second;
+      scopes[0].attrs.emplace_back(c.first, b);
+    }
+    PipeCompiler pc{*this, scopes, R::PipeProg(), 0, {}};
+    int cur = 0;
+    bool ok = true;
+    for (auto it = stages.rbegin(); it != stages.rend() && ok; ++it) {
+      const Json &st = **it;
+      const std::string k = st["operator"].as_string();
+      if (k == "select") {
+        pc.emit(st["p"], cur, R::BOOL);
+        pc.push(R::OP_FILTER, R::BOOL, 0, 0);
+      } else if (k == "project") {
+        Scope ns;
+        const std::string rel = st["relName"].is_string() ? st["relName"].as_string() : "project";
+        int i = 0;
+        for (auto &e : st["e"].items()) {
+          Binding b;
+          b.expr = &e;
+          b.scope = cur;
+          ns.attrs.emplace_back(reg_attr(e, rel, "expr" + std::to_string(i++)), b);
+        }
+        scopes.push_back(ns);
+        cur = (int)scopes.size() - 1;
+      } else {
+        JoinSides js = join_sides(st);
+        Rel b = exec(*js.build_in, na);
+        Compiler kb(*this, b);
+        if (kb.infer(*js.build_k) != R::I32 || pc.infer(*js.probe_k, cur) != R::I32) {
+          ok = false;
+          break;
+        }
+        Col bk = eval_as(*js.build_k, b, R::I32);
+        Col ids;
+        ids.owner = alloc(std::max<uint64_t>(b.n, 1) * 4);
+        ids.dev = ids.owner->p;
+        ck(R::iota(ctx_, (uint32_t *)ids.owner->p, b.n));
+        pj_dim *dim = nullptr;
+        int rc = pj_dim_create(ctx_, (const int32_t *)bk.dev, (const int32_t *)ids.dev, b.n, &dim);
+        if (rc == PJ_E_ARG) {  // duplicate or reserved build keys
+          ok = false;
+          break;
+        }
+        ck(rc);
+        dims.push_back(dim);
+        R::PipeTable t;
+        ck(R::dim_table(dim, &t));
+        const int tj = (int)pc.p.tables.size();
+        pc.p.tables.push_back(t);
+        pc.emit(*js.probe_k, cur, R::I32);
+        pc.push(R::OP_PROBE, R::I32, tj, 0);
+        // the join's output = probe attributes + build attributes
+        Scope ns = scopes[cur];
+        for (auto &c : b.cols) {
+          Binding bb;
+          bb.level = tj + 1;
+          bb.col = c.second;
+          ns.attrs.emplace_back(c.first, bb);
+        }
+        scopes.push_back(ns);
+        cur = (int)scopes.size() - 1;
+        builds.push_back(std::move(b));
+      }
+    }
+    if (!ok) {
+      drop();
+      no_pipeline_ = true;
+      Rel r = exec(top, need);
+      no_pipeline_ = false;
+      *out = std::move(r);
+      return true;
+    }
+    // outputs: the attributes of the final scope read above the chain
+    std::vector<std::pair<Attr, Col>> outs;
+    std::vector<void *> optr;
+    int j = 0;
+    for (auto &a : scopes[cur].attrs) {
+      if (!need.has(a.first.name)) continue;
+      bool dup = false;
+      for (auto &o : outs) dup |= o.first.rel == a.first.rel && o.first.name == a.first.name;
+      if (dup) continue;
+      Json ref = Json::object();
+      Json attr = Json::object();
+      attr.set("relName", Json::string(a.first.rel));
+      attr.set("attrName", Json::string(a.first.name));
+      ref.set("expression", Json::string("recordProjection"));
+      ref.set("attribute", attr);
+      int t = pc.infer(ref, cur);
+      pc.emit(ref, cur, t);
+      pc.push(R::OP_OUT, t, j++, 0);
+      Col c;
+      c.t = t;
+      const std::shared_ptr<Dict> *d = pc.dict_of(ref, cur);
+      if (d) c.dict = *d;
+      c.owner = alloc(std::max<uint64_t>(src.n, 1) * R::col_size(t));
+      c.dev = c.owner->p;
+      optr.push_back(c.owner->p);
+      pc.p.out_t.push_back(t);
+      outs.emplace_back(a.first, c);
+    }
+    if (outs.empty()) {  // nothing read above (e.g. COUNT only): one dummy column
+      pc.push(R::OP_CONST, R::BOOL, 0, 1);
+      pc.push(R::OP_OUT, R::BOOL, 0, 0);
+      Col c;
+      c.t = R::BOOL;
+      c.owner = alloc(std::max<uint64_t>(src.n, 1));
+      c.dev = c.owner->p;
+      optr.push_back(c.owner->p);
+      pc.p.out_t.push_back(R::BOOL);
+      outs.emplace_back(Attr{"", "__row"}, c);
+    }
+    uint64_t m = 0;
+    int rc = R::pipeline(ctx_, pc.p, src.n, optr.data(), &m);
+    drop();
+    ck(rc);
+    out->n = m;
+    out->cols = std::move(outs);
+    return true;
   }
 
   // ---- hash-rearrange ------------------------------------------------
