@@ -139,6 +139,7 @@ def main():
                                             "value": NS / zsec / 1e9, "result": zres}}
     if not a.no_extra:
         line["groupby"] = groupby_c3(ctx, a.steps)
+        line["star"] = star_c4(ctx, a.steps)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
             x.free()
@@ -201,6 +202,39 @@ def groupby_c3(ctx, steps):
     v.free()
     w.free()
     return out
+
+
+def star_c4(ctx, steps):
+    """Star-schema plan (fact 6e8 rows x 7 int32 columns, four dimensions,
+    filters 1/7, 1/5, 1/25, ~800 groups of SUM(price*(100-discount)))
+    through pj_plan_execute, fused probe pipeline; the operator-at-a-time
+    execution of the same plan is run once as a cross-check.  Algorithmic
+    bytes: the 28 B of fact columns per row."""
+    import json as _json
+    from proteus_amd import star
+    n = 600_000_000
+    ctx.clear_catalog()
+    s = star.Star(ctx, n, SEED)
+    text = _json.dumps(s.plan())
+    ref = ctx.execute_plan(text)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = ctx.execute_plan(text)
+    sec = (time.perf_counter() - t0) / steps
+    os.environ["PROTEUS_AMD_NO_PIPELINE"] = "1"
+    try:
+        t1 = time.perf_counter()
+        mat = ctx.execute_plan(text)
+        sec_mat = time.perf_counter() - t1
+    finally:
+        del os.environ["PROTEUS_AMD_NO_PIPELINE"]
+    rows = [ln for ln in got.splitlines() if ln]
+    s.close()
+    gbs = 28.0 * n / sec / 1e9
+    return {"fact_rows": n, "ms": sec * 1e3, "rows_per_s": n / sec, "groups": len(rows),
+            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "operator_at_a_time_ms": sec_mat * 1e3,
+            "identical": got == ref == mat}
 
 
 def batched_mix(ctx, nq, qps):

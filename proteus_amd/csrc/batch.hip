@@ -83,7 +83,7 @@ __global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__rest
       continue;
     }
     const unsigned long long w = (unsigned long long)(uint32_t)key << 32 | (uint32_t)(v ? v[i] : 0);
-    uint64_t h = hash_i32(key) & mask;
+    uint64_t h = dim_hash(key) & mask;
     for (;;) {
       unsigned long long old = atomicCAS(&t[h], kEmptySlot, w);
       if (old == kEmptySlot) break;
@@ -145,7 +145,7 @@ template <bool LDS_TAB, bool LDS_GRP>
 __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long long *ltab,
                                           uint32_t *gcnt, unsigned long long *gsum, int32_t k,
                                           int32_t v, int32_t g, uint32_t &bad) {
-  uint64_t h = hash_i32(k) & q.mask;
+  uint64_t h = dim_hash(k) & q.mask;
   int32_t b = 0;
   for (;;) {
     const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];

@@ -63,6 +63,10 @@ __host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
   return h;
 }
 
+// Slot hash of the resident dimension tables (build and every probe use
+// it): the 32-bit murmur3 finaliser, cheaper than the 64-bit hashInt32.
+__host__ __device__ __forceinline__ uint32_t dim_hash(int32_t k) { return fmix32((uint32_t)k); }
+
 // ---------------------------------------------------------------------------
 // counter-based RNG (splitmix64 finaliser; Steele, Lea, Flood 2014)
 // ---------------------------------------------------------------------------

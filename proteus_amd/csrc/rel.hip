@@ -8,6 +8,8 @@
 // generation (core/olap/lib/expressions/expressions-generator.cpp) without a
 // JIT.
 #include <cstring>
+#include <memory>
+#include <type_traits>
 
 #include "pj_internal.hpp"
 #include "rel_api.hpp"
@@ -519,123 +521,490 @@ struct PipeArgs {
   const int32_t *out_t;
   PipeTable tables[kPipeMaxLevels];
   uint64_t n;
-  unsigned long long *cursor;
+  uint32_t *counts;  // survivors per workgroup
+  uint64_t region;   // output slots per workgroup: block b writes from b * region
   int32_t nins, nout, depth, nlev;
 };
 
 constexpr unsigned long long kPipeEmpty = 0x8000000000000000ULL;  // key INT32_MIN
+constexpr int kPipeTile = 256;  // rows per wave pass
+constexpr int kPipeWave = 64;   // one wave per workgroup
 
-// One row per lane; the program is uniform, so every lane of a wave walks
-// the same instruction.  A wave whose lanes have all ended skips the rest
-// of the program.  Survivors take consecutive output positions from one
-// atomic per wave.
-__global__ __launch_bounds__(kVmBlock) void k_pipe(PipeArgs a) {
+// Tile-at-a-time interpretation, one wave per workgroup: each instruction
+// runs over the surviving rows of a kPipeTile-row tile (a list of tile
+// positions in LDS), so its dispatch is paid once per tile, and a CU keeps
+// ~20 tiles in flight.  A lane owns the survivors lane + 64 j (j < PER); all
+// of its loads of one instruction are issued before any result is used, and
+// a probe advances all of the lane's keys one slot per round, so a tile
+// waits for one memory round trip per probe round, not per row.  PROBE /
+// FILTER compact the survivor list in order with one ballot per j (the
+// stack is empty there: the host checks it); build row ids live per tile row
+// and level; the outputs of a tile take consecutive positions from one
+// atomic.
+template <class T>
+__device__ __forceinline__ void load_col(const T *c, int64_t *d, const uint16_t *sel,
+                                         const uint32_t *rl, bool lv0, uint64_t base, uint32_t n,
+                                         int lane) {
+  constexpr int PER = kPipeTile / kPipeWave;
+  T v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t i = lane + j * kPipeWave;
+    if (i < n) {
+      const uint32_t r = sel[i];
+      v[j] = c[lv0 ? base + r : (uint64_t)rl[r]];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t i = lane + j * kPipeWave;
+    if (i < n) {
+      if constexpr (sizeof(T) == 8) d[i] = __builtin_bit_cast(int64_t, v[j]);
+      else if constexpr (std::is_same<T, float>::value) d[i] = of_f((double)v[j]);
+      else if constexpr (std::is_same<T, uint8_t>::value) d[i] = v[j] != 0;
+      else d[i] = (int64_t)v[j];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
   extern __shared__ int64_t sh[];
-  int64_t *st = sh + threadIdx.x;
-  int64_t *rid = sh + (size_t)a.depth * kVmBlock + threadIdx.x;
-  int64_t *ov = sh + (size_t)(a.depth + a.nlev) * kVmBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  for (uint64_t base = (uint64_t)blockIdx.x * kVmBlock; base < a.n;
-       base += (uint64_t)gridDim.x * kVmBlock) {
-    const uint64_t row = base + threadIdx.x;
-    bool alive = row < a.n;
-    rid[0] = (int64_t)row;
+  int64_t *st = sh;                                                   // depth x TILE
+  uint32_t *rid = (uint32_t *)(sh + (size_t)a.depth * kPipeTile);     // nlev x TILE (level 0 unused)
+  uint16_t *sel = (uint16_t *)(rid + (size_t)a.nlev * kPipeTile);     // TILE
+  const int lane = threadIdx.x;
+  constexpr int PER = kPipeTile / kPipeWave;
+  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+  uint64_t written = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * kPipeTile; base < a.n;
+       base += (uint64_t)gridDim.x * kPipeTile) {
+    const uint32_t rows = (uint32_t)(a.n - base < (uint64_t)kPipeTile ? a.n - base : kPipeTile);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) sel[lane + j * kPipeWave] = (uint16_t)(lane + j * kPipeWave);
+    __syncthreads();
+    uint32_t n = rows;
     int sp = 0;
-    for (int k = 0; k < a.nins; ++k) {
+    bool reserved = false;
+    unsigned long long p0 = 0;
+    for (int k = 0; k < a.nins && n; ++k) {
       const Instr in = a.ins[k];
-      bool check = false;
       switch (in.op) {
         case OP_COL: {
-          int64_t v = 0;
-          if (alive) v = ld_slot(a.cols[in.a], a.ctype[in.a], (uint64_t)rid[a.clevel[in.a] * kVmBlock]);
-          st[sp++ * kVmBlock] = v;
+          const void *c = a.cols[in.a];
+          const int lv = a.clevel[in.a];
+          const uint32_t *rl = rid + (size_t)lv * kPipeTile;
+          int64_t *d = st + (size_t)sp * kPipeTile;
+          switch (a.ctype[in.a]) {
+            case I32: load_col((const int32_t *)c, d, sel, rl, lv == 0, base, n, lane); break;
+            case I64:
+            case F64: load_col((const int64_t *)c, d, sel, rl, lv == 0, base, n, lane); break;
+            case BOOL: load_col((const uint8_t *)c, d, sel, rl, lv == 0, base, n, lane); break;
+            default: load_col((const float *)c, d, sel, rl, lv == 0, base, n, lane); break;
+          }
+          ++sp;
           break;
         }
-        case OP_CONST: st[sp++ * kVmBlock] = in.imm; break;
+        case OP_CONST: {
+          int64_t *d = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) d[i] = in.imm;
+          ++sp;
+          break;
+        }
         case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD: {
-          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] = arith(in.op, in.t, x, b);
+          --sp;
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile, *y = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) x[i] = arith(in.op, in.t, x[i], y[i]);
           break;
         }
         case OP_NEG: {
-          int64_t x = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] =
-              in.t == F64 ? of_f(-as_f(x)) : (in.t == I32 ? wrap32(0 - x) : (int64_t)(0 - (uint64_t)x));
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave)
+            x[i] = in.t == F64 ? of_f(-as_f(x[i])) : (in.t == I32 ? wrap32(0 - x[i]) : (int64_t)(0 - (uint64_t)x[i]));
           break;
         }
         case OP_EQ: case OP_NE: case OP_LT: case OP_LE: case OP_GT: case OP_GE: {
-          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] = cmp(in.op, in.t, x, b);
+          --sp;
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile, *y = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) x[i] = cmp(in.op, in.t, x[i], y[i]);
           break;
         }
-        case OP_AND: {
-          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] = (x != 0) & (b != 0);
+        case OP_AND: case OP_OR: {
+          --sp;
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile, *y = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave)
+            x[i] = in.op == OP_AND ? ((x[i] != 0) & (y[i] != 0)) : ((x[i] != 0) | (y[i] != 0));
           break;
         }
-        case OP_OR: {
-          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] = (x != 0) | (b != 0);
+        case OP_NOT: {
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) x[i] = x[i] == 0;
           break;
         }
-        case OP_NOT: st[(sp - 1) * kVmBlock] = st[(sp - 1) * kVmBlock] == 0; break;
-        case OP_CAST: st[(sp - 1) * kVmBlock] = cast(in.a, in.t, st[(sp - 1) * kVmBlock]); break;
+        case OP_CAST: {
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) x[i] = cast(in.a, in.t, x[i]);
+          break;
+        }
         case OP_IF: {
-          int64_t e = st[--sp * kVmBlock], t = st[--sp * kVmBlock], c = st[(sp - 1) * kVmBlock];
-          st[(sp - 1) * kVmBlock] = c ? t : e;
+          sp -= 2;
+          int64_t *c = st + (size_t)(sp - 1) * kPipeTile, *t = st + (size_t)sp * kPipeTile,
+                  *e = st + (size_t)(sp + 1) * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) c[i] = c[i] ? t[i] : e[i];
           break;
         }
         case OP_MIN: case OP_MAX: {
-          int64_t b = st[--sp * kVmBlock], x = st[(sp - 1) * kVmBlock];
-          bool lt = in.t == F64 ? as_f(x) < as_f(b) : x < b;
-          st[(sp - 1) * kVmBlock] = (in.op == OP_MIN) == lt ? x : b;
-          break;
-        }
-        case OP_PROBE: {
-          const int32_t key = (int32_t)st[--sp * kVmBlock];
-          if (alive) {
-            const PipeTable tb = a.tables[in.a];
-            uint64_t h = hash_i32(key) & tb.mask;
-            bool hit = false;
-            for (;;) {
-              const unsigned long long s = tb.table[h];
-              if ((int32_t)(s >> 32) == key && s != kPipeEmpty) {
-                rid[(in.a + 1) * kVmBlock] = (int64_t)(uint32_t)s;
-                hit = true;
-                break;
-              }
-              if (s == kPipeEmpty) break;
-              h = (h + 1) & tb.mask;
-            }
-            alive = hit;
+          --sp;
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile, *y = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) {
+            bool lt2 = in.t == F64 ? as_f(x[i]) < as_f(y[i]) : x[i] < y[i];
+            x[i] = (in.op == OP_MIN) == lt2 ? x[i] : y[i];
           }
-          check = true;
           break;
         }
+        case OP_PROBE:
         case OP_FILTER: {
-          const bool c = st[--sp * kVmBlock] != 0;
-          alive = alive && c;
-          check = true;
+          --sp;  // the stack is empty after the pop
+          bool hit[PER];
+          uint16_t rr[PER];
+#pragma unroll
+          for (int j = 0; j < PER; ++j) {
+            const uint32_t i = lane + j * kPipeWave;
+            hit[j] = false;
+            rr[j] = i < n ? sel[i] : 0;
+          }
+          if (in.op == OP_FILTER) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              const uint32_t i = lane + j * kPipeWave;
+              hit[j] = i < n && st[i] != 0;
+            }
+          } else {
+            const PipeTable tb = a.tables[in.a];
+            uint32_t *rl = rid + (size_t)(in.a + 1) * kPipeTile;
+            int32_t key[PER];
+            uint64_t h[PER];
+            bool act[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              const uint32_t i = lane + j * kPipeWave;
+              act[j] = i < n;
+              key[j] = act[j] ? (int32_t)st[i] : 0;
+              h[j] = dim_hash(key[j]) & tb.mask;
+            }
+            // all of the lane's keys advance one slot per round
+            for (;;) {
+              bool any = false;
+              unsigned long long sv[PER];
+#pragma unroll
+              for (int j = 0; j < PER; ++j) sv[j] = act[j] ? tb.table[h[j]] : 0ULL;
+#pragma unroll
+              for (int j = 0; j < PER; ++j) {
+                if (!act[j]) continue;
+                if (sv[j] == kPipeEmpty) {
+                  act[j] = false;
+                } else if ((int32_t)(sv[j] >> 32) == key[j]) {
+                  rl[rr[j]] = (uint32_t)sv[j];
+                  hit[j] = true;
+                  act[j] = false;
+                } else {
+                  h[j] = (h[j] + 1) & tb.mask;
+                  any = true;
+                }
+              }
+              if (!any) break;
+            }
+          }
+          __syncthreads();  // every lane has read sel and st
+          uint32_t o = 0;
+#pragma unroll
+          for (int j = 0; j < PER; ++j) {
+            const unsigned long long m = __ballot(hit[j]);
+            if (hit[j]) sel[o + __popcll(m & lt)] = rr[j];
+            o += __popcll(m);
+          }
+          n = o;
           break;
         }
-        case OP_OUT: ov[in.a * kVmBlock] = st[--sp * kVmBlock]; break;
+        case OP_OUT: {
+          --sp;
+          if (!reserved) {
+            p0 = (unsigned long long)blockIdx.x * a.region + written;
+            written += n;
+            reserved = true;
+          }
+          const int64_t *x = st + (size_t)sp * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) st_slot(a.outs[in.a], a.out_t[in.a], p0 + i, x[i]);
+          break;
+        }
         default: break;
       }
-      if (check && __ballot(alive) == 0) break;
+      __syncthreads();
     }
-    const unsigned long long m = __ballot(alive);
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
-      const unsigned long long below = lane ? (m & ((1ULL << lane) - 1)) : 0ULL;
-      unsigned long long b0 = 0;
-      if (lane == leader) b0 = atomicAdd(a.cursor, (unsigned long long)__popcll(m));
-      b0 = __shfl(b0, leader, 64);
-      if (alive) {
-        const uint64_t pos = b0 + __popcll(below);
-        for (int j = 0; j < a.nout; ++j) st_slot(a.outs[j], a.out_t[j], pos, ov[j * kVmBlock]);
+    __syncthreads();
+  }
+  if (lane == 0) a.counts[blockIdx.x] = (uint32_t)written;
+}
+
+// Probe chains without interpretation.  The common pipeline shape -- int32
+// key column -> probe, column cmp constant -> filter, then plain columns out
+// -- runs row-at-a-time: each lane carries kChainRows independent rows
+// through the stages with no LDS and no barriers (a row ends at its first
+// miss; a wave stops when all its rows have), which keeps far more probes in
+// flight than a tile-synchronous interpreter.
+constexpr int kChainRows = 4;
+constexpr int kChainMaxStages = 8;
+constexpr int kChainMaxCols = 32;
+struct ChainStage {
+  int32_t kind;  // 0 probe table `tab` with key column `col`; 1 filter col cmp imm
+  int32_t col;
+  int32_t tab;
+  int32_t op;    // comparison (filters)
+  int64_t imm;
+};
+struct ChainOut {
+  void *out;
+  int32_t col;
+  int32_t out_t;
+};
+struct ChainArgs {
+  const void *cols[kChainMaxCols];
+  int32_t ctype[kChainMaxCols];
+  int32_t clevel[kChainMaxCols];
+  ChainStage st[kChainMaxStages];
+  ChainOut outs[kPipeMaxOut];
+  PipeTable tables[kPipeMaxLevels];
+  uint64_t n;
+  uint32_t *counts;  // survivors per workgroup
+  uint64_t region;   // output slots per workgroup: block b writes from b * region
+  int32_t nst, nout;
+};
+
+// Loads of R values of one column at row indices ix (branch-free: the type
+// switch is outside the per-row loop, so all loads are in flight together).
+template <class T, int R>
+__device__ __forceinline__ void gather_rows(const void *c, const uint32_t (&ix)[R], int64_t (&v)[R]) {
+  const T *p = (const T *)c;
+  T x[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) x[k] = p[ix[k]];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if constexpr (sizeof(T) == 8) v[k] = __builtin_bit_cast(int64_t, x[k]);
+    else if constexpr (std::is_same<T, float>::value) v[k] = of_f((double)x[k]);
+    else if constexpr (std::is_same<T, uint8_t>::value) v[k] = x[k] != 0;
+    else v[k] = (int64_t)x[k];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void gather_col(const void *c, int ct, const uint32_t (&ix)[R], int64_t (&v)[R]) {
+  switch (ct) {
+    case I32: gather_rows<int32_t, R>(c, ix, v); break;
+    case I64:
+    case F64: gather_rows<int64_t, R>(c, ix, v); break;
+    case BOOL: gather_rows<uint8_t, R>(c, ix, v); break;
+    default: gather_rows<float, R>(c, ix, v); break;
+  }
+}
+
+// Per-lane state of R rows.  Build row ids of level L >= 1 for row slot k of
+// lane t live in LDS at ridl[((L - 1) * R + k) * 256 + t]; level 0 is the
+// source row.
+template <int R>
+struct ChainRows {
+  uint32_t row[R];
+  bool alive[R];
+  uint32_t *ridl;
+  __device__ __forceinline__ void at(int lv, uint32_t (&ix)[R]) const {
+    if (lv == 0) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) ix[k] = alive[k] ? row[k] : 0u;
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t r = ridl[((lv - 1) * R + k) * 256 + threadIdx.x];
+        ix[k] = alive[k] ? r : 0u;
       }
     }
   }
+};
+
+// Stage S of K (template recursion: every stage's descriptor is a
+// compile-time index, so it stays in registers).
+template <int S, int K, int R>
+__device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &cr, bool &live) {
+  if constexpr (S < K) {
+    if (live) {
+      const ChainStage sg = a.st[S];
+      const void *c = a.cols[sg.col];
+      const int ct = a.ctype[sg.col];
+      int64_t v[R];
+      uint32_t ix[R];
+      cr.at(a.clevel[sg.col], ix);
+      gather_col<R>(c, ct, ix, v);
+      if (sg.kind == 1) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          cr.alive[k] = cr.alive[k] && cmp(sg.op, ct == F64 || ct == F32 ? F64 : I64, v[k], sg.imm);
+      } else {
+        const PipeTable tb = a.tables[sg.tab];
+        const uint32_t mask = (uint32_t)tb.mask;
+        uint32_t h[R], found[R];
+        bool act[R], hit[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          act[k] = cr.alive[k];
+          hit[k] = false;
+          found[k] = 0;
+          h[k] = dim_hash((int32_t)v[k]) & mask;
+        }
+        for (;;) {
+          unsigned long long sv[R];
+#pragma unroll
+          for (int k = 0; k < R; ++k) sv[k] = tb.table[act[k] ? h[k] : 0u];
+          bool any = false;
+#pragma unroll
+          for (int k = 0; k < R; ++k) {
+            const bool empty = sv[k] == kPipeEmpty;
+            const bool eq = (int32_t)(sv[k] >> 32) == (int32_t)v[k];
+            const bool done = empty || eq;
+            const bool got = act[k] && !empty && eq;
+            hit[k] = hit[k] || got;
+            found[k] = got ? (uint32_t)sv[k] : found[k];
+            h[k] = (act[k] && !done) ? ((h[k] + 1) & mask) : h[k];
+            act[k] = act[k] && !done;
+            any = any || act[k];
+          }
+          if (!any) break;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          cr.alive[k] = hit[k];
+          if (hit[k]) cr.ridl[(sg.tab * R + k) * 256 + threadIdx.x] = found[k];
+        }
+      }
+      bool any_alive = false;
+#pragma unroll
+      for (int k = 0; k < R; ++k) any_alive |= cr.alive[k];
+      live = __any(any_alive);
+    }
+    chain_stages<S + 1, K, R>(a, cr, live);
+  }
+}
+
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_chain(const ChainArgs *__restrict__ ap) {
+  extern __shared__ uint32_t ridl[];
+  __shared__ uint32_t s_cnt;
+  const ChainArgs &a = *ap;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+  const uint32_t n32 = (uint32_t)a.n;
+  const uint32_t stride = gridDim.x * 256u * R;
+  ChainRows<R> cr;
+  cr.ridl = ridl;
+  for (uint32_t base = blockIdx.x * 256u * R; base < n32; base += stride) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      cr.row[k] = base + k * 256u + threadIdx.x;
+      cr.alive[k] = cr.row[k] < n32;
+    }
+    bool live = true;
+    chain_stages<0, K, R>(a, cr, live);
+    if (!live) continue;
+    // outputs: positions from one LDS atomic per wave
+    uint32_t cnt = 0;
+    unsigned long long m[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      m[k] = __ballot(cr.alive[k]);
+      cnt += __popcll(m[k]);
+    }
+    if (!cnt) continue;
+    unsigned long long p0 = 0;
+    if (lane == 0) p0 = (unsigned long long)blockIdx.x * a.region + atomicAdd(&s_cnt, cnt);
+    p0 = __shfl(p0, 0, 64);
+    uint64_t pos[R];
+    uint32_t off = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      pos[k] = p0 + off + __popcll(m[k] & lt);
+      off += __popcll(m[k]);
+    }
+    for (int j = 0; j < a.nout; ++j) {
+      const ChainOut o = a.outs[j];
+      int64_t v[R];
+      uint32_t ix[R];
+      cr.at(a.clevel[o.col], ix);
+      gather_col<R>(a.cols[o.col], a.ctype[o.col], ix, v);
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (cr.alive[k]) st_slot(o.out, o.out_t, pos[k], v[k]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
+}
+
+// Moves every workgroup's output segment to its place in the dense output.
+__global__ void k_seg_compact(const void *src, int esize, const uint32_t *counts,
+                              const unsigned long long *offs, uint64_t region, void *dst) {
+  const uint32_t b = blockIdx.y;
+  const uint32_t c = counts[b];
+  const char *s = (const char *)src + (uint64_t)b * region * esize;
+  char *d = (char *)dst + offs[b] * esize;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
+    if (esize == 8) ((uint64_t *)d)[i] = ((const uint64_t *)s)[i];
+    else if (esize == 4) ((uint32_t *)d)[i] = ((const uint32_t *)s)[i];
+    else d[i] = s[i];
+  }
+}
+
+// The program as a probe chain, or false.
+static bool as_chain(const PipeProg &p, void *const *outs, ChainArgs *a) {
+  if (p.cols.size() > (size_t)kChainMaxCols) return false;
+  for (size_t c = 0; c < p.cols.size(); ++c) {
+    a->cols[c] = p.cols[c];
+    a->ctype[c] = p.ctype[c];
+    a->clevel[c] = p.clevel[c];
+  }
+  a->nst = a->nout = 0;
+  const auto &I = p.ins;
+  size_t k = 0;
+  while (k < I.size()) {
+    if (k + 1 < I.size() && I[k].op == OP_COL && I[k + 1].op == OP_PROBE) {
+      if (p.ctype[I[k].a] != I32 || a->nst >= kChainMaxStages) return false;
+      a->st[a->nst++] = ChainStage{0, I[k].a, I[k + 1].a, 0, 0};
+      k += 2;
+    } else if (k + 3 < I.size() && I[k].op == OP_COL && I[k + 1].op == OP_CONST &&
+               I[k + 2].op >= OP_EQ && I[k + 2].op <= OP_GE && I[k + 3].op == OP_FILTER) {
+      const int ct = p.ctype[I[k].a];
+      const bool f = ct == F64 || ct == F32;
+      // the comparison must be the column's own type against a constant of it
+      if (I[k + 2].t != (f ? F64 : I[k + 2].t) || (f && I[k + 2].t != F64) ||
+          (!f && I[k + 2].t == F64) || a->nst >= kChainMaxStages)
+        return false;
+      a->st[a->nst++] = ChainStage{1, I[k].a, 0, I[k + 2].op, I[k + 1].imm};
+      k += 4;
+    } else {
+      break;
+    }
+  }
+  while (k < I.size()) {
+    if (k + 1 < I.size() && I[k].op == OP_COL && I[k + 1].op == OP_OUT) {
+      const int ct = p.ctype[I[k].a];
+      const int ot = p.out_t[I[k + 1].a];
+      const int nat = ct == F32 ? F64 : ct;
+      if (ot != nat || a->nout >= kPipeMaxOut || I[k + 1].a != a->nout) return false;
+      a->outs[a->nout++] = ChainOut{outs[I[k + 1].a], I[k].a, ot};
+      k += 2;
+    } else {
+      return false;
+    }
+  }
+  return true;
 }
 
 // Host side -----------------------------------------------------------------
@@ -802,27 +1171,117 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
   return PJ_OK;
 }
 
+// Per-workgroup output regions of a pipeline and their compaction into the
+// dense outputs (block order, so the output order is deterministic).
+struct Segments {
+  std::vector<std::unique_ptr<Scratch>> bufs;
+  Scratch counts, offs;
+  uint64_t grid = 0, region = 0;
+  int init(pj_ctx *ctx, const std::vector<int32_t> &types, uint64_t g, uint64_t r) {
+    grid = g;
+    region = r;
+    for (int32_t t : types) {
+      bufs.emplace_back(new Scratch());
+      PJ_TRY(bufs.back()->alloc(ctx, g * r * col_size(t)));
+    }
+    PJ_TRY(counts.alloc(ctx, g * 4));
+    PJ_TRY(offs.alloc(ctx, (g + 1) * 8));
+    return PJ_OK;
+  }
+  int compact(pj_ctx *ctx, const std::vector<int32_t> &types, void *const *outs, uint64_t *n_out) {
+    PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, counts.as<const uint32_t>(), grid,
+              offs.as<unsigned long long>());
+    unsigned long long total = 0;
+    PJ_TRY(d2h(ctx, &total, offs.as<unsigned long long>() + grid, 8));
+    for (size_t j = 0; j < types.size() && total; ++j) {
+      const uint32_t gx = (uint32_t)std::min<uint64_t>(ceil_div(region, 256), 16);
+      hipLaunchKernelGGL(k_seg_compact, dim3(gx, (uint32_t)grid), dim3(256), 0, ctx->stream, bufs[j]->p,
+                         col_size(types[j]), counts.as<const uint32_t>(),
+                         offs.as<const unsigned long long>(), region, outs[j]);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return fail(PJ_E_HIP, "pipeline compaction: %s", hipGetErrorString(e));
+    }
+    PJ_TRY(sync(ctx));
+    *n_out = total;
+    return PJ_OK;
+  }
+};
+
 int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint64_t *n_out) {
   *n_out = 0;
   const int nlev = (int)p.tables.size() + 1;
   if (nlev > kPipeMaxLevels) return fail(PJ_E_UNSUPPORTED, "pipeline of more than %d joins", kPipeMaxLevels - 1);
   if ((int)p.out_t.size() > kPipeMaxOut) return fail(PJ_E_UNSUPPORTED, "pipeline with more than %d outputs", kPipeMaxOut);
   if (p.depth < 1 || p.depth > kMaxDepth || p.ins.empty()) return fail(PJ_E_ARG, "bad pipeline program");
-  for (auto &i : p.ins) {
-    if (i.op == OP_COL && (i.a < 0 || i.a >= (int)p.cols.size() || p.clevel[i.a] < 0 || p.clevel[i.a] >= nlev))
-      return fail(PJ_E_ARG, "pipeline column out of range");
-    if (i.op == OP_PROBE && (i.a < 0 || i.a >= (int)p.tables.size())) return fail(PJ_E_ARG, "pipeline table out of range");
-    if (i.op == OP_OUT && (i.a < 0 || i.a >= (int)p.out_t.size())) return fail(PJ_E_ARG, "pipeline output out of range");
+  {
+    int sp = 0;
+    bool out_seen = false;
+    for (auto &i : p.ins) {
+      if (i.op == OP_COL && (i.a < 0 || i.a >= (int)p.cols.size() || p.clevel[i.a] < 0 || p.clevel[i.a] >= nlev))
+        return fail(PJ_E_ARG, "pipeline column out of range");
+      if (i.op == OP_PROBE && (i.a < 0 || i.a >= (int)p.tables.size())) return fail(PJ_E_ARG, "pipeline table out of range");
+      if (i.op == OP_OUT && (i.a < 0 || i.a >= (int)p.out_t.size())) return fail(PJ_E_ARG, "pipeline output out of range");
+      switch (i.op) {
+        case OP_COL: case OP_CONST: ++sp; break;
+        case OP_NEG: case OP_NOT: case OP_CAST: break;
+        case OP_IF: sp -= 2; break;
+        default: --sp; break;
+      }
+      if (sp < 0 || sp > p.depth) return fail(PJ_E_ARG, "pipeline stack out of range");
+      if ((i.op == OP_PROBE || i.op == OP_FILTER) && (sp != 0 || out_seen))
+        return fail(PJ_E_ARG, "pipeline probes and filters need an empty stack, before the outputs");
+      if (i.op == OP_OUT) out_seen = true;
+    }
+    if (sp != 0) return fail(PJ_E_ARG, "pipeline leaves values on the stack");
+  }
+  if (n >= (1ULL << 32)) return fail(PJ_E_CAPACITY, "pipeline source above 2^32-1 rows");
+  if (!n) return PJ_OK;
+  {
+    ChainArgs ca{};
+    std::vector<void *> sparse(p.out_t.size());
+    if (as_chain(p, sparse.data(), &ca)) {
+      for (size_t j = 0; j < p.tables.size(); ++j) ca.tables[j] = p.tables[j];
+      const uint64_t tile = 256 * kChainRows;
+      const uint64_t grid = std::min<uint64_t>(ceil_div(n, tile), (uint64_t)ctx->num_cus * 8);
+      ca.n = n;
+      ca.region = ceil_div(ceil_div(n, tile), grid) * tile;
+      Segments seg;
+      PJ_TRY(seg.init(ctx, p.out_t, grid, ca.region));
+      for (size_t j = 0; j < p.out_t.size(); ++j) ca.outs[j].out = seg.bufs[j]->p;
+      ca.counts = seg.counts.as<uint32_t>();
+      Scratch dargs;
+      PJ_TRY(dargs.alloc(ctx, sizeof(ChainArgs)));
+      PJ_TRY(h2d(ctx, dargs.p, &ca, sizeof(ChainArgs)));
+      const size_t lds = std::max<size_t>(1, p.tables.size()) * kChainRows * 256 * 4;
+      const ChainArgs *da = dargs.as<const ChainArgs>();
+      switch (ca.nst) {
+        case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<1, kChainRows>), grid, 256, lds, da); break;
+        case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<2, kChainRows>), grid, 256, lds, da); break;
+        case 3: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<3, kChainRows>), grid, 256, lds, da); break;
+        case 4: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<4, kChainRows>), grid, 256, lds, da); break;
+        case 5: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<5, kChainRows>), grid, 256, lds, da); break;
+        case 6: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<6, kChainRows>), grid, 256, lds, da); break;
+        case 7: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<7, kChainRows>), grid, 256, lds, da); break;
+        default: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<8, kChainRows>), grid, 256, lds, da); break;
+      }
+      return seg.compact(ctx, p.out_t, outs, n_out);
+    }
   }
   if (!n) return PJ_OK;
   // program, column tables and output tables in one device block
   const size_t ni = p.ins.size() * sizeof(Instr), nc = p.cols.size(), no = p.out_t.size();
+  const uint64_t grid = std::min<uint64_t>(ceil_div(n, kPipeTile), (uint64_t)ctx->num_cus * 32);
+  const uint64_t region = ceil_div(ceil_div(n, kPipeTile), grid) * kPipeTile;
+  Segments seg;
+  PJ_TRY(seg.init(ctx, p.out_t, grid, region));
+  std::vector<void *> sp(no);
+  for (size_t j = 0; j < no; ++j) sp[j] = seg.bufs[j]->p;
   const size_t bytes = ni + nc * 8 + nc * 4 + nc * 4 + no * 8 + no * 4 + 64;
   std::vector<char> h(bytes, 0);
   size_t off = 0;
   auto put = [&](const void *src, size_t len) {
     size_t at = off;
-    memcpy(h.data() + off, src, len);
+    if (len) memcpy(h.data() + off, src, len);
     off = (off + len + 7) & ~(size_t)7;
     return at;
   };
@@ -830,35 +1289,31 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
   const size_t o_cols = put(p.cols.data(), nc * 8);
   const size_t o_ct = put(p.ctype.data(), nc * 4);
   const size_t o_cl = put(p.clevel.data(), nc * 4);
-  const size_t o_outs = put(outs, no * 8);
+  const size_t o_sp = put(sp.data(), no * 8);
   const size_t o_ot = put(p.out_t.data(), no * 4);
-  Scratch dev, cur;
+  Scratch dev;
   PJ_TRY(dev.alloc(ctx, off));
   PJ_TRY(h2d(ctx, dev.p, h.data(), off));
-  PJ_TRY(cur.alloc(ctx, 8));
-  PJ_TRY(zero(ctx, cur.p, 8));
   PipeArgs a{};
   char *d = (char *)dev.p;
   a.ins = (const Instr *)(d + o_ins);
   a.cols = (const void *const *)(d + o_cols);
   a.ctype = (const int32_t *)(d + o_ct);
   a.clevel = (const int32_t *)(d + o_cl);
-  a.outs = (void *const *)(d + o_outs);
+  a.outs = (void *const *)(d + o_sp);
   a.out_t = (const int32_t *)(d + o_ot);
   for (size_t j = 0; j < p.tables.size(); ++j) a.tables[j] = p.tables[j];
   a.n = n;
-  a.cursor = cur.as<unsigned long long>();
+  a.counts = seg.counts.as<uint32_t>();
+  a.region = region;
   a.nins = (int32_t)p.ins.size();
   a.nout = (int32_t)no;
   a.depth = p.depth;
   a.nlev = nlev;
-  const size_t lds = (size_t)(p.depth + nlev + no) * kVmBlock * 8;
+  const size_t lds = (size_t)p.depth * kPipeTile * 8 + (size_t)nlev * kPipeTile * 4 + kPipeTile * 2;
   if (lds > 64 * 1024) return fail(PJ_E_UNSUPPORTED, "pipeline needs %zu bytes of LDS", lds);
-  PJ_LAUNCH(ctx, "rel_pipeline", k_pipe, grid_for(ctx, n, kVmBlock), kVmBlock, lds, a);
-  unsigned long long m = 0;
-  PJ_TRY(d2h(ctx, &m, cur.p, 8));
-  *n_out = m;
-  return PJ_OK;
+  PJ_LAUNCH(ctx, "rel_pipeline", k_pipe, grid, kPipeWave, lds, a);
+  return seg.compact(ctx, p.out_t, outs, n_out);
 }
 
 int hash_buckets(pj_ctx *ctx, const int32_t *k, uint64_t n, uint32_t buckets, uint32_t *out) {
