@@ -88,8 +88,9 @@ struct JoinUnit {
   uint32_t r_len, s_len;
 };
 
-// Per-unit LDS table + probe.  Units [wg_begin[b], wg_begin[b+1]) belong to
-// workgroup b; consecutive units of one build partition reuse the table.
+// Per-unit LDS table + probe.  Workgroups claim units from a counter (the
+// host orders them largest first); a workgroup that claims another unit of
+// the build partition it has just built reuses the table.
 //
 // The table is bucket-sorted ("CSR"): tab[b] packs the [start, end) range of
 // bucket b in ent[] (16 + 16 bits) and entries are (key, value) pairs grouped
@@ -106,7 +107,7 @@ template <int BLOCK, int H, int NCAP, bool HASHED>
 __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
-    const JoinUnit *__restrict__ units, const uint32_t *__restrict__ wg_begin,
+    const JoinUnit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ next_unit,
     unsigned long long *__restrict__ acc_out) {
   static_assert(NCAP < 65536, "bucket ranges are 16-bit");
   static_assert(NCAP % BLOCK == 0 && H % BLOCK == 0, "whole rows per thread");
@@ -121,9 +122,15 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
   const int tid = threadIdx.x;
   JoinAcc acc;
   acc.init();
-  const uint32_t u0 = wg_begin[blockIdx.x], u1 = wg_begin[blockIdx.x + 1];
+  __shared__ uint32_t s_unit;
   unsigned long long built_r = ~0ULL;
-  for (uint32_t u = u0; u < u1; ++u) {
+  for (;;) {
+    // units are claimed from a counter (largest first: the host sorts them)
+    if (tid == 0) s_unit = atomicAdd(next_unit, 1u);
+    __syncthreads();
+    const uint32_t u = s_unit;
+    __syncthreads();
+    if (u >= nunits) break;
     const JoinUnit U = units[u];
     const uint32_t rounds = (U.r_len + NCAP - 1) / NCAP;
     for (uint32_t round = 0; round < rounds; ++round) {
@@ -241,26 +248,30 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
           uint32_t rg[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
-          int2 e0[4];
+          int2 e0[4], e1[4], e2[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu;  // == NCAP when the tail buckets are empty
             e0[c] = ent[st < (uint32_t)NCAP ? st : 0u];
+            e1[c] = ent[st + 1 < (uint32_t)NCAP ? st + 1 : 0u];
+            e2[c] = ent[st + 2 < (uint32_t)NCAP ? st + 2 : 0u];
           }
-          // one-entry buckets without branches; longer ones in probe_multi
+          // buckets of up to three entries without branches (~98% of probes
+          // at the planner's load); longer ones finish in probe_multi
           bool multi = false;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
-            const bool hit = (en == st + 1) & (e0[c].x == kk[c]);
-            acc.add1_if(hit, e0[c].y, vv[c]);
-            multi |= en > st + 1;
+            acc.add1_if((en > st) & (e0[c].x == kk[c]), e0[c].y, vv[c]);
+            acc.add1_if((en > st + 1) & (e1[c].x == kk[c]), e1[c].y, vv[c]);
+            acc.add1_if((en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
+            multi |= en > st + 3;
           }
           if (multi) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
               const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
-              if (en > st + 1) probe_multi(kk[c], vv[c], st, en);
+              if (en > st + 3) probe_multi(kk[c], vv[c], st + 3, en);
             }
           }
           k4 = kn;
@@ -324,12 +335,18 @@ static int join_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
 }
 
 // One side of the join, partitioned into 2^(b1+b2) dense partitions.
+// The four partition arrays start at different odd multiples of 4 KiB
+// within their allocations: pass 2 reads and writes the same offsets of
+// equally sized buffers, and bases that coincide modulo the HBM channel
+// interleave would send its read and write streams to the same channels.
 struct Side {
   RadixPlan plan;
   RadixHist hist;
   Scratch k1, v1, k2, v2;
   const int32_t *fk = nullptr, *fv = nullptr;  // final partitioned columns
 };
+constexpr uint64_t kSkew[4] = {0, 17 * 4096, 33 * 4096, 49 * 4096};
+static int32_t *skewed(Scratch &b, int i) { return (int32_t *)((char *)b.p + kSkew[i]); }
 
 static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
                           uint64_t n, int b1, int b2, const char *tag) {
@@ -338,22 +355,22 @@ static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_
   PJ_TRY(s.hist.alloc(ctx, s.plan));
   PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, true));
   JointHashed j{32 - B};
-  PJ_TRY(s.k1.alloc(ctx, n * 4 + 256));
-  PJ_TRY(s.v1.alloc(ctx, n * 4 + 256));
+  PJ_TRY(s.k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
+  PJ_TRY(s.v1.alloc(ctx, n * 4 + 256 + kSkew[1]));
   char name[64];
   snprintf(name, sizeof(name), "radix_scatter1_%s", tag);
-  PJ_TRY(join_pass_bits(ctx, name, b1, false, d_k, d_v, s.k1.as<int32_t>(),
-                        s.v1.as<int32_t>(), n, s.plan, j, s.hist));
-  s.fk = s.k1.as<int32_t>();
-  s.fv = s.v1.as<int32_t>();
+  PJ_TRY(join_pass_bits(ctx, name, b1, false, d_k, d_v, skewed(s.k1, 0), skewed(s.v1, 1), n,
+                        s.plan, j, s.hist));
+  s.fk = skewed(s.k1, 0);
+  s.fv = skewed(s.v1, 1);
   if (b2) {
-    PJ_TRY(s.k2.alloc(ctx, n * 4 + 256));
-    PJ_TRY(s.v2.alloc(ctx, n * 4 + 256));
+    PJ_TRY(s.k2.alloc(ctx, n * 4 + 256 + kSkew[2]));
+    PJ_TRY(s.v2.alloc(ctx, n * 4 + 256 + kSkew[3]));
     snprintf(name, sizeof(name), "radix_scatter2_%s", tag);
-    PJ_TRY(join_pass_bits(ctx, name, b2, true, s.fk, s.fv, s.k2.as<int32_t>(),
-                          s.v2.as<int32_t>(), n, s.plan, j, s.hist));
-    s.fk = s.k2.as<int32_t>();
-    s.fv = s.v2.as<int32_t>();
+    PJ_TRY(join_pass_bits(ctx, name, b2, true, s.fk, s.fv, skewed(s.k2, 2), skewed(s.v2, 3), n,
+                          s.plan, j, s.hist));
+    s.fk = skewed(s.k2, 2);
+    s.fv = skewed(s.v2, 3);
   }
   return PJ_OK;
 }
@@ -378,48 +395,32 @@ constexpr int kJoinBlock = 512;
 template <int H, int NCAP>
 static int launch_pjoin_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
                           const int32_t *sk, const int32_t *sv, const JoinUnit *units,
-                          const uint32_t *wgb, uint32_t grid, unsigned long long *acc) {
+                          uint32_t nunits, uint32_t *next, uint32_t grid, unsigned long long *acc) {
   if (hashed)
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, true>), grid, kJoinBlock, 0,
-              rk, rv, sk, sv, units, wgb, acc);
+              rk, rv, sk, sv, units, nunits, next, acc);
   else
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, false>), grid, kJoinBlock, 0,
-              rk, rv, sk, sv, units, wgb, acc);
+              rk, rv, sk, sv, units, nunits, next, acc);
   return PJ_OK;
 }
 
 static int launch_pjoin(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk,
                         const int32_t *rv, const int32_t *sk, const int32_t *sv,
-                        const JoinUnit *units, const uint32_t *wgb, uint32_t grid,
+                        const JoinUnit *units, uint32_t nunits, uint32_t *next, uint32_t grid,
                         unsigned long long *acc) {
   switch (lds_log) {
     case 11:
-      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
+      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
     case 12:
-      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
+      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
     case 13:
-      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
+      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
     case 14:
-      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, wgb, grid, acc);
+      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
     default:
       return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", lds_log);
   }
-}
-
-static void balance_units(const std::vector<JoinUnit> &units, uint32_t grid,
-                          std::vector<uint32_t> &wgb) {
-  wgb.assign(grid + 1, (uint32_t)units.size());
-  uint64_t total = 0;
-  for (auto &u : units) total += (uint64_t)u.s_len + u.r_len;
-  uint64_t per = ceil_div(total ? total : 1, grid);
-  uint64_t acc = 0;
-  uint32_t w = 0;
-  wgb[0] = 0;
-  for (uint32_t i = 0; i < units.size(); ++i) {
-    while (w + 1 <= grid && acc >= per * (w + 1)) wgb[++w] = i;
-    acc += (uint64_t)units[i].s_len + units[i].r_len;
-  }
-  while (w + 1 <= grid) wgb[++w] = (uint32_t)units.size();
 }
 
 static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv,
@@ -469,17 +470,20 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
     uint64_t g = (uint64_t)ctx->num_cus * geo.per_cu *
                  (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
     uint32_t grid = (uint32_t)std::min<uint64_t>(g, units.size());
-    std::vector<uint32_t> wgb;
-    balance_units(units, grid, wgb);
+    // largest units first, so the tail of the dynamic schedule is short
+    std::stable_sort(units.begin(), units.end(), [](const JoinUnit &a, const JoinUnit &b) {
+      return (uint64_t)a.s_len + a.r_len > (uint64_t)b.s_len + b.r_len;
+    });
+    std::vector<uint32_t> ctr(1, 0u);
     std::vector<uint8_t> buf;
-    size_t o_units = pack_add(buf, units), o_wgb = pack_add(buf, wgb);
+    size_t o_units = pack_add(buf, units), o_ctr = pack_add(buf, ctr);
     Scratch dplan;
     PJ_TRY(dplan.alloc(ctx, buf.size()));
     PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
     uint8_t *dp = dplan.as<uint8_t>();
     PJ_TRY(launch_pjoin(ctx, geo.lds_log, B > 0, rk, rv, sk, sv,
-                        reinterpret_cast<const JoinUnit *>(dp + o_units),
-                        reinterpret_cast<const uint32_t *>(dp + o_wgb), grid,
+                        reinterpret_cast<const JoinUnit *>(dp + o_units), (uint32_t)units.size(),
+                        reinterpret_cast<uint32_t *>(dp + o_ctr), grid,
                         acc.as<unsigned long long>()));
   }
   return join_acc_read(ctx, acc.as<unsigned long long>(), out);
