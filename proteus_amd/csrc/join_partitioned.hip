@@ -262,9 +262,8 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
-            acc.add1_if((en > st) & (e0[c].x == kk[c]), e0[c].y, vv[c]);
-            acc.add1_if((en > st + 1) & (e1[c].x == kk[c]), e1[c].y, vv[c]);
-            acc.add1_if((en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
+            acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y, (en > st + 1) & (e1[c].x == kk[c]),
+                        e1[c].y, (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
             multi |= en > st + 3;
           }
           if (multi) {

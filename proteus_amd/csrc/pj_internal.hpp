@@ -266,6 +266,21 @@ struct JoinAcc {
     min_p = hit ? min(min_p, pv) : min_p;
     max_p = hit ? max(max_p, pv) : max_p;
   }
+  // up to three matches of one probe row (flags h0..h2 over build values
+  // b0..b2), without branches
+  __device__ __forceinline__ void add3_if(bool h0, int32_t b0, bool h1, int32_t b1, bool h2,
+                                          int32_t b2, int32_t pv) {
+    const uint32_t m = (uint32_t)h0 + (uint32_t)h1 + (uint32_t)h2;
+    n1 += m;
+    sum_b += (int64_t)(h0 ? b0 : 0) + (int64_t)(h1 ? b1 : 0) + (int64_t)(h2 ? b2 : 0);
+    sum_p += (int64_t)pv * (int64_t)m;
+    int32_t mn = min(h0 ? b0 : INT32_MAX, min(h1 ? b1 : INT32_MAX, h2 ? b2 : INT32_MAX));
+    int32_t mx = max(h0 ? b0 : INT32_MIN, max(h1 ? b1 : INT32_MIN, h2 ? b2 : INT32_MIN));
+    min_b = min(min_b, mn);
+    max_b = max(max_b, mx);
+    min_p = m ? min(min_p, pv) : min_p;
+    max_p = m ? max(max_p, pv) : max_p;
+  }
   // `m` matches of one probe row whose build values sum to sb and span
   // [mnb, mxb].
   __device__ __forceinline__ void add(int64_t m, int64_t sb, int32_t mnb, int32_t mxb,
