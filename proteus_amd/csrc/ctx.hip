@@ -121,6 +121,24 @@ int sync(pj_ctx *ctx) {
   return PJ_OK;
 }
 
+int aux_fork(pj_ctx *ctx, hipStream_t *aux) {
+  if (!ctx->aux) {
+    PJ_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    PJ_HIP(hipEventCreateWithFlags(&ctx->aux_fork, hipEventDisableTiming));
+    PJ_HIP(hipEventCreateWithFlags(&ctx->aux_join, hipEventDisableTiming));
+  }
+  PJ_HIP(hipEventRecord(ctx->aux_fork, ctx->stream));
+  PJ_HIP(hipStreamWaitEvent(ctx->aux, ctx->aux_fork, 0));
+  *aux = ctx->aux;
+  return PJ_OK;
+}
+
+int aux_join(pj_ctx *ctx) {
+  PJ_HIP(hipEventRecord(ctx->aux_join, ctx->aux));
+  PJ_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_join, 0));
+  return PJ_OK;
+}
+
 // Clears use a fill kernel, never hipMemsetAsync, so callers may capture the
 // work into a hipGraph.
 __global__ void k_fill_u32(uint32_t *__restrict__ d, uint32_t v, uint64_t n) {
@@ -184,6 +202,7 @@ void prof_end(pj_ctx *ctx, const char *name, hipEvent_t a) {
 static void prof_collect(pj_ctx *ctx) {
   if (ctx->prof_pending.empty()) return;
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
   for (auto &r : ctx->prof_pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
@@ -453,6 +472,12 @@ int pj_ctx_destroy(pj_ctx *ctx) {
   }
   for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
   if (ctx->pinned_small) (void)hipHostFree(ctx->pinned_small);
+  if (ctx->aux) {
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipEventDestroy(ctx->aux_fork);
+    (void)hipEventDestroy(ctx->aux_join);
+    (void)hipStreamDestroy(ctx->aux);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PJ_OK;

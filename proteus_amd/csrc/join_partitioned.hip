@@ -107,8 +107,11 @@ template <int BLOCK, int H, int NCAP, bool HASHED>
 __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
-    const JoinUnit *__restrict__ units, uint32_t nunits, uint32_t *__restrict__ next_unit,
+    const JoinUnit *__restrict__ units, uint32_t *__restrict__ unit_hdr,
     unsigned long long *__restrict__ acc_out) {
+  // unit_hdr[0] = number of units, unit_hdr[1] = claim counter
+  const uint32_t nunits = unit_hdr[0];
+  uint32_t *next_unit = unit_hdr + 1;
   static_assert(NCAP < 65536, "bucket ranges are 16-bit");
   static_assert(NCAP % BLOCK == 0 && H % BLOCK == 0, "whole rows per thread");
   constexpr int RPT = NCAP / BLOCK;  // build rows per thread
@@ -282,6 +285,72 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
   join_acc_flush(acc, acc_out);
 }
 
+// Join work units from the two sides' partition sizes and starts, built on
+// the device so the join follows the partitioning without a host round trip.
+// Partition q with both sides non-empty gives ceil(|S_q| / probe_unit)
+// units.  Units are ordered largest first (|R_q| + probe tuples) by a
+// counting sort over size classes of a quarter octave, so the tail of the
+// join's dynamic schedule is short; the order within a class is arbitrary
+// (the aggregate does not depend on it).  One workgroup.
+constexpr int kUnitBlock = 1024;
+constexpr int kUnitClasses = 160;  // 4 per octave of a 40-bit weight
+__device__ __forceinline__ uint32_t unit_class(uint64_t w) {
+  const int lg = 63 - __clzll((long long)(w | 1));
+  const uint32_t sub = lg >= 2 ? (uint32_t)(w >> (lg - 2)) & 3u : 0u;
+  const uint32_t c = (uint32_t)lg * 4 + sub;
+  return c < kUnitClasses ? c : kUnitClasses - 1;
+}
+__global__ __launch_bounds__(kUnitBlock) void k_join_units(
+    const uint32_t *__restrict__ rT, const unsigned long long *__restrict__ rS,
+    const uint32_t *__restrict__ sT, const unsigned long long *__restrict__ sS, uint32_t nb,
+    uint32_t pu, JoinUnit *__restrict__ units, uint32_t *__restrict__ unit_hdr) {
+  __shared__ uint32_t cnt[kUnitClasses];
+  const int tid = threadIdx.x;
+  for (int c = tid; c < kUnitClasses; c += kUnitBlock) cnt[c] = 0;
+  __syncthreads();
+  for (uint32_t q = tid; q < nb; q += kUnitBlock) {
+    const uint32_t rl = rT[q], sl = sT[q];
+    if (!rl || !sl) continue;
+    const uint32_t full = sl / pu, tail = sl % pu;
+    if (full) atomicAdd(&cnt[unit_class((uint64_t)pu + rl)], full);
+    if (tail) atomicAdd(&cnt[unit_class((uint64_t)tail + rl)], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {  // class starts, largest class first
+    uint32_t run = 0;
+    for (int c = kUnitClasses - 1; c >= 0; --c) {
+      const uint32_t x = cnt[c];
+      cnt[c] = run;
+      run += x;
+    }
+    unit_hdr[0] = run;
+    unit_hdr[1] = 0;
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < nb; q += kUnitBlock) {
+    const uint32_t rl = rT[q], sl = sT[q];
+    if (!rl || !sl) continue;
+    const uint32_t full = sl / pu, tail = sl % pu;
+    JoinUnit u;
+    u.r_begin = rS[q];
+    u.r_len = rl;
+    if (full) {
+      uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)pu + rl)], full);
+      for (uint32_t k = 0; k < full; ++k) {
+        u.s_begin = sS[q] + (unsigned long long)k * pu;
+        u.s_len = pu;
+        units[at + k] = u;
+      }
+    }
+    if (tail) {
+      const uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)tail + rl)], 1u);
+      u.s_begin = sS[q] + (unsigned long long)full * pu;
+      u.s_len = tail;
+      units[at] = u;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -347,12 +416,14 @@ struct Side {
 constexpr uint64_t kSkew[4] = {0, 17 * 4096, 33 * 4096, 49 * 4096};
 static int32_t *skewed(Scratch &b, int i) { return (int32_t *)((char *)b.p + kSkew[i]); }
 
+// Queues the side's histograms and passes on ctx->stream; the partition
+// sizes are read back separately.
 static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
                           uint64_t n, int b1, int b2, const char *tag) {
   const int B = b1 + b2;
   s.plan = radix_make_plan(ctx, n, b1, b2, 1 << B, 2);
   PJ_TRY(s.hist.alloc(ctx, s.plan));
-  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, true));
+  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, false));
   JointHashed j{32 - B};
   PJ_TRY(s.k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
   PJ_TRY(s.v1.alloc(ctx, n * 4 + 256 + kSkew[1]));
@@ -394,29 +465,29 @@ constexpr int kJoinBlock = 512;
 template <int H, int NCAP>
 static int launch_pjoin_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
                           const int32_t *sk, const int32_t *sv, const JoinUnit *units,
-                          uint32_t nunits, uint32_t *next, uint32_t grid, unsigned long long *acc) {
+                          uint32_t *hdr, uint32_t grid, unsigned long long *acc) {
   if (hashed)
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, true>), grid, kJoinBlock, 0,
-              rk, rv, sk, sv, units, nunits, next, acc);
+              rk, rv, sk, sv, units, hdr, acc);
   else
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, false>), grid, kJoinBlock, 0,
-              rk, rv, sk, sv, units, nunits, next, acc);
+              rk, rv, sk, sv, units, hdr, acc);
   return PJ_OK;
 }
 
 static int launch_pjoin(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk,
                         const int32_t *rv, const int32_t *sk, const int32_t *sv,
-                        const JoinUnit *units, uint32_t nunits, uint32_t *next, uint32_t grid,
+                        const JoinUnit *units, uint32_t *hdr, uint32_t grid,
                         unsigned long long *acc) {
   switch (lds_log) {
     case 11:
-      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
+      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 12:
-      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
+      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 13:
-      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
+      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 14:
-      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, nunits, next, grid, acc);
+      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     default:
       return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", lds_log);
   }
@@ -430,11 +501,13 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
   PJ_TRY(acc.alloc(ctx, 64));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
   const Geometry geo = geometry_for(cfg.lds_log);
-  std::vector<JoinUnit> units;
   Side R, S;
+  Scratch dunits;  // unit header (2 x u32, 256 B), then the units
+  uint64_t max_units = 0;
   const int32_t *rk = d_bk, *rv = d_bv, *sk = d_pk, *sv = d_pv;
   if (nb && np) {
     if (B == 0) {
+      std::vector<JoinUnit> units;
       for (uint64_t s0 = 0; s0 < np; s0 += cfg.probe_unit) {
         JoinUnit u;
         u.r_begin = 0;
@@ -443,47 +516,56 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
         u.s_len = (uint32_t)std::min<uint64_t>(cfg.probe_unit, np - s0);
         units.push_back(u);
       }
+      std::vector<uint32_t> hdr{(uint32_t)units.size(), 0u};
+      std::vector<uint8_t> buf;
+      pack_add(buf, hdr);
+      pack_add(buf, units);  // at offset 256
+      PJ_TRY(dunits.alloc(ctx, buf.size()));
+      PJ_TRY(h2d(ctx, dunits.p, buf.data(), buf.size()));
+      max_units = units.size();
     } else {
-      PJ_TRY(partition_side(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build"));
-      PJ_TRY(partition_side(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe"));
+      // The build side is partitioned on the auxiliary stream while the
+      // probe side is on the main one: the smaller side's passes alone
+      // leave HBM bandwidth unused (short grids, launch gaps).
+      hipStream_t aux;
+      PJ_TRY(aux_fork(ctx, &aux));
+      int rc;
+      {
+        StreamSwap sw(ctx, aux);
+        rc = partition_side(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build");
+      }
+      if (rc == PJ_OK) rc = partition_side(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe");
+      // join even on failure, so no scratch is released under running work
+      const int rj = aux_join(ctx);
+      if (rc != PJ_OK) {
+        (void)hipStreamSynchronize(aux);
+        return rc;
+      }
+      PJ_TRY(rj);
       rk = R.fk;
       rv = d_bv ? R.fv : nullptr;
       sk = S.fk;
       sv = d_pv ? S.fv : nullptr;
-      const int NB = 1 << B;
-      for (int q = 0; q < NB; ++q) {
-        const uint64_t rl = R.hist.hT[q], sl = S.hist.hT[q];
-        if (!rl || !sl) continue;
-        for (uint64_t s0 = 0; s0 < sl; s0 += cfg.probe_unit) {
-          JoinUnit u;
-          u.r_begin = R.hist.hS[q];
-          u.r_len = (uint32_t)rl;
-          u.s_begin = S.hist.hS[q] + s0;
-          u.s_len = (uint32_t)std::min<uint64_t>(cfg.probe_unit, sl - s0);
-          units.push_back(u);
-        }
-      }
+      const uint32_t NB = 1u << B;
+      max_units = NB + np / cfg.probe_unit;
+      PJ_TRY(dunits.alloc(ctx, 256 + max_units * sizeof(JoinUnit)));
+      uint8_t *du = dunits.as<uint8_t>();
+      PJ_LAUNCH(ctx, "pjoin_units", k_join_units, 1, kUnitBlock, 0,
+                R.hist.T.as<const uint32_t>(), R.hist.S.as<const unsigned long long>(),
+                S.hist.T.as<const uint32_t>(), S.hist.S.as<const unsigned long long>(), NB,
+                cfg.probe_unit, reinterpret_cast<JoinUnit *>(du + 256),
+                reinterpret_cast<uint32_t *>(du));
     }
   }
-  if (!units.empty()) {
-    uint64_t g = (uint64_t)ctx->num_cus * geo.per_cu *
-                 (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
-    uint32_t grid = (uint32_t)std::min<uint64_t>(g, units.size());
-    // largest units first, so the tail of the dynamic schedule is short
-    std::stable_sort(units.begin(), units.end(), [](const JoinUnit &a, const JoinUnit &b) {
-      return (uint64_t)a.s_len + a.r_len > (uint64_t)b.s_len + b.r_len;
-    });
-    std::vector<uint32_t> ctr(1, 0u);
-    std::vector<uint8_t> buf;
-    size_t o_units = pack_add(buf, units), o_ctr = pack_add(buf, ctr);
-    Scratch dplan;
-    PJ_TRY(dplan.alloc(ctx, buf.size()));
-    PJ_TRY(h2d(ctx, dplan.p, buf.data(), buf.size()));
-    uint8_t *dp = dplan.as<uint8_t>();
+  if (max_units) {
+    // workgroups beyond the number of units exit at their first claim
+    const uint64_t g = (uint64_t)ctx->num_cus * geo.per_cu *
+                       (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(g, max_units);
+    uint8_t *du = dunits.as<uint8_t>();
     PJ_TRY(launch_pjoin(ctx, geo.lds_log, B > 0, rk, rv, sk, sv,
-                        reinterpret_cast<const JoinUnit *>(dp + o_units), (uint32_t)units.size(),
-                        reinterpret_cast<uint32_t *>(dp + o_ctr), grid,
-                        acc.as<unsigned long long>()));
+                        reinterpret_cast<const JoinUnit *>(du + 256),
+                        reinterpret_cast<uint32_t *>(du), grid, acc.as<unsigned long long>()));
   }
   return join_acc_read(ctx, acc.as<unsigned long long>(), out);
 }

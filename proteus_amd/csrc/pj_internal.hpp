@@ -136,6 +136,10 @@ struct pj_ctx {
   std::map<pj::TuneKey, pj_radix_cfg> tuned;
   // batched-query stream slots (owned by batch.hip)
   void *batch = nullptr;
+  // second stream for work that overlaps the main stream's (created on first
+  // use), with a fork and a join event
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_fork = nullptr, aux_join = nullptr;
 };
 
 namespace pj {
@@ -184,6 +188,20 @@ void prof_end(pj_ctx *ctx, const char *name, hipEvent_t a);
 int h2d(pj_ctx *ctx, void *d, const void *h, uint64_t bytes);
 int d2h(pj_ctx *ctx, void *h, const void *d, uint64_t bytes);
 int sync(pj_ctx *ctx);
+// Returns the auxiliary stream after making it wait for the work queued so
+// far on ctx->stream.
+int aux_fork(pj_ctx *ctx, hipStream_t *aux);
+// Makes ctx->stream wait for the work queued so far on the auxiliary stream.
+int aux_join(pj_ctx *ctx);
+// Sends launches through ctx to another stream while in scope.
+struct StreamSwap {
+  pj_ctx *ctx;
+  hipStream_t saved;
+  StreamSwap(pj_ctx *c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+  ~StreamSwap() { ctx->stream = saved; }
+  StreamSwap(const StreamSwap &) = delete;
+  StreamSwap &operator=(const StreamSwap &) = delete;
+};
 int zero(pj_ctx *ctx, void *d, uint64_t bytes);
 int fill_u32(pj_ctx *ctx, uint32_t *d, uint32_t v, uint64_t n);
 

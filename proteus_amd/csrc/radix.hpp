@@ -488,6 +488,15 @@ struct RadixHist {
   }
 };
 
+// Partition sizes and starts of h to the host (waits for ctx->stream).
+inline int radix_readback(pj_ctx *ctx, const RadixPlan &p, RadixHist &h) {
+  h.hT.resize(p.nb);
+  h.hS.resize(p.nb + 1);
+  PJ_TRY(d2h(ctx, h.hT.data(), h.T.p, (uint64_t)p.nb * 4));
+  PJ_TRY(d2h(ctx, h.hS.data(), h.S.p, (uint64_t)(p.nb + 1) * 8));
+  return PJ_OK;
+}
+
 // Histogram + scans for one side into h (device), optionally reading T and S
 // back to the host.
 template <class Joint>
@@ -506,12 +515,7 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
   PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, 1024, 0, h.T.as<const uint32_t>(), p.nb,
             h.S.as<unsigned long long>());
   PJ_TRY(fill_u32(ctx, h.ctr.as<uint32_t>(), 0, 1));
-  if (readback) {
-    h.hT.resize(p.nb);
-    h.hS.resize(p.nb + 1);
-    PJ_TRY(d2h(ctx, h.hT.data(), h.T.p, (uint64_t)p.nb * 4));
-    PJ_TRY(d2h(ctx, h.hS.data(), h.S.p, (uint64_t)(p.nb + 1) * 8));
-  }
+  if (readback) PJ_TRY(radix_readback(ctx, p, h));
   return PJ_OK;
 }
 
