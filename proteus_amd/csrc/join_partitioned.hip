@@ -29,43 +29,91 @@
 
 namespace pj {
 
-__global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb,
-                          uint32_t *__restrict__ E, uint32_t *__restrict__ T) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_colsum(const uint32_t *__restrict__ H, int G, int nb,
+                                                int slab, uint32_t *__restrict__ P) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
   if (d >= nb) return;
-  uint32_t run = 0;
-  for (int c = 0; c < G; ++c) {
-    const uint64_t i = (uint64_t)c * nb + d;
-    const uint32_t h = H[i];
-    E[i] = run;
-    run += h;
+  const int c0 = blockIdx.y * slab, c1 = min(c0 + slab, G);
+  uint32_t s = 0;
+  for (int c = c0; c < c1; c += 8) {  // 8 independent loads in flight
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = c + j < c1 ? H[(uint64_t)(c + j) * nb + d] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += x[j];
   }
-  T[d] = run;
+  P[(uint64_t)blockIdx.y * nb + d] = s;
+}
+
+__global__ __launch_bounds__(256) void k_colscan(const uint32_t *__restrict__ H, int G, int nb,
+                                                 int slab, const uint32_t *__restrict__ P,
+                                                 uint32_t *__restrict__ E,
+                                                 uint32_t *__restrict__ T) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= nb) return;
+  const int y = blockIdx.y;
+  const int c0 = y * slab, c1 = min(c0 + slab, G);
+  uint32_t run = 0;
+  for (int q = 0; q < y; q += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = q + j < y ? P[(uint64_t)(q + j) * nb + d] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) run += x[j];
+  }
+  for (int c = c0; c < c1; c += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = c + j < c1 ? H[(uint64_t)(c + j) * nb + d] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (c + j < c1) {
+        E[(uint64_t)(c + j) * nb + d] = run;
+        run += x[j];
+      }
+  }
+  if (c1 == G) T[d] = run;
 }
 
 __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__ T, int nb,
                                                     unsigned long long *__restrict__ S) {
-  __shared__ unsigned long long part[1024];
+  __shared__ unsigned long long tmp[16];
   const int tid = threadIdx.x;
   const int per = (nb + 1023) / 1024;
-  const int lo = tid * per, hi = lo + per < nb ? lo + per : nb;
+  const int lo = min(tid * per, nb), hi = min(lo + per, nb);
   unsigned long long sum = 0;
-  for (int i = lo; i < hi; ++i) sum += T[i];
-  part[tid] = sum;
-  __syncthreads();
-  // Hillis-Steele scan of the 1024 partial sums
-  for (int o = 1; o < 1024; o <<= 1) {
-    unsigned long long x = tid >= o ? part[tid - o] : 0ULL;
-    __syncthreads();
-    part[tid] += x;
-    __syncthreads();
+  for (int i = lo; i < hi; i += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = i + j < hi ? T[i + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += x[j];
   }
-  unsigned long long run = part[tid] - sum;
-  for (int i = lo; i < hi; ++i) {
-    S[i] = run;
-    run += T[i];
+  unsigned long long tot;
+  unsigned long long run = block_excl_scan<1024, unsigned long long>(sum, tmp, &tot);
+  for (int i = lo; i < hi; i += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = i + j < hi ? T[i + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (i + j < hi) {
+        S[i + j] = run;
+        run += x[j];
+      }
   }
-  if (tid == 1023) S[nb] = part[1023];
+  if (tid == 0) S[nb] = tot;
+}
+
+int colscan_slabs(const pj_ctx *ctx, int G, int nb, int *slab) {
+  // about 2048 threads per CU over (bin, slab) pairs, slabs of >= 8 chunks
+  const uint64_t want = (uint64_t)ctx->num_cus * 2048;
+  uint64_t ns = want / (uint64_t)(nb > 0 ? nb : 1);
+  if (ns < 1) ns = 1;
+  int sl = (int)ceil_div((uint64_t)(G > 0 ? G : 1), ns);
+  if (sl < 8) sl = 8;
+  *slab = sl;
+  return (int)ceil_div((uint64_t)(G > 0 ? G : 1), (uint64_t)sl);
 }
 
 RadixPlan radix_make_plan(pj_ctx *ctx, uint64_t n, int b1, int b2, int nb, int per_cu) {
@@ -251,13 +299,16 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
           uint32_t rg[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
+          // entries are read only by the lanes whose bucket holds them
+          // (exec-masked LDS reads: most buckets hold zero or one entry)
           int2 e0[4], e1[4], e2[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            const uint32_t st = rg[c] & 0xffffu;  // == NCAP when the tail buckets are empty
-            e0[c] = ent[st < (uint32_t)NCAP ? st : 0u];
-            e1[c] = ent[st + 1 < (uint32_t)NCAP ? st + 1 : 0u];
-            e2[c] = ent[st + 2 < (uint32_t)NCAP ? st + 2 : 0u];
+            const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+            e0[c] = e1[c] = e2[c] = make_int2(0, 0);
+            if (en > st) e0[c] = ent[st];
+            if (en > st + 1) e1[c] = ent[st + 1];
+            if (en > st + 2) e2[c] = ent[st + 2];
           }
           // buckets of up to three entries without branches (~98% of probes
           // at the planner's load); longer ones finish in probe_multi
@@ -362,20 +413,19 @@ static size_t pack_add(std::vector<uint8_t> &buf, const std::vector<T> &v) {
   return off;
 }
 
-// Tile / block of each pass by fan-out: larger fan-outs need more LDS for
-// the write-combining lines, so they run one 512-thread workgroup per CU with
-// 8192-tuple tiles; smaller ones two 256-thread workgroups with 4096.
+// Tile / block of each pass by fan-out.  Every fan-out writes 32-tuple
+// (128-byte) lines: 64-byte lines measured 15-20% slower.  Fan-out 256 needs
+// 64 KiB of line buffers, so it runs one 1024-thread workgroup per CU with
+// 8192-tuple tiles; smaller ones two 512-thread workgroups with 4096.
 template <int F>
 static int join_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
                      const int32_t *iv, int32_t *ok, int32_t *ov, uint64_t n,
                      const RadixPlan &p, JointHashed j, RadixHist &h) {
   // pass 1 replaces keys by fmix32(key); pass 2 and the join read digits,
   // buckets and equality off the hashes
-  // LDS write-combining; fan-out 256 uses 16-tuple (64 B) lines to keep two
-  // workgroups per CU
-  constexpr int T = 4096;
-  constexpr int BLOCK = 512;
-  constexpr int LINE = F >= 256 ? 16 : 32;
+  constexpr int T = F >= 256 ? 8192 : 4096;
+  constexpr int BLOCK = F >= 256 ? 1024 : 512;
+  constexpr int LINE = 32;
   if (pass2)
     return radix_launch_pass<F, T, BLOCK, JointHashed, int32_t, NoVal, false, LINE, false>(
         ctx, name, true, ik, iv, nullptr, ok, ov, nullptr, n, p, j, h);
@@ -416,14 +466,19 @@ struct Side {
 constexpr uint64_t kSkew[4] = {0, 17 * 4096, 33 * 4096, 49 * 4096};
 static int32_t *skewed(Scratch &b, int i) { return (int32_t *)((char *)b.p + kSkew[i]); }
 
-// Queues the side's histograms and passes on ctx->stream; the partition
-// sizes are read back separately.
-static int partition_side(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
-                          uint64_t n, int b1, int b2, const char *tag) {
+// Queues the side's histograms on ctx->stream.
+static int side_histograms(pj_ctx *ctx, Side &s, const int32_t *d_k, uint64_t n, int b1,
+                           int b2) {
   const int B = b1 + b2;
   s.plan = radix_make_plan(ctx, n, b1, b2, 1 << B, 2);
   PJ_TRY(s.hist.alloc(ctx, s.plan));
-  PJ_TRY(radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, false));
+  return radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, false);
+}
+
+// Queues the side's radix passes on ctx->stream (after its histograms).
+static int side_passes(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
+                       uint64_t n, int b1, int b2, const char *tag) {
+  const int B = b1 + b2;
   JointHashed j{32 - B};
   PJ_TRY(s.k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
   PJ_TRY(s.v1.alloc(ctx, n * 4 + 256 + kSkew[1]));
@@ -532,9 +587,11 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       int rc;
       {
         StreamSwap sw(ctx, aux);
-        rc = partition_side(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build");
+        rc = side_histograms(ctx, R, d_bk, nb, cfg.bits1, cfg.bits2);
+        if (rc == PJ_OK) rc = side_passes(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build");
       }
-      if (rc == PJ_OK) rc = partition_side(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe");
+      if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, cfg.bits1, cfg.bits2);
+      if (rc == PJ_OK) rc = side_passes(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe");
       // join even on failure, so no scratch is released under running work
       const int rj = aux_join(ctx);
       if (rc != PJ_OK) {
@@ -579,7 +636,10 @@ static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
     c.bits1 = B;
     c.bits2 = 0;
   } else {
-    c.bits1 = B / 2 + (B & 1);
+    // the odd bit goes to pass 2: a fan-out of 256 runs closer to copy speed
+    // over pass 1's output (runs already grouped by the first digit) than
+    // over the original input
+    c.bits1 = B / 2;
     c.bits2 = B - c.bits1;
   }
 }

@@ -54,23 +54,22 @@ struct JointMod {  // hashInt32(key) URem buckets (gpu-hash-rearrange.cpp:338-34
 };
 
 // Block-wide exclusive scan of one value per thread.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp,
-                                                    uint32_t *total) {
+template <int BLOCK, class U>
+__device__ __forceinline__ U block_excl_scan(U v, U *tmp, U *total) {
   constexpr int NW = BLOCK / 64;
   int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t incl = v;
+  U incl = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    uint32_t x = __shfl_up(incl, o, 64);
+    U x = __shfl_up(incl, o, 64);
     if (lane >= o) incl += x;
   }
   if (lane == 63) tmp[wid] = incl;
   __syncthreads();
-  uint32_t pre = 0, tot = 0;
+  U pre = 0, tot = 0;
 #pragma unroll
   for (int w = 0; w < NW; ++w) {
-    uint32_t t = tmp[w];
+    U t = tmp[w];
     pre += (w < wid) ? t : 0;
     tot += t;
   }
@@ -155,10 +154,17 @@ __global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__
   for (int i = threadIdx.x; i < nb; i += 1024) row[i] = hsh[i];
 }
 
-// Column-wise exclusive scan over chunks: E[c][d] = sum_{c'<c} H[c'][d];
-// T[d] = sum over all chunks.
-__global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb,
-                          uint32_t *__restrict__ E, uint32_t *__restrict__ T);
+// Column-wise exclusive scan over chunks, E[c][d] = sum_{c'<c} H[c'][d] and
+// T[d] = sum over all chunks, as two launches over (bin, slab of `slab`
+// chunks) threads: k_colsum writes each slab's per-bin sum to P[slab][d];
+// k_colscan adds the sums of the earlier slabs and walks its own.
+__global__ void k_colsum(const uint32_t *__restrict__ H, int G, int nb, int slab,
+                         uint32_t *__restrict__ P);
+__global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb, int slab,
+                          const uint32_t *__restrict__ P, uint32_t *__restrict__ E,
+                          uint32_t *__restrict__ T);
+// Number of slabs (return) and chunks per slab for G chunks and nb bins.
+int colscan_slabs(const pj_ctx *ctx, int G, int nb, int *slab);
 // Exclusive scan of T[0..nb) into S[0..nb] (one workgroup of 1024 threads).
 __global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
                             unsigned long long *__restrict__ S);
@@ -475,10 +481,13 @@ RadixPlan radix_make_plan(pj_ctx *ctx, uint64_t n, int b1, int b2, int nb, int p
 
 // Device scratch of one side's histograms.
 struct RadixHist {
-  Scratch H, E, T, S, ctr;
+  Scratch H, E, T, S, P, ctr;
   std::vector<uint32_t> hT;            // partition sizes (host)
   std::vector<unsigned long long> hS;  // partition starts (host), nb + 1
+  int slab = 0, nslab = 0;             // column-scan geometry
   int alloc(pj_ctx *ctx, const RadixPlan &p) {
+    nslab = colscan_slabs(ctx, p.G, p.nb, &slab);
+    PJ_TRY(P.alloc(ctx, (uint64_t)nslab * p.nb * 4));
     PJ_TRY(H.alloc(ctx, (uint64_t)p.G * p.nb * 4));
     PJ_TRY(E.alloc(ctx, (uint64_t)p.G * p.nb * 4));
     PJ_TRY(T.alloc(ctx, (uint64_t)p.nb * 4));
@@ -510,8 +519,11 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
   }
   PJ_LAUNCH(ctx, "radix_hist", k_chunk_hist<Joint>, p.G, 1024, (size_t)p.nb * 4, d_k, n,
             p.chunk, p.nb, joint, h.H.as<uint32_t>());
-  PJ_LAUNCH(ctx, "radix_colscan", k_colscan, ceil_div(p.nb, 256), 256, 0,
-            h.H.as<const uint32_t>(), p.G, p.nb, h.E.as<uint32_t>(), h.T.as<uint32_t>());
+  const dim3 cgrid((uint32_t)ceil_div(p.nb, 256), (uint32_t)h.nslab);
+  PJ_LAUNCH(ctx, "radix_colsum", k_colsum, cgrid, 256, 0, h.H.as<const uint32_t>(), p.G, p.nb,
+            h.slab, h.P.as<uint32_t>());
+  PJ_LAUNCH(ctx, "radix_colscan", k_colscan, cgrid, 256, 0, h.H.as<const uint32_t>(), p.G, p.nb,
+            h.slab, h.P.as<const uint32_t>(), h.E.as<uint32_t>(), h.T.as<uint32_t>());
   PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, 1024, 0, h.T.as<const uint32_t>(), p.nb,
             h.S.as<unsigned long long>());
   PJ_TRY(fill_u32(ctx, h.ctr.as<uint32_t>(), 0, 1));

@@ -143,6 +143,7 @@ CFGS = [
     dict(bits1=3, bits2=0, lds_log=11, probe_unit=2048, grid_mul=1),  # single pass
     dict(bits1=4, bits2=3, lds_log=12, probe_unit=8192, grid_mul=1),  # two passes
     dict(bits1=8, bits2=7, lds_log=12, probe_unit=65536, grid_mul=0), # max fan-out
+    dict(bits1=7, bits2=8, lds_log=13, probe_unit=65536, grid_mul=0), # fan-out 256 in pass 2
     dict(bits1=1, bits2=0, lds_log=11, probe_unit=1024, grid_mul=2),  # LDS rounds (overflow)
 ]
 
