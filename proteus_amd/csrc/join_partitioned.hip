@@ -279,23 +279,20 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
           const unsigned long long x = tid < tail_lo ? s0 + tid : a1 + (tid - tail_lo);
           probe1(sk[x], sv ? sv[x] : 0);
         }
+        // Three 4-tuple steps rotate through registers A, B, C: a step's
+        // loads are issued two steps before it is probed.  The loads are
+        // unconditional (indices clamped to the last vector, results past
+        // the end unused): a load under a branch, or a register copy of a
+        // loaded value, makes the compiler wait for outstanding loads there,
+        // which would serialise the prefetch with the probe.  Without probe
+        // values the key column stands in for them.
         const int4 *kq = reinterpret_cast<const int4 *>(sk + a0);
-        const int4 *vq = sv ? reinterpret_cast<const int4 *>(sv + a0) : nullptr;
-        const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);
-        uint32_t q = tid;
-        int4 k4 = make_int4(0, 0, 0, 0), v4 = make_int4(0, 0, 0, 0);
-        if (q < nvec) {
-          k4 = ld_nt4(kq + q);
-          if (vq) v4 = ld_nt4(vq + q);
-        }
-        for (; q < nvec; q += BLOCK) {
-          int4 kn = make_int4(0, 0, 0, 0), vn = make_int4(0, 0, 0, 0);
-          if (q + BLOCK < nvec) {
-            kn = ld_nt4(kq + q + BLOCK);
-            if (vq) vn = ld_nt4(vq + q + BLOCK);
-          }
+        const int4 *vq = reinterpret_cast<const int4 *>((sv ? sv : sk) + a0);
+        const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);  // >= 1 here
+        auto cl = [nvec](uint32_t x) { return x < nvec ? x : nvec - 1; };
+        auto probe4 = [&](const int4 k4, const int4 v4) {
           const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-          const int32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+          const int32_t vv[4] = {sv ? v4.x : 0, sv ? v4.y : 0, sv ? v4.z : 0, sv ? v4.w : 0};
           uint32_t rg[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
@@ -311,14 +308,36 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
             if (en > st + 2) e2[c] = ent[st + 2];
           }
           // buckets of up to three entries without branches (~98% of probes
-          // at the planner's load); longer ones finish in probe_multi
-          bool multi = false;
+          // at the planner's load); longer ones finish in probe_multi.  A
+          // row with one match among them (always, for unique build keys)
+          // takes the single-match accumulate; rows matching twice or more
+          // take the general one.
+          bool multi = false, dup = false;
+          bool h1m[4];
+          int32_t bs[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
-            acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y, (en > st + 1) & (e1[c].x == kk[c]),
-                        e1[c].y, (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
+            const bool h0 = (en > st) & (e0[c].x == kk[c]);
+            const bool h1 = (en > st + 1) & (e1[c].x == kk[c]);
+            const bool h2 = (en > st + 2) & (e2[c].x == kk[c]);
+            h1m[c] = h0 | h1 | h2;
+            // the matching entry's value when there is one match
+            bs[c] = (e0[c].y & -(int32_t)h0) | (e1[c].y & -(int32_t)h1) |
+                    (e2[c].y & -(int32_t)h2);
+            dup |= (h0 & h1) | (h0 & h2) | (h1 & h2);
             multi |= en > st + 3;
+          }
+          if (!dup) {
+            acc.add4_1(h1m, bs, vv);
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+              acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y,
+                          (en > st + 1) & (e1[c].x == kk[c]), e1[c].y,
+                          (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
+            }
           }
           if (multi) {
 #pragma unroll
@@ -327,9 +346,24 @@ __global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) voi
               if (en > st + 3) probe_multi(kk[c], vv[c], st + 3, en);
             }
           }
-          k4 = kn;
-          v4 = vn;
+        };
+        uint32_t q = tid;
+        int4 kA = ld_nt4(kq + cl(q)), vA = ld_nt4(vq + cl(q));
+        int4 kB = ld_nt4(kq + cl(q + BLOCK)), vB = ld_nt4(vq + cl(q + BLOCK));
+        // whole triples of steps: a straight-line body, so the compiler's
+        // waits stay precise across the back edge
+        for (; q + 2 * BLOCK < nvec; q += 3 * BLOCK) {
+          const int4 kC = ld_nt4(kq + q + 2 * BLOCK), vC = ld_nt4(vq + q + 2 * BLOCK);
+          probe4(kA, vA);
+          kA = ld_nt4(kq + cl(q + 3 * BLOCK));
+          vA = ld_nt4(vq + cl(q + 3 * BLOCK));
+          probe4(kB, vB);
+          kB = ld_nt4(kq + cl(q + 4 * BLOCK));
+          vB = ld_nt4(vq + cl(q + 4 * BLOCK));
+          probe4(kC, vC);
         }
+        if (q < nvec) probe4(kA, vA);
+        if (q + BLOCK < nvec) probe4(kB, vB);
       }
     }
   }

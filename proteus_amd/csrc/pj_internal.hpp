@@ -274,16 +274,6 @@ struct JoinAcc {
     min_p = min(min_p, pv);
     max_p = max(max_p, pv);
   }
-  // one match if `hit`, without branching
-  __device__ __forceinline__ void add1_if(bool hit, int32_t b, int32_t pv) {
-    n1 += hit ? 1u : 0u;
-    sum_b += hit ? (int64_t)b : 0;
-    sum_p += hit ? (int64_t)pv : 0;
-    min_b = hit ? min(min_b, b) : min_b;
-    max_b = hit ? max(max_b, b) : max_b;
-    min_p = hit ? min(min_p, pv) : min_p;
-    max_p = hit ? max(max_p, pv) : max_p;
-  }
   // up to three matches of one probe row (flags h0..h2 over build values
   // b0..b2), without branches
   __device__ __forceinline__ void add3_if(bool h0, int32_t b0, bool h1, int32_t b1, bool h2,
@@ -298,6 +288,27 @@ struct JoinAcc {
     max_b = max(max_b, mx);
     min_p = m ? min(min_p, pv) : min_p;
     max_p = m ? max(max_p, pv) : max_p;
+  }
+  // Four probe rows with at most one match each: row c matched iff h[c],
+  // with build value b[c] and probe value p[c].
+  __device__ __forceinline__ void add4_1(const bool h[4], const int32_t b[4],
+                                         const int32_t p[4]) {
+    int32_t mnb = INT32_MAX, mxb = INT32_MIN, mnp = INT32_MAX, mxp = INT32_MIN;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t m = -(int32_t)h[c];  // all ones on a match
+      n1 += (uint32_t)h[c];
+      sum_b += (int64_t)(b[c] & m);
+      sum_p += (int64_t)(p[c] & m);
+      mnb = min(mnb, h[c] ? b[c] : INT32_MAX);
+      mxb = max(mxb, h[c] ? b[c] : INT32_MIN);
+      mnp = min(mnp, h[c] ? p[c] : INT32_MAX);
+      mxp = max(mxp, h[c] ? p[c] : INT32_MIN);
+    }
+    min_b = min(min_b, mnb);
+    max_b = max(max_b, mxb);
+    min_p = min(min_p, mnp);
+    max_p = max(max_p, mxp);
   }
   // `m` matches of one probe row whose build values sum to sb and span
   // [mnb, mxb].
