@@ -114,6 +114,25 @@ struct ValIO {
   __device__ static __forceinline__ bool aligned(const V *p, uint64_t base) {
     return ((uintptr_t)(p + base) & 15) == 0;
   }
+  // Elements 4*vi .. 4*vi+3 of a 16-byte-aligned column, unconditionally.
+  __device__ static __forceinline__ void load4_vec(const V *__restrict__ p, uint64_t vi,
+                                                   V out[4]) {
+    const char *q = reinterpret_cast<const char *>(p + vi * 4);
+    if constexpr (sizeof(V) == 4) {
+      int4 a = ld_nt4(q);
+      out[0] = __builtin_bit_cast(V, a.x);
+      out[1] = __builtin_bit_cast(V, a.y);
+      out[2] = __builtin_bit_cast(V, a.z);
+      out[3] = __builtin_bit_cast(V, a.w);
+    } else {
+      int4 a = ld_nt4(q);
+      int4 b = ld_nt4(q + 16);
+      out[0] = __builtin_bit_cast(V, make_int2(a.x, a.y));
+      out[1] = __builtin_bit_cast(V, make_int2(a.z, a.w));
+      out[2] = __builtin_bit_cast(V, make_int2(b.x, b.y));
+      out[3] = __builtin_bit_cast(V, make_int2(b.z, b.w));
+    }
+  }
 };
 template <>
 struct ValIO<NoVal> {
@@ -121,6 +140,7 @@ struct ValIO<NoVal> {
   __device__ static __forceinline__ void load4(const NoVal *, uint64_t, uint32_t, uint32_t,
                                                uint32_t, bool, NoVal *) {}
   __device__ static __forceinline__ bool aligned(const NoVal *, uint64_t) { return true; }
+  __device__ static __forceinline__ void load4_vec(const NoVal *, uint64_t, NoVal *) {}
 };
 
 // Per-chunk joint histogram: chunk c = [c*chunk, min(n, (c+1)*chunk)) into
@@ -174,7 +194,7 @@ __global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
 // of the pass-1 output from a counter and partition them on the pass-2 digit.
 // ROWID: the V1 (int32) column is the input row index.
 template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, bool PASS2,
-          int LINE = kLine, bool HASHKEYS = false>
+          int LINE = kLine, bool HASHKEYS = false, bool VEC = false>
 __global__ __launch_bounds__(BLOCK) void k_radix_pass(
     const int32_t *__restrict__ in_k, const V1 *__restrict__ in_v,
     const V2 *__restrict__ in_w, int32_t *__restrict__ out_k, V1 *__restrict__ out_v,
@@ -287,35 +307,55 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
     const uint64_t span = seg_hi > base0 ? seg_hi - base0 : 0;
     const uint32_t ntiles = (uint32_t)((span + T - 1) / T);
 
-    int32_t k[IPT], kn[IPT];
-    V1 v[IPT], vn[IPT];
-    V2 w[IPT], wn[IPT];
+    // Tiles alternate between register sets A and B: tile t + 1 streams in
+    // while tile t is shuffled and written.  Alternating (instead of copying
+    // the next tile's registers over the current ones) and, with VEC,
+    // unconditional loads keep the compiler from waiting for the prefetch
+    // before the current tile is processed.
+    // VEC (host-checked: every column 16-byte aligned, n a multiple of 4):
+    // whole aligned vectors, indices clamped to the column's last vector;
+    // elements outside the tile's [lo, len) are loaded but never used.
     auto load_tile = [&](uint32_t t, int32_t *kk, V1 *vv, V2 *ww) {
       const uint64_t base = base0 + (uint64_t)t * T;
-      const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
-      const uint32_t lo = t == 0 ? lo0 : 0;
-      const bool vec = (((uintptr_t)(in_k + base) & 15) == 0) &&
-                       (ROWID || ValIO<V1>::aligned(in_v, base)) &&
-                       ValIO<V2>::aligned(in_w, base);
+      if constexpr (VEC) {
+        const uint64_t vlast = n / 4 - 1;
 #pragma unroll
-      for (int q = 0; q < IPT / 4; ++q) {
-        uint32_t e = (uint32_t)(q * BLOCK + tid) * 4;
-        ValIO<int32_t>::load4(in_k, base, e, lo, len, vec, &kk[4 * q]);
-        if constexpr (ROWID) {
+        for (int q = 0; q < IPT / 4; ++q) {
+          const uint32_t e = (uint32_t)(q * BLOCK + tid) * 4;
+          uint64_t vi = (base + e) / 4;
+          vi = vi < vlast ? vi : vlast;
+          ValIO<int32_t>::load4_vec(in_k, vi, &kk[4 * q]);
+          if constexpr (ROWID) {
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
-            vv[4 * q + c] = __builtin_bit_cast(V1, (int32_t)(base + e + c));
-        } else {
-          ValIO<V1>::load4(in_v, base, e, lo, len, vec, &vv[4 * q]);
+            for (int c = 0; c < 4; ++c)
+              vv[4 * q + c] = __builtin_bit_cast(V1, (int32_t)(base + e + c));
+          } else {
+            ValIO<V1>::load4_vec(in_v, vi, &vv[4 * q]);
+          }
+          ValIO<V2>::load4_vec(in_w, vi, &ww[4 * q]);
         }
-        ValIO<V2>::load4(in_w, base, e, lo, len, vec, &ww[4 * q]);
+      } else {
+        const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
+        const uint32_t lo = t == 0 ? lo0 : 0;
+        const bool vec = (((uintptr_t)(in_k + base) & 15) == 0) &&
+                         (ROWID || ValIO<V1>::aligned(in_v, base)) &&
+                         ValIO<V2>::aligned(in_w, base);
+#pragma unroll
+        for (int q = 0; q < IPT / 4; ++q) {
+          uint32_t e = (uint32_t)(q * BLOCK + tid) * 4;
+          ValIO<int32_t>::load4(in_k, base, e, lo, len, vec, &kk[4 * q]);
+          if constexpr (ROWID) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              vv[4 * q + c] = __builtin_bit_cast(V1, (int32_t)(base + e + c));
+          } else {
+            ValIO<V1>::load4(in_v, base, e, lo, len, vec, &vv[4 * q]);
+          }
+          ValIO<V2>::load4(in_w, base, e, lo, len, vec, &ww[4 * q]);
+        }
       }
     };
-    if (ntiles) load_tile(0, k, v, w);
-
-    for (uint32_t t = 0; t < ntiles; ++t) {
-      // the next tile streams in while this one is shuffled and written
-      if (t + 1 < ntiles) load_tile(t + 1, kn, vn, wn);
+    auto process_tile = [&](uint32_t t, int32_t *k, V1 *v, V2 *w) {
       const uint64_t base = base0 + (uint64_t)t * T;
       const uint32_t len = (uint32_t)((seg_hi - base) < (uint64_t)T ? (seg_hi - base) : T);
       const uint32_t lo = t == 0 ? lo0 : 0;
@@ -444,15 +484,20 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
         hist[tid] = 0;
       }
       __syncthreads();
-      if (t + 1 < ntiles) {
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-          k[i] = kn[i];
-          v[i] = vn[i];
-          if constexpr (HW) w[i] = wn[i];
-        }
-      }
+    };
+    int32_t kA[IPT], kB[IPT];
+    V1 vA[IPT], vB[IPT];
+    V2 wA[IPT], wB[IPT];
+    if (ntiles) load_tile(0, kA, vA, wA);
+    uint32_t t = 0;
+    for (; t + 1 < ntiles; t += 2) {
+      load_tile(t + 1, kB, vB, wB);
+      process_tile(t, kA, vA, wA);
+      // past the last tile: the last tile again (cached), never processed
+      load_tile(t + 2 < ntiles ? t + 2 : ntiles - 1, kA, vA, wA);
+      process_tile(t + 1, kB, vB, wB);
     }
+    if (t < ntiles) process_tile(t, kA, vA, wA);
     // ---- flush the remainders of this segment --------------------------------
     for (int x = tid; x < F * LINE; x += BLOCK) {
       const int d = x / LINE, s = x % LINE;
@@ -532,13 +577,13 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
 }
 
 // One pass launch (pass2 selects the segment-walking variant).
-template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE = kLine,
-          bool HASHKEYS = false>
-int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
-                      const V1 *iv, const V2 *iw, int32_t *ok, V1 *ov, V2 *ow, uint64_t n,
-                      const RadixPlan &p, Joint joint, RadixHist &h) {
+template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE,
+          bool HASHKEYS, bool VEC>
+int radix_launch_pass_v(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
+                        const V1 *iv, const V2 *iw, int32_t *ok, V1 *ov, V2 *ow, uint64_t n,
+                        const RadixPlan &p, Joint joint, RadixHist &h) {
   if (!pass2) {
-    PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE, HASHKEYS>),
+    PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE, HASHKEYS, VEC>),
               p.G, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, 1u, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
               h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
   } else {
@@ -555,7 +600,7 @@ int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *
       ngrp = (uint32_t)ceil_div(p.G, cpg);
       uint64_t nseg = (uint64_t)ngrp * F1;
       if (grid > nseg) grid = nseg;
-      PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE, HASHKEYS>),
+      PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE, HASHKEYS, VEC>),
                 grid, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, cpg, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
                 h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
     } else {
@@ -563,6 +608,23 @@ int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *
     }
   }
   return PJ_OK;
+}
+
+// Picks the whole-vector load variant when every input column is 16-byte
+// aligned and n is a multiple of 4.
+template <int F, int T, int BLOCK, class Joint, class V1, class V2, bool ROWID, int LINE = kLine,
+          bool HASHKEYS = false>
+int radix_launch_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
+                      const V1 *iv, const V2 *iw, int32_t *ok, V1 *ov, V2 *ow, uint64_t n,
+                      const RadixPlan &p, Joint joint, RadixHist &h) {
+  auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+  const bool vec = n >= 4 && n % 4 == 0 && a16(ik) && (ROWID || a16(iv)) &&
+                   (!ValIO<V2>::present || a16(iw));
+  if (vec)
+    return radix_launch_pass_v<F, T, BLOCK, Joint, V1, V2, ROWID, LINE, HASHKEYS, true>(
+        ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, joint, h);
+  return radix_launch_pass_v<F, T, BLOCK, Joint, V1, V2, ROWID, LINE, HASHKEYS, false>(
+      ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, joint, h);
 }
 
 }  // namespace pj
