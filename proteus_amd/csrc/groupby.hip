@@ -23,6 +23,7 @@
 // (Proteus float is FP64, core/olap/lib/values/expressionTypes.cpp:87-89).
 #include <algorithm>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "radix.hpp"
@@ -31,7 +32,7 @@ namespace pj {
 
 constexpr int kGbSlots = 2048;   // LDS table slots (96 KiB)
 constexpr int kGbCap = 1024;     // groups per table the planner aims for
-constexpr int kGbBlock = 512;
+constexpr int kGbBlock = 1024;  // one workgroup per CU (96 KiB of LDS), 16 waves
 constexpr int kGbMergeFanIn = 64;
 
 struct GbEntries {  // SoA entry arrays of one level
@@ -68,14 +69,32 @@ __device__ __forceinline__ bool gb_insert(unsigned long long *tk, int32_t key,
   return false;
 }
 
+// Adds row (key, v, w) to the LDS table; false when the table is full.
+template <bool HV, bool HW>
+__device__ __forceinline__ bool gb_add_row(unsigned long long *tk, unsigned long long *tcnt,
+                                           long long *tsum, long long *tmin, long long *tmax,
+                                           double *tfs, int32_t key, int64_t v, float w) {
+  uint32_t s;
+  if (!gb_insert(tk, key, s)) return false;
+  atomicAdd(&tcnt[s], 1ULL);
+  if (HV) {
+    atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)v);
+    atomicMin(&tmin[s], (long long)v);
+    atomicMax(&tmax[s], (long long)v);
+  }
+  if (HW) atomicAdd(&tfs[s], (double)w);
+  return true;
+}
+
 // One unit per workgroup iteration.  MERGE = inputs are entries of the
-// previous level's units; otherwise raw rows (k, v?, w?).
-template <bool MERGE>
+// previous level's units; otherwise raw rows (k, v if HV, w if HW).
+template <bool MERGE, bool HV, bool HW>
 __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
     const GbUnit *__restrict__ units, uint32_t nunits,
     const int32_t *__restrict__ rk, const int64_t *__restrict__ rv,
     const float *__restrict__ rw, GbEntries in, const uint32_t *__restrict__ in_count,
-    GbEntries out, uint32_t *__restrict__ out_count, uint32_t *__restrict__ overflow) {
+    GbEntries out, uint32_t *__restrict__ out_count, uint32_t *__restrict__ overflow,
+    bool vec) {
   __shared__ unsigned long long tk[kGbSlots];
   __shared__ unsigned long long tcnt[kGbSlots];
   __shared__ long long tsum[kGbSlots], tmin[kGbSlots], tmax[kGbSlots];
@@ -96,24 +115,73 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
     const GbUnit U = units[u];
     bool ovf = false;
     if (!MERGE) {
-      const uint64_t b = U.begin;
-      const uint32_t n = U.len;
-      for (uint32_t j = tid; j < n; j += kGbBlock) {
-        int32_t key = rk[b + j];
-        int64_t v = rv ? rv[b + j] : 0;
-        double w = rw ? (double)rw[b + j] : 0.0;
-        uint32_t s;
-        if (!gb_insert(tk, key, s)) {
-          ovf = true;
-          continue;
+      // Rows [s0, s1): with `vec` (every column 16-byte aligned) the
+      // 4-aligned interior is read as vectors of 4 rows
+      // (key int4, v 2 x int4, w float4), three register sets rotating so
+      // that a step's loads are issued two steps before it is aggregated;
+      // the loads are unconditional (indices clamped to the last vector)
+      // so the compiler keeps them in flight.  The <= 3 rows before and
+      // after the interior are added one by one.
+      const uint64_t s0 = U.begin, s1 = U.begin + U.len;
+      const uint64_t a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
+      auto row = [&](uint64_t x) {
+        ovf |= !gb_add_row<HV, HW>(tk, tcnt, tsum, tmin, tmax, tfs, rk[x], HV ? rv[x] : 0,
+                                   HW ? rw[x] : 0.f);
+      };
+      if (!vec || a0 >= a1) {
+        for (uint64_t x = s0 + tid; x < s1; x += kGbBlock) row(x);
+      } else {
+        const int head = (int)(a0 - s0), tail = (int)(s1 - a1);
+        if (tid < head + tail) row(tid < head ? s0 + tid : a1 + (tid - head));
+        struct Vec {
+          int4 k;
+          int4 v0, v1;
+          float4 w;
+        };
+        const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);  // >= 1 here
+        const int4 *kq = reinterpret_cast<const int4 *>(rk + a0);
+        const int4 *vq = HV ? reinterpret_cast<const int4 *>(rv + a0) : nullptr;
+        const float4 *wq = HW ? reinterpret_cast<const float4 *>(rw + a0) : nullptr;
+        auto load = [&](uint32_t q) {
+          q = q < nvec ? q : nvec - 1;
+          Vec r;
+          r.k = ld_nt4(kq + q);
+          if (HV) {
+            r.v0 = ld_nt4(vq + 2 * q);
+            r.v1 = ld_nt4(vq + 2 * q + 1);
+          }
+          if (HW) {
+            const int4 t = ld_nt4(wq + q);
+            r.w = make_float4(__int_as_float(t.x), __int_as_float(t.y), __int_as_float(t.z),
+                              __int_as_float(t.w));
+          }
+          return r;
+        };
+        auto agg4 = [&](const Vec &r) {
+          const int32_t kk[4] = {r.k.x, r.k.y, r.k.z, r.k.w};
+          const int64_t vv[4] = {
+              HV ? (int64_t)(((uint64_t)(uint32_t)r.v0.y << 32) | (uint32_t)r.v0.x) : 0,
+              HV ? (int64_t)(((uint64_t)(uint32_t)r.v0.w << 32) | (uint32_t)r.v0.z) : 0,
+              HV ? (int64_t)(((uint64_t)(uint32_t)r.v1.y << 32) | (uint32_t)r.v1.x) : 0,
+              HV ? (int64_t)(((uint64_t)(uint32_t)r.v1.w << 32) | (uint32_t)r.v1.z) : 0};
+          const float ww[4] = {HW ? r.w.x : 0.f, HW ? r.w.y : 0.f, HW ? r.w.z : 0.f,
+                               HW ? r.w.w : 0.f};
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            ovf |= !gb_add_row<HV, HW>(tk, tcnt, tsum, tmin, tmax, tfs, kk[c], vv[c], ww[c]);
+        };
+        uint32_t q = tid;
+        Vec A = load(q), B = load(q + kGbBlock);
+        for (; q + 2 * kGbBlock < nvec; q += 3 * kGbBlock) {
+          const Vec C = load(q + 2 * kGbBlock);
+          agg4(A);
+          A = load(q + 3 * kGbBlock);
+          agg4(B);
+          B = load(q + 4 * kGbBlock);
+          agg4(C);
         }
-        atomicAdd(&tcnt[s], 1ULL);
-        if (rv) {
-          atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)v);
-          atomicMin(&tmin[s], (long long)v);
-          atomicMax(&tmax[s], (long long)v);
-        }
-        if (rw) atomicAdd(&tfs[s], w);
+        if (q < nvec) agg4(A);
+        if (q + kGbBlock < nvec) agg4(B);
       }
     } else {
       for (uint32_t q = 0; q < U.len; ++q) {
@@ -308,10 +376,23 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
   {
     uint32_t grid = (uint32_t)std::min<uint64_t>(units.size(), (uint64_t)ctx->num_cus * 2);
     GbEntries none{};
-    PJ_LAUNCH(ctx, "gb_agg", k_gb_agg<false>, grid, kGbBlock, 0,
-              dunits.as<const GbUnit>(), (uint32_t)units.size(), pk, pv, pw, none,
-              (const uint32_t *)nullptr, level->e, level->count.as<uint32_t>(),
-              ovf.as<uint32_t>());
+    // The vector loads need 16-byte-aligned columns; otherwise every row is
+    // added one by one.
+    auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    const bool vec = a16(pk) && (!pv || a16(pv)) && (!pw || a16(pw));
+    auto launch = [&](auto hv, auto hw) -> int {
+      PJ_LAUNCH(ctx, "gb_agg", (k_gb_agg<false, decltype(hv)::value, decltype(hw)::value>), grid,
+                kGbBlock, 0, dunits.as<const GbUnit>(), (uint32_t)units.size(), pk, pv, pw,
+                none, (const uint32_t *)nullptr, level->e, level->count.as<uint32_t>(),
+                ovf.as<uint32_t>(), vec);
+      return PJ_OK;
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (pv && pw) PJ_TRY(launch(T_{}, T_{}));
+    else if (pv) PJ_TRY(launch(T_{}, F_{}));
+    else if (pw) PJ_TRY(launch(F_{}, T_{}));
+    else PJ_TRY(launch(F_{}, F_{}));
   }
   // ---- merge levels: fold units of the same partition ------------------
   // Units are ordered by partition.  Each level merges up to kGbMergeFanIn
@@ -339,10 +420,10 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     PJ_TRY(dn.alloc(ctx, next.size() * sizeof(GbUnit)));
     PJ_TRY(h2d(ctx, dn.p, next.data(), next.size() * sizeof(GbUnit)));
     uint32_t grid = (uint32_t)std::min<uint64_t>(next.size(), (uint64_t)ctx->num_cus * 2);
-    PJ_LAUNCH(ctx, "gb_merge", k_gb_agg<true>, grid, kGbBlock, 0, dn.as<const GbUnit>(),
+    PJ_LAUNCH(ctx, "gb_merge", (k_gb_agg<true, true, true>), grid, kGbBlock, 0, dn.as<const GbUnit>(),
               (uint32_t)next.size(), (const int32_t *)nullptr, (const int64_t *)nullptr,
               (const float *)nullptr, level->e, level->count.as<const uint32_t>(), nl->e,
-              nl->count.as<uint32_t>(), ovf.as<uint32_t>());
+              nl->count.as<uint32_t>(), ovf.as<uint32_t>(), false);
     PJ_TRY(sync(ctx));
     units.swap(next);
     level.swap(nl);

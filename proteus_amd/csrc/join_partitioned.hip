@@ -151,8 +151,14 @@ struct JoinUnit {
 // HASHED: keys are fmix32(original key) (written by the radix passes); the
 // bucket is their low bits and equality of hashes is equality of keys.
 // Otherwise keys are raw and hashed here.
+// Four waves per SIMD: two 512-thread workgroups per CU, or one of 1024 for
+// the largest table.
+template <int BLOCK, int H, int NCAP>
+constexpr int pjoin_waves_per_simd() {
+  return (H * 4 + NCAP * 8 <= 80 * 1024 || BLOCK >= 1024) ? 4 : 2;
+}
 template <int BLOCK, int H, int NCAP, bool HASHED>
-__global__ __launch_bounds__(BLOCK, (H * 4 + NCAP * 8 <= 80 * 1024 ? 4 : 2)) void k_pjoin(
+__global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
     const JoinUnit *__restrict__ units, uint32_t *__restrict__ unit_hdr,
@@ -538,28 +544,27 @@ static int side_passes(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *
 struct Geometry {
   int lds_log;
   int ncap;
-  int per_cu;  // workgroups per CU the LDS allows
+  int per_cu;         // workgroups per CU the LDS allows
+  int block = 512;    // threads per workgroup
 };
 static Geometry geometry_for(int lds_log) {
   switch (lds_log) {
     case 11: return {11, 3072, 4};
     case 12: return {12, 6144, 2};
-    case 14: return {14, 11264, 1};
+    case 14: return {14, 11264, 1, 1024};
     default: return {13, 5632, 2};
   }
 }
 
-constexpr int kJoinBlock = 512;
-
-template <int H, int NCAP>
+template <int BLOCK, int H, int NCAP>
 static int launch_pjoin_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
                           const int32_t *sk, const int32_t *sv, const JoinUnit *units,
                           uint32_t *hdr, uint32_t grid, unsigned long long *acc) {
   if (hashed)
-    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, true>), grid, kJoinBlock, 0,
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<BLOCK, H, NCAP, true>), grid, BLOCK, 0,
               rk, rv, sk, sv, units, hdr, acc);
   else
-    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<kJoinBlock, H, NCAP, false>), grid, kJoinBlock, 0,
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<BLOCK, H, NCAP, false>), grid, BLOCK, 0,
               rk, rv, sk, sv, units, hdr, acc);
   return PJ_OK;
 }
@@ -570,13 +575,13 @@ static int launch_pjoin(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk
                         unsigned long long *acc) {
   switch (lds_log) {
     case 11:
-      return launch_pjoin_g<2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_g<512, 2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 12:
-      return launch_pjoin_g<4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_g<512, 4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 13:
-      return launch_pjoin_g<8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_g<512, 8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     case 14:
-      return launch_pjoin_g<16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_g<1024, 16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
     default:
       return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", lds_log);
   }
