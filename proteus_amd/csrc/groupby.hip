@@ -327,23 +327,35 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     PJ_TRY(rh.alloc(ctx, plan));
     JointRadix j{32 - pbits};
     PJ_TRY(radix_histograms(ctx, d_k, n, plan, j, rh, true));
-    PJ_TRY(k1.alloc(ctx, n * 4 + 256));
-    PJ_TRY(v1.alloc(ctx, n * 8 + 256));
-    PJ_TRY(w1.alloc(ctx, n * 4 + 256));
-    PJ_TRY(gb_pass_bits(ctx, "gb_scatter1", b1, false, pk, pv, pw, k1.as<int32_t>(),
-                        v1.as<int64_t>(), w1.as<float>(), n, plan, j, rh));
-    pk = k1.as<int32_t>();
-    pv = v1.as<int64_t>();
-    pw = w1.as<float>();
+    // The six partition arrays start at different odd multiples of 4 KiB
+    // inside their allocations: pass 2 reads and writes the same offsets of
+    // equally sized arrays, and bases that coincide modulo the HBM channel
+    // interleave send its read and write streams to the same channels.
+    constexpr uint64_t kSkew[6] = {0, 17 * 4096, 33 * 4096, 49 * 4096, 65 * 4096, 81 * 4096};
+    auto at = [](Scratch &b, int i) { return (char *)b.p + kSkew[i]; };
+    PJ_TRY(k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
+    PJ_TRY(v1.alloc(ctx, n * 8 + 256 + kSkew[1]));
+    PJ_TRY(w1.alloc(ctx, n * 4 + 256 + kSkew[2]));
+    int32_t *ok1 = (int32_t *)at(k1, 0);
+    int64_t *ov1 = (int64_t *)at(v1, 1);
+    float *ow1 = (float *)at(w1, 2);
+    PJ_TRY(gb_pass_bits(ctx, "gb_scatter1", b1, false, pk, pv, pw, ok1, ov1, ow1, n, plan, j,
+                        rh));
+    pk = ok1;
+    pv = ov1;
+    pw = ow1;
     if (b2) {
-      PJ_TRY(k2.alloc(ctx, n * 4 + 256));
-      PJ_TRY(v2.alloc(ctx, n * 8 + 256));
-      PJ_TRY(w2.alloc(ctx, n * 4 + 256));
-      PJ_TRY(gb_pass_bits(ctx, "gb_scatter2", b2, true, pk, pv, pw, k2.as<int32_t>(),
-                          v2.as<int64_t>(), w2.as<float>(), n, plan, j, rh));
-      pk = k2.as<int32_t>();
-      pv = v2.as<int64_t>();
-      pw = w2.as<float>();
+      PJ_TRY(k2.alloc(ctx, n * 4 + 256 + kSkew[3]));
+      PJ_TRY(v2.alloc(ctx, n * 8 + 256 + kSkew[4]));
+      PJ_TRY(w2.alloc(ctx, n * 4 + 256 + kSkew[5]));
+      int32_t *ok2 = (int32_t *)at(k2, 3);
+      int64_t *ov2 = (int64_t *)at(v2, 4);
+      float *ow2 = (float *)at(w2, 5);
+      PJ_TRY(gb_pass_bits(ctx, "gb_scatter2", b2, true, pk, pv, pw, ok2, ov2, ow2, n, plan, j,
+                          rh));
+      pk = ok2;
+      pv = ov2;
+      pw = ow2;
     }
     const int NB = 1 << pbits;
     part_start.assign(rh.hS.begin(), rh.hS.begin() + NB);
