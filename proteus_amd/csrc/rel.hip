@@ -771,6 +771,12 @@ struct ChainArgs {
   int32_t ctype[kChainMaxCols];
   int32_t clevel[kChainMaxCols];
   ChainStage st[kChainMaxStages];
+  // Key column of stage s when it is a probe on a source-row int32 column
+  // (pf_on[s] = 1); such keys are loaded one stage ahead, beside the
+  // previous stage's probes.  Otherwise pf[s] points at a one-element
+  // dummy read at index 0.
+  const int32_t *pf[kChainMaxStages];
+  int32_t pf_on[kChainMaxStages];
   ChainOut outs[kPipeMaxOut];
   PipeTable tables[kPipeMaxLevels];
   uint64_t n;
@@ -831,17 +837,42 @@ struct ChainRows {
 
 // Stage S of K (template recursion: every stage's descriptor is a
 // compile-time index, so it stays in registers).
+// Loads stage S's prefetchable keys (unconditionally: rows that are dead or
+// not prefetchable read index 0 of the stage's column or dummy).
 template <int S, int K, int R>
-__device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &cr, bool &live) {
+__device__ __forceinline__ void chain_prefetch(const ChainArgs &a, const ChainRows<R> &cr,
+                                               int32_t (&pre)[R]) {
   if constexpr (S < K) {
+    const int32_t *c = a.pf[S];
+    const bool on = a.pf_on[S] != 0;
+    uint32_t ix[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) ix[k] = (on && cr.alive[k]) ? cr.row[k] : 0u;
+#pragma unroll
+    for (int k = 0; k < R; ++k) pre[k] = c[ix[k]];
+  }
+}
+
+template <int S, int K, int R>
+__device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &cr, bool &live,
+                                             int32_t (&pre)[R]) {
+  if constexpr (S < K) {
+    int32_t nxt[R];
     if (live) {
       const ChainStage sg = a.st[S];
       const void *c = a.cols[sg.col];
       const int ct = a.ctype[sg.col];
       int64_t v[R];
-      uint32_t ix[R];
-      cr.at(a.clevel[sg.col], ix);
-      gather_col<R>(c, ct, ix, v);
+      if (a.pf_on[S]) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = pre[k];
+      } else {
+        uint32_t ix[R];
+        cr.at(a.clevel[sg.col], ix);
+        gather_col<R>(c, ct, ix, v);
+      }
+      // the next stage's keys stream in beside this stage's probes
+      chain_prefetch<S + 1, K, R>(a, cr, nxt);
       if (sg.kind == 1) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
@@ -888,7 +919,30 @@ __device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &c
       for (int k = 0; k < R; ++k) any_alive |= cr.alive[k];
       live = __any(any_alive);
     }
-    chain_stages<S + 1, K, R>(a, cr, live);
+    chain_stages<S + 1, K, R>(a, cr, live, nxt);
+  }
+}
+
+// Loads R rows of an output column as raw 64-bit slots (I32 sign-extended,
+// F32 widened to FP64 bits, I64/F64 as stored), with no branch around the
+// loads: 4-byte columns read their word twice.
+template <int R>
+__device__ __forceinline__ void gather_out(const void *c, int ct, const uint32_t (&ix)[R],
+                                           int64_t (&v)[R]) {
+  const bool wide = ct == I64 || ct == F64;
+  const uint32_t *w = (const uint32_t *)c;
+  uint32_t lo[R], hi[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const uint64_t e = wide ? 2ull * ix[k] : (uint64_t)ix[k];
+    lo[k] = w[e];
+    hi[k] = w[wide ? e + 1 : e];
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int64_t x64 = (int64_t)(((uint64_t)hi[k] << 32) | lo[k]);
+    const int64_t x32 = ct == F32 ? of_f((double)__uint_as_float(lo[k])) : (int64_t)(int32_t)lo[k];
+    v[k] = wide ? x64 : x32;
   }
 }
 
@@ -912,7 +966,9 @@ __global__ __launch_bounds__(256) void k_chain(const ChainArgs *__restrict__ ap)
       cr.alive[k] = cr.row[k] < n32;
     }
     bool live = true;
-    chain_stages<0, K, R>(a, cr, live);
+    int32_t pre[R];
+    chain_prefetch<0, K, R>(a, cr, pre);
+    chain_stages<0, K, R>(a, cr, live, pre);
     if (!live) continue;
     // outputs: positions from one LDS atomic per wave
     uint32_t cnt = 0;
@@ -933,15 +989,27 @@ __global__ __launch_bounds__(256) void k_chain(const ChainArgs *__restrict__ ap)
       pos[k] = p0 + off + __popcll(m[k] & lt);
       off += __popcll(m[k]);
     }
-    for (int j = 0; j < a.nout; ++j) {
-      const ChainOut o = a.outs[j];
-      int64_t v[R];
-      uint32_t ix[R];
-      cr.at(a.clevel[o.col], ix);
-      gather_col<R>(a.cols[o.col], a.ctype[o.col], ix, v);
+    // outputs four at a time: all their loads in flight together (past the
+    // last output the last one is loaded again and not stored)
+    for (int j0 = 0; j0 < a.nout; j0 += 4) {
+      int64_t v[4][R];
 #pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (cr.alive[k]) st_slot(o.out, o.out_t, pos[k], v[k]);
+      for (int u = 0; u < 4; ++u) {
+        const int j = min(j0 + u, a.nout - 1);
+        const int col = a.outs[j].col;
+        uint32_t ix[R];
+        cr.at(a.clevel[col], ix);
+        gather_out<R>(a.cols[col], a.ctype[col], ix, v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (j0 + u < a.nout) {
+          const ChainOut o = a.outs[j0 + u];
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            if (cr.alive[k]) st_slot(o.out, o.out_t, pos[k], v[u][k]);
+        }
+      }
     }
   }
   __syncthreads();
@@ -997,7 +1065,8 @@ static bool as_chain(const PipeProg &p, void *const *outs, ChainArgs *a) {
       const int ct = p.ctype[I[k].a];
       const int ot = p.out_t[I[k + 1].a];
       const int nat = ct == F32 ? F64 : ct;
-      if (ot != nat || a->nout >= kPipeMaxOut || I[k + 1].a != a->nout) return false;
+      if (ct == BOOL || ot != nat || a->nout >= kPipeMaxOut || I[k + 1].a != a->nout)
+        return false;
       a->outs[a->nout++] = ChainOut{outs[I[k + 1].a], I[k].a, ot};
       k += 2;
     } else {
@@ -1249,9 +1318,19 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       PJ_TRY(seg.init(ctx, p.out_t, grid, ca.region));
       for (size_t j = 0; j < p.out_t.size(); ++j) ca.outs[j].out = seg.bufs[j]->p;
       ca.counts = seg.counts.as<uint32_t>();
+      // the argument block carries a zeroed dummy key after ChainArgs
       Scratch dargs;
-      PJ_TRY(dargs.alloc(ctx, sizeof(ChainArgs)));
-      PJ_TRY(h2d(ctx, dargs.p, &ca, sizeof(ChainArgs)));
+      PJ_TRY(dargs.alloc(ctx, sizeof(ChainArgs) + 256));
+      const int32_t *dummy =
+          reinterpret_cast<const int32_t *>(dargs.as<char>() + ((sizeof(ChainArgs) + 15) & ~15));
+      for (int st = 0; st < kChainMaxStages; ++st) {
+        const bool on = st < ca.nst && ca.st[st].kind == 0 && ca.clevel[ca.st[st].col] == 0;
+        ca.pf_on[st] = on ? 1 : 0;
+        ca.pf[st] = on ? (const int32_t *)ca.cols[ca.st[st].col] : dummy;
+      }
+      std::vector<uint8_t> blob(sizeof(ChainArgs) + 256, 0);
+      std::memcpy(blob.data(), &ca, sizeof(ChainArgs));
+      PJ_TRY(h2d(ctx, dargs.p, blob.data(), blob.size()));
       const size_t lds = std::max<size_t>(1, p.tables.size()) * kChainRows * 256 * 4;
       const ChainArgs *da = dargs.as<const ChainArgs>();
       switch (ca.nst) {
