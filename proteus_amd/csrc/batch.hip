@@ -149,11 +149,11 @@ __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long lo
   int32_t b = 0;
   for (;;) {
     const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];
+    if (s == kEmptySlot) return;  // before the key test: a probe key INT32_MIN is a miss
     if ((int32_t)(s >> 32) == k) {
       b = (int32_t)(uint32_t)s;
       break;
     }
-    if (s == kEmptySlot) return;
     h = (h + 1) & q.mask;
   }
   if ((uint32_t)g >= q.groups) {
@@ -167,6 +167,64 @@ __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long lo
   } else {
     atomicAdd((unsigned long long *)&q.cnt[g], 1ULL);
     atomicAdd((unsigned long long *)&q.sum[g], (unsigned long long)sv);
+  }
+}
+
+// Eight rows of one lane: their probe sequences advance together, so the
+// table loads of all eight are in flight at once (a row at a time, each
+// probe would wait for the previous row's), then the matches aggregate.
+template <bool LDS_TAB, bool LDS_GRP>
+__device__ __forceinline__ void probe8(const QDesc &q, const unsigned long long *ltab,
+                                       uint32_t *gcnt, unsigned long long *gsum,
+                                       const int32_t (&k)[8], const int32_t (&v)[8],
+                                       const int32_t (&g)[8], uint32_t &bad) {
+  const uint32_t mask = (uint32_t)q.mask;
+  uint32_t h[8];
+  int32_t b[8];
+  bool act[8], hit[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    h[j] = dim_hash(k[j]) & mask;
+    act[j] = true;
+    hit[j] = false;
+    b[j] = 0;
+  }
+  for (;;) {
+    unsigned long long s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t at = act[j] ? h[j] : 0u;
+      s[j] = LDS_TAB ? ltab[at] : q.table[at];
+    }
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool empty = s[j] == kEmptySlot;
+      const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
+      const bool got = act[j] && eq;
+      hit[j] = hit[j] || got;
+      b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
+      h[j] = (act[j] && !empty && !eq) ? ((h[j] + 1) & mask) : h[j];
+      act[j] = act[j] && !empty && !eq;
+      any = any || act[j];
+    }
+    if (!any) break;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (!hit[j]) continue;
+    if ((uint32_t)g[j] >= q.groups) {
+      ++bad;
+      continue;
+    }
+    const long long sv = (long long)b[j] + (long long)v[j];
+    if (LDS_GRP) {
+      atomicAdd(&gcnt[g[j]], 1u);
+      atomicAdd(&gsum[g[j]], (unsigned long long)sv);
+    } else {
+      atomicAdd((unsigned long long *)&q.cnt[g[j]], 1ULL);
+      atomicAdd((unsigned long long *)&q.sum[g[j]], (unsigned long long)sv);
+    }
   }
 }
 
@@ -189,13 +247,10 @@ __device__ __forceinline__ void probe_rows(const QDesc &q, uint64_t r0, uint64_t
         v[h] = ld_nt4(q.val + base + 4 * h);
         g[h] = ld_nt4(q.grp + base + 4 * h);
       }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].x, v[h].x, g[h].x, bad);
-        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].y, v[h].y, g[h].y, bad);
-        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].z, v[h].z, g[h].z, bad);
-        probe_row<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, k[h].w, v[h].w, g[h].w, bad);
-      }
+      const int32_t kk[8] = {k[0].x, k[0].y, k[0].z, k[0].w, k[1].x, k[1].y, k[1].z, k[1].w};
+      const int32_t vv[8] = {v[0].x, v[0].y, v[0].z, v[0].w, v[1].x, v[1].y, v[1].z, v[1].w};
+      const int32_t gg[8] = {g[0].x, g[0].y, g[0].z, g[0].w, g[1].x, g[1].y, g[1].z, g[1].w};
+      probe8<LDS_TAB, LDS_GRP>(q, ltab, gcnt, gsum, kk, vv, gg, bad);
     }
     r = full;
   }
