@@ -219,9 +219,13 @@ typedef struct {
   int64_t *d_sum;
 } pj_query;
 
+/* Zero fields take the context's autotuned value (pj_batch_autotune), or
+ * the default in parentheses before any tuning. */
 typedef struct {
-  uint32_t tile_rows;      /* probe rows per work item (0: 65536)            */
-  int wg_per_cu;           /* persistent workgroups per CU (0: 2)            */
+  uint32_t tile_rows;      /* probe rows per work item (65536)               */
+  int wg_per_cu;           /* persistent workgroups per CU (2)               */
+  int lds_dim;             /* dimensions of <= lds_dim rows are probed from  */
+                           /* LDS, the rest from HBM; -1: none (2048, max)   */
 } pj_batch_cfg;
 
 /* One query, one launch sequence, synchronous (the plain path). */
@@ -234,6 +238,12 @@ int pj_batch_launch(pj_ctx *ctx, int slot, const pj_query *qs, uint32_t nq,
                     const pj_batch_cfg *cfg);
 int pj_batch_poll(pj_ctx *ctx, int slot, int *done);
 int pj_batch_wait(pj_ctx *ctx, int slot);
+/* Times batches of the given queries (their outputs are overwritten) over
+ * tile sizes, workgroups per CU and the LDS-vs-HBM table threshold, one
+ * knob at a time, and keeps the fastest as the context's default for zero
+ * fields of pj_batch_cfg; *best receives it.  Queries with gen_seed need
+ * their probe columns allocated (d_fk, d_val, d_grp). */
+int pj_batch_autotune(pj_ctx *ctx, const pj_query *qs, uint32_t nq, pj_batch_cfg *best);
 
 /* Serving loop: queries arrive at arrival_s[i] seconds after the call
  * starts; mode 0 admits every waiting query whose probe inputs fit the HBM

@@ -63,7 +63,10 @@ class Query(C.Structure):
 
 
 class BatchCfg(C.Structure):
-    _fields_ = [("tile_rows", C.c_uint32), ("wg_per_cu", C.c_int)]
+    _fields_ = [("tile_rows", C.c_uint32), ("wg_per_cu", C.c_int), ("lds_dim", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class ServeCfg(C.Structure):
@@ -121,6 +124,7 @@ _SIGS = {
     "pj_batch_launch": ([vp, i32, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
     "pj_batch_poll": ([vp, i32, C.POINTER(C.c_int)], i32),
     "pj_batch_wait": ([vp, i32], i32),
+    "pj_batch_autotune": ([vp, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
     "pj_serve_run": ([vp, C.POINTER(Query), C.POINTER(C.c_double), u32, C.POINTER(ServeCfg),
                       C.POINTER(C.c_double), C.POINTER(ServeStats)], i32),
     "pj_register_column": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, i32], i32),
@@ -392,14 +396,23 @@ class Context:
     def batch_wait(self, slot):
         _check(_lib.pj_batch_wait(self.h, slot))
 
+    def batch_autotune(self, queries):
+        """Tunes tile rows, workgroups per CU and the LDS table threshold on
+        these queries (outputs overwritten; probe columns allocated) and makes
+        the result the context's default."""
+        arr = (Query * len(queries))(*queries)
+        best = BatchCfg()
+        _check(_lib.pj_batch_autotune(self.h, arr, len(queries), C.byref(best)))
+        return best.as_dict()
+
     def serve_run(self, queries, arrivals, mode=0, slots=3, max_batch=4096, hbm_budget=0,
-                  tile_rows=0, wg_per_cu=0, heavy_rows=0):
+                  tile_rows=0, wg_per_cu=0, heavy_rows=0, lds_dim=0):
         """Runs the serving loop; returns (completion seconds per query, stats)."""
         n = len(queries)
         arr = (Query * n)(*queries)
         arrv = (C.c_double * n)(*arrivals)
         done = (C.c_double * n)()
-        cfg = ServeCfg(mode, slots, max_batch, hbm_budget, heavy_rows, BatchCfg(tile_rows, wg_per_cu))
+        cfg = ServeCfg(mode, slots, max_batch, hbm_budget, heavy_rows, BatchCfg(tile_rows, wg_per_cu, lds_dim))
         st = ServeStats()
         _check(_lib.pj_serve_run(self.h, arr, arrv, n, C.byref(cfg), done, C.byref(st)))
         return list(done), {"batches": st.batches, "max_in_flight_bytes": st.max_in_flight_bytes,

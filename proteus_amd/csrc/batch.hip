@@ -329,6 +329,7 @@ struct Slot {
 
 struct Batches {
   std::vector<Slot> slots;
+  pj_batch_cfg tuned{};  // defaults for zero fields (pj_batch_autotune)
 };
 
 constexpr size_t kBufBytes = 256 + sizeof(QDesc) * kMaxBatch;
@@ -363,18 +364,28 @@ static int fill_desc(const pj_query &q, QDesc *d) {
   return PJ_OK;
 }
 
-static uint32_t tile_rows_of(const pj_batch_cfg *cfg) {
-  return cfg && cfg->tile_rows ? cfg->tile_rows : 65536u;
+static uint32_t tile_rows_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
+  if (cfg && cfg->tile_rows) return cfg->tile_rows;
+  const uint32_t t = batches(ctx).tuned.tile_rows;
+  return t ? t : 65536u;
 }
 static uint32_t grid_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
-  int per = cfg && cfg->wg_per_cu > 0 ? cfg->wg_per_cu : 2;
+  int per = cfg && cfg->wg_per_cu > 0 ? cfg->wg_per_cu : batches(ctx).tuned.wg_per_cu;
+  if (per <= 0) per = 2;
   return (uint32_t)(ctx->num_cus * per);
+}
+// Largest dimension probed from LDS (0: none).
+static uint32_t lds_dim_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
+  int v = cfg && cfg->lds_dim ? cfg->lds_dim : batches(ctx).tuned.lds_dim;
+  if (v == 0) return kLdsDim;
+  if (v < 0) return 0;
+  return std::min<uint32_t>((uint32_t)v, kLdsDim);
 }
 
 // Writes header + descriptors into a host buffer.  Queries whose dimension
 // fits LDS come first (their items run in the LDS-table kernel); every query
 // owns at least one item (its first item clears its outputs).
-static int pack(const pj_query *qs, uint32_t nq, uint32_t T, void *h_buf) {
+static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, void *h_buf) {
   BatchHdr *hdr = (BatchHdr *)h_buf;
   QDesc *d = (QDesc *)((char *)h_buf + 256);
   uint64_t tiles = 0, lds_tiles = 0;
@@ -382,7 +393,7 @@ static int pack(const pj_query *qs, uint32_t nq, uint32_t T, void *h_buf) {
   for (int pass = 0; pass < 2; ++pass)
     for (uint32_t i = 0; i < nq; ++i) {
       if (!qs[i].dim) return fail(PJ_E_ARG, "query without a dimension");
-      const bool lds = qs[i].dim->n <= kLdsDim;
+      const bool lds = lds_max && qs[i].dim->n <= lds_max;
       if (lds != (pass == 0)) continue;
       PJ_TRY(fill_desc(qs[i], &d[j]));
       d[j].tile0 = tiles;
@@ -515,10 +526,10 @@ extern "C" int pj_dim_info(pj_ctx *ctx, const pj_dim *dim, uint64_t *rows, uint6
 
 extern "C" int pj_query_run(pj_ctx *ctx, const pj_query *q, const pj_batch_cfg *cfg) {
   std::lock_guard<std::mutex> g(ctx->mu);
-  const uint32_t T = tile_rows_of(cfg);
+  const uint32_t T = tile_rows_of(ctx, cfg);
   char *h = (char *)ctx->pinned_small;
   static_assert(256 + sizeof(QDesc) <= (1 << 20), "pinned staging too small");
-  PJ_TRY(pack(q, 1, T, h));
+  PJ_TRY(pack(q, 1, T, lds_dim_of(ctx, cfg), h));
   Scratch dbuf;
   PJ_TRY(dbuf.alloc(ctx, 256 + sizeof(QDesc)));
   PJ_TRY(enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, cfg), 256 + sizeof(QDesc)));
@@ -545,8 +556,8 @@ extern "C" int pj_batch_launch(pj_ctx *ctx, int slot, const pj_query *qs, uint32
   if (nq < 1 || nq > kMaxBatch) return fail(PJ_E_ARG, "batch of %u queries not in [1, %u]", nq, kMaxBatch);
   Slot &sl = b.slots[slot];
   if (sl.busy) return fail(PJ_E_ARG, "batch slot %d is busy", slot);
-  const uint32_t T = tile_rows_of(cfg);
-  PJ_TRY(pack(qs, nq, T, sl.h_buf));
+  const uint32_t T = tile_rows_of(ctx, cfg);
+  PJ_TRY(pack(qs, nq, T, lds_dim_of(ctx, cfg), sl.h_buf));
   const uint32_t grid = grid_of(ctx, cfg);
   if (!sl.exec || sl.graph_grid != grid) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -599,4 +610,71 @@ extern "C" int pj_batch_wait(pj_ctx *ctx, int slot) {
   PJ_HIP(hipEventSynchronize(sl.done));
   sl.busy = false;
   return check_bad(sl.h_buf);
+}
+
+// Autotuner: each candidate runs the whole batch on the context stream (no
+// graph) twice and keeps the second time; knobs are tuned one after the
+// other (tile rows, then workgroups per CU, then the LDS threshold).
+extern "C" int pj_batch_autotune(pj_ctx *ctx, const pj_query *qs, uint32_t nq,
+                                 pj_batch_cfg *best) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (nq < 1 || nq > kMaxBatch) return fail(PJ_E_ARG, "batch of %u queries not in [1, %u]", nq, kMaxBatch);
+  for (uint32_t i = 0; i < nq; ++i)
+    if (qs[i].n && (!qs[i].d_fk || !qs[i].d_val || !qs[i].d_grp))
+      return fail(PJ_E_ARG, "autotune query %u without probe columns", i);
+  static_assert(kBufBytes <= (1 << 20), "pinned staging too small");
+  char *h = (char *)ctx->pinned_small;
+  Scratch dbuf;
+  PJ_TRY(dbuf.alloc(ctx, kBufBytes));
+  hipEvent_t ev[2];
+  PJ_HIP(hipEventCreate(&ev[0]));
+  PJ_HIP(hipEventCreate(&ev[1]));
+  auto time_cfg = [&](const pj_batch_cfg &c, float *ms) -> int {
+    const size_t bytes = 256 + sizeof(QDesc) * nq;
+    for (int rep = 0; rep < 2; ++rep) {
+      PJ_TRY(pack(qs, nq, tile_rows_of(ctx, &c), lds_dim_of(ctx, &c), h));
+      PJ_HIP(hipEventRecord(ev[0], ctx->stream));
+      PJ_TRY(enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, &c), bytes));
+      PJ_HIP(hipEventRecord(ev[1], ctx->stream));
+      PJ_HIP(hipEventSynchronize(ev[1]));
+      PJ_HIP(hipEventElapsedTime(ms, ev[0], ev[1]));
+    }
+    return PJ_OK;
+  };
+  pj_batch_cfg cur{65536u, 2, 0};
+  float best_ms = 0.f;
+  int rc = time_cfg(cur, &best_ms);
+  auto try_knob = [&](pj_batch_cfg c) {
+    float ms = 0.f;
+    if (rc == PJ_OK) rc = time_cfg(c, &ms);
+    if (rc == PJ_OK && ms < best_ms) {
+      best_ms = ms;
+      cur = c;
+    }
+  };
+  for (uint32_t t : {16384u, 32768u, 131072u, 262144u}) {
+    pj_batch_cfg c = cur;
+    c.tile_rows = t;
+    try_knob(c);
+  }
+  const pj_batch_cfg after_t = cur;
+  for (int w : {1, 4}) {
+    pj_batch_cfg c = after_t;
+    c.wg_per_cu = w;
+    try_knob(c);
+  }
+  const pj_batch_cfg after_w = cur;
+  for (int l : {-1, 512}) {
+    pj_batch_cfg c = after_w;
+    c.lds_dim = l;
+    try_knob(c);
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  PJ_TRY(rc);
+  PJ_TRY(d2h(ctx, h, dbuf.p, sizeof(BatchHdr)));  // the last run's group check
+  PJ_TRY(check_bad(h));
+  batches(ctx).tuned = cur;
+  if (best) *best = cur;
+  return PJ_OK;
 }

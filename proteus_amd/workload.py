@@ -56,6 +56,27 @@ class Mix:
                             self.out_cnt[i].ptr, self.out_sum[i].ptr))
         return qs
 
+    def autotune(self, k=64, max_rows=1 << 26):
+        """Tunes the context's batch defaults (tile rows, workgroups per CU,
+        LDS table threshold) on a seeded sample of k queries of at most
+        max_rows probe rows, with their probe columns allocated for the run.
+        Returns the tuned configuration."""
+        idx = np.flatnonzero(self.n <= max_rows)
+        pick = _rng(self.seed, 3).choice(idx, size=min(k, len(idx)), replace=False)
+        qs, bufs = [], []
+        for i in sorted(pick.tolist()):
+            n = int(self.n[i])
+            cols = [self.ctx.empty(n, np.int32) for _ in range(3)]
+            bufs += cols
+            qs.append(Query(self.dims[int(self.u[i])], cols[0].ptr, cols[1].ptr, cols[2].ptr, n,
+                            int(self.groups[i]), 0, int(self.gen_seed[i]),
+                            int(self.fk_range[i]), self.out_cnt[i].ptr, self.out_sum[i].ptr))
+        try:
+            return self.ctx.batch_autotune(qs)
+        finally:
+            for b in bufs:
+                b.free()
+
     def result(self, i):
         return self.out_cnt[i].numpy(), self.out_sum[i].numpy()
 

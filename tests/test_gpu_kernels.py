@@ -159,6 +159,36 @@ def test_join_partitioned_parity(ctx, ci, dup):
     _free(*d)
 
 
+def _shifted(ctx, a):
+    """A device copy of `a` that starts one element past a 16-byte boundary
+    (the owner, to free, and the view)."""
+    d = ctx.to_device(np.concatenate([a[:1], a]))
+    return d, pa.DeviceArray(ctx, d.ptr + a.itemsize, len(a), a.dtype)
+
+
+def test_unaligned_columns_and_ragged_sizes(ctx):
+    """Row counts that are not multiples of 4 and columns that start off a
+    16-byte boundary take the element-wise load paths of the radix passes and
+    of the group-by; the results equal the oracle's."""
+    rk, rv, sk, sv = _join_inputs(41, 20001, 150003, 30000, False)
+    rc, want = O.join_chained(rk, rv, sk, sv, 16, threads=8)
+    d = _dev(ctx, rk, rv, sk, sv)
+    assert ctx.join_partitioned_reduce(d[0], d[1], d[2], d[3]) == want
+    _free(*d)
+    sh = [_shifted(ctx, a) for a in (rk, rv, sk, sv)]
+    got = ctx.join_partitioned_reduce(*[v for _, v in sh])
+    assert got == want
+    _free(*[o for o, _ in sh])
+    n = 50001
+    k = O.gen_uniform_i32(n, 3, 1, 0, 3000)
+    v = O.gen_uniform_i64(n, 3, 2, -(1 << 40), 1 << 41)
+    w = O.gen_unit_f32(n, 3, 3)
+    sh = [_shifted(ctx, a) for a in (k, v, w)]
+    got = ctx.groupby_i32(sh[0][1], sh[1][1], sh[2][1], 4096)
+    _check_groups(got, k, v, w)
+    _free(*[o for o, _ in sh])
+
+
 def test_join_partitioned_zipf_and_skewed_build(ctx):
     nr, ns, seed = 1 << 18, 1 << 21, 77
     rk = ctx.gen_perm_i32(nr, seed)

@@ -134,3 +134,33 @@ def test_batch_and_serving_equal_alone(ctx):
             assert np.array_equal(got[i][0], alone[i][0]), (mode, i)
             assert np.array_equal(got[i][1], alone[i][1]), (mode, i)
     mix.close()
+
+
+@pytest.mark.gpu
+def test_autotune_and_table_placement_keep_results(ctx):
+    """The tuned configuration and every LDS-vs-HBM table threshold give the
+    results of the default configuration."""
+    mix = workload.Mix(ctx, 24, seed=13, u_lo=10, u_hi=14)
+    qs, bufs = _alloc_inputs(ctx, mix)
+    ctx.batch_slots(1)
+    ctx.batch_launch(0, qs, pa.BatchCfg(65536, 2, 0))
+    ctx.batch_wait(0)
+    want = mix.results()
+    tuned = mix.autotune(k=12, max_rows=1 << 20)
+    assert tuned["tile_rows"] in (16384, 32768, 65536, 131072, 262144)
+    assert tuned["wg_per_cu"] in (1, 2, 4) and tuned["lds_dim"] in (0, -1, 512)
+    for cfg in (None, pa.BatchCfg(16384, 1, -1), pa.BatchCfg(32768, 4, 512),
+                pa.BatchCfg(4096, 2, 1 << 11)):
+        ctx.batch_launch(0, qs, cfg)
+        ctx.batch_wait(0)
+        got = mix.results()
+        for i in range(mix.nq):
+            assert np.array_equal(got[i][0], want[i][0]) and np.array_equal(got[i][1], want[i][1])
+    for q in qs[:4]:
+        ctx.query_run(q, pa.BatchCfg(8192, 1, -1))
+    got = mix.results()
+    for i in range(4):
+        assert np.array_equal(got[i][0], want[i][0]) and np.array_equal(got[i][1], want[i][1])
+    for b in bufs:
+        b.free()
+    mix.close()
