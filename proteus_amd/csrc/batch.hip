@@ -476,7 +476,12 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   if (n >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
   pj_dim *dm = new pj_dim();
   dm->n = n;
-  dm->slots = std::max<uint64_t>(64, 2ULL << ilog2_ceil(n ? n : 1));
+  // load <= 1/2; tables of mid-sized dimensions (that still fit the L2s)
+  // at load <= 1/8, which shortens the longest probe sequence a wave waits
+  // for.  Dimensions probed from LDS keep <= kLdsSlots slots.
+  const int lg = ilog2_ceil(n ? n : 1);
+  const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
+  dm->slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
   int rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;
