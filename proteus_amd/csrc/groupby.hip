@@ -101,6 +101,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
   __shared__ double tfs[kGbSlots];
   __shared__ uint32_t wave_off[kGbBlock / 64 + 1];
   const int tid = threadIdx.x;
+  // units round-robin over the grid (the host makes many similar units)
   for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     __syncthreads();
     for (int i = tid; i < kGbSlots; i += kGbBlock) {
@@ -364,7 +365,9 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     if (!d_w) pw = nullptr;
   }
   // ---- level 0 units -------------------------------------------------------
-  const uint64_t target_units = (uint64_t)ctx->num_cus * 2;
+  // about eight units per workgroup (one per CU): with units taken
+  // round-robin, the last round is at most one unit of eight
+  const uint64_t target_units = (uint64_t)ctx->num_cus * 8;
   uint64_t unit_rows = std::max<uint64_t>(ceil_div(n ? n : 1, target_units), 16384);
   std::vector<GbUnit> units;
   std::vector<uint32_t> unit_part;
@@ -386,7 +389,8 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
   PJ_TRY(dunits.alloc(ctx, units.size() * sizeof(GbUnit)));
   PJ_TRY(h2d(ctx, dunits.p, units.data(), units.size() * sizeof(GbUnit)));
   {
-    uint32_t grid = (uint32_t)std::min<uint64_t>(units.size(), (uint64_t)ctx->num_cus * 2);
+    // one 1024-thread workgroup per CU (96 KiB of LDS each)
+    uint32_t grid = (uint32_t)std::min<uint64_t>(units.size(), (uint64_t)ctx->num_cus);
     GbEntries none{};
     // The vector loads need 16-byte-aligned columns; otherwise every row is
     // added one by one.
@@ -431,7 +435,7 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     Scratch dn;
     PJ_TRY(dn.alloc(ctx, next.size() * sizeof(GbUnit)));
     PJ_TRY(h2d(ctx, dn.p, next.data(), next.size() * sizeof(GbUnit)));
-    uint32_t grid = (uint32_t)std::min<uint64_t>(next.size(), (uint64_t)ctx->num_cus * 2);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(next.size(), (uint64_t)ctx->num_cus);
     PJ_LAUNCH(ctx, "gb_merge", (k_gb_agg<true, true, true>), grid, kGbBlock, 0, dn.as<const GbUnit>(),
               (uint32_t)next.size(), (const int32_t *)nullptr, (const int64_t *)nullptr,
               (const float *)nullptr, level->e, level->count.as<const uint32_t>(), nl->e,
