@@ -515,11 +515,10 @@ static int side_histograms(pj_ctx *ctx, Side &s, const int32_t *d_k, uint64_t n,
   return radix_histograms(ctx, d_k, n, s.plan, JointRadix{32 - B}, s.hist, false);
 }
 
-// Queues the side's radix passes on ctx->stream (after its histograms).
-static int side_passes(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
-                       uint64_t n, int b1, int b2, const char *tag) {
-  const int B = b1 + b2;
-  JointHashed j{32 - B};
+// Queues the side's first radix pass on ctx->stream (after its histograms).
+static int side_pass1(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *d_v,
+                      uint64_t n, int b1, int b2, const char *tag) {
+  JointHashed j{32 - (b1 + b2)};
   PJ_TRY(s.k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
   PJ_TRY(s.v1.alloc(ctx, n * 4 + 256 + kSkew[1]));
   char name[64];
@@ -528,15 +527,21 @@ static int side_passes(pj_ctx *ctx, Side &s, const int32_t *d_k, const int32_t *
                         s.plan, j, s.hist));
   s.fk = skewed(s.k1, 0);
   s.fv = skewed(s.v1, 1);
-  if (b2) {
-    PJ_TRY(s.k2.alloc(ctx, n * 4 + 256 + kSkew[2]));
-    PJ_TRY(s.v2.alloc(ctx, n * 4 + 256 + kSkew[3]));
-    snprintf(name, sizeof(name), "radix_scatter2_%s", tag);
-    PJ_TRY(join_pass_bits(ctx, name, b2, true, s.fk, s.fv, skewed(s.k2, 2), skewed(s.v2, 3), n,
-                          s.plan, j, s.hist));
-    s.fk = skewed(s.k2, 2);
-    s.fv = skewed(s.v2, 3);
-  }
+  return PJ_OK;
+}
+
+// Queues the side's second radix pass (if any) on ctx->stream.
+static int side_pass2(pj_ctx *ctx, Side &s, uint64_t n, int b1, int b2, const char *tag) {
+  if (!b2) return PJ_OK;
+  JointHashed j{32 - (b1 + b2)};
+  PJ_TRY(s.k2.alloc(ctx, n * 4 + 256 + kSkew[2]));
+  PJ_TRY(s.v2.alloc(ctx, n * 4 + 256 + kSkew[3]));
+  char name[64];
+  snprintf(name, sizeof(name), "radix_scatter2_%s", tag);
+  PJ_TRY(join_pass_bits(ctx, name, b2, true, s.fk, s.fv, skewed(s.k2, 2), skewed(s.v2, 3), n,
+                        s.plan, j, s.hist));
+  s.fk = skewed(s.k2, 2);
+  s.fv = skewed(s.v2, 3);
   return PJ_OK;
 }
 
@@ -623,14 +628,19 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       // leave HBM bandwidth unused (short grids, launch gaps).
       hipStream_t aux;
       PJ_TRY(aux_fork(ctx, &aux));
+      // (Starting the build side's later passes only beside the probe
+      // side's second pass measured the same.)
+      const int b1 = cfg.bits1, b2 = cfg.bits2;
       int rc;
       {
         StreamSwap sw(ctx, aux);
-        rc = side_histograms(ctx, R, d_bk, nb, cfg.bits1, cfg.bits2);
-        if (rc == PJ_OK) rc = side_passes(ctx, R, d_bk, d_bv, nb, cfg.bits1, cfg.bits2, "build");
+        rc = side_histograms(ctx, R, d_bk, nb, b1, b2);
+        if (rc == PJ_OK) rc = side_pass1(ctx, R, d_bk, d_bv, nb, b1, b2, "build");
+        if (rc == PJ_OK) rc = side_pass2(ctx, R, nb, b1, b2, "build");
       }
-      if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, cfg.bits1, cfg.bits2);
-      if (rc == PJ_OK) rc = side_passes(ctx, S, d_pk, d_pv, np, cfg.bits1, cfg.bits2, "probe");
+      if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, b1, b2);
+      if (rc == PJ_OK) rc = side_pass1(ctx, S, d_pk, d_pv, np, b1, b2, "probe");
+      if (rc == PJ_OK) rc = side_pass2(ctx, S, np, b1, b2, "probe");
       // join even on failure, so no scratch is released under running work
       const int rj = aux_join(ctx);
       if (rc != PJ_OK) {

@@ -193,6 +193,7 @@ int sync(pj_ctx *ctx);
 int aux_fork(pj_ctx *ctx, hipStream_t *aux);
 // Makes ctx->stream wait for the work queued so far on the auxiliary stream.
 int aux_join(pj_ctx *ctx);
+
 // Sends launches through ctx to another stream while in scope.
 struct StreamSwap {
   pj_ctx *ctx;
