@@ -141,8 +141,9 @@ struct JoinUnit {
 // the build partition it has just built reuses the table.
 //
 // The table is bucket-sorted ("CSR"): tab[b] packs the [start, end) range of
-// bucket b in ent[] (16 + 16 bits) and entries are (key, value) pairs grouped
-// by bucket.  Building it is count -> scan -> scatter with LDS atomics;
+// bucket b in ent[] (start in the high, end in the low 16 bits) and entries
+// are (key, value) pairs grouped by bucket.  Building it is count -> scan ->
+// scatter with LDS atomics (the scatter's cursor is the low half);
 // probing is one LDS read for the range plus the bucket's entries, which sit
 // next to each other -- the reference's chained LDS table (head / next,
 // gpu-partitioned-hash-join-chained.cpp:832-1037) needs one dependent read per
@@ -223,8 +224,10 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           uint32_t tot;
           uint32_t run = block_excl_scan<BLOCK>(sum, scan_tmp, &tot);
 #pragma unroll
+          // the start in both halves: the scatter's cursor increments the
+          // low half, which ends at the bucket's end
           for (int x = 0; x < BPT; ++x) {
-            tab[tid * BPT + x] = run;
+            tab[tid * BPT + x] = run | (run << 16);
             run += c[x];
           }
         }
@@ -232,23 +235,9 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
 #pragma unroll
         for (int q = 0; q < RPT; ++q)
           if ((uint32_t)(q * BLOCK + tid) < rl) {
-            const uint32_t pos = atomicAdd(&tab[bb[q]], 1u);
+            const uint32_t pos = atomicAdd(&tab[bb[q]], 1u) & 0xffffu;
             ent[pos] = make_int2(bk[q], bv[q]);
           }
-        __syncthreads();
-        {  // tab[b] now holds the end of bucket b: pack (start, end)
-          uint32_t e[BPT];
-          const uint32_t prev = tid ? tab[tid * BPT - 1] : 0u;
-#pragma unroll
-          for (int x = 0; x < BPT; ++x) e[x] = tab[tid * BPT + x];
-          __syncthreads();
-          uint32_t st = prev;
-#pragma unroll
-          for (int x = 0; x < BPT; ++x) {
-            tab[tid * BPT + x] = st | (e[x] << 16);
-            st = e[x];
-          }
-        }
         __syncthreads();
         built_r = rounds == 1 ? U.r_begin : ~0ULL;
       }
@@ -274,7 +263,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
       };
       auto probe1 = [&](int32_t key, int32_t pv) {
         const uint32_t rg = tab[bucket(key)];
-        const uint32_t st = rg & 0xffffu, en = rg >> 16;
+        const uint32_t st = rg >> 16, en = rg & 0xffffu;
         if (st < en) probe_multi(key, pv, st, en);
       };
       if (a0 >= a1) {  // short unit: every tuple on its own
@@ -307,7 +296,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           int2 e0[4], e1[4], e2[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+            const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
             e0[c] = e1[c] = e2[c] = make_int2(0, 0);
             if (en > st) e0[c] = ent[st];
             if (en > st + 1) e1[c] = ent[st + 1];
@@ -323,7 +312,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           int32_t bs[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+            const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
             const bool h0 = (en > st) & (e0[c].x == kk[c]);
             const bool h1 = (en > st + 1) & (e1[c].x == kk[c]);
             const bool h2 = (en > st + 2) & (e2[c].x == kk[c]);
@@ -339,7 +328,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-              const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+              const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
               acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y,
                           (en > st + 1) & (e1[c].x == kk[c]), e1[c].y,
                           (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
@@ -348,7 +337,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           if (multi) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-              const uint32_t st = rg[c] & 0xffffu, en = rg[c] >> 16;
+              const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
               if (en > st + 3) probe_multi(kk[c], vv[c], st + 3, en);
             }
           }
