@@ -388,12 +388,24 @@ __global__ __launch_bounds__(kUnitBlock) void k_join_units(
   const int tid = threadIdx.x;
   for (int c = tid; c < kUnitClasses; c += kUnitBlock) cnt[c] = 0;
   __syncthreads();
-  for (uint32_t q = tid; q < nb; q += kUnitBlock) {
-    const uint32_t rl = rT[q], sl = sT[q];
-    if (!rl || !sl) continue;
-    const uint32_t full = sl / pu, tail = sl % pu;
-    if (full) atomicAdd(&cnt[unit_class((uint64_t)pu + rl)], full);
-    if (tail) atomicAdd(&cnt[unit_class((uint64_t)tail + rl)], 1u);
+  // eight partitions per thread per step, their loads in flight together
+  // (out-of-range slots read partition 0 and are skipped)
+  constexpr int K = 8;
+  for (uint32_t q0 = tid; q0 < nb; q0 += K * kUnitBlock) {
+    uint32_t rl[K], sl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t q = q0 + k * kUnitBlock;
+      rl[k] = rT[q < nb ? q : 0u];
+      sl[k] = sT[q < nb ? q : 0u];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (q0 + k * kUnitBlock >= nb || !rl[k] || !sl[k]) continue;
+      const uint32_t full = sl[k] / pu, tail = sl[k] % pu;
+      if (full) atomicAdd(&cnt[unit_class((uint64_t)pu + rl[k])], full);
+      if (tail) atomicAdd(&cnt[unit_class((uint64_t)tail + rl[k])], 1u);
+    }
   }
   __syncthreads();
   if (tid == 0) {  // class starts, largest class first
@@ -407,26 +419,38 @@ __global__ __launch_bounds__(kUnitBlock) void k_join_units(
     unit_hdr[1] = 0;
   }
   __syncthreads();
-  for (uint32_t q = tid; q < nb; q += kUnitBlock) {
-    const uint32_t rl = rT[q], sl = sT[q];
-    if (!rl || !sl) continue;
-    const uint32_t full = sl / pu, tail = sl % pu;
-    JoinUnit u;
-    u.r_begin = rS[q];
-    u.r_len = rl;
-    if (full) {
-      uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)pu + rl)], full);
-      for (uint32_t k = 0; k < full; ++k) {
-        u.s_begin = sS[q] + (unsigned long long)k * pu;
-        u.s_len = pu;
-        units[at + k] = u;
-      }
+  for (uint32_t q0 = tid; q0 < nb; q0 += K * kUnitBlock) {
+    uint32_t rl[K], sl[K];
+    unsigned long long rs[K], ss[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t q = q0 + k * kUnitBlock < nb ? q0 + k * kUnitBlock : 0u;
+      rl[k] = rT[q];
+      sl[k] = sT[q];
+      rs[k] = rS[q];
+      ss[k] = sS[q];
     }
-    if (tail) {
-      const uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)tail + rl)], 1u);
-      u.s_begin = sS[q] + (unsigned long long)full * pu;
-      u.s_len = tail;
-      units[at] = u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (q0 + k * kUnitBlock >= nb || !rl[k] || !sl[k]) continue;
+      const uint32_t full = sl[k] / pu, tail = sl[k] % pu;
+      JoinUnit u;
+      u.r_begin = rs[k];
+      u.r_len = rl[k];
+      if (full) {
+        const uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)pu + rl[k])], full);
+        for (uint32_t f = 0; f < full; ++f) {
+          u.s_begin = ss[k] + (unsigned long long)f * pu;
+          u.s_len = pu;
+          units[at + f] = u;
+        }
+      }
+      if (tail) {
+        const uint32_t at = atomicAdd(&cnt[unit_class((uint64_t)tail + rl[k])], 1u);
+        u.s_begin = ss[k] + (unsigned long long)full * pu;
+        u.s_len = tail;
+        units[at] = u;
+      }
     }
   }
 }
