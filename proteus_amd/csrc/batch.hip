@@ -55,6 +55,16 @@ struct QDesc {
   uint64_t fk_range;
   uint32_t dn;         // dimension rows
   uint32_t groups;
+  // Staged aggregation of queries with more groups than LDS holds: matches
+  // go to per-partition runs of (group & 2047, value) words, partition =
+  // group >> 11, and k_batch_agg folds each chunk of a run in LDS.  stage
+  // is null for the other queries (and when no scratch was available).
+  unsigned long long *stage;  // parts x cap words
+  uint32_t *pcur;             // parts cursors
+  uint64_t cap;               // words per partition run
+  uint64_t agg0;              // first k_batch_agg item of this query
+  uint32_t parts;
+  uint32_t cpp;               // k_batch_agg chunks per partition
 };
 
 struct BatchHdr {
@@ -64,7 +74,16 @@ struct BatchHdr {
   uint64_t lds_tiles;  // items [0, lds_tiles) belong to LDS-table queries
   uint32_t gen_ctr, probe_ctr_lds, probe_ctr_hbm;
   uint32_t bad_group;  // rows whose group key is outside [0, groups)
+  uint64_t total_agg;  // k_batch_agg items
+  uint32_t agg_ctr;
 };
+
+constexpr uint32_t kStageGroups = 2048;         // groups per staged partition (LDS)
+constexpr uint32_t kStageMaxParts = 32;         // staging for <= 65536 groups
+constexpr uint32_t kAggChunk = 1u << 18;        // staged words per k_batch_agg item
+__device__ __forceinline__ unsigned long long stage_word(uint32_t gl, long long sv) {
+  return ((unsigned long long)gl << 53) | ((unsigned long long)sv & ((1ULL << 53) - 1));
+}
 
 __global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -123,11 +142,14 @@ __global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d
     __syncthreads();
     if (item >= total) break;
     const QDesc &q = d[find_query(d, nq, item)];
-    if (item == q.tile0)
+    if (item == q.tile0) {
       for (uint32_t i = threadIdx.x; i < q.groups; i += blockDim.x) {
         q.cnt[i] = 0;
         q.sum[i] = 0;
       }
+      if (q.stage)
+        for (uint32_t i = threadIdx.x; i < q.parts; i += blockDim.x) q.pcur[i] = 0;
+    }
     if (!q.gen_seed) continue;
     const uint64_t r0 = (item - q.tile0) * T;
     const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
@@ -228,6 +250,109 @@ __device__ __forceinline__ void probe8(const QDesc &q, const unsigned long long 
   }
 }
 
+// The probe of probe8 for a staged query: the eight rows' matches are
+// appended to their partitions' runs (ranks from LDS counters, one global
+// reservation per partition per step); a match that finds its run full
+// falls back to the global atomics.  Every thread of the workgroup calls
+// this the same number of times (rows past the end pass valid = false).
+__device__ __forceinline__ void stage8(const QDesc &q, bool valid, const int32_t (&k)[8],
+                                       const int32_t (&v)[8], const int32_t (&g)[8],
+                                       uint32_t &bad, uint32_t *s_pcnt,
+                                       unsigned long long *s_pbase) {
+  const uint32_t mask = (uint32_t)q.mask;
+  uint32_t h[8];
+  int32_t b[8];
+  bool act[8], hit[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    h[j] = dim_hash(k[j]) & mask;
+    act[j] = valid;
+    hit[j] = false;
+    b[j] = 0;
+  }
+  for (;;) {
+    unsigned long long s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = q.table[act[j] ? h[j] : 0u];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool empty = s[j] == kEmptySlot;
+      const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
+      const bool got = act[j] && eq;
+      hit[j] = hit[j] || got;
+      b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
+      h[j] = (act[j] && !empty && !eq) ? ((h[j] + 1) & mask) : h[j];
+      act[j] = act[j] && !empty && !eq;
+      any = any || act[j];
+    }
+    if (!any) break;
+  }
+  uint32_t rank[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (hit[j] && (uint32_t)g[j] >= q.groups) {
+      ++bad;
+      hit[j] = false;
+    }
+    rank[j] = hit[j] ? atomicAdd(&s_pcnt[(uint32_t)g[j] / kStageGroups], 1u) : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x < q.parts)
+    s_pbase[threadIdx.x] =
+        s_pcnt[threadIdx.x] ? atomicAdd(&q.pcur[threadIdx.x], s_pcnt[threadIdx.x]) : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (!hit[j]) continue;
+    const uint32_t p = (uint32_t)g[j] / kStageGroups;
+    const unsigned long long pos = s_pbase[p] + rank[j];
+    const long long sv = (long long)b[j] + (long long)v[j];
+    if (pos < q.cap) {
+      q.stage[(uint64_t)p * q.cap + pos] = stage_word((uint32_t)g[j] % kStageGroups, sv);
+    } else {
+      atomicAdd((unsigned long long *)&q.cnt[g[j]], 1ULL);
+      atomicAdd((unsigned long long *)&q.sum[g[j]], (unsigned long long)sv);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < q.parts) s_pcnt[threadIdx.x] = 0;
+  __syncthreads();
+}
+
+// Rows [r0, r1) of a staged query's item: steps of 8 rows per lane that
+// every thread takes (the steps synchronise the workgroup); the rows after
+// the last whole step of the workgroup, or all rows when the columns are
+// not 16-byte aligned, go through probe_row's global atomics.
+__device__ __forceinline__ void stage_rows(const QDesc &q, uint64_t r0, uint64_t r1, uint32_t &bad,
+                                           uint32_t *s_pcnt, unsigned long long *s_pbase) {
+  const int tid = threadIdx.x;
+  const bool vec = ((((uintptr_t)q.fk | (uintptr_t)q.val | (uintptr_t)q.grp) & 15) == 0) && (r0 % 4 == 0);
+  uint64_t r = r0;
+  if (vec) {
+    const uint64_t full = r0 + (r1 - r0) / 8 * 8;
+    for (uint64_t sb = r0; sb < full; sb += (uint64_t)kBatchBlock * 8) {
+      const uint64_t base = sb + (uint64_t)tid * 8;
+      const bool valid = base < full;
+      const uint64_t at = valid ? base : full - 8;
+      int4 k[2], v[2], g[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        k[h] = ld_nt4(q.fk + at + 4 * h);
+        v[h] = ld_nt4(q.val + at + 4 * h);
+        g[h] = ld_nt4(q.grp + at + 4 * h);
+      }
+      const int32_t kk[8] = {k[0].x, k[0].y, k[0].z, k[0].w, k[1].x, k[1].y, k[1].z, k[1].w};
+      const int32_t vv[8] = {v[0].x, v[0].y, v[0].z, v[0].w, v[1].x, v[1].y, v[1].z, v[1].w};
+      const int32_t gg[8] = {g[0].x, g[0].y, g[0].z, g[0].w, g[1].x, g[1].y, g[1].z, g[1].w};
+      stage8(q, valid, kk, vv, gg, bad, s_pcnt, s_pbase);
+    }
+    r = full;
+  }
+  for (uint64_t i = r + tid; i < r1; i += kBatchBlock)
+    probe_row<false, false>(q, nullptr, nullptr, nullptr, q.fk[i], q.val[i], q.grp[i], bad);
+}
+
 // Rows [r0, r1) of one item: 8 rows per lane per step, 16-byte loads where
 // the columns allow.
 template <bool LDS_TAB, bool LDS_GRP>
@@ -268,6 +393,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, cons
   __shared__ unsigned long long gsum[kLdsGroups];
   __shared__ uint64_t s_item;
   __shared__ uint32_t s_bad;
+  __shared__ uint32_t s_pcnt[kStageMaxParts];
+  __shared__ unsigned long long s_pbase[kStageMaxParts];
+  if (threadIdx.x < kStageMaxParts) s_pcnt[threadIdx.x] = 0;
   const uint32_t nq = hdr->nq;
   const uint64_t first = LDS_TAB ? 0 : hdr->lds_tiles;
   const uint64_t last = LDS_TAB ? hdr->lds_tiles : hdr->total_tiles;
@@ -300,6 +428,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, cons
     uint32_t bad = 0;
     if (r0 < r1) {
       if (lds_grp) probe_rows<LDS_TAB, true>(q, r0, r1, ltab, gcnt, gsum, bad);
+      else if (!LDS_TAB && q.stage) stage_rows(q, r0, r1, bad, s_pcnt, s_pbase);
       else probe_rows<LDS_TAB, false>(q, r0, r1, ltab, gcnt, gsum, bad);
     }
     if (bad) atomicAdd(&s_bad, bad);
@@ -315,6 +444,63 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, cons
   }
 }
 
+// Folds the staged runs: item = (query, partition, chunk of kAggChunk
+// words); the chunk's words are counted and summed per group in LDS and the
+// non-zero groups are added to the query's outputs (one atomic pair per
+// group per chunk instead of one per match).
+__device__ __forceinline__ uint32_t find_agg_query(const QDesc *d, uint32_t nq, uint64_t item) {
+  uint32_t lo = 0, hi = nq;  // last q with agg0 <= item
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (d[mid].agg0 <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(kBatchBlock) void k_batch_agg(BatchHdr *hdr, const QDesc *d) {
+  __shared__ uint32_t acnt[kStageGroups];
+  __shared__ unsigned long long asum[kStageGroups];
+  __shared__ uint64_t s_item;
+  const uint32_t nq = hdr->nq;
+  const uint64_t total = hdr->total_agg;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(&hdr->agg_ctr, 1u);
+    __syncthreads();
+    const uint64_t item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const QDesc &q = d[find_agg_query(d, nq, item)];
+    const uint64_t local = item - q.agg0;
+    const uint32_t p = (uint32_t)(local / q.cpp);
+    const uint64_t c0 = (local % q.cpp) * kAggChunk;
+    const uint64_t used = min((unsigned long long)q.pcur[p], (unsigned long long)q.cap);
+    if (c0 >= used) continue;  // uniform: every thread read the same item
+    const uint64_t c1 = min((unsigned long long)(c0 + kAggChunk), (unsigned long long)used);
+    for (uint32_t i = tid; i < kStageGroups; i += kBatchBlock) {
+      acnt[i] = 0;
+      asum[i] = 0;
+    }
+    __syncthreads();
+    const unsigned long long *w = q.stage + (uint64_t)p * q.cap;
+    for (uint64_t i = c0 + tid; i < c1; i += kBatchBlock) {
+      const unsigned long long x = w[i];
+      const uint32_t gl = (uint32_t)(x >> 53);
+      const long long sv = (long long)(x << 11) >> 11;
+      atomicAdd(&acnt[gl], 1u);
+      atomicAdd(&asum[gl], (unsigned long long)sv);
+    }
+    __syncthreads();
+    const uint32_t gbase = p * kStageGroups;
+    for (uint32_t i = tid; i < kStageGroups; i += kBatchBlock)
+      if (acnt[i]) {
+        atomicAdd((unsigned long long *)&q.cnt[gbase + i], (unsigned long long)acnt[i]);
+        atomicAdd((unsigned long long *)&q.sum[gbase + i], asum[i]);
+      }
+    __syncthreads();
+  }
+}
+
 // One stream slot: pinned + device descriptor buffers, its graph.
 struct Slot {
   hipStream_t stream = nullptr;
@@ -325,6 +511,7 @@ struct Slot {
   hipGraphExec_t exec = nullptr;
   uint32_t graph_grid = 0;
   bool busy = false;
+  std::vector<void *> stage;  // staging of the batch in flight
 };
 
 struct Batches {
@@ -385,10 +572,46 @@ static uint32_t lds_dim_of(pj_ctx *ctx, const pj_batch_cfg *cfg) {
 // Writes header + descriptors into a host buffer.  Queries whose dimension
 // fits LDS come first (their items run in the LDS-table kernel); every query
 // owns at least one item (its first item clears its outputs).
-static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, void *h_buf) {
+// Staging geometry of a query with more groups than LDS holds (parts == 0:
+// not staged): partitions of kStageGroups groups, runs sized for uniform
+// group keys plus slack (a fuller run spills to atomics).
+struct StageGeo {
+  uint32_t parts = 0;
+  uint64_t cap = 0;
+  uint64_t bytes() const { return parts ? parts * cap * 8 + 256 : 0; }
+};
+static StageGeo stage_geo(const pj_query &q) {
+  StageGeo g;
+  if (q.groups <= kLdsGroups || q.groups > kStageMaxParts * kStageGroups || !q.n) return g;
+  g.parts = (q.groups + kStageGroups - 1) / kStageGroups;
+  g.cap = (ceil_div(q.n, g.parts) + 4096 + 3) & ~3ULL;
+  return g;
+}
+// Staging for the queries that want it, from the pool; a query whose
+// staging cannot be allocated aggregates with atomics instead.
+static void stage_alloc(pj_ctx *ctx, const pj_query *qs, uint32_t nq, std::vector<void *> &ptr,
+                        std::vector<void *> &owned) {
+  ptr.assign(nq, nullptr);
+  for (uint32_t i = 0; i < nq; ++i) {
+    const StageGeo g = stage_geo(qs[i]);
+    if (!g.parts) continue;
+    void *p = nullptr;
+    if (dalloc(ctx, g.bytes(), &p) == PJ_OK) {
+      ptr[i] = p;
+      owned.push_back(p);
+    }
+  }
+}
+static void stage_free(pj_ctx *ctx, std::vector<void *> &owned) {
+  for (void *p : owned) (void)dfree(ctx, p);
+  owned.clear();
+}
+
+static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, void *h_buf,
+                const std::vector<void *> *stage) {
   BatchHdr *hdr = (BatchHdr *)h_buf;
   QDesc *d = (QDesc *)((char *)h_buf + 256);
-  uint64_t tiles = 0, lds_tiles = 0;
+  uint64_t tiles = 0, lds_tiles = 0, aggs = 0;
   uint32_t j = 0;
   for (int pass = 0; pass < 2; ++pass)
     for (uint32_t i = 0; i < nq; ++i) {
@@ -399,6 +622,24 @@ static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, v
       d[j].tile0 = tiles;
       tiles += std::max<uint64_t>(1, ceil_div(qs[i].n, T));
       if (lds) lds_tiles = tiles;
+      // staging only on the HBM-table path (the LDS-table kernel keeps atomics)
+      void *sp = (stage && !lds) ? (*stage)[i] : nullptr;
+      const StageGeo g = stage_geo(qs[i]);
+      d[j].agg0 = aggs;
+      if (sp && g.parts) {
+        d[j].stage = (unsigned long long *)sp;
+        d[j].parts = g.parts;
+        d[j].cap = g.cap;
+        d[j].pcur = (uint32_t *)((char *)sp + g.parts * g.cap * 8);
+        d[j].cpp = (uint32_t)ceil_div(g.cap, kAggChunk);
+        aggs += (uint64_t)g.parts * d[j].cpp;
+      } else {
+        d[j].stage = nullptr;
+        d[j].pcur = nullptr;
+        d[j].parts = 0;
+        d[j].cap = 0;
+        d[j].cpp = 0;
+      }
       ++j;
     }
   hdr->nq = nq;
@@ -406,6 +647,8 @@ static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, v
   hdr->total_tiles = tiles;
   hdr->lds_tiles = lds_tiles;
   hdr->gen_ctr = hdr->probe_ctr_lds = hdr->probe_ctr_hbm = hdr->bad_group = 0;
+  hdr->total_agg = aggs;
+  hdr->agg_ctr = 0;
   return PJ_OK;
 }
 
@@ -418,6 +661,7 @@ static int enqueue(pj_ctx *ctx, hipStream_t s, void *h_buf, void *d_buf, uint32_
   hipLaunchKernelGGL(k_batch_gen, dim3(grid * 2), dim3(256), 0, s, hdr, d);
   hipLaunchKernelGGL(k_batch_probe<true>, dim3(grid), dim3(kBatchBlock), 0, s, hdr, d);
   hipLaunchKernelGGL(k_batch_probe<false>, dim3(grid * 2), dim3(kBatchBlock), 0, s, hdr, d);
+  hipLaunchKernelGGL(k_batch_agg, dim3(grid * 2), dim3(kBatchBlock), 0, s, hdr, d);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PJ_E_HIP, "batch launch: %s", hipGetErrorString(e));
   return PJ_OK;
@@ -438,6 +682,7 @@ static void slot_free(pj_ctx *ctx, Slot &sl) {
   if (sl.done) (void)hipEventDestroy(sl.done);
   if (sl.h_buf) (void)hipHostFree(sl.h_buf);
   if (sl.d_buf) (void)dfree(ctx, sl.d_buf);
+  stage_free(ctx, sl.stage);
   if (sl.stream) (void)hipStreamDestroy(sl.stream);
   sl = Slot();
 }
@@ -534,12 +779,19 @@ extern "C" int pj_query_run(pj_ctx *ctx, const pj_query *q, const pj_batch_cfg *
   const uint32_t T = tile_rows_of(ctx, cfg);
   char *h = (char *)ctx->pinned_small;
   static_assert(256 + sizeof(QDesc) <= (1 << 20), "pinned staging too small");
-  PJ_TRY(pack(q, 1, T, lds_dim_of(ctx, cfg), h));
+  std::vector<void *> sp, owned;
+  stage_alloc(ctx, q, 1, sp, owned);
+  int rc = pack(q, 1, T, lds_dim_of(ctx, cfg), h, &sp);
   Scratch dbuf;
-  PJ_TRY(dbuf.alloc(ctx, 256 + sizeof(QDesc)));
-  PJ_TRY(enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, cfg), 256 + sizeof(QDesc)));
-  PJ_HIP(hipMemcpyAsync(h, dbuf.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, ctx->stream));
-  PJ_TRY(sync(ctx));
+  if (rc == PJ_OK) rc = dbuf.alloc(ctx, 256 + sizeof(QDesc));
+  if (rc == PJ_OK) rc = enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, cfg), 256 + sizeof(QDesc));
+  if (rc == PJ_OK &&
+      hipMemcpyAsync(h, dbuf.p, sizeof(BatchHdr), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = fail(PJ_E_HIP, "query header copy failed");
+  const int rs = sync(ctx);
+  stage_free(ctx, owned);
+  PJ_TRY(rc);
+  PJ_TRY(rs);
   return check_bad(h);
 }
 
@@ -562,7 +814,16 @@ extern "C" int pj_batch_launch(pj_ctx *ctx, int slot, const pj_query *qs, uint32
   Slot &sl = b.slots[slot];
   if (sl.busy) return fail(PJ_E_ARG, "batch slot %d is busy", slot);
   const uint32_t T = tile_rows_of(ctx, cfg);
-  PJ_TRY(pack(qs, nq, T, lds_dim_of(ctx, cfg), sl.h_buf));
+  stage_free(ctx, sl.stage);  // left by a launch that failed (the slot is idle)
+  std::vector<void *> sp;
+  stage_alloc(ctx, qs, nq, sp, sl.stage);
+  {
+    const int rc = pack(qs, nq, T, lds_dim_of(ctx, cfg), sl.h_buf, &sp);
+    if (rc != PJ_OK) {
+      stage_free(ctx, sl.stage);
+      return rc;
+    }
+  }
   const uint32_t grid = grid_of(ctx, cfg);
   if (!sl.exec || sl.graph_grid != grid) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -603,6 +864,7 @@ extern "C" int pj_batch_poll(pj_ctx *ctx, int slot, int *done) {
   }
   if (e != hipSuccess) return fail(PJ_E_HIP, "batch: %s", hipGetErrorString(e));
   sl.busy = false;
+  stage_free(ctx, sl.stage);
   return check_bad(sl.h_buf);
 }
 
@@ -614,6 +876,7 @@ extern "C" int pj_batch_wait(pj_ctx *ctx, int slot) {
   if (!sl.busy) return PJ_OK;
   PJ_HIP(hipEventSynchronize(sl.done));
   sl.busy = false;
+  stage_free(ctx, sl.stage);
   return check_bad(sl.h_buf);
 }
 
@@ -634,10 +897,12 @@ extern "C" int pj_batch_autotune(pj_ctx *ctx, const pj_query *qs, uint32_t nq,
   hipEvent_t ev[2];
   PJ_HIP(hipEventCreate(&ev[0]));
   PJ_HIP(hipEventCreate(&ev[1]));
+  std::vector<void *> sp, owned;
+  stage_alloc(ctx, qs, nq, sp, owned);
   auto time_cfg = [&](const pj_batch_cfg &c, float *ms) -> int {
     const size_t bytes = 256 + sizeof(QDesc) * nq;
     for (int rep = 0; rep < 2; ++rep) {
-      PJ_TRY(pack(qs, nq, tile_rows_of(ctx, &c), lds_dim_of(ctx, &c), h));
+      PJ_TRY(pack(qs, nq, tile_rows_of(ctx, &c), lds_dim_of(ctx, &c), h, &sp));
       PJ_HIP(hipEventRecord(ev[0], ctx->stream));
       PJ_TRY(enqueue(ctx, ctx->stream, h, dbuf.p, grid_of(ctx, &c), bytes));
       PJ_HIP(hipEventRecord(ev[1], ctx->stream));
@@ -676,6 +941,8 @@ extern "C" int pj_batch_autotune(pj_ctx *ctx, const pj_query *qs, uint32_t nq,
   }
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
+  (void)sync(ctx);
+  stage_free(ctx, owned);
   PJ_TRY(rc);
   PJ_TRY(d2h(ctx, h, dbuf.p, sizeof(BatchHdr)));  // the last run's group check
   PJ_TRY(check_bad(h));

@@ -56,14 +56,51 @@ def test_query_matches_oracle(ctx, u, groups):
     fk, val, grp = _inputs(ctx, n)
     cnt, sm = ctx.empty(groups, np.int64), ctx.empty(groups, np.int64)
     q = pa.Query(dim, fk.ptr, val.ptr, grp.ptr, n, groups, 0, seed, fk_range, cnt.ptr, sm.ptr)
-    for tile in (0, 4096):
-        ctx.query_run(q, pa.BatchCfg(tile, 0))
-        wc, ws = expected(dk.numpy(), dv.numpy(), n, groups, seed, fk_range)
-        assert np.array_equal(cnt.numpy(), wc)
-        assert np.array_equal(sm.numpy(), ws)
+    wc, ws = expected(dk.numpy(), dv.numpy(), n, groups, seed, fk_range)
+    # LDS or HBM dimension table (lds_dim 2048 / -1); more than 2048 groups
+    # on the HBM table take the staged aggregation
+    for tile, lds in ((0, 2048), (4096, 2048), (0, -1), (4096, -1)):
+        ctx.query_run(q, pa.BatchCfg(tile, 0, lds))
+        assert np.array_equal(cnt.numpy(), wc), (tile, lds)
+        assert np.array_equal(sm.numpy(), ws), (tile, lds)
     # the same columns again, given (no generation)
     q.gen_seed = 0
     ctx.query_run(q)
+    assert np.array_equal(cnt.numpy(), wc) and np.array_equal(sm.numpy(), ws)
+    for a in (fk, val, grp, cnt, sm):
+        a.free()
+    ctx.dim_destroy(dim)
+
+
+@pytest.mark.gpu
+def test_staged_groups_with_one_hot_group(ctx):
+    """Every probe row in one group of 65536: that group's staged run
+    overflows and the rest of its matches fall back to atomics; the result
+    is exact, alone and batched."""
+    nb = 1 << 14
+    dk = ctx.gen_perm_i32(nb, 3)
+    dv = ctx.gen_uniform_i32(nb, 3, 7, 0, 1 << 31)
+    dim = ctx.dim_create(dk, dv)
+    n, groups = 300001, 1 << 16
+    fk_h = O.gen_uniform_i32(n, 9, 1, 0, int(1.33 * nb))
+    val_h = O.gen_uniform_i32(n, 9, 2, -(1 << 31), 1 << 32)
+    grp_h = np.full(n, 4099, np.int32)
+    fk, val, grp = (ctx.to_device(a) for a in (fk_h, val_h, grp_h))
+    cnt, sm = ctx.empty(groups, np.int64), ctx.empty(groups, np.int64)
+    q = pa.Query(dim, fk.ptr, val.ptr, grp.ptr, n, groups, 0, 0, 0, cnt.ptr, sm.ptr)
+    pos = np.full(nb, -1, np.int64)
+    pos[dk.numpy()] = np.arange(nb)
+    hit = fk_h < nb
+    b = dv.numpy().astype(np.int64)[pos[fk_h[hit]]]
+    wc = np.zeros(groups, np.int64)
+    ws = np.zeros(groups, np.int64)
+    wc[4099] = int(hit.sum())
+    ws[4099] = int((b + val_h[hit].astype(np.int64)).sum())
+    ctx.query_run(q, pa.BatchCfg(8192, 0, -1))
+    assert np.array_equal(cnt.numpy(), wc) and np.array_equal(sm.numpy(), ws)
+    ctx.batch_slots(1)
+    ctx.batch_launch(0, [q], pa.BatchCfg(65536, 0, -1))
+    ctx.batch_wait(0)
     assert np.array_equal(cnt.numpy(), wc) and np.array_equal(sm.numpy(), ws)
     for a in (fk, val, grp, cnt, sm):
         a.free()
