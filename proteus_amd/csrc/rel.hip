@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 // through the stages with no LDS and no barriers (a row ends at its first
 // miss; a wave stops when all its rows have), which keeps far more probes in
 // flight than a tile-synchronous interpreter.
-constexpr int kChainRows = 4;
+constexpr int kChainRows = 8;
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
