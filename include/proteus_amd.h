@@ -214,7 +214,7 @@ typedef struct {
   uint32_t groups;         /*   1, 2, 3 of gen_seed)                          */
   uint32_t flags;          /* 0                                               */
   uint64_t gen_seed;
-  uint64_t fk_range;
+  uint64_t fk_range;        /* 1 .. 2^31 when gen_seed != 0                    */
   int64_t *d_cnt;          /* outputs, `groups` entries each                  */
   int64_t *d_sum;
 } pj_query;

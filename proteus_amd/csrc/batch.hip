@@ -154,10 +154,16 @@ __global__ __launch_bounds__(256) void k_batch_gen(BatchHdr *hdr, const QDesc *d
     const uint64_t r0 = (item - q.tile0) * T;
     const uint64_t r1 = r0 + T < q.n ? r0 + T : q.n;
     const uint64_t k1 = rng_key(q.gen_seed, 1), k2 = rng_key(q.gen_seed, 2), k3 = rng_key(q.gen_seed, 3);
-    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-      q.gfk[r] = (int32_t)mulhi64(rng_at(k1, r), q.fk_range);
-      q.gval[r] = (int32_t)mulhi64(rng_at(k2, r), 1ULL << 31);
-      q.ggrp[r] = (int32_t)mulhi64(rng_at(k3, r), q.groups);
+    // rng_at(k, r) = mix64(k + kGolden (r + 1)): the counter term advances
+    // by kGolden * blockDim per step (mod 2^64, as in rng_at); ranges are
+    // below 2^32, so the scaling takes two 32-bit products
+    const uint32_t fr = (uint32_t)q.fk_range, gr = q.groups;
+    uint64_t ctr = kGolden * (r0 + threadIdx.x + 1);
+    const uint64_t step = kGolden * blockDim.x;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x, ctr += step) {
+      q.gfk[r] = (int32_t)mulhi64_32(mix64(k1 + ctr), fr);
+      q.gval[r] = (int32_t)mulhi64_32(mix64(k2 + ctr), 1u << 31);
+      q.ggrp[r] = (int32_t)mulhi64_32(mix64(k3 + ctr), gr);
     }
   }
 }
@@ -531,7 +537,8 @@ static int fill_desc(const pj_query &q, QDesc *d) {
   if (!q.d_cnt || !q.d_sum) return fail(PJ_E_ARG, "query without output arrays");
   if (!q.groups) return fail(PJ_E_ARG, "query with 0 groups");
   if (q.n && (!q.d_fk || !q.d_val || !q.d_grp)) return fail(PJ_E_ARG, "query without probe columns");
-  if (q.gen_seed && !q.fk_range) return fail(PJ_E_ARG, "generated query needs fk_range");
+  if (q.gen_seed && (!q.fk_range || q.fk_range > (1ULL << 31)))
+    return fail(PJ_E_ARG, "generated query needs fk_range in [1, 2^31]");
   if (q.n >= (1ULL << 40)) return fail(PJ_E_ARG, "query probe side too large");
   d->table = q.dim->table;
   d->mask = q.dim->slots - 1;

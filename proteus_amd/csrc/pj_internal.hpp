@@ -90,6 +90,14 @@ __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
   return (uint64_t)(((unsigned __int128)a * b) >> 64);
 #endif
 }
+// mulhi64(a, b) for b < 2^32 with two 32 x 32 -> 64 products: with
+// a = ah 2^32 + al, (a b) >> 64 = (ah b + ((al b) >> 32)) >> 32 exactly (the
+// low 32 bits of al b cannot carry into bit 64), and ah b + 2^32 - 1 < 2^64.
+__host__ __device__ __forceinline__ uint64_t mulhi64_32(uint64_t a, uint32_t b) {
+  const uint64_t hi = (a >> 32) * (uint64_t)b;
+  const uint64_t lo = (a & 0xffffffffULL) * (uint64_t)b;
+  return (hi + (lo >> 32)) >> 32;
+}
 
 // ---------------------------------------------------------------------------
 // device memory: caching pool with an HBM budget

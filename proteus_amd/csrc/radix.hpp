@@ -154,20 +154,50 @@ __global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__
   __syncthreads();
   const uint64_t lo = (uint64_t)blockIdx.x * chunk;
   const uint64_t hi = lo + chunk < n ? lo + chunk : n;
-  if (lo < hi) {
-    const bool vec = ((uintptr_t)(keys + lo) & 15) == 0;
-    uint64_t i = lo + (uint64_t)threadIdx.x * 4;
-    if (vec) {
-      for (; i + 3 < hi; i += 4096) {
-        int4 k = ld_nt4(keys + i);
-        atomicAdd(&hsh[joint(k.x)], 1u);
-        atomicAdd(&hsh[joint(k.y)], 1u);
-        atomicAdd(&hsh[joint(k.z)], 1u);
-        atomicAdd(&hsh[joint(k.w)], 1u);
+  const uint64_t nvec = lo < hi ? (hi - lo) >> 2 : 0;
+  if (nvec && ((uintptr_t)(keys + lo) & 15) == 0) {
+    // Sets of U vectors per thread, two sets in flight: the loads of the next
+    // set are issued before the LDS atomics of the current one.  Loads are
+    // unconditional (clamped to the last vector) so the compiler keeps them
+    // in flight; a clamped duplicate adds 0.
+    constexpr int U = 4;
+    const int4 *kv = reinterpret_cast<const int4 *>(keys + lo);
+    const uint64_t last = nvec - 1;
+    const uint64_t nsets = (nvec + U * 1024 - 1) / (U * 1024);
+    auto vix = [&](uint64_t set, int u) {
+      return (set * U + u) * 1024 + threadIdx.x;
+    };
+    auto load = [&](uint64_t set, int4 (&k)[U]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t v = vix(set, u);
+        k[u] = ld_nt4(kv + (v < last ? v : last));
       }
+    };
+    auto count = [&](uint64_t set, const int4 (&k)[U]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t inc = vix(set, u) < nvec ? 1u : 0u;
+        atomicAdd(&hsh[joint(k[u].x)], inc);
+        atomicAdd(&hsh[joint(k[u].y)], inc);
+        atomicAdd(&hsh[joint(k[u].z)], inc);
+        atomicAdd(&hsh[joint(k[u].w)], inc);
+      }
+    };
+    int4 A[U], B[U];
+    load(0, A);
+    uint64_t s = 0;
+    for (; s + 1 < nsets; s += 2) {
+      load(s + 1, B);
+      count(s, A);
+      load(s + 2 < nsets ? s + 2 : nsets - 1, A);
+      count(s + 1, B);
     }
-    for (; i < hi; i += 4096)
-      for (uint64_t j = i; j < i + 4 && j < hi; ++j) atomicAdd(&hsh[joint(keys[j])], 1u);
+    if (s < nsets) count(s, A);
+    const uint64_t j = lo + nvec * 4 + threadIdx.x;  // < 4 tail keys
+    if (j < hi) atomicAdd(&hsh[joint(keys[j])], 1u);
+  } else if (lo < hi) {
+    for (uint64_t j = lo + threadIdx.x; j < hi; j += 1024) atomicAdd(&hsh[joint(keys[j])], 1u);
   }
   __syncthreads();
   uint32_t *row = H + (uint64_t)blockIdx.x * nb;
