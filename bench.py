@@ -260,8 +260,13 @@ def batched_mix(ctx, nq, qps):
     zeros = np.zeros(nq)
     res = {}
     for mode, name in ((0, "batched"), (1, "plain")):
+        m0 = ctx.mem_stats()
         done, st = ctx.serve_run(mix.queries(), zeros, mode=mode, slots=3, hbm_budget=budget,
                                  **(plain_cfg if mode else {}))
+        m1 = ctx.mem_stats()
+        # device allocations and cache releases during the run (each release
+        # is a hipFree, which waits for the whole device)
+        out[name + "_pool"] = {k: m1[k] - m0[k] for k in ("mallocs", "evictions")}
         res[mode] = mix.results()
         out[name + "_qps"] = nq / st["wall_s"]
         out[name + "_probe_rows_per_s"] = mix.probe_rows() / st["wall_s"]

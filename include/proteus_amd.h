@@ -54,6 +54,11 @@ int pj_memcpy_h2d(pj_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int pj_memcpy_d2h(pj_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 int pj_memset(pj_ctx *ctx, void *d_dst, int byte, uint64_t bytes);
 int pj_mem_in_use(pj_ctx *ctx, uint64_t *bytes);
+/* Pool counters: out[0] bytes handed out, out[1] bytes cached for reuse,
+ * out[2] hipMalloc calls (cache misses), out[3] cached blocks released.
+ * Host-side bookkeeping of this library; the reference's memory managers
+ * (core/platform/lib/memory/memory-manager.cpp) keep no such counters. */
+int pj_mem_stats(pj_ctx *ctx, uint64_t out[4]);
 /* Device-to-device copy with a streaming HIP kernel (the reference's
  * gpu memcpy helper, core/platform/lib/memory/buffer-manager.cu:846,908). */
 int pj_copy(pj_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);

@@ -94,6 +94,7 @@ _SIGS = {
     "pj_memcpy_d2h": ([vp, vp, vp, u64], i32),
     "pj_memset": ([vp, vp, i32, u64], i32),
     "pj_mem_in_use": ([vp, C.POINTER(u64)], i32),
+    "pj_mem_stats": ([vp, C.POINTER(u64)], i32),
     "pj_copy": ([vp, vp, vp, u64], i32),
     "pj_prof_enable": ([vp, i32], i32),
     "pj_prof_reset": ([vp], i32),
@@ -236,6 +237,13 @@ class Context:
         b = u64()
         _check(_lib.pj_mem_in_use(self.h, C.byref(b)))
         return b.value
+
+    def mem_stats(self):
+        """Pool counters: bytes in use and cached, hipMalloc calls, cached
+        blocks released."""
+        b = (u64 * 4)()
+        _check(_lib.pj_mem_stats(self.h, b))
+        return {"in_use": b[0], "cached": b[1], "mallocs": b[2], "evictions": b[3]}
 
     # ---- profiling -------------------------------------------------------
     def prof_enable(self, on=True):

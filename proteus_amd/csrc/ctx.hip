@@ -47,7 +47,7 @@ int dalloc(pj_ctx *ctx, uint64_t bytes, void **p) {
   // reuse a cached block of exactly this size class, or up to 2x larger
   auto it = pool.free_blocks.lower_bound(sz);
   if (it != pool.free_blocks.end() && it->first <= sz * 2) {
-    *p = it->second;
+    *p = it->second.p;
     uint64_t got = it->first;
     pool.free_blocks.erase(it);
     pool.cached -= got;
@@ -62,7 +62,7 @@ int dalloc(pj_ctx *ctx, uint64_t bytes, void **p) {
   hipError_t e = hipMalloc(p, sz);
   if (e != hipSuccess) {
     // release cached blocks and retry once
-    for (auto &kv : pool.free_blocks) (void)hipFree(kv.second);
+    for (auto &kv : pool.free_blocks) (void)hipFree(kv.second.p);
     pool.free_blocks.clear();
     pool.cached = 0;
     (void)hipGetLastError();
@@ -76,6 +76,7 @@ int dalloc(pj_ctx *ctx, uint64_t bytes, void **p) {
   }
   pool.live[*p] = sz;
   pool.in_use += sz;
+  ++pool.mallocs;
   return PJ_OK;
 }
 
@@ -87,17 +88,23 @@ int dfree(pj_ctx *ctx, void *p) {
   uint64_t sz = it->second;
   pool.live.erase(it);
   pool.in_use -= sz;
-  pool.free_blocks.emplace(sz, p);
+  pool.free_blocks.emplace(sz, Pool::Cached{p, ++pool.clock});
   pool.cached += sz;
-  // keep the cache bounded: drop the largest blocks beyond a quarter of HBM
+  // Keep the cache within a quarter of HBM by dropping the least recently
+  // freed blocks: the blocks of an earlier workload go first, and a workload
+  // whose own blocks fit stops evicting (every hipFree waits for the whole
+  // device, which would serialise overlapping batches).
   const uint64_t cap = std::max<uint64_t>(8ULL << 30, ctx->hbm_bytes / 4);
   while (pool.cached > cap && !pool.free_blocks.empty()) {
-    auto last = std::prev(pool.free_blocks.end());
+    auto old = pool.free_blocks.begin();
+    for (auto j = pool.free_blocks.begin(); j != pool.free_blocks.end(); ++j)
+      if (j->second.stamp < old->second.stamp) old = j;
     // all work on this context is stream-ordered; free after the stream drains
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(last->second);
-    pool.cached -= last->first;
-    pool.free_blocks.erase(last);
+    (void)hipFree(old->second.p);
+    pool.cached -= old->first;
+    pool.free_blocks.erase(old);
+    ++pool.evictions;
   }
   return PJ_OK;
 }
@@ -465,7 +472,7 @@ int pj_ctx_destroy(pj_ctx *ctx) {
   pj_batch_release(ctx);
   pj_engine_release_catalog(ctx);
   for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
-  for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
+  for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second.p);
   for (auto &r : ctx->prof_pending) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -555,6 +562,15 @@ int pj_copy(pj_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes) {
 
 int pj_mem_in_use(pj_ctx *ctx, uint64_t *bytes) {
   *bytes = ctx->pool.in_use;
+  return PJ_OK;
+}
+
+int pj_mem_stats(pj_ctx *ctx, uint64_t out[4]) {
+  if (!ctx || !out) return fail(PJ_E_ARG, "null argument");
+  out[0] = ctx->pool.in_use;
+  out[1] = ctx->pool.cached;
+  out[2] = ctx->pool.mallocs;
+  out[3] = ctx->pool.evictions;
   return PJ_OK;
 }
 

@@ -106,7 +106,14 @@ struct Pool {
   uint64_t budget = 0;      // 0 = unlimited
   uint64_t in_use = 0;      // bytes handed out (user + scratch)
   uint64_t cached = 0;      // bytes held in free lists
-  std::multimap<uint64_t, void *> free_blocks;  // size -> ptr
+  uint64_t clock = 0;       // frees so far: the age stamp of cached blocks
+  uint64_t mallocs = 0;     // hipMalloc calls (cache misses)
+  uint64_t evictions = 0;   // cached blocks returned with hipFree
+  struct Cached {
+    void *p;
+    uint64_t stamp;  // clock when the block was freed
+  };
+  std::multimap<uint64_t, Cached> free_blocks;  // size -> block
   std::unordered_map<void *, uint64_t> live;    // ptr -> size
 };
 
