@@ -75,11 +75,13 @@ __global__ __launch_bounds__(256) void k_colscan(const uint32_t *__restrict__ H,
   if (c1 == G) T[d] = run;
 }
 
-__global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__ T, int nb,
-                                                    unsigned long long *__restrict__ S) {
-  __shared__ unsigned long long tmp[16];
+__global__ __launch_bounds__(kScanBlock) void k_excl_scan(const uint32_t *__restrict__ T, int nb,
+                                                          unsigned long long *__restrict__ S,
+                                                          uint32_t *__restrict__ ctr) {
+  __shared__ unsigned long long tmp[kScanBlock / 64];
   const int tid = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
+  if (tid == 0) *ctr = 0;
+  const int per = (nb + kScanBlock - 1) / kScanBlock;
   const int lo = min(tid * per, nb), hi = min(lo + per, nb);
   unsigned long long sum = 0;
   for (int i = lo; i < hi; i += 8) {
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
     for (int j = 0; j < 8; ++j) sum += x[j];
   }
   unsigned long long tot;
-  unsigned long long run = block_excl_scan<1024, unsigned long long>(sum, tmp, &tot);
+  unsigned long long run = block_excl_scan<kScanBlock, unsigned long long>(sum, tmp, &tot);
   for (int i = lo; i < hi; i += 8) {
     uint32_t x[8];
 #pragma unroll
@@ -642,7 +644,8 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       hipStream_t aux;
       PJ_TRY(aux_fork(ctx, &aux));
       // (Starting the build side's later passes only beside the probe
-      // side's second pass measured the same.)
+      // side's second pass, or all of them only after the probe side's
+      // histograms and scans, measured the same.)
       const int b1 = cfg.bits1, b2 = cfg.bits2;
       int rc;
       {

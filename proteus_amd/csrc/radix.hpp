@@ -215,9 +215,12 @@ __global__ void k_colscan(const uint32_t *__restrict__ H, int G, int nb, int sla
                           uint32_t *__restrict__ T);
 // Number of slabs (return) and chunks per slab for G chunks and nb bins.
 int colscan_slabs(const pj_ctx *ctx, int G, int nb, int *slab);
-// Exclusive scan of T[0..nb) into S[0..nb] (one workgroup of 1024 threads).
+// Exclusive scan of T[0..nb) into S[0..nb], and *ctr = 0 (the pass-2 item
+// counter), in one workgroup of kScanBlock threads: a small workgroup finds a
+// free slot sooner while the other side's passes hold the CUs.
+constexpr int kScanBlock = 256;
 __global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
-                            unsigned long long *__restrict__ S);
+                            unsigned long long *__restrict__ S, uint32_t *__restrict__ ctr);
 
 // One radix pass.  PASS2 = false: workgroup c partitions input chunk c on the
 // pass-1 digit.  PASS2 = true: workgroups take (d1, group of cpg chunks) items
@@ -599,9 +602,8 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
             h.slab, h.P.as<uint32_t>());
   PJ_LAUNCH(ctx, "radix_colscan", k_colscan, cgrid, 256, 0, h.H.as<const uint32_t>(), p.G, p.nb,
             h.slab, h.P.as<const uint32_t>(), h.E.as<uint32_t>(), h.T.as<uint32_t>());
-  PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, 1024, 0, h.T.as<const uint32_t>(), p.nb,
-            h.S.as<unsigned long long>());
-  PJ_TRY(fill_u32(ctx, h.ctr.as<uint32_t>(), 0, 1));
+  PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, kScanBlock, 0, h.T.as<const uint32_t>(), p.nb,
+            h.S.as<unsigned long long>(), h.ctr.as<uint32_t>());
   if (readback) PJ_TRY(radix_readback(ctx, p, h));
   return PJ_OK;
 }
