@@ -277,6 +277,29 @@ int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s,
                  uint32_t nq, const pj_serve_cfg *cfg, double *done_s,
                  pj_serve_stats *stats);
 
+/* ---- bloom filters ---------------------------------------------------------
+ * BloomFilterBuild / BloomFilterProbe / BloomFilterRepack
+ * (core/olap/lib/operators/bloom-filter/bloom-filter-build.cpp:29-58,
+ * bloom-filter-probe.cpp:29-52, bloom-filter-repack.cpp:43-470) as called by
+ * RelBuilder::bloomfilter_build / _probe / _repack (relbuilder.cpp:957-975).
+ * A filter has filter_size entries (a power of two, 1 .. 2^32); key k maps to
+ * entry (uint32)k & (filter_size - 1), as BloomFilter::findInFilter
+ * (bloom-filter.cpp:85-107).  One filter per bloom_id per context, like the
+ * reference's registry (setBloomFilter / getBloomFilter, bloom-filter.cpp:36-49):
+ * a build makes a fresh filter and replaces the one of the same id.
+ * Probe writes the ascending row ids whose key's entry is set (d_idx holds n
+ * entries); repack also gathers ncols columns of esize[c] bytes (1, 4 or 8)
+ * at those rows into d_out[c] (n rows of room each).  Probing an id that was
+ * never built returns PJ_E_ARG. */
+int pj_bloom_build_i32(pj_ctx *ctx, uint64_t bloom_id, uint64_t filter_size,
+                       const int32_t *d_keys, uint64_t n);
+int pj_bloom_probe_i32(pj_ctx *ctx, uint64_t bloom_id, const int32_t *d_keys,
+                       uint64_t n, uint32_t *d_idx, uint64_t *n_out);
+int pj_bloom_repack(pj_ctx *ctx, uint64_t bloom_id, const int32_t *d_keys,
+                    uint64_t n, int ncols, const void *const *d_cols,
+                    const int *esize, void *const *d_out, uint64_t *n_out);
+int pj_bloom_release(pj_ctx *ctx, uint64_t bloom_id);
+
 /* ---- plan API ----------------------------------------------------------------
  * PlanExecutor::parseOperator (core/olap/lib/plan/plan-parser.cpp:238-1675)
  * + PreparedStatement::execute (prepared-statement.cpp:41-140): parse a

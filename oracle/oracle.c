@@ -13,6 +13,8 @@
  *   core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained-helpers.cpp:49-56
  *   core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained.cpp:595-610
  *   core/olap/lib/operators/gpu/gpu-hash-rearrange.cpp:338-348
+ *   core/olap/lib/operators/bloom-filter/bloom-filter.cpp:36-107,
+ *     bloom-filter-build.cpp:29-58, bloom-filter-probe.cpp:29-52
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -840,4 +842,32 @@ void or_histogram(const int32_t *k, uint64_t n, int shift, int bits,
   uint32_t mask = (bits >= 32) ? 0xffffffffU : ((1U << bits) - 1U);
   memset(hist, 0, ((uint64_t)mask + 1) * 8);
   for (uint64_t i = 0; i < n; ++i) hist[(fmix32u((uint32_t)k[i]) >> shift) & mask]++;
+}
+
+/* ======================================================================= */
+/* bloom filters                                                            */
+/* ======================================================================= */
+
+/* BloomFilterBuild (bloom-filter-build.cpp:29-58): a zeroed array of `size`
+ * one-byte flags (produce_ memsets it), and per input row
+ * filter[key & (size - 1)] = true (findInFilter, bloom-filter.cpp:85-107:
+ * the key itself, masked; size a power of two).  The mask is applied to the
+ * int32 key's bits, so negative keys select high entries. */
+void or_bloom_build(uint8_t *filter, uint64_t size, const int32_t *keys,
+                    uint64_t n) {
+  const uint32_t mask = (uint32_t)(size - 1);
+  memset(filter, 0, size);
+  for (uint64_t i = 0; i < n; ++i) filter[(uint32_t)keys[i] & mask] = 1;
+}
+
+/* BloomFilterProbe (bloom-filter-probe.cpp:43-52): the row passes when its
+ * key's flag is set; rows keep their order.  Returns the number passed and
+ * writes their row ids. */
+uint64_t or_bloom_probe(const uint8_t *filter, uint64_t size,
+                        const int32_t *keys, uint64_t n, uint32_t *out_idx) {
+  const uint32_t mask = (uint32_t)(size - 1);
+  uint64_t m = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if (filter[(uint32_t)keys[i] & mask]) out_idx[m++] = (uint32_t)i;
+  return m;
 }

@@ -15,6 +15,9 @@
  *   - or_groupby_chained: dense-array oracle, brute force, first-appearance
  *     order, exact FP64 sums for u/2^24 inputs.
  *   - generators: bijection / range / distribution invariants.
+ *   - or_bloom_build / or_bloom_probe: set-based semi-join on the masked keys
+ *     (brute force), no false negatives, exact semi-join when the filter
+ *     covers every key, a one-entry filter passing every row.
  */
 #ifndef PROTEUS_ORACLE_H
 #define PROTEUS_ORACLE_H
@@ -106,6 +109,12 @@ void or_reduce_i64(const int64_t *v, uint64_t n, int64_t *sum, int64_t *min,
 /* ---- radix partition invariants ---------------------------------------- */
 void or_histogram(const int32_t *k, uint64_t n, int shift, int bits,
                   uint64_t *hist);
+
+/* ---- bloom filters ------------------------------------------------------ */
+void or_bloom_build(uint8_t *filter, uint64_t size, const int32_t *keys,
+                    uint64_t n);
+uint64_t or_bloom_probe(const uint8_t *filter, uint64_t size,
+                        const int32_t *keys, uint64_t n, uint32_t *out_idx);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,10 @@ def _setup(L):
     u64, i32, i64, vp = C.c_uint64, C.c_int32, C.c_int64, C.c_void_p
     L.or_hash_i32.restype = u64
     L.or_hash_i32.argtypes = [i32]
+    L.or_bloom_build.restype = None
+    L.or_bloom_build.argtypes = [vp, u64, vp, u64]
+    L.or_bloom_probe.restype = u64
+    L.or_bloom_probe.argtypes = [vp, u64, vp, u64, vp]
     L.or_hash_i64.restype = u64
     L.or_hash_i64.argtypes = [i64]
     L.or_hash_combine.restype = u64
@@ -260,3 +264,22 @@ def histogram(k, shift, bits):
     h = np.zeros(1 << bits, np.uint64)
     lib().or_histogram(_p(k), len(k), shift, bits, _p(h))
     return h
+
+
+def bloom_build(keys, size):
+    """or_bloom_build: the reference's one-byte-per-entry filter
+    (bloom-filter-build.cpp:29-58) of `size` entries over the int32 keys."""
+    keys = _i32(keys)
+    f = np.zeros(int(size), np.uint8)
+    lib().or_bloom_build(f.ctypes.data, int(size), keys.ctypes.data, len(keys))
+    return f
+
+
+def bloom_probe(filt, keys):
+    """or_bloom_probe: ascending row ids of the keys whose flag is set
+    (bloom-filter-probe.cpp:43-52)."""
+    keys = _i32(keys)
+    out = np.empty(max(len(keys), 1), np.uint32)
+    m = lib().or_bloom_probe(filt.ctypes.data, len(filt), keys.ctypes.data, len(keys),
+                             out.ctypes.data)
+    return out[:m].copy()

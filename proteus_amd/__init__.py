@@ -120,6 +120,11 @@ _SIGS = {
     "pj_dim_create": ([vp, vp, vp, u64, C.POINTER(vp)], i32),
     "pj_dim_destroy": ([vp, vp], i32),
     "pj_dim_info": ([vp, vp, C.POINTER(u64), C.POINTER(u64)], i32),
+    "pj_bloom_build_i32": ([vp, u64, u64, vp, u64], i32),
+    "pj_bloom_probe_i32": ([vp, u64, vp, u64, vp, C.POINTER(u64)], i32),
+    "pj_bloom_repack": ([vp, u64, vp, u64, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp),
+                         C.POINTER(u64)], i32),
+    "pj_bloom_release": ([vp, u64], i32),
     "pj_query_run": ([vp, C.POINTER(Query), C.POINTER(BatchCfg)], i32),
     "pj_batch_slots": ([vp, i32], i32),
     "pj_batch_launch": ([vp, i32, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
@@ -376,6 +381,44 @@ class Context:
             self._keep.append(a)
             _check(_lib.pj_register_column(self.h, rel.encode(), attr.encode(), type_.encode(),
                                            a.ctypes.data, len(a), 0))
+
+    # ---- bloom filters (bloomfilter_build / _probe / _repack) ---------------
+    def bloom_build(self, bloom_id, filter_size, keys):
+        """A fresh filter of filter_size entries (a power of two) with the
+        entries of the int32 device column `keys` set; replaces the filter of
+        the same id."""
+        _check(_lib.pj_bloom_build_i32(self.h, bloom_id, filter_size, _ptr(keys), len(keys)))
+
+    def bloom_probe(self, bloom_id, keys):
+        """Ascending row ids (numpy uint32) of the keys whose entry is set."""
+        n = len(keys)
+        idx = self.empty(max(n, 1), np.uint32)
+        m = u64()
+        try:
+            _check(_lib.pj_bloom_probe_i32(self.h, bloom_id, _ptr(keys), n, idx.ptr, C.byref(m)))
+            return idx.numpy()[:m.value].copy()
+        finally:
+            idx.free()
+
+    def bloom_repack(self, bloom_id, keys, cols):
+        """The rows of the device columns `cols` whose key's entry is set, in
+        row order (numpy arrays)."""
+        n = len(keys)
+        outs = [self.empty(max(n, 1), c.dtype) for c in cols]
+        m = u64()
+        src = (vp * max(len(cols), 1))(*[c.ptr for c in cols])
+        dst = (vp * max(len(cols), 1))(*[o.ptr for o in outs])
+        es = (i32 * max(len(cols), 1))(*[c.dtype.itemsize for c in cols])
+        try:
+            _check(_lib.pj_bloom_repack(self.h, bloom_id, _ptr(keys), n, len(cols), src, es, dst,
+                                        C.byref(m)))
+            return [o.numpy()[:m.value].copy() for o in outs]
+        finally:
+            for o in outs:
+                o.free()
+
+    def bloom_release(self, bloom_id):
+        _check(_lib.pj_bloom_release(self.h, bloom_id))
 
     # ---- resident dimensions, batched queries, serving --------------------
     def dim_create(self, keys, vals=None):

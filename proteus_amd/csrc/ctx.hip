@@ -464,12 +464,14 @@ int pj_ctx_create(int device, uint64_t hbm_budget_bytes, pj_ctx **out) {
 
 void pj_engine_release_catalog(pj_ctx *ctx);
 void pj_batch_release(pj_ctx *ctx);
+void pj_bloom_release_all(pj_ctx *ctx);
 
 int pj_ctx_destroy(pj_ctx *ctx) {
   if (!ctx) return PJ_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   pj_batch_release(ctx);
+  pj_bloom_release_all(ctx);
   pj_engine_release_catalog(ctx);
   for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
   for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second.p);

@@ -151,6 +151,8 @@ struct pj_ctx {
   std::map<pj::TuneKey, pj_radix_cfg> tuned;
   // batched-query stream slots (owned by batch.hip)
   void *batch = nullptr;
+  // bloom filters by id (owned by bloom.hip)
+  void *bloom = nullptr;
   // second stream for work that overlaps the main stream's (created on first
   // use), with a fork and a join event
   hipStream_t aux = nullptr;

@@ -116,6 +116,13 @@ __device__ __forceinline__ int64_t cast(int from, int to, int64_t v) {
   }
 }
 
+// OP_BLOOM: bit (uint32)key & (2^log2 - 1) of a bit-packed filter.
+__device__ __forceinline__ int64_t bloom_bit(int64_t words, int log2, int64_t key) {
+  const uint32_t mask = log2 >= 32 ? 0xffffffffu : (1u << log2) - 1u;
+  const uint32_t h = (uint32_t)key & mask;
+  return (((const uint32_t *)words)[h >> 5] >> (h & 31)) & 1u;
+}
+
 // One row of the program; st points at this lane's slot 0, stride BLOCK.
 template <int BLOCK>
 __device__ __forceinline__ int64_t run_prog(const Prog &p, uint64_t row, int64_t *st) {
@@ -164,6 +171,7 @@ __device__ __forceinline__ int64_t run_prog(const Prog &p, uint64_t row, int64_t
         st[(sp - 1) * BLOCK] = (in.op == OP_MIN) == lt ? a : b;
         break;
       }
+      case OP_BLOOM: st[(sp - 1) * BLOCK] = bloom_bit(in.imm, in.a, st[(sp - 1) * BLOCK]); break;
       default: break;
     }
   }
@@ -661,6 +669,11 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
           }
           break;
         }
+        case OP_BLOOM: {
+          int64_t *x = st + (size_t)(sp - 1) * kPipeTile;
+          for (uint32_t i = lane; i < n; i += kPipeWave) x[i] = bloom_bit(in.imm, in.a, x[i]);
+          break;
+        }
         case OP_PROBE:
         case OP_FILTER: {
           --sp;  // the stack is empty after the pop
@@ -755,7 +768,8 @@ constexpr int kChainRows = 8;
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
-  int32_t kind;  // 0 probe table `tab` with key column `col`; 1 filter col cmp imm
+  int32_t kind;  // 0 probe table `tab` with key column `col`; 1 filter col cmp imm;
+                 // 2 bloom filter test of int32 column `col` (words at imm, log2 size op)
   int32_t col;
   int32_t tab;
   int32_t op;    // comparison (filters)
@@ -771,8 +785,8 @@ struct ChainArgs {
   int32_t ctype[kChainMaxCols];
   int32_t clevel[kChainMaxCols];
   ChainStage st[kChainMaxStages];
-  // Key column of stage s when it is a probe on a source-row int32 column
-  // (pf_on[s] = 1); such keys are loaded one stage ahead, beside the
+  // Key column of stage s when it is a probe or a bloom filter test on a
+  // source-row int32 column (pf_on[s] = 1); such keys are loaded one stage ahead, beside the
   // previous stage's probes.  Otherwise pf[s] points at a one-element
   // dummy read at index 0.
   const int32_t *pf[kChainMaxStages];
@@ -877,6 +891,13 @@ __device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &c
 #pragma unroll
         for (int k = 0; k < R; ++k)
           cr.alive[k] = cr.alive[k] && cmp(sg.op, ct == F64 || ct == F32 ? F64 : I64, v[k], sg.imm);
+      } else if (sg.kind == 2) {
+        // a dead row's key is still a valid int32, so every lane tests a bit
+        int64_t bit[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) bit[k] = bloom_bit(sg.imm, sg.op, v[k]);
+#pragma unroll
+        for (int k = 0; k < R; ++k) cr.alive[k] = cr.alive[k] && bit[k] != 0;
       } else {
         const PipeTable tb = a.tables[sg.tab];
         const uint32_t mask = (uint32_t)tb.mask;
@@ -1046,6 +1067,11 @@ static bool as_chain(const PipeProg &p, void *const *outs, ChainArgs *a) {
       if (p.ctype[I[k].a] != I32 || a->nst >= kChainMaxStages) return false;
       a->st[a->nst++] = ChainStage{0, I[k].a, I[k + 1].a, 0, 0};
       k += 2;
+    } else if (k + 2 < I.size() && I[k].op == OP_COL && I[k + 1].op == OP_BLOOM &&
+               I[k + 2].op == OP_FILTER) {
+      if (p.ctype[I[k].a] != I32 || a->nst >= kChainMaxStages) return false;
+      a->st[a->nst++] = ChainStage{2, I[k].a, 0, I[k + 1].a, I[k + 1].imm};
+      k += 3;
     } else if (k + 3 < I.size() && I[k].op == OP_COL && I[k + 1].op == OP_CONST &&
                I[k + 2].op >= OP_EQ && I[k + 2].op <= OP_GE && I[k + 3].op == OP_FILTER) {
       const int ct = p.ctype[I[k].a];
@@ -1088,9 +1114,12 @@ static int check_prog(const Prog &p) {
   if (p.n <= 0 || p.n > kMaxInstr) return fail(PJ_E_ARG, "expression program of %d instructions", p.n);
   if (p.depth <= 0 || p.depth > kMaxDepth)
     return fail(PJ_E_UNSUPPORTED, "expression needs %d stack slots (limit %d)", p.depth, kMaxDepth);
-  for (int k = 0; k < p.n; ++k)
+  for (int k = 0; k < p.n; ++k) {
     if (p.ins[k].op == OP_COL && (p.ins[k].a < 0 || p.ins[k].a >= p.ncols || !p.cols[p.ins[k].a]))
       return fail(PJ_E_ARG, "expression reads an unbound column");
+    if (p.ins[k].op == OP_BLOOM && (p.ins[k].a < 0 || p.ins[k].a > 32 || !p.ins[k].imm))
+      return fail(PJ_E_ARG, "expression bloom filter without words or with log2 size %d", p.ins[k].a);
+  }
   return PJ_OK;
 }
 
@@ -1292,10 +1321,12 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       if (i.op == OP_OUT && (i.a < 0 || i.a >= (int)p.out_t.size())) return fail(PJ_E_ARG, "pipeline output out of range");
       switch (i.op) {
         case OP_COL: case OP_CONST: ++sp; break;
-        case OP_NEG: case OP_NOT: case OP_CAST: break;
+        case OP_NEG: case OP_NOT: case OP_CAST: case OP_BLOOM: break;
         case OP_IF: sp -= 2; break;
         default: --sp; break;
       }
+      if (i.op == OP_BLOOM && (i.a < 0 || i.a > 32 || !i.imm))
+        return fail(PJ_E_ARG, "pipeline bloom filter without words or with log2 size %d", i.a);
       if (sp < 0 || sp > p.depth) return fail(PJ_E_ARG, "pipeline stack out of range");
       if ((i.op == OP_PROBE || i.op == OP_FILTER) && (sp != 0 || out_seen))
         return fail(PJ_E_ARG, "pipeline probes and filters need an empty stack, before the outputs");
@@ -1324,7 +1355,8 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       const int32_t *dummy =
           reinterpret_cast<const int32_t *>(dargs.as<char>() + ((sizeof(ChainArgs) + 15) & ~15));
       for (int st = 0; st < kChainMaxStages; ++st) {
-        const bool on = st < ca.nst && ca.st[st].kind == 0 && ca.clevel[ca.st[st].col] == 0;
+        const bool on = st < ca.nst && (ca.st[st].kind == 0 || ca.st[st].kind == 2) &&
+                        ca.clevel[ca.st[st].col] == 0;
         ca.pf_on[st] = on ? 1 : 0;
         ca.pf[st] = on ? (const int32_t *)ca.cols[ca.st[st].col] : dummy;
       }

@@ -36,6 +36,9 @@ enum OpCode : int32_t {
               // a hit sets the row id of level a + 1
   OP_FILTER,  // pops a bool; false ends the row
   OP_OUT,     // pops into output column a
+  // any program
+  OP_BLOOM,   // pops an int32 key k, pushes bit (uint32)k & (2^a - 1) of the
+              // filter words at address imm (a = log2 of the filter size)
 };
 
 struct Instr {
@@ -108,6 +111,10 @@ struct PipeProg {
 int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint64_t *n_out);
 // The table of a dimension built with pj_dim_create.
 int dim_table(const pj_dim *dim, PipeTable *t);
+
+// The words and log2 size of the context's bloom filter `id` (built with
+// pj_bloom_build_i32), for OP_BLOOM.
+int bloom_lookup(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_size);
 
 // bucket[i] = hashInt32(k[i]) mod buckets (the reference's hash-rearrange
 // target, gpu-hash-rearrange.cpp).

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <memory>
 #include <set>
 #include <sstream>
@@ -278,7 +279,7 @@ class Exec {
         if (sp > p.depth) p.depth = sp;
       } else if (op == R::OP_IF) {
         sp -= 2;
-      } else if (op != R::OP_NEG && op != R::OP_NOT && op != R::OP_CAST) {
+      } else if (op != R::OP_NEG && op != R::OP_NOT && op != R::OP_CAST && op != R::OP_BLOOM) {
         --sp;
       }
     }
@@ -501,6 +502,12 @@ class Exec {
       return join(op, need);
     }
     if (k == "hash-rearrange" || k == "hash-pack") return rearrange(op, need);
+    if (k == "bloomfilter-build") return bloom_build(op, need);
+    if (k == "bloomfilter-probe" || k == "bloomfilter-repack") {
+      Rel r;
+      if (!no_pipeline_ && pipeline(op, need, &r)) return r;
+      return bloom_filter(op, need);
+    }
     if (k == "sort") return sort(op, need);
     if (k == "print") return print(op, need);
     raise(PJ_E_UNSUPPORTED, "operator '" + k + "' is not supported on the device path");
@@ -730,6 +737,73 @@ class Exec {
       Col c = eval(e, in);
       if (Compiler(*this, in).is_col(e)) c.dict = Compiler(*this, in).column(e).dict;
       out.add(reg_attr(e, rel, "expr" + std::to_string(i++)), c);
+    }
+    return out;
+  }
+
+  // ---- bloom filters -------------------------------------------------
+  // bloomfilter-build / -probe / -repack {input, e, filterSize, bloomId
+  // [, projections]}: the JSON form of RelBuilder::bloomfilter_build /
+  // _probe / _repack (relbuilder.cpp:957-975; the reference's plan parser has
+  // no bloom operators).  The key expression is int32, as the reference's
+  // `e & (int32)(filterSize - 1)` (bloom-filter.cpp:85-107).
+  static uint64_t bloom_id(const Json &op) {
+    if (!op["bloomId"].is_number()) raise(PJ_E_PARSE, "bloom filter operator without bloomId");
+    return (uint64_t)op["bloomId"].as_int();
+  }
+  static uint64_t bloom_size(const Json &op) {
+    if (!op["filterSize"].is_number() || op["filterSize"].as_int() <= 0)
+      raise(PJ_E_PARSE, "bloom filter operator without a positive filterSize");
+    return (uint64_t)op["filterSize"].as_int();
+  }
+  // The filter of a probe or repack, checked against its filterSize.
+  void bloom_of(const Json &op, const uint32_t **words, int *lg) {
+    ck(R::bloom_lookup(ctx_, bloom_id(op), words, lg));
+    const uint64_t size = bloom_size(op);
+    if ((*lg >= 64 ? 0 : (1ULL << *lg)) != size)
+      raise(PJ_E_ARG, "bloom filter " + std::to_string(bloom_id(op)) + " was built with " +
+                          std::to_string(1ULL << *lg) + " entries, probed with " +
+                          std::to_string(size));
+  }
+
+  Rel bloom_build(const Json &op, const Need &need) {
+    Rel in = exec(op["input"], need_below(op, need));
+    if (Compiler(*this, in).infer(op["e"]) != R::I32)
+      raise(PJ_E_UNSUPPORTED, "bloom filter key must be an int32 expression");
+    Col k = eval_as(op["e"], in, R::I32);
+    ck(pj_bloom_build_i32(ctx_, bloom_id(op), bloom_size(op), (const int32_t *)k.dev, in.n));
+    return in;
+  }
+
+  // The rows whose key's filter entry is set; repack keeps the projections.
+  Rel bloom_filter(const Json &op, const Need &need) {
+    // a repack without projections keeps every input attribute, as
+    // RelBuilder::bloomfilter_repack(pred, filterSize, bloomId) (relbuilder.hpp:427-436)
+    const bool repack = op["operator"].as_string() == "bloomfilter-repack" && op["projections"].is_array();
+    Need below = need_below(op, need);
+    Rel in = exec(op["input"], below);
+    const uint32_t *words = nullptr;
+    int lg = 0;
+    bloom_of(op, &words, &lg);
+    Compiler c(*this, in);
+    if (c.infer(op["e"]) != R::I32) raise(PJ_E_UNSUPPORTED, "bloom filter key must be an int32 expression");
+    c.emit(op["e"], R::I32);
+    c.push(R::OP_BLOOM, R::BOOL, lg, (int64_t)(uintptr_t)words);
+    Col idx;
+    idx.t = R::I32;
+    idx.owner = alloc(std::max<uint64_t>(in.n, 1) * 4);
+    idx.dev = idx.owner->p;
+    uint64_t m = 0;
+    ck(R::select(ctx_, c.p, in.n, (uint32_t *)idx.owner->p, &m));
+    if (!repack) return gather_rel(in, idx, m, need);
+    Rel sel = gather_rel(in, idx, m, below);
+    Rel out;
+    out.n = m;
+    int i = 0;
+    for (auto &e : op["projections"].items()) {
+      Col col = eval(e, sel);
+      if (Compiler(*this, sel).is_col(e)) col.dict = Compiler(*this, sel).column(e).dict;
+      out.add(reg_attr(e, "repack", "expr" + std::to_string(i++)), col);
     }
     return out;
   }
@@ -1200,7 +1274,7 @@ class Exec {
         sp -= 2;
       } else if (op == R::OP_PROBE || op == R::OP_FILTER || op == R::OP_OUT) {
         --sp;
-      } else if (op != R::OP_NEG && op != R::OP_NOT && op != R::OP_CAST) {
+      } else if (op != R::OP_NEG && op != R::OP_NOT && op != R::OP_CAST && op != R::OP_BLOOM) {
         --sp;
       }
     }
@@ -1341,7 +1415,7 @@ class Exec {
         node = join_sides(*node).probe_in;
         continue;
       }
-      if (k == "select" || k == "project") {
+      if (k == "select" || k == "project" || k == "bloomfilter-probe" || k == "bloomfilter-repack") {
         stages.push_back(node);
         node = &(*node)["input"];
         continue;
@@ -1351,7 +1425,15 @@ class Exec {
     if (!joins || joins >= R::kPipeMaxLevels) return false;
     Need na = need;
     for (const Json *st : stages) na = need_below(*st, na);
-    // execute the source and every build side
+    // Every build side runs before the source, as the reference runs build
+    // pipelines before the probe pipeline: a bloom filter built on a build
+    // side can be probed below.
+    std::map<const Json *, Rel> built;
+    for (auto it = stages.rbegin(); it != stages.rend(); ++it) {
+      const std::string k = (**it)["operator"].as_string();
+      if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join")
+        built.emplace(*it, exec(*join_sides(**it).build_in, na));
+    }
     Rel src = exec(*node, na);
     std::vector<Rel> builds;
     std::vector<pj_dim *> dims;
@@ -1387,9 +1469,30 @@ class Exec {
         }
         scopes.push_back(ns);
         cur = (int)scopes.size() - 1;
+      } else if (k == "bloomfilter-probe" || k == "bloomfilter-repack") {
+        const uint32_t *words = nullptr;
+        int lg = 0;
+        bloom_of(st, &words, &lg);
+        if (pc.infer(st["e"], cur) != R::I32)
+          raise(PJ_E_UNSUPPORTED, "bloom filter key must be an int32 expression");
+        pc.emit(st["e"], cur, R::I32);
+        pc.push(R::OP_BLOOM, R::BOOL, lg, (int64_t)(uintptr_t)words);
+        pc.push(R::OP_FILTER, R::BOOL, 0, 0);
+        if (k == "bloomfilter-repack" && st["projections"].is_array()) {  // the projections, by their names
+          Scope ns;
+          int i = 0;
+          for (auto &e : st["projections"].items()) {
+            Binding b;
+            b.expr = &e;
+            b.scope = cur;
+            ns.attrs.emplace_back(reg_attr(e, "repack", "expr" + std::to_string(i++)), b);
+          }
+          scopes.push_back(ns);
+          cur = (int)scopes.size() - 1;
+        }
       } else {
         JoinSides js = join_sides(st);
-        Rel b = exec(*js.build_in, na);
+        Rel b = std::move(built.at(*it));
         Compiler kb(*this, b);
         if (kb.infer(*js.build_k) != R::I32 || pc.infer(*js.probe_k, cur) != R::I32) {
           ok = false;

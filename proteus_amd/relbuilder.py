@@ -4,7 +4,8 @@ runs them through pj_plan_execute.
 Mirrors the call shapes of the reference's C++ RelBuilder
 (core/olap/include/olap/operators/relbuilder.hpp:110-609): scan, to_gpu,
 to_cpu, unpack, pack, filter, project, join, groupby, reduce, sort, print,
-prepare -> PreparedStatement.execute() -> QueryResult.  Expressions are the
+bloomfilter_build/_probe/_repack, prepare -> PreparedStatement.execute() ->
+QueryResult.  Expressions are the
 plan language's JSON objects (core/olap/lib/plan/expression-parser.cpp), made
 with the helpers below.  `prepare` lowers to the C ABI instead of a JIT.
 """
@@ -154,6 +155,25 @@ class RelBuilder:
 
     def reduce(self, exprs, accs):
         return self._on("reduce", e=exprs, accumulator=accs, p=lit(True))
+
+    # bloom filters: RelBuilder::bloomfilter_build / _probe / _repack
+    # (relbuilder.hpp:427-462, relbuilder.cpp:957-975), as the operators
+    # bloomfilter-build / -probe / -repack.  pred is the int32 key expression.
+    def bloomfilter_build(self, pred, filter_size, bloom_id):
+        """Sets the key's entry of filter bloom_id; passes every row on."""
+        return self._on("bloomfilter-build", e=pred, filterSize=filter_size, bloomId=bloom_id)
+
+    def bloomfilter_probe(self, pred, filter_size, bloom_id):
+        """Keeps the rows whose key's entry is set."""
+        return self._on("bloomfilter-probe", e=pred, filterSize=filter_size, bloomId=bloom_id)
+
+    def bloomfilter_repack(self, pred, filter_size, bloom_id, attrs=None):
+        """Keeps the rows whose key's entry is set, repacked to attrs (every
+        input attribute when attrs is None)."""
+        fields = dict(e=pred, filterSize=filter_size, bloomId=bloom_id)
+        if attrs is not None:
+            fields["projections"] = attrs
+        return self._on("bloomfilter-repack", **fields)
 
     def sort(self, exprs, dirs):
         return self._on("sort", e=[{"expression": x, "direction": d} for x, d in zip(exprs, dirs)])

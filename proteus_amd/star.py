@@ -14,6 +14,13 @@ Query: SELECT s_nation, c_city, SUM(price * (100 - discount)) (int64)
        GROUP BY s_nation, c_city -> 5 x 160 = 800 groups.
 Fact columns are generated on the device from (seed, column); the
 dimensions are closed forms of the row id.
+
+With bloom_size > 0 the plan is the shape of the reference's ssb100_bloom
+queries (apps/benchmarks/olap/prepared-queries/ssb100_bloom/q4_1.sql.json.cpp):
+the filtered supplier and part dimensions build bloom filters on their keys
+(bloomfilter_build, ids 2 and 3) and the fact side is probed against both
+before the join chain (bloomfilter_probe).  The joins still decide every
+match, so the result is the same for any filter size.
 """
 import numpy as np
 
@@ -40,8 +47,9 @@ def dims():
     }
 
 
-def plan(sort=True):
-    """The query as a JSON plan (dict)."""
+def plan(sort=True, bloom_size=0):
+    """The query as a JSON plan (dict); bloom_size > 0 adds the bloom
+    filters (a power of two entries each)."""
     b = rb.RelBuilder()
     col = rb.col
 
@@ -57,6 +65,11 @@ def plan(sort=True):
                rb.eq(col("star/part", "p_category"), rb.lit(CATEGORY)))
     cust = dim("star/customer", ["c_custkey", "c_city"])
     f = b.scan(FACT, [c[0] for c in FACT_COLS]).to_gpu().unpack()
+    if bloom_size:
+        supp = supp.bloomfilter_build(col("star/supplier", "s_suppkey"), bloom_size, 2)
+        part = part.bloomfilter_build(col("star/part", "p_partkey"), bloom_size, 3)
+        f = f.bloomfilter_probe(col(FACT, "fk_part"), bloom_size, 3) \
+             .bloomfilter_probe(col(FACT, "fk_supp"), bloom_size, 2)
     j = f.join(date, col("star/date", "d_datekey"), col(FACT, "fk_date"), 13, N_DATE)
     j = j.join(supp, col("star/supplier", "s_suppkey"), col(FACT, "fk_supp"), 19, N_SUPP)
     j = j.join(part, col("star/part", "p_partkey"), col(FACT, "fk_part"), 23, N_PART)
@@ -85,8 +98,8 @@ class Star:
                 self.dim_dev.append(d)
                 ctx.register_column(rel, name, "int", d)
 
-    def plan(self, sort=True):
-        return plan(sort)
+    def plan(self, sort=True, bloom_size=0):
+        return plan(sort, bloom_size)
 
     def oracle(self):
         """numpy result (host copies of the fact columns)."""
