@@ -208,8 +208,9 @@ def star_c4(ctx, steps):
     """Star-schema plan (fact 6e8 rows x 7 int32 columns, four dimensions,
     filters 1/7, 1/5, 1/25, ~800 groups of SUM(price*(100-discount)))
     through pj_plan_execute, fused probe pipeline; the operator-at-a-time
-    execution of the same plan is run once as a cross-check.  Algorithmic
-    bytes: the 28 B of fact columns per row."""
+    execution of the same plan is run once as a cross-check, and the plan
+    with bloom filters on the supplier and part keys is timed beside it.
+    Algorithmic bytes: the 28 B of fact columns per row."""
     import json as _json
     from proteus_amd import star
     n = 600_000_000
@@ -229,12 +230,21 @@ def star_c4(ctx, steps):
         sec_mat = time.perf_counter() - t1
     finally:
         del os.environ["PROTEUS_AMD_NO_PIPELINE"]
+    # the ssb100_bloom shape: bloom filters (2^22 entries) built on the
+    # filtered supplier and part keys, the fact side probed before the joins
+    btext = _json.dumps(s.plan(bloom_size=1 << 22))
+    bref = ctx.execute_plan(btext)
+    ctx.sync()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        bgot = ctx.execute_plan(btext)
+    sec_bloom = (time.perf_counter() - t2) / steps
     rows = [ln for ln in got.splitlines() if ln]
     s.close()
     gbs = 28.0 * n / sec / 1e9
     return {"fact_rows": n, "ms": sec * 1e3, "rows_per_s": n / sec, "groups": len(rows),
             "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "operator_at_a_time_ms": sec_mat * 1e3,
-            "identical": got == ref == mat}
+            "bloom_ms": sec_bloom * 1e3, "identical": got == ref == mat == bref == bgot}
 
 
 def batched_mix(ctx, nq, qps):
