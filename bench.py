@@ -140,6 +140,7 @@ def main():
     if not a.no_extra:
         line["groupby"] = groupby_c3(ctx, a.steps)
         line["star"] = star_c4(ctx, a.steps)
+        line["sort"] = sort_plan(ctx, a.steps)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
             x.free()
@@ -202,6 +203,35 @@ def groupby_c3(ctx, steps):
     v.free()
     w.free()
     return out
+
+
+def sort_plan(ctx, steps):
+    """The plan operator sort on the device: 2^25 rows ordered by an int32
+    key ascending (2^20 distinct values) then an FP64 key descending, through
+    pj_plan_execute (scan -> sort -> reduce COUNT)."""
+    import json as _json
+    from proteus_amd import relbuilder as rb
+    n = 1 << 25
+    k = ctx.gen_uniform_i32(n, SEED, 20, 0, 1 << 20)
+    w = ctx.gen_unit_f32(n, SEED, 21)
+    wd = ctx.to_device(w.numpy().astype("float64"))
+    ctx.clear_catalog()
+    ctx.register_column("sortbench", "k", "int", k)
+    ctx.register_column("sortbench", "w", "float", wd)
+    r = rb.RelBuilder(ctx).scan("sortbench", ["k", "w"]).to_gpu().unpack()
+    q = r.sort([rb.col("sortbench", "k"), rb.col("sortbench", "w")], ["ASC", "DESC"]) \
+         .reduce([rb.int64(1)], ["sum"])
+    text = _json.dumps(q.plan())
+    assert int(ctx.execute_plan(text).strip()) == n
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.execute_plan(text)
+    sec = (time.perf_counter() - t0) / steps
+    ctx.clear_catalog()
+    for x in (k, w, wd):
+        x.free()
+    return {"rows": n, "keys": "int32 ASC, FP64 DESC", "ms": sec * 1e3, "rows_per_s": n / sec}
 
 
 def star_c4(ctx, steps):

@@ -112,6 +112,20 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
 // The table of a dimension built with pj_dim_create.
 int dim_table(const pj_dim *dim, PipeTable *t);
 
+// Stable sort permutation of n rows by nk keys (the first most
+// significant): dir 1 ascending, -1 descending, 0 ignored; rank (int32 keys
+// only, may be null) maps a dictionary code in [0, nrank) to its position in
+// string order (codes outside sort after all others).  perm (n entries)
+// receives the row ids in sorted order.
+struct SortKey {
+  const void *col;
+  int32_t t;
+  int32_t dir;
+  const int32_t *rank;
+  uint32_t nrank;
+};
+int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm);
+
 // The words and log2 size of the context's bloom filter `id` (built with
 // pj_bloom_build_i32), for OP_BLOOM.
 int bloom_lookup(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_size);

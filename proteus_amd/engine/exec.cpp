@@ -1649,29 +1649,36 @@ class Exec {
       const std::string d = x["direction"].is_string() ? x["direction"].as_string() : "ASC";
       dir.push_back(d == "DESC" ? -1 : d == "NONE" ? 0 : 1);
     }
-    std::vector<std::vector<double>> fv(cols.size());
-    std::vector<std::vector<int64_t>> iv(cols.size());
-    for (size_t k = 0; k < cols.size(); ++k) load_host(cols[k], in.n, &iv[k], &fv[k]);
-    std::vector<uint64_t> perm(in.n);
-    for (uint64_t i = 0; i < in.n; ++i) perm[i] = i;
-    std::stable_sort(perm.begin(), perm.end(), [&](uint64_t a, uint64_t b) {
-      for (size_t k = 0; k < cols.size(); ++k) {
-        if (!dir[k]) continue;
-        int c = 0;
-        if (cols[k].dict) {
-          const std::string &x = (*cols[k].dict)[iv[k][a]], &y = (*cols[k].dict)[iv[k][b]];
-          c = x < y ? -1 : x > y ? 1 : 0;
-        } else if (cols[k].t == R::F64 || cols[k].t == R::F32) {
-          c = fv[k][a] < fv[k][b] ? -1 : fv[k][a] > fv[k][b] ? 1 : 0;
-        } else {
-          c = iv[k][a] < iv[k][b] ? -1 : iv[k][a] > iv[k][b] ? 1 : 0;
+    // Stable device sort of a row permutation (sort.hip).  Dictionary
+    // columns sort by their strings: each code maps to its rank in string
+    // order (equal strings, equal ranks).
+    std::vector<Col> ranks(cols.size());
+    std::vector<R::SortKey> keys;
+    for (size_t k = 0; k < cols.size(); ++k) {
+      R::SortKey sk{cols[k].dev, cols[k].t, dir[k], nullptr, 0};
+      if (cols[k].dict && dir[k]) {
+        const Dict &d = *cols[k].dict;
+        std::vector<uint32_t> order(d.size());
+        for (size_t c = 0; c < d.size(); ++c) order[c] = (uint32_t)c;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return d[a] < d[b]; });
+        std::vector<int32_t> rank(std::max<size_t>(d.size(), 1), 0);
+        int32_t r = 0;
+        for (size_t c = 0; c < order.size(); ++c) {
+          if (c && d[order[c]] != d[order[c - 1]]) ++r;
+          rank[order[c]] = r;
         }
-        if (c) return dir[k] * c < 0;
+        ranks[k] = upload(R::I32, rank.data(), rank.size());
+        sk.rank = (const int32_t *)ranks[k].dev;
+        sk.nrank = (uint32_t)d.size();
       }
-      return false;
-    });
-    std::vector<uint32_t> p32(perm.begin(), perm.end());
-    Col idx = upload(R::I32, p32.data(), in.n);
+      keys.push_back(sk);
+    }
+    Col idx;
+    idx.t = R::I32;
+    idx.owner = alloc(std::max<uint64_t>(in.n, 1) * 4);
+    idx.dev = idx.owner->p;
+    ck(R::sort_perm(ctx_, keys.data(), (int)keys.size(), in.n, (uint32_t *)idx.owner->p));
     Rel out;
     out.n = in.n;
     const Json &rt = op["rowType"];

@@ -244,3 +244,45 @@ def test_plan_errors(ctx):
         ctx.execute_plan(json.dumps({"operator": "scan", "plugin": {"name": "nope", "type": "block",
                                                                      "projections": []}}), ".")
     assert e.value.code == pa.PJ_E_PARSE
+
+
+# ---- GPU: the device sort --------------------------------------------------
+@pytest.mark.gpu
+def test_device_sort_matches_stable_lexsort(ctx):
+    """sort over int32 (ties, negatives), FP64 (-0.0 and +0.0 equal,
+    negatives) and int64 (NONE: carried, ties keep input order)."""
+    rs = np.random.default_rng(12)
+    n = 50000
+    a = rs.integers(-50, 50, n).astype(np.int32)
+    c = rs.choice(np.array([-2.5, -0.0, 0.0, 1.0, 3.25, -1e300, 7e-300]), n)
+    b = np.arange(n, dtype=np.int64) * 7 - 3
+    ctx.clear_catalog()
+    ctx.register_column("T", "a", "int", a)
+    ctx.register_column("T", "c", "float", c)
+    ctx.register_column("T", "b", "int64", b)
+    r = rb.RelBuilder(ctx).scan("T", ["a", "c", "b"]).to_gpu().unpack()
+    q = r.sort([rb.col("T", "a"), rb.col("T", "c"), rb.col("T", "b")], ["ASC", "DESC", "NONE"])
+    rows = q.print([rb.col("T", "a"), rb.col("T", "c"), rb.col("T", "b")]).prepare().execute().rows()
+    order = np.lexsort((-c, a))  # stable: last key primary
+    assert [int(x[0]) for x in rows] == a[order].tolist()
+    assert [float(x[1]) for x in rows] == [float(v) for v in c[order]]
+    assert [int(x[2]) for x in rows] == b[order].tolist()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_device_sort_dictionary_strings(ctx, tmp_path):
+    """string columns sort by their text, not their dictionary codes."""
+    names = ["pear", "apple", "fig", "apple", "kiwi", "fig", "banana", "pear", "apple"]
+    score = [3, 1, 4, 1, 5, 9, 2, 6, 5]
+    (tmp_path / "fruit.csv").write_text("".join("%s;%d\n" % x for x in zip(names, score)))
+    ctx.clear_catalog()
+    b = rb.RelBuilder(ctx, str(tmp_path))
+    r = b.scan_csv("fruit.csv", [("name", "string", 1), ("score", "int", 2)])
+    q = r.sort([rb.col("fruit.csv", "name"), rb.col("fruit.csv", "score")], ["DESC", "ASC"]) \
+         .print([rb.col("fruit.csv", "name"), rb.col("fruit.csv", "score")])
+    rows = [(x[0], int(x[1])) for x in q.prepare().execute().rows()]
+    want = sorted(zip(names, score), key=lambda t: t[1])
+    want = sorted(want, key=lambda t: t[0], reverse=True)
+    assert rows == want
+    ctx.clear_catalog()
