@@ -151,6 +151,22 @@ int pj_join_partitioned_reduce(pj_ctx *ctx, const int32_t *d_bk,
                                uint64_t max_probe, const pj_radix_cfg *cfg,
                                pj_join_agg *out);
 
+/* out-of-gpu-join -> reduce (plan-parser.cpp:531-850 with ZipCollect /
+ * packet-zip.cpp and the CPU HashRearrange into numOfBuckets buckets): the
+ * inputs h_* are in host memory and may exceed HBM.  Both sides are
+ * bucketed by hashInt32(key) URem num_buckets on the device in chunks that
+ * fit hbm_budget bytes (0: half of HBM), through a pinned host arena, and
+ * each bucket pair is joined by the radix-partitioned join; the aggregates
+ * are those of pj_join_partitioned_reduce over the whole inputs.
+ * num_buckets = 0 picks the fewest buckets that fit the budget; a bucket
+ * that still does not fit (skewed keys) returns PJ_E_CAPACITY.
+ * h_bv / h_pv may be NULL (value 0). */
+int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk,
+                              const int32_t *h_bv, uint64_t nb,
+                              const int32_t *h_pk, const int32_t *h_pv,
+                              uint64_t np, uint32_t num_buckets,
+                              uint64_t hbm_budget, pj_join_agg *out);
+
 /* The configuration pj_join_partitioned_reduce picks for (nb, np) when cfg
  * is NULL (the autotuner's choice; runs a timed sweep the first time a size
  * class is seen if tuning is enabled with pj_autotune_enable). */

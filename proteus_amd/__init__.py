@@ -111,6 +111,8 @@ _SIGS = {
                               i32),
     "pj_join_partitioned_reduce": ([vp, vp, vp, u64, vp, vp, u64, u64, u64, C.POINTER(RadixCfg),
                                     C.POINTER(JoinAgg)], i32),
+    "pj_join_out_of_gpu_reduce": ([vp, vp, vp, u64, vp, vp, u64, C.c_uint32, u64,
+                                   C.POINTER(JoinAgg)], i32),
     "pj_join_partitioned_config": ([vp, u64, u64, C.POINTER(RadixCfg)], i32),
     "pj_autotune_enable": ([vp, i32], i32),
     "pj_groupby_i32": ([vp, vp, vp, vp, u64, i32, u64, vp, vp, vp, vp, vp, vp, C.POINTER(u64)],
@@ -327,6 +329,17 @@ class Context:
             self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk), _ptr(pv), np_,
             nb if max_build is None else max_build, np_ if max_probe is None else max_probe,
             C.byref(c) if c is not None else None, C.byref(out)))
+        return out.as_dict()
+
+    def join_out_of_gpu_reduce(self, bk, bv, pk, pv, num_buckets=0, hbm_budget=0):
+        """out-of-gpu-join -> reduce over host (numpy int32) columns."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+                for a in (bk, bv, pk, pv)]
+        hp = [None if a is None else a.ctypes.data for a in arrs]
+        out = JoinAgg()
+        _check(_lib.pj_join_out_of_gpu_reduce(self.h, hp[0], hp[1], len(arrs[0]), hp[2], hp[3],
+                                              len(arrs[2]), num_buckets, hbm_budget,
+                                              C.byref(out)))
         return out.as_dict()
 
     def join_partitioned_config(self, nb, np_):

@@ -1,0 +1,165 @@
+// Out-of-GPU co-partitioned join -> reduce: the reference's "out-of-gpu-join"
+// (core/olap/lib/plan/plan-parser.cpp:531-850, core/olap/lib/operators/
+// packet-zip.cpp): both inputs live in host memory and may exceed the HBM
+// budget.  The reference hash-rearranges them on CPU threads into
+// numOfBuckets buckets (bucket = hashInt32(key) URem buckets, HashRearrange)
+// and ZipCollect ships each bucket pair to a GpuPartitionedHashJoinChained.
+//
+// Here the GPU does the bucketing too, streaming: each side goes to HBM in
+// chunks that fit the budget, is rearranged by bucket on the device
+// (pj_hash_rearrange_i32, the same bucket function), and comes back to a
+// pinned host arena, where bucket b of the side is a list of pieces (one per
+// chunk).  Then each bucket's pieces of both sides are copied to HBM and
+// joined with the radix-partitioned join; the per-bucket aggregates combine
+// exactly (int64 counts and sums add, minima and maxima combine), since
+// equal keys fall in the same bucket.  PCIe carries 24 B per tuple (8 in,
+// 8 back, 8 in again).
+#include <algorithm>
+#include <vector>
+
+#include "pj_internal.hpp"
+
+namespace pj {
+
+namespace {
+
+struct Piece {
+  uint64_t off, len;  // in the side's host arena
+};
+
+struct DevBuf {
+  pj_ctx *ctx;
+  void *p = nullptr;
+  explicit DevBuf(pj_ctx *c) : ctx(c) {}
+  int alloc(uint64_t bytes) { return pj_malloc(ctx, bytes ? bytes : 4, &p); }
+  ~DevBuf() {
+    if (p) (void)pj_free(ctx, p);
+  }
+};
+
+struct HostArena {
+  void *p = nullptr;
+  int alloc(uint64_t bytes) {
+    PJ_HIP(hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault));
+    return PJ_OK;
+  }
+  ~HostArena() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// One side: host (k, v) -> arena keys / values grouped by bucket per chunk.
+int bucket_side(pj_ctx *ctx, const int32_t *hk, const int32_t *hv, uint64_t n, uint32_t buckets,
+                uint64_t chunk, int32_t *ak, int32_t *av, std::vector<std::vector<Piece>> &pieces) {
+  pieces.assign(buckets, {});
+  if (!n) return PJ_OK;
+  const uint64_t c = std::min(chunk, n);
+  DevBuf ik(ctx), iv(ctx), ok(ctx), ov(ctx);
+  PJ_TRY(ik.alloc(c * 4));
+  PJ_TRY(iv.alloc(c * 4));
+  PJ_TRY(ok.alloc(c * 4));
+  PJ_TRY(ov.alloc(c * 4));
+  std::vector<uint64_t> off(buckets + 1);
+  for (uint64_t base = 0; base < n; base += c) {
+    const uint64_t len = std::min(c, n - base);
+    PJ_TRY(pj_memcpy_h2d(ctx, ik.p, hk + base, len * 4));
+    if (hv) PJ_TRY(pj_memcpy_h2d(ctx, iv.p, hv + base, len * 4));
+    PJ_TRY(pj_hash_rearrange_i32(ctx, (const int32_t *)ik.p, hv ? (const int32_t *)iv.p : nullptr,
+                                 len, buckets, (int32_t *)ok.p, hv ? (int32_t *)ov.p : nullptr,
+                                 off.data()));
+    PJ_TRY(pj_memcpy_d2h(ctx, ak + base, ok.p, len * 4));
+    if (hv) PJ_TRY(pj_memcpy_d2h(ctx, av + base, ov.p, len * 4));
+    for (uint32_t b = 0; b < buckets; ++b)
+      if (off[b + 1] > off[b]) pieces[b].push_back(Piece{base + off[b], off[b + 1] - off[b]});
+  }
+  return PJ_OK;
+}
+
+uint64_t total(const std::vector<Piece> &ps) {
+  uint64_t t = 0;
+  for (const Piece &p : ps) t += p.len;
+  return t;
+}
+
+// The pieces of one bucket, concatenated into device memory.
+int gather_pieces(pj_ctx *ctx, const int32_t *arena, const std::vector<Piece> &ps, int32_t *dst) {
+  uint64_t at = 0;
+  for (const Piece &p : ps) {
+    PJ_TRY(pj_memcpy_h2d(ctx, dst + at, arena + p.off, p.len * 4));
+    at += p.len;
+  }
+  return PJ_OK;
+}
+
+}  // namespace
+}  // namespace pj
+
+using namespace pj;
+
+extern "C" int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk, const int32_t *h_bv,
+                                         uint64_t nb, const int32_t *h_pk, const int32_t *h_pv,
+                                         uint64_t np, uint32_t num_buckets, uint64_t hbm_budget,
+                                         pj_join_agg *out) {
+  if (!ctx || !out) return fail(PJ_E_ARG, "null argument");
+  *out = pj_join_agg{0, 0, 0, INT64_MAX, INT64_MIN, INT64_MAX, INT64_MIN};
+  if ((nb && !h_bk) || (np && !h_pk)) return fail(PJ_E_ARG, "out-of-GPU join without keys");
+  if (!nb || !np) return PJ_OK;
+  if (!hbm_budget) hbm_budget = ctx->hbm_bytes / 2;
+  // a bucket pair in HBM takes its 8 B per tuple plus the partitioned join's
+  // scratch, about 3x in all; buckets sized for half that, for skew
+  constexpr uint64_t kJoinBytes = 24;
+  uint32_t buckets = num_buckets;
+  if (!buckets) {
+    const uint64_t need = (nb + np) * kJoinBytes * 2;
+    buckets = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, ceil_div(need, hbm_budget)), 1u << 16);
+  }
+  if (buckets > (1u << 16)) return fail(PJ_E_ARG, "out-of-GPU join with %u buckets (limit 65536)", buckets);
+  // chunks of the bucketing phase: 16 B per tuple in HBM (input + output)
+  const uint64_t chunk = std::max<uint64_t>(1 << 16, hbm_budget / 32);
+  HostArena bk, bv, pk, pv;
+  PJ_TRY(bk.alloc(nb * 4));
+  PJ_TRY(pk.alloc(np * 4));
+  if (h_bv) PJ_TRY(bv.alloc(nb * 4));
+  if (h_pv) PJ_TRY(pv.alloc(np * 4));
+  std::vector<std::vector<Piece>> bpieces, ppieces;
+  PJ_TRY(bucket_side(ctx, h_bk, h_bv, nb, buckets, chunk, (int32_t *)bk.p, (int32_t *)bv.p, bpieces));
+  PJ_TRY(bucket_side(ctx, h_pk, h_pv, np, buckets, chunk, (int32_t *)pk.p, (int32_t *)pv.p, ppieces));
+  uint64_t max_nb = 0, max_np = 0;
+  for (uint32_t b = 0; b < buckets; ++b) {
+    const uint64_t rb = total(bpieces[b]), sb = total(ppieces[b]);
+    if ((rb + sb) * kJoinBytes > hbm_budget)
+      return fail(PJ_E_CAPACITY,
+                  "out-of-GPU join: bucket %u holds %llu + %llu tuples, beyond the HBM budget "
+                  "of %llu bytes (use more buckets)",
+                  b, (unsigned long long)rb, (unsigned long long)sb, (unsigned long long)hbm_budget);
+    max_nb = std::max(max_nb, rb);
+    max_np = std::max(max_np, sb);
+  }
+  DevBuf drk(ctx), drv(ctx), dsk(ctx), dsv(ctx);
+  PJ_TRY(drk.alloc(max_nb * 4));
+  PJ_TRY(dsk.alloc(max_np * 4));
+  if (h_bv) PJ_TRY(drv.alloc(max_nb * 4));
+  if (h_pv) PJ_TRY(dsv.alloc(max_np * 4));
+  for (uint32_t b = 0; b < buckets; ++b) {
+    const uint64_t rb = total(bpieces[b]), sb = total(ppieces[b]);
+    if (!rb || !sb) continue;
+    PJ_TRY(gather_pieces(ctx, (const int32_t *)bk.p, bpieces[b], (int32_t *)drk.p));
+    PJ_TRY(gather_pieces(ctx, (const int32_t *)pk.p, ppieces[b], (int32_t *)dsk.p));
+    if (h_bv) PJ_TRY(gather_pieces(ctx, (const int32_t *)bv.p, bpieces[b], (int32_t *)drv.p));
+    if (h_pv) PJ_TRY(gather_pieces(ctx, (const int32_t *)pv.p, ppieces[b], (int32_t *)dsv.p));
+    pj_join_agg a;
+    PJ_TRY(pj_join_partitioned_reduce(ctx, (const int32_t *)drk.p,
+                                      h_bv ? (const int32_t *)drv.p : nullptr, rb,
+                                      (const int32_t *)dsk.p,
+                                      h_pv ? (const int32_t *)dsv.p : nullptr, sb, rb, sb,
+                                      nullptr, &a));
+    out->count += a.count;
+    out->sum_b += a.sum_b;
+    out->sum_p += a.sum_p;
+    out->min_b = std::min(out->min_b, a.min_b);
+    out->max_b = std::max(out->max_b, a.max_b);
+    out->min_p = std::min(out->min_p, a.min_p);
+    out->max_p = std::max(out->max_p, a.max_p);
+  }
+  return PJ_OK;
+}

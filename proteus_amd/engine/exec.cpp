@@ -109,6 +109,15 @@ static bool is_pass_through(const std::string &op) {
   return false;
 }
 
+// Join operators.  out-of-gpu-join (plan-parser.cpp:531-850) bucket-joins
+// inputs that live in host memory; here every column of a plan is already in
+// HBM, so it runs as the radix-partitioned join (pj_join_out_of_gpu_reduce
+// is the call for host-resident inputs).
+static bool is_join_op(const std::string &k) {
+  return k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join" ||
+         k == "out-of-gpu-join";
+}
+
 static const char *kInputs[] = {"input", "build_input", "probe_input", "leftInput", "rightInput"};
 
 // Every attrName mentioned in j outside operator inputs.
@@ -496,7 +505,7 @@ class Exec {
     }
     if (k == "reduce") return reduce(op, need);
     if (k == "groupby" || k == "hashgroupby-chained") return groupby(op, need);
-    if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join") {
+    if (is_join_op(k)) {
       Rel r;
       if (!no_pipeline_ && pipeline(op, need, &r)) return r;
       return join(op, need);
@@ -882,7 +891,7 @@ class Exec {
     while (below->is_object() && is_pass_through((*below)["operator"].as_string()))
       below = &(*below)["input"];
     const std::string bk = (*below)["operator"].is_string() ? (*below)["operator"].as_string() : "";
-    if (bk == "hashjoin-chained" || bk == "partitioned-hashjoin-chained" || bk == "join") {
+    if (is_join_op(bk)) {
       Rel r;
       if (fused_join_reduce(op, *below, need, &r)) return r;
     }
@@ -918,6 +927,13 @@ class Exec {
       const Json &p = j["p"];
       if (p["expression"].as_string() != "eq") raise(PJ_E_UNSUPPORTED, "join predicate is not an equality");
       return JoinSides{&j["leftInput"], &j["rightInput"], &p["left"], &p["right"]};
+    }
+    if (k == "out-of-gpu-join") {  // the key is the first of build_e / probe_e
+      const Json &be = j["build_e"], &pe = j["probe_e"];
+      if (!be.is_array() || !pe.is_array() || !be.size() || !pe.size())
+        raise(PJ_E_PARSE, "out-of-gpu-join without build_e / probe_e");
+      return JoinSides{&j["build_input"], &j["probe_input"], &be[(size_t)0]["original"],
+                       &pe[(size_t)0]["original"]};
     }
     return JoinSides{&j["build_input"], &j["probe_input"], &j["build_k"], &j["probe_k"]};
   }
@@ -1409,7 +1425,7 @@ class Exec {
         node = &(*node)["input"];
         continue;
       }
-      if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join") {
+      if (is_join_op(k)) {
         stages.push_back(node);
         ++joins;
         node = join_sides(*node).probe_in;
@@ -1431,8 +1447,7 @@ class Exec {
     std::map<const Json *, Rel> built;
     for (auto it = stages.rbegin(); it != stages.rend(); ++it) {
       const std::string k = (**it)["operator"].as_string();
-      if (k == "hashjoin-chained" || k == "partitioned-hashjoin-chained" || k == "join")
-        built.emplace(*it, exec(*join_sides(**it).build_in, na));
+      if (is_join_op(k)) built.emplace(*it, exec(*join_sides(**it).build_in, na));
     }
     Rel src = exec(*node, na);
     std::vector<Rel> builds;

@@ -147,6 +147,16 @@ class RelBuilder:
              "maxBuildInputSize": max_build, "maxProbeInputSize": max_probe}
         return self._wrap(n)
 
+    def out_of_gpu_join(self, build, build_e, probe_e, num_buckets, max_build, max_probe):
+        """out-of-gpu-join (plan-parser.cpp:531-850); self is the probe side.
+        build_e / probe_e: the attributes carried, the first one the join key."""
+        n = {"operator": "out-of-gpu-join", "build_input": build.node, "probe_input": self.node,
+             "numOfBuckets": num_buckets, "numPartitioners": 1, "numConcurrent": 1,
+             "maxBuildInputSize": max_build, "maxProbeInputSize": max_probe, "slack": 8,
+             "build_e": [{"original": e} for e in build_e],
+             "probe_e": [{"original": e} for e in probe_e]}
+        return self._wrap(n)
+
     def groupby(self, k, e, hash_bits, max_input_size):
         """k: [expr]; e: [(expr, monoid)]"""
         return self._on("groupby", k=k, e=[{"e": x, "m": m, "packet": i + 1, "offset": 0}
