@@ -656,6 +656,8 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       }
       if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, b1, b2);
       if (rc == PJ_OK) rc = side_pass1(ctx, S, d_pk, d_pv, np, b1, b2, "probe");
+      // (Making the probe side's pass 2 wait for the build side, so that it
+      // runs alone, measured 0.07 ms slower per join.)
       if (rc == PJ_OK) rc = side_pass2(ctx, S, np, b1, b2, "probe");
       // join even on failure, so no scratch is released under running work
       const int rj = aux_join(ctx);
