@@ -50,11 +50,8 @@ __global__ __launch_bounds__(256) void k_bloom_set(uint32_t *__restrict__ words,
   }
 }
 
-namespace rel {
-
-int bloom_lookup(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_size) {
-  if (!ctx || !words || !log2_size) return fail(PJ_E_ARG, "null argument");
-  std::lock_guard<std::mutex> g(ctx->mu);
+// Filter `id` of the context; the caller holds ctx->mu.
+static int lookup_locked(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_size) {
   BloomMap &m = blooms(ctx);
   auto it = m.find(id);
   if (it == m.end())
@@ -62,6 +59,14 @@ int bloom_lookup(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_siz
   *words = (const uint32_t *)it->second.words;
   *log2_size = it->second.log2;
   return PJ_OK;
+}
+
+namespace rel {
+
+int bloom_lookup(pj_ctx *ctx, uint64_t id, const uint32_t **words, int *log2_size) {
+  if (!ctx || !words || !log2_size) return fail(PJ_E_ARG, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  return lookup_locked(ctx, id, words, log2_size);
 }
 
 }  // namespace rel
@@ -106,14 +111,19 @@ extern "C" int pj_bloom_build_i32(pj_ctx *ctx, uint64_t bloom_id, uint64_t filte
   f.log2 = log2;
   const uint64_t nw = bloom_words(log2);
   PJ_TRY(dalloc(ctx, nw * 4, &f.words));
-  int rc = fill_u32(ctx, (uint32_t *)f.words, 0, nw);
-  if (rc == PJ_OK && n) {
-    uint64_t grid = ceil_div(n, (uint64_t)256);
-    const uint64_t cap = (uint64_t)ctx->num_cus * 16;
-    if (grid > cap) grid = cap;
-    PJ_LAUNCH(ctx, "bloom_set", k_bloom_set, grid, 256, 0, (uint32_t *)f.words,
-              bloom_mask(log2), d_keys, n);
-  }
+  // a launch that fails returns from the lambda, and the block is freed below
+  auto set_bits = [&]() -> int {
+    PJ_TRY(fill_u32(ctx, (uint32_t *)f.words, 0, nw));
+    if (n) {
+      uint64_t grid = ceil_div(n, (uint64_t)256);
+      const uint64_t cap = (uint64_t)ctx->num_cus * 16;
+      if (grid > cap) grid = cap;
+      PJ_LAUNCH(ctx, "bloom_set", k_bloom_set, grid, 256, 0, (uint32_t *)f.words,
+                bloom_mask(log2), d_keys, n);
+    }
+    return PJ_OK;
+  };
+  const int rc = set_bits();
   if (rc != PJ_OK) {
     (void)dfree(ctx, f.words);
     return rc;
@@ -134,11 +144,11 @@ extern "C" int pj_bloom_probe_i32(pj_ctx *ctx, uint64_t bloom_id, const int32_t 
   if (!ctx || !n_out) return fail(PJ_E_ARG, "null argument");
   *n_out = 0;
   if (n && (!d_keys || !d_idx)) return fail(PJ_E_ARG, "bloom filter probe without keys or output");
+  std::lock_guard<std::mutex> g(ctx->mu);
   const uint32_t *words = nullptr;
   int log2 = 0;
-  PJ_TRY(rel::bloom_lookup(ctx, bloom_id, &words, &log2));
+  PJ_TRY(lookup_locked(ctx, bloom_id, &words, &log2));
   if (!n) return PJ_OK;
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   PJ_TRY(rel::select(ctx, bloom_prog(d_keys, words, log2), n, d_idx, n_out));
   return sync(ctx);
@@ -155,11 +165,11 @@ extern "C" int pj_bloom_repack(pj_ctx *ctx, uint64_t bloom_id, const int32_t *d_
     if (esize[c] != 1 && esize[c] != 4 && esize[c] != 8)
       return fail(PJ_E_ARG, "bloom filter repack of %d-byte elements", esize[c]);
   if (n && !d_keys) return fail(PJ_E_ARG, "bloom filter repack without keys");
+  std::lock_guard<std::mutex> g(ctx->mu);
   const uint32_t *words = nullptr;
   int log2 = 0;
-  PJ_TRY(rel::bloom_lookup(ctx, bloom_id, &words, &log2));
+  PJ_TRY(lookup_locked(ctx, bloom_id, &words, &log2));
   if (!n) return PJ_OK;
-  std::lock_guard<std::mutex> g(ctx->mu);
   (void)hipSetDevice(ctx->device);
   Scratch idx;
   PJ_TRY(idx.alloc(ctx, n * 4));
