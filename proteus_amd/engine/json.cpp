@@ -1,6 +1,7 @@
 // JSON parsing and serialisation for plans.
 #include "json.hpp"
 
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -249,7 +250,9 @@ class JsonParser {
         return true;
       }
     }
-    *out = Json::real(strtod(t.c_str(), nullptr));
+    const double d = strtod(t.c_str(), nullptr);
+    if (!std::isfinite(d)) return bad("number out of range");  // no JSON spelling for inf
+    *out = Json::real(d);
     return true;
   }
   bool value(Json *out, int depth) {
