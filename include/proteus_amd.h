@@ -283,6 +283,9 @@ typedef struct {
   uint64_t heavy_rows;     /* with >= 2 slots, slot 0 serves only queries of */
                            /* fewer probe rows (0: 2^22)                     */
   pj_batch_cfg batch;
+  uint64_t max_batch_rows; /* probe rows per batch: a batch takes no further  */
+                           /* query that would pass it (0: 2^26); its first   */
+                           /* query always fits                              */
 } pj_serve_cfg;
 typedef struct {
   uint64_t batches;

@@ -71,7 +71,8 @@ class BatchCfg(C.Structure):
 
 class ServeCfg(C.Structure):
     _fields_ = [("mode", C.c_int), ("slots", C.c_int), ("max_batch", C.c_uint32),
-                ("hbm_budget", C.c_uint64), ("heavy_rows", C.c_uint64), ("batch", BatchCfg)]
+                ("hbm_budget", C.c_uint64), ("heavy_rows", C.c_uint64), ("batch", BatchCfg),
+                ("max_batch_rows", C.c_uint64)]
 
 
 class ServeStats(C.Structure):
@@ -470,13 +471,14 @@ class Context:
         return best.as_dict()
 
     def serve_run(self, queries, arrivals, mode=0, slots=3, max_batch=4096, hbm_budget=0,
-                  tile_rows=0, wg_per_cu=0, heavy_rows=0, lds_dim=0):
+                  tile_rows=0, wg_per_cu=0, heavy_rows=0, lds_dim=0, max_batch_rows=0):
         """Runs the serving loop; returns (completion seconds per query, stats)."""
         n = len(queries)
         arr = (Query * n)(*queries)
         arrv = (C.c_double * n)(*arrivals)
         done = (C.c_double * n)()
-        cfg = ServeCfg(mode, slots, max_batch, hbm_budget, heavy_rows, BatchCfg(tile_rows, wg_per_cu, lds_dim))
+        cfg = ServeCfg(mode, slots, max_batch, hbm_budget, heavy_rows,
+                       BatchCfg(tile_rows, wg_per_cu, lds_dim), max_batch_rows)
         st = ServeStats()
         _check(_lib.pj_serve_run(self.h, arr, arrv, n, C.byref(cfg), done, C.byref(st)))
         return list(done), {"batches": st.batches, "max_in_flight_bytes": st.max_in_flight_bytes,

