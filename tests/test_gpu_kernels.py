@@ -235,6 +235,21 @@ def test_join_partitioned_c2_scaled(ctx):
     _free(rk, rv, sv)
 
 
+def test_join_partitioned_c2_full_size(ctx):
+    """The benchmark's join at its full size, 2^27 x 2^30 (uniform fk over
+    1.33 x 2^27), against the dense oracle on the host: bit-exact."""
+    nr, ns, seed = 1 << 27, 1 << 30, 20251203
+    rk = ctx.gen_perm_i32(nr, seed)
+    rv = ctx.gen_uniform_i32(nr, seed, 1, 0, 1 << 31)
+    sk = ctx.gen_uniform_i32(ns, seed, 2, 0, int(1.33 * nr))
+    sv = ctx.gen_uniform_i32(ns, seed, 3, 0, 1 << 31)
+    got = ctx.join_partitioned_reduce(rk, rv, sk, sv)
+    rc, want = O.join_dense(rk.numpy(), rv.numpy(), sk.numpy(), sv.numpy())
+    assert rc == O.OR_OK and got == want
+    assert 0.74 < got["count"] / ns < 0.76
+    _free(rk, rv, sk, sv)
+
+
 def test_join_partitioned_capacity_and_empty(ctx):
     k = ctx.to_device(np.arange(5000, dtype=np.int32))
     e = ctx.to_device(np.zeros(0, np.int32))
