@@ -13,7 +13,7 @@
 // joined with the radix-partitioned join; the per-bucket aggregates combine
 // exactly (int64 counts and sums add, minima and maxima combine), since
 // equal keys fall in the same bucket.  PCIe carries 24 B per tuple (8 in,
-// 8 back, 8 in again).
+// 8 back, 8 in again); a chunk's copy back overlaps the next chunk's upload.
 #include <algorithm>
 #include <vector>
 
@@ -49,29 +49,61 @@ struct HostArena {
 };
 
 // One side: host (k, v) -> arena keys / values grouped by bucket per chunk.
+// Each chunk's rearranged output goes back to the pinned arena on a copy
+// stream while the next chunk comes in (the link is full duplex); two output
+// buffer sets alternate, each reused only after its copy has finished.
+struct CopyStream {
+  hipStream_t s = nullptr;
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int init() {
+    PJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto &e : done) PJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return PJ_OK;
+  }
+  ~CopyStream() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (auto &e : done)
+      if (e) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 int bucket_side(pj_ctx *ctx, const int32_t *hk, const int32_t *hv, uint64_t n, uint32_t buckets,
                 uint64_t chunk, int32_t *ak, int32_t *av, std::vector<std::vector<Piece>> &pieces) {
   pieces.assign(buckets, {});
   if (!n) return PJ_OK;
   const uint64_t c = std::min(chunk, n);
-  DevBuf ik(ctx), iv(ctx), ok(ctx), ov(ctx);
+  DevBuf ik(ctx), iv(ctx), ok0(ctx), ov0(ctx), ok1(ctx), ov1(ctx);
   PJ_TRY(ik.alloc(c * 4));
-  PJ_TRY(iv.alloc(c * 4));
-  PJ_TRY(ok.alloc(c * 4));
-  PJ_TRY(ov.alloc(c * 4));
+  PJ_TRY(ok0.alloc(c * 4));
+  PJ_TRY(ok1.alloc(c * 4));
+  if (hv) {
+    PJ_TRY(iv.alloc(c * 4));
+    PJ_TRY(ov0.alloc(c * 4));
+    PJ_TRY(ov1.alloc(c * 4));
+  }
+  DevBuf *ok[2] = {&ok0, &ok1}, *ov[2] = {&ov0, &ov1};
+  CopyStream cs;
+  PJ_TRY(cs.init());
   std::vector<uint64_t> off(buckets + 1);
-  for (uint64_t base = 0; base < n; base += c) {
+  uint32_t idx = 0;
+  for (uint64_t base = 0; base < n; base += c, ++idx) {
+    const int b = idx & 1;
     const uint64_t len = std::min(c, n - base);
+    if (idx >= 2) PJ_HIP(hipEventSynchronize(cs.done[b]));  // buffer set b is free
     PJ_TRY(pj_memcpy_h2d(ctx, ik.p, hk + base, len * 4));
     if (hv) PJ_TRY(pj_memcpy_h2d(ctx, iv.p, hv + base, len * 4));
     PJ_TRY(pj_hash_rearrange_i32(ctx, (const int32_t *)ik.p, hv ? (const int32_t *)iv.p : nullptr,
-                                 len, buckets, (int32_t *)ok.p, hv ? (int32_t *)ov.p : nullptr,
-                                 off.data()));
-    PJ_TRY(pj_memcpy_d2h(ctx, ak + base, ok.p, len * 4));
-    if (hv) PJ_TRY(pj_memcpy_d2h(ctx, av + base, ov.p, len * 4));
-    for (uint32_t b = 0; b < buckets; ++b)
-      if (off[b + 1] > off[b]) pieces[b].push_back(Piece{base + off[b], off[b + 1] - off[b]});
+                                 len, buckets, (int32_t *)ok[b]->p,
+                                 hv ? (int32_t *)ov[b]->p : nullptr, off.data()));
+    // the rearrange has synchronised: its output is complete
+    PJ_HIP(hipMemcpyAsync(ak + base, ok[b]->p, len * 4, hipMemcpyDeviceToHost, cs.s));
+    if (hv) PJ_HIP(hipMemcpyAsync(av + base, ov[b]->p, len * 4, hipMemcpyDeviceToHost, cs.s));
+    PJ_HIP(hipEventRecord(cs.done[b], cs.s));
+    for (uint32_t q = 0; q < buckets; ++q)
+      if (off[q + 1] > off[q]) pieces[q].push_back(Piece{base + off[q], off[q + 1] - off[q]});
   }
+  PJ_HIP(hipStreamSynchronize(cs.s));
   return PJ_OK;
 }
 
@@ -114,8 +146,9 @@ extern "C" int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk, const
     buckets = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, ceil_div(need, hbm_budget)), 1u << 16);
   }
   if (buckets > (1u << 16)) return fail(PJ_E_ARG, "out-of-GPU join with %u buckets (limit 65536)", buckets);
-  // chunks of the bucketing phase: 16 B per tuple in HBM (input + output)
-  const uint64_t chunk = std::max<uint64_t>(1 << 16, hbm_budget / 32);
+  // chunks of the bucketing phase: 24 B per tuple in HBM (input and two
+  // output sets), several chunks per side so that copies overlap
+  const uint64_t chunk = std::max<uint64_t>(1 << 16, hbm_budget / 64);
   HostArena bk, bv, pk, pv;
   PJ_TRY(bk.alloc(nb * 4));
   PJ_TRY(pk.alloc(np * 4));
