@@ -656,13 +656,12 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       }
       if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, b1, b2);
       if (rc == PJ_OK) rc = side_pass1(ctx, S, d_pk, d_pv, np, b1, b2, "probe");
-      // The build side has finished when the probe side's pass 2 starts: it
-      // overlaps the histogram and pass 1, and pass 2 runs alone (with the
-      // build side's last pass beside it, pass 2 took 4.55 ms instead of
-      // 4.10 and the join 0.07 ms less).  Joined even on failure, so no
-      // scratch is released under running work.
+      // The build side's last pass ends beside the probe side's pass 2.
+      // (Making pass 2 wait for it moves the contention to pass 1, which
+      // then took 4.1-4.9 ms instead of 4.2, for a join 0.04-0.07 ms longer.)
+      if (rc == PJ_OK) rc = side_pass2(ctx, S, np, b1, b2, "probe");
+      // join even on failure, so no scratch is released under running work
       const int rj = aux_join(ctx);
-      if (rc == PJ_OK && rj == PJ_OK) rc = side_pass2(ctx, S, np, b1, b2, "probe");
       if (rc != PJ_OK) {
         (void)hipStreamSynchronize(aux);
         return rc;
