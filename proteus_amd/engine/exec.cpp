@@ -725,11 +725,16 @@ class Exec {
   Rel select(const Json &op, const Need &need) {
     Need below = need_below(op, need);
     Rel in = exec(op["input"], below);
+    return filter_rel(in, op["p"], need);
+  }
+
+  // The rows of `in` where `pred` holds, in order.
+  Rel filter_rel(const Rel &in, const Json &pred, const Need &need) {
     Compiler c(*this, in);
-    c.emit(op["p"], R::BOOL);
+    c.emit(pred, R::BOOL);
     Col idx;
     idx.t = R::I32;
-    idx.owner = alloc(in.n * 4);
+    idx.owner = alloc(std::max<uint64_t>(in.n, 1) * 4);
     idx.dev = idx.owner->p;
     uint64_t m = 0;
     ck(R::select(ctx_, c.p, in.n, (uint32_t *)idx.owner->p, &m));
@@ -881,8 +886,11 @@ class Exec {
   }
 
   Rel reduce(const Json &op, const Need &need) {
-    if (op["p"].is_object() && !(op["p"]["expression"].as_string() == "bool" && op["p"]["v"].as_bool()))
-      raise(PJ_E_UNSUPPORTED, "reduce with a non-true predicate");
+    // p: only the rows where it holds are accumulated (opt::Reduce's
+    // predicate, reduce-opt.cpp); a missing p or literal true takes them all
+    const Json &p = op["p"];
+    const bool filtered =
+        p.is_object() && !(p["expression"].as_string() == "bool" && p["v"].as_bool());
     const Json &es = op["e"], &accs = op["accumulator"];
     if (!es.is_array() || !accs.is_array() || es.size() != accs.size())
       raise(PJ_E_PARSE, "reduce needs as many accumulators as expressions");
@@ -891,11 +899,13 @@ class Exec {
     while (below->is_object() && is_pass_through((*below)["operator"].as_string()))
       below = &(*below)["input"];
     const std::string bk = (*below)["operator"].is_string() ? (*below)["operator"].as_string() : "";
-    if (is_join_op(bk)) {
+    if (!filtered && is_join_op(bk)) {
       Rel r;
       if (fused_join_reduce(op, *below, need, &r)) return r;
     }
-    Rel in = exec(op["input"], need_below(op, need));
+    const Need nb = need_below(op, need);
+    Rel in = exec(op["input"], nb);
+    if (filtered) in = filter_rel(in, p, nb);
     std::vector<std::pair<Attr, Scalar>> vals;
     for (size_t i = 0; i < es.size(); ++i) {
       const Json &e = es[i];

@@ -163,8 +163,9 @@ class RelBuilder:
                                            for i, (x, m) in enumerate(e)],
                         hash_bits=hash_bits, maxInputSize=max_input_size)
 
-    def reduce(self, exprs, accs):
-        return self._on("reduce", e=exprs, accumulator=accs, p=lit(True))
+    def reduce(self, exprs, accs, pred=None):
+        """Aggregates over the rows where pred holds (all rows by default)."""
+        return self._on("reduce", e=exprs, accumulator=accs, p=lit(True) if pred is None else pred)
 
     # bloom filters: RelBuilder::bloomfilter_build / _probe / _repack
     # (relbuilder.hpp:427-462, relbuilder.cpp:957-975), as the operators

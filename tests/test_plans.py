@@ -286,3 +286,28 @@ def test_device_sort_dictionary_strings(ctx, tmp_path):
     want = sorted(want, key=lambda t: t[0], reverse=True)
     assert rows == want
     ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_reduce_with_predicate(ctx):
+    """reduce's predicate p: only the rows where it holds are accumulated,
+    over a scan and over a join (which then is not fused)."""
+    nr, ns = 1 << 12, 1 << 16
+    rk, rv, sk, sv = _synthetic(ctx, nr, ns, 13)
+    b = rb.RelBuilder(ctx)
+    S = b.scan("S", ["k", "v"]).to_gpu().unpack()
+    q = S.reduce([rb.int64(1), rb.cast(rb.col("S", "v"), "int64"), rb.col("S", "k")],
+                 ["sum", "sum", "max"], pred=rb.lt(rb.col("S", "k"), rb.lit(1000)))
+    got = [int(x) for x in str(q.prepare().execute()).strip().split(",")]
+    m = sk < 1000
+    assert got == [int(m.sum()), int(sv[m].astype(np.int64).sum()), int(sk[m].max())]
+    R = b.scan("R", ["k", "v"]).to_gpu().unpack()
+    j = S.join(R, rb.col("R", "k"), rb.col("S", "k"), hash_bits=13, max_build=nr)
+    qj = j.reduce([rb.int64(1), rb.cast(rb.col("R", "v"), "int64")], ["sum", "sum"],
+                  pred=rb.gt(rb.col("S", "v"), rb.lit(1 << 30)))
+    got = [int(x) for x in str(qj.prepare().execute()).strip().split(",")]
+    pos = np.full(nr, -1, np.int64)
+    pos[rk] = np.arange(nr)
+    hit = (sk < nr) & (sv > (1 << 30))
+    assert got == [int(hit.sum()), int(rv[pos[sk[hit]]].astype(np.int64).sum())]
+    ctx.clear_catalog()
