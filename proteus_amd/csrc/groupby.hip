@@ -33,7 +33,8 @@ namespace pj {
 constexpr int kGbSlots = 2048;   // LDS table slots (96 KiB)
 constexpr int kGbCap = 1024;     // groups per table the planner aims for
 constexpr int kGbBlock = 1024;  // one workgroup per CU (96 KiB of LDS), 16 waves
-constexpr int kGbMergeFanIn = 64;
+constexpr int kGbMergeFanIn = 64;     // merge fan-in with many partitions
+constexpr int kGbMergeFanInFew = 8;   // with at most 64 partitions
 
 struct GbEntries {  // SoA entry arrays of one level
   int32_t *k;
@@ -411,8 +412,12 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     else PJ_TRY(launch(F_{}, F_{}));
   }
   // ---- merge levels: fold units of the same partition ------------------
-  // Units are ordered by partition.  Each level merges up to kGbMergeFanIn
-  // units of one partition into one, until every partition has one unit.
+  // Units are ordered by partition.  Each level merges up to `fan` units of
+  // one partition into one, until every partition has one unit.  With few
+  // partitions (many units each) a small fan-in keeps the first levels wide
+  // enough for every CU (2^10 groups: merges 0.32 -> 0.12 ms); with many
+  // partitions most need no merge, and a large fan-in saves levels.
+  const size_t fan = part_start.size() <= 64 ? (size_t)kGbMergeFanInFew : (size_t)kGbMergeFanIn;
   while (true) {
     std::vector<GbUnit> next;
     bool single = true;
@@ -420,10 +425,10 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
       size_t j = i;
       while (j < units.size() && units[j].part == units[i].part) ++j;
       if (j - i > 1) single = false;
-      for (size_t a = i; a < j; a += kGbMergeFanIn) {
+      for (size_t a = i; a < j; a += fan) {
         GbUnit m;
         m.begin = a;
-        m.len = (uint32_t)std::min<size_t>(kGbMergeFanIn, j - a);
+        m.len = (uint32_t)std::min<size_t>(fan, j - a);
         m.part = units[i].part;
         next.push_back(m);
       }
