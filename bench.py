@@ -59,8 +59,9 @@ def kernel_table(ctx, names):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(the sharding design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported (the join "
+              "would shard as a broadcast build side with 1/N of the probe side per GPU)"
+              % a.gpus, file=sys.stderr)
         sys.exit(2)
     import proteus_amd as pa
 

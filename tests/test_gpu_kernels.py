@@ -81,7 +81,7 @@ def test_join_chained_reduce_parity(ctx, case):
 
 
 def test_join_chained_c1_full_size(ctx):
-    """configs[0] shape: R = 2^20 permuted keys, S = 2^24 uniform over R's keys;
+    """The plain chained join at full size: R = 2^20 permuted keys, S = 2^24 uniform over R's keys;
     COUNT = 2^24 exactly and the int64 checksum equals the dense oracle's."""
     nr, ns, seed = 1 << 20, 1 << 24, 2024
     rk = ctx.gen_perm_i32(nr, seed)
@@ -214,7 +214,7 @@ def test_join_partitioned_zipf_and_skewed_build(ctx):
 
 
 def test_join_partitioned_c2_scaled(ctx):
-    """configs[1] at 1/8 scale: R = 2^24 permuted keys, S = 2^27 with ~25%
+    """The radix join at 1/8 of the benchmark size: R = 2^24 permuted keys, S = 2^27 with ~25%
     misses, uniform and Zipf(1.0); dense oracle, bit-exact."""
     nr, ns, seed = 1 << 24, 1 << 27, 99
     rng = int(1.33 * nr)
@@ -330,7 +330,7 @@ def test_groupby_optional_columns_and_capacity(ctx):
 
 
 def test_groupby_c3_exact_fsum(ctx):
-    """configs[2] shape at 1/16 scale: FP64 sums of u/2^24 floats are exact."""
+    """The group-by workload at 1/16 scale: FP64 sums of u/2^24 floats are exact."""
     n, G, seed = 1 << 25, 1 << 16, 8
     k = ctx.gen_zipf_i32(n, seed, 1, G, 0.8, seed)
     v = ctx.gen_uniform_i64(n, seed, 2, 0, 1 << 20)
