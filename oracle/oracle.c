@@ -2,7 +2,7 @@
  * oracle.c -- CPU restatement of Proteus's GPU join / group-by hot path.
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg as the checker.  Never linked by the product.
+ * bench.py's cpu_baseline leg, to check results.  Never linked by the product.
  *
  * Reference files restated (paths relative to the reference repository):
  *   core/olap/lib/expressions/expressions-hasher.cpp:39-84,560-580

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package never
+cpu_baseline leg of bench.py, to check results.  The product package never
 imports this module.  Each wrapped function restates the reference file:line
 cited in oracle.c.
 """

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker:
+"""Pins the CPU oracle (oracle/oracle.c) before results are checked against it:
 the reference's own golden outputs, closed forms, brute force on tiny inputs
 and a library routine.  CPU only."""
 import json
