@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of Proteus's join / group-by / reduce hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the
- * checker (or the timed CPU baseline).  The product library never links it.
+ * cpu_baseline leg of bench.py may load this library, and only to check
+ * results (or as the timed CPU baseline).  The product library never links it.
  *
  * Every function cites the reference file:line whose behaviour it restates.
  * Parity pins (see tests/test_oracle.py):
