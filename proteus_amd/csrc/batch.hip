@@ -28,8 +28,10 @@
 
 struct pj_dim {
   uint64_t n = 0;
-  uint64_t slots = 0;  // power of two
+  uint64_t slots = 0;  // table entries: a power of two (hash) or `range` (direct)
   unsigned long long *table = nullptr;
+  int32_t lo = 0;      // direct tables: smallest key
+  uint32_t range = 0;  // direct tables: max key - lo + 1; 0 = hash table
 };
 
 namespace pj {
@@ -43,7 +45,10 @@ constexpr uint32_t kMaxBatch = 4096;    // queries per batch
 
 struct QDesc {
   const unsigned long long *table;
-  uint64_t mask;
+  uint64_t mask;       // hash table: slots - 1
+  int32_t dlo;         // direct table: smallest key
+  uint32_t drange;     // direct table: slots; 0 = hash table
+  uint32_t dslots;     // table entries
   const int32_t *fk;
   const int32_t *val;
   const int32_t *grp;
@@ -93,7 +98,8 @@ __global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
 
 // flags: bit 0 duplicate key, bit 1 reserved key INT32_MIN
 __global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__restrict__ v, uint64_t n,
-                            unsigned long long *t, uint64_t mask, unsigned int *flags) {
+                            unsigned long long *t, uint64_t mask, int32_t lo, uint32_t range,
+                            unsigned int *flags) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int32_t key = k[i];
@@ -102,11 +108,16 @@ __global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__rest
       continue;
     }
     const unsigned long long w = (unsigned long long)(uint32_t)key << 32 | (uint32_t)(v ? v[i] : 0);
-    uint64_t h = dim_hash(key) & mask;
+    bool in;
+    uint64_t h = dim_start(key, mask, lo, range, &in);
+    if (!in) {  // cannot happen: lo and range cover every key
+      atomicOr(flags, 4u);
+      continue;
+    }
     for (;;) {
       unsigned long long old = atomicCAS(&t[h], kEmptySlot, w);
       if (old == kEmptySlot) break;
-      if ((int32_t)(old >> 32) == key) {
+      if ((int32_t)(old >> 32) == key || range) {  // a direct slot holds only its key
         atomicOr(flags, 1u);
         break;
       }
@@ -173,7 +184,9 @@ template <bool LDS_TAB, bool LDS_GRP>
 __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long long *ltab,
                                           uint32_t *gcnt, unsigned long long *gsum, int32_t k,
                                           int32_t v, int32_t g, uint32_t &bad) {
-  uint64_t h = dim_hash(k) & q.mask;
+  bool in;
+  uint64_t h = dim_start(k, q.mask, q.dlo, q.drange, &in);
+  if (!in) return;
   int32_t b = 0;
   for (;;) {
     const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];
@@ -182,6 +195,7 @@ __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long lo
       b = (int32_t)(uint32_t)s;
       break;
     }
+    if (q.drange) return;  // a direct slot holds only its own key
     h = (h + 1) & q.mask;
   }
   if ((uint32_t)g >= q.groups) {
@@ -207,13 +221,13 @@ __device__ __forceinline__ void probe8(const QDesc &q, const unsigned long long 
                                        const int32_t (&k)[8], const int32_t (&v)[8],
                                        const int32_t (&g)[8], uint32_t &bad) {
   const uint32_t mask = (uint32_t)q.mask;
+  const bool direct = q.drange != 0;
   uint32_t h[8];
   int32_t b[8];
   bool act[8], hit[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    h[j] = dim_hash(k[j]) & mask;
-    act[j] = true;
+    h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &act[j]);
     hit[j] = false;
     b[j] = 0;
   }
@@ -230,10 +244,11 @@ __device__ __forceinline__ void probe8(const QDesc &q, const unsigned long long 
       const bool empty = s[j] == kEmptySlot;
       const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
       const bool got = act[j] && eq;
+      const bool more = act[j] && !empty && !eq && !direct;
       hit[j] = hit[j] || got;
       b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
-      h[j] = (act[j] && !empty && !eq) ? ((h[j] + 1) & mask) : h[j];
-      act[j] = act[j] && !empty && !eq;
+      h[j] = more ? ((h[j] + 1) & mask) : h[j];
+      act[j] = more;
       any = any || act[j];
     }
     if (!any) break;
@@ -266,13 +281,15 @@ __device__ __forceinline__ void stage8(const QDesc &q, bool valid, const int32_t
                                        uint32_t &bad, uint32_t *s_pcnt,
                                        unsigned long long *s_pbase) {
   const uint32_t mask = (uint32_t)q.mask;
+  const bool direct = q.drange != 0;
   uint32_t h[8];
   int32_t b[8];
   bool act[8], hit[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    h[j] = dim_hash(k[j]) & mask;
-    act[j] = valid;
+    bool in;
+    h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &in);
+    act[j] = valid && in;
     hit[j] = false;
     b[j] = 0;
   }
@@ -286,10 +303,11 @@ __device__ __forceinline__ void stage8(const QDesc &q, bool valid, const int32_t
       const bool empty = s[j] == kEmptySlot;
       const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
       const bool got = act[j] && eq;
+      const bool more = act[j] && !empty && !eq && !direct;
       hit[j] = hit[j] || got;
       b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
-      h[j] = (act[j] && !empty && !eq) ? ((h[j] + 1) & mask) : h[j];
-      act[j] = act[j] && !empty && !eq;
+      h[j] = more ? ((h[j] + 1) & mask) : h[j];
+      act[j] = more;
       any = any || act[j];
     }
     if (!any) break;
@@ -421,7 +439,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_probe(BatchHdr *hdr, cons
     const QDesc &q = d[qi];
     const bool lds_grp = q.groups <= kLdsGroups;
     if (LDS_TAB && qi != last_q)
-      for (uint32_t i = tid; i <= (uint32_t)q.mask; i += kBatchBlock) ltab[i] = q.table[i];
+      for (uint32_t i = tid; i < q.dslots; i += kBatchBlock) ltab[i] = q.table[i];
     last_q = qi;
     if (lds_grp)
       for (uint32_t i = tid; i < q.groups; i += kBatchBlock) {
@@ -541,7 +559,10 @@ static int fill_desc(const pj_query &q, QDesc *d) {
     return fail(PJ_E_ARG, "generated query needs fk_range in [1, 2^31]");
   if (q.n >= (1ULL << 40)) return fail(PJ_E_ARG, "query probe side too large");
   d->table = q.dim->table;
-  d->mask = q.dim->slots - 1;
+  d->mask = q.dim->range ? 0 : q.dim->slots - 1;
+  d->dlo = q.dim->lo;
+  d->drange = q.dim->range;
+  d->dslots = (uint32_t)q.dim->slots;
   d->dn = (uint32_t)std::min<uint64_t>(q.dim->n, 0xffffffffULL);
   d->fk = q.d_fk;
   d->val = q.d_val;
@@ -706,7 +727,9 @@ static int check_bad(const void *h_buf) {
 int pj::rel::dim_table(const pj_dim *dim, PipeTable *t) {
   if (!dim) return fail(PJ_E_ARG, "null dimension");
   t->table = dim->table;
-  t->mask = dim->slots - 1;
+  t->mask = dim->range ? 0 : dim->slots - 1;
+  t->lo = dim->lo;
+  t->range = dim->range;
   return PJ_OK;
 }
 
@@ -728,12 +751,34 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   if (n >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
   pj_dim *dm = new pj_dim();
   dm->n = n;
-  // load <= 1/2; tables of mid-sized dimensions (that still fit the L2s)
-  // at load <= 1/8, which shortens the longest probe sequence a wave waits
-  // for.  Dimensions probed from LDS keep <= kLdsSlots slots.
-  const int lg = ilog2_ceil(n ? n : 1);
-  const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
-  dm->slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
+  // Keys spanning at most twice the row count (dense keys, as primary keys
+  // usually are) get a direct table indexed by key - min: one load per
+  // probe, and keys outside the span miss without one.  Otherwise open
+  // addressing at load <= 1/2; tables of mid-sized dimensions (that still
+  // fit the L2s) at load <= 1/8, which shortens the longest probe sequence
+  // a wave waits for.  Dimensions probed from LDS keep <= kLdsSlots slots.
+  if (n) {
+    int64_t kmin = 0, kmax = 0;
+    double unused = 0;
+    int rc = rel::reduce(ctx, d_keys, rel::I32, n, rel::M_MIN, &kmin, &unused);
+    if (rc == PJ_OK) rc = rel::reduce(ctx, d_keys, rel::I32, n, rel::M_MAX, &kmax, &unused);
+    if (rc != PJ_OK) {
+      delete dm;
+      return rc;
+    }
+    const uint64_t span = (uint64_t)(kmax - kmin) + 1;
+    if (kmin > INT32_MIN && span <= 2 * n) {
+      dm->lo = (int32_t)kmin;
+      dm->range = (uint32_t)span;
+    }
+  }
+  if (dm->range) {
+    dm->slots = dm->range;
+  } else {
+    const int lg = ilog2_ceil(n ? n : 1);
+    const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
+    dm->slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
+  }
   int rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;
@@ -753,13 +798,14 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
   if (n)
     hipLaunchKernelGGL(k_dim_build, dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
-                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->table, dm->slots - 1,
-                       flags.as<unsigned int>());
+                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->table,
+                       dm->range ? 0 : dm->slots - 1, dm->lo, dm->range, flags.as<unsigned int>());
   if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
   unsigned int f = 0;
   if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
   if (f & 2) return fin(fail(PJ_E_ARG, "dimension key INT32_MIN is reserved"));
   if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
+  if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
   *out = dm;
   return PJ_OK;
 }

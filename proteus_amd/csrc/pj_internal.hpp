@@ -67,6 +67,21 @@ __host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
 // it): the 32-bit murmur3 finaliser, cheaper than the 64-bit hashInt32.
 __host__ __device__ __forceinline__ uint32_t dim_hash(int32_t k) { return fmix32((uint32_t)k); }
 
+// First slot of key k in a resident dimension table.  A direct table
+// (range > 0) holds key k only at slot k - lo, so one load decides; a key
+// outside [lo, lo + range) is a miss without a load (*in = false, slot 0).
+// A hash table starts at dim_hash(k) & mask and is probed linearly.
+__host__ __device__ __forceinline__ uint32_t dim_start(int32_t k, uint64_t mask, int32_t lo,
+                                                      uint32_t range, bool *in) {
+  if (range) {
+    const uint32_t o = (uint32_t)k - (uint32_t)lo;
+    *in = o < range;
+    return *in ? o : 0u;
+  }
+  *in = true;
+  return dim_hash(k) & (uint32_t)mask;
+}
+
 // ---------------------------------------------------------------------------
 // counter-based RNG (splitmix64 finaliser; Steele, Lea, Flood 2014)
 // ---------------------------------------------------------------------------

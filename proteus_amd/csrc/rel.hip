@@ -700,9 +700,10 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
               const uint32_t i = lane + j * kPipeWave;
-              act[j] = i < n;
-              key[j] = act[j] ? (int32_t)st[i] : 0;
-              h[j] = dim_hash(key[j]) & tb.mask;
+              key[j] = i < n ? (int32_t)st[i] : 0;
+              bool in;
+              h[j] = dim_start(key[j], tb.mask, tb.lo, tb.range, &in);
+              act[j] = i < n && in;
             }
             // all of the lane's keys advance one slot per round
             for (;;) {
@@ -718,6 +719,8 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
                 } else if ((int32_t)(sv[j] >> 32) == key[j]) {
                   rl[rr[j]] = (uint32_t)sv[j];
                   hit[j] = true;
+                  act[j] = false;
+                } else if (tb.range) {  // a direct slot holds only its own key
                   act[j] = false;
                 } else {
                   h[j] = (h[j] + 1) & tb.mask;
@@ -901,14 +904,16 @@ __device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &c
       } else {
         const PipeTable tb = a.tables[sg.tab];
         const uint32_t mask = (uint32_t)tb.mask;
+        const bool direct = tb.range != 0;
         uint32_t h[R], found[R];
         bool act[R], hit[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          act[k] = cr.alive[k];
+          bool in;
+          h[k] = dim_start((int32_t)v[k], tb.mask, tb.lo, tb.range, &in);
+          act[k] = cr.alive[k] && in;
           hit[k] = false;
           found[k] = 0;
-          h[k] = dim_hash((int32_t)v[k]) & mask;
         }
         for (;;) {
           unsigned long long sv[R];
@@ -919,7 +924,7 @@ __device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &c
           for (int k = 0; k < R; ++k) {
             const bool empty = sv[k] == kPipeEmpty;
             const bool eq = (int32_t)(sv[k] >> 32) == (int32_t)v[k];
-            const bool done = empty || eq;
+            const bool done = empty || eq || direct;  // a direct slot holds only its key
             const bool got = act[k] && !empty && eq;
             hit[k] = hit[k] || got;
             found[k] = got ? (uint32_t)sv[k] : found[k];

@@ -96,7 +96,9 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
 // rows.
 struct PipeTable {
   const unsigned long long *table;
-  uint64_t mask;
+  uint64_t mask;   // hash tables: slots - 1
+  int32_t lo;      // direct tables: the smallest key
+  uint32_t range;  // direct tables: slots (keys lo .. lo + range - 1); 0 = hash table
 };
 constexpr int kPipeMaxLevels = 8;  // source + 7 joins
 constexpr int kPipeMaxOut = 24;

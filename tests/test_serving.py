@@ -38,9 +38,18 @@ def test_dimension_errors(ctx):
     with pytest.raises(pa.PjError) as e:
         ctx.dim_create(k2)
     assert e.value.code == pa.PJ_E_ARG
+    # dense keys: a direct table of one slot per key in the span
     d = ctx.dim_create(ctx.to_device(np.arange(100, dtype=np.int32)))
+    assert ctx.dim_info(d) == (100, 100 * 8)
+    ctx.dim_destroy(d)
+    # sparse keys: open addressing at load <= 1/2
+    d = ctx.dim_create(ctx.to_device(np.arange(100, dtype=np.int32) * 3))
     assert ctx.dim_info(d) == (100, 256 * 8)
     ctx.dim_destroy(d)
+    # a duplicate in a dense span is still an error
+    with pytest.raises(pa.PjError) as e:
+        ctx.dim_create(ctx.to_device(np.array([5, 6, 7, 6], np.int32)))
+    assert e.value.code == pa.PJ_E_ARG
 
 
 @pytest.mark.gpu
@@ -203,3 +212,45 @@ def test_autotune_and_table_placement_keep_results(ctx):
     for b in bufs:
         b.free()
     mix.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lo,step,groups", [(7, 3, 16), (-50000, 5, 3000), (-(1 << 31) + 1, 1, 16)])
+def test_direct_and_hashed_dimensions(ctx, lo, step, groups):
+    """Dimensions of dense keys (step 1: a direct table, negative keys and
+    the lowest key above the reserved INT32_MIN included) and of sparse keys
+    (open addressing), probed alone and batched, against numpy: probe keys
+    cover the key span and beyond on both sides."""
+    rs = np.random.default_rng(abs(lo) + step)
+    nb = 5000
+    keys = (lo + step * rs.permutation(nb)).astype(np.int64)
+    vals = rs.integers(-(1 << 31), 1 << 31, nb).astype(np.int32)
+    dim = ctx.dim_create(ctx.to_device(keys.astype(np.int32)), ctx.to_device(vals))
+    span = step * nb
+    n = 200003
+    fk = np.clip(lo - 1000 + rs.integers(0, span + 2000, n), -(1 << 31), (1 << 31) - 1)
+    fk[:3] = [-(1 << 31), (1 << 31) - 1, lo]
+    fk = fk.astype(np.int32)
+    val = rs.integers(-(1 << 31), 1 << 31, n).astype(np.int32)
+    grp = rs.integers(0, groups, n).astype(np.int32)
+    look = dict(zip(keys.tolist(), vals.astype(np.int64).tolist()))
+    hit = np.array([int(k) in look for k in fk.tolist()])
+    b = np.array([look.get(int(k), 0) for k in fk.tolist()], np.int64)
+    wc = np.zeros(groups, np.int64)
+    ws = np.zeros(groups, np.int64)
+    np.add.at(wc, grp[hit], 1)
+    np.add.at(ws, grp[hit], b[hit] + val[hit].astype(np.int64))
+    cols = [ctx.to_device(a) for a in (fk, val, grp)]
+    cnt, sm = ctx.empty(groups, np.int64), ctx.empty(groups, np.int64)
+    q = pa.Query(dim, cols[0].ptr, cols[1].ptr, cols[2].ptr, n, groups, 0, 0, 0, cnt.ptr, sm.ptr)
+    for tile, lds in ((0, 2048), (0, -1), (4096, -1)):
+        ctx.query_run(q, pa.BatchCfg(tile, 0, lds))
+        assert np.array_equal(cnt.numpy(), wc), (tile, lds)
+        assert np.array_equal(sm.numpy(), ws), (tile, lds)
+    ctx.batch_slots(1)
+    ctx.batch_launch(0, [q, q])  # the same query twice: twice the sums
+    ctx.batch_wait(0)
+    assert np.array_equal(cnt.numpy(), 2 * wc) and np.array_equal(sm.numpy(), 2 * ws)
+    for a in cols + [cnt, sm]:
+        a.free()
+    ctx.dim_destroy(dim)
