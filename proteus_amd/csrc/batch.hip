@@ -751,12 +751,16 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   if (n >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
   pj_dim *dm = new pj_dim();
   dm->n = n;
-  // Keys spanning at most twice the row count (dense keys, as primary keys
-  // usually are) get a direct table indexed by key - min: one load per
-  // probe, and keys outside the span miss without one.  Otherwise open
-  // addressing at load <= 1/2; tables of mid-sized dimensions (that still
-  // fit the L2s) at load <= 1/8, which shortens the longest probe sequence
-  // a wave waits for.  Dimensions probed from LDS keep <= kLdsSlots slots.
+  // Open addressing at load <= 1/2; tables of mid-sized dimensions (that
+  // still fit the L2s) at load <= 1/16, which shortens the longest probe
+  // sequence a wave waits for.  Dimensions probed from LDS keep <=
+  // kLdsSlots slots.  Keys whose span is no larger than that table (dense
+  // keys, as primary keys usually are, or a filtered subset of them) get a
+  // direct table indexed by key - min instead: one load per probe, and keys
+  // outside the span miss without one.
+  const int lg = ilog2_ceil(n ? n : 1);
+  const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
+  const uint64_t hash_slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
   if (n) {
     int64_t kmin = 0, kmax = 0;
     double unused = 0;
@@ -767,18 +771,12 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
       return rc;
     }
     const uint64_t span = (uint64_t)(kmax - kmin) + 1;
-    if (kmin > INT32_MIN && span <= 2 * n) {
+    if (kmin > INT32_MIN && span <= std::max<uint64_t>(2 * n, hash_slots)) {
       dm->lo = (int32_t)kmin;
       dm->range = (uint32_t)span;
     }
   }
-  if (dm->range) {
-    dm->slots = dm->range;
-  } else {
-    const int lg = ilog2_ceil(n ? n : 1);
-    const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
-    dm->slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
-  }
+  dm->slots = dm->range ? dm->range : hash_slots;
   int rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;

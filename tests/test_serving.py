@@ -215,11 +215,13 @@ def test_autotune_and_table_placement_keep_results(ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lo,step,groups", [(7, 3, 16), (-50000, 5, 3000), (-(1 << 31) + 1, 1, 16)])
+@pytest.mark.parametrize("lo,step,groups", [(7, 3, 16), (-50000, 5, 3000), (-(1 << 31) + 1, 1, 16),
+                                            (11, 100, 16), (-50000, 97, 3000)])
 def test_direct_and_hashed_dimensions(ctx, lo, step, groups):
-    """Dimensions of dense keys (step 1: a direct table, negative keys and
-    the lowest key above the reserved INT32_MIN included) and of sparse keys
-    (open addressing), probed alone and batched, against numpy: probe keys
+    """Dimensions of dense keys (step 1), of keys with holes (steps 3 and 5:
+    direct tables with empty slots) and of sparse keys (steps 97 and 100:
+    open addressing), negative keys and the lowest key above the reserved
+    INT32_MIN included, probed alone and batched, against numpy: probe keys
     cover the key span and beyond on both sides."""
     rs = np.random.default_rng(abs(lo) + step)
     nb = 5000
@@ -227,6 +229,8 @@ def test_direct_and_hashed_dimensions(ctx, lo, step, groups):
     vals = rs.integers(-(1 << 31), 1 << 31, nb).astype(np.int32)
     dim = ctx.dim_create(ctx.to_device(keys.astype(np.int32)), ctx.to_device(vals))
     span = step * nb
+    direct = span - step + 1 <= (1 << 17)  # the hash table of 5000 keys: 2^17 slots
+    assert ctx.dim_info(dim)[1] == ((span - step + 1) * 8 if direct else (1 << 17) * 8)
     n = 200003
     fk = np.clip(lo - 1000 + rs.integers(0, span + 2000, n), -(1 << 31), (1 << 31) - 1)
     fk[:3] = [-(1 << 31), (1 << 31) - 1, lo]
