@@ -354,6 +354,22 @@ def batched_mix(ctx, nq, qps):
         out["identical"] = out["identical"] and all(
             np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(got, res[1]))
     mix.close()
+    # The same closed loop over small queries only (builds of 2^10..2^16
+    # rows): per-query launches rather than bytes bound the plain path here.
+    # Same tuned configuration as above.
+    small = workload.Mix(ctx, nq, seed=SEED + 1, u_lo=10, u_hi=16)
+    sres = {}
+    sm = {"u_range": [10, 16], "probe_rows": small.probe_rows()}
+    for mode, name in ((0, "batched"), (1, "plain")):
+        done, st = ctx.serve_run(small.queries(), zeros, mode=mode, slots=3, hbm_budget=budget,
+                                 **(plain_cfg if mode else {}))
+        sres[mode] = small.results()
+        sm[name + "_qps"] = nq / st["wall_s"]
+    sm["speedup"] = sm["batched_qps"] / sm["plain_qps"]
+    sm["identical"] = all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+                          for x, y in zip(sres[0], sres[1]))
+    small.close()
+    out["small_queries"] = sm
     return out
 
 
