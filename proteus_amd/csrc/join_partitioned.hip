@@ -643,22 +643,29 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       // leave HBM bandwidth unused (short grids, launch gaps).
       hipStream_t aux;
       PJ_TRY(aux_fork(ctx, &aux));
-      // (Starting the build side's later passes only beside the probe
-      // side's second pass, or all of them only after the probe side's
-      // histograms and scans, measured the same.)
+      // (Starting the whole build side only after the probe side's
+      // histograms and scans measured the same.)
       const int b1 = cfg.bits1, b2 = cfg.bits2;
       int rc;
       {
         StreamSwap sw(ctx, aux);
         rc = side_histograms(ctx, R, d_bk, nb, b1, b2);
         if (rc == PJ_OK) rc = side_pass1(ctx, R, d_bk, d_bv, nb, b1, b2, "build");
-        if (rc == PJ_OK) rc = side_pass2(ctx, R, nb, b1, b2, "build");
       }
       if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, b1, b2);
       if (rc == PJ_OK) rc = side_pass1(ctx, S, d_pk, d_pv, np, b1, b2, "probe");
-      // The build side's last pass ends beside the probe side's pass 2.
-      // (Making pass 2 wait for it moves the contention to pass 1, which
-      // then took 4.1-4.9 ms instead of 4.2, for a join 0.04-0.07 ms longer.)
+      // The build side's last pass (one 1024-thread workgroup per CU) starts
+      // after the probe side's pass 1 and runs beside its pass 2.  Left
+      // free, it sometimes took the CUs that the build side's pass 1 left
+      // while the probe side's pass 1 still ran: that pass then took 4.9-6.9
+      // ms instead of 4.2 and the join 13.1 ms instead of 12.5.  (Making the
+      // probe side's pass 2 wait for the build side instead moves the
+      // contention into its pass 1 every time.)
+      if (rc == PJ_OK) rc = aux_fork(ctx, &aux);
+      if (rc == PJ_OK) {
+        StreamSwap sw(ctx, aux);
+        rc = side_pass2(ctx, R, nb, b1, b2, "build");
+      }
       if (rc == PJ_OK) rc = side_pass2(ctx, S, np, b1, b2, "probe");
       // join even on failure, so no scratch is released under running work
       const int rj = aux_join(ctx);
