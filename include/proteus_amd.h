@@ -119,6 +119,23 @@ int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
                            uint64_t np, int hash_bits, uint64_t max_build,
                            pj_join_agg *out);
 
+/* Allocation-free variant of pj_join_chained_reduce.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors, but the table and the
+ * accumulator live in a static 32 MiB + 256 B device array of this library
+ * (one per GPU, held until the process ends) instead of the context pool:
+ * the table (max(2^hash_bits, 2 * next_pow2(nb)) 8-byte slots) must fit
+ * 2^22 slots, i.e. nb <= 2^21 and hash_bits <= 22, else PJ_E_CAPACITY.
+ * No hipMalloc / hipFree or other allocation happens in the call.  Every
+ * call on a GPU shares that GPU's array, from every context on it: calls
+ * that overlap on one GPU race (calls on one context are serialized). */
+int pj_join_chained_reduce_noalloc(pj_ctx *ctx, const int32_t *d_bk,
+                                   const int32_t *d_bv, uint64_t nb,
+                                   const int32_t *d_pk, const int32_t *d_pv,
+                                   uint64_t np, int hash_bits,
+                                   uint64_t max_build, pj_join_agg *out);
+
 /* hashjoin-chained with materialized output: every (build row, probe row)
  * pair the reference's probe loop hands to its parent
  * (hash-join-chained.cpp:321-566).  Writes up to `cap` pairs to d_out_b /

@@ -108,6 +108,8 @@ _SIGS = {
     "pj_gen_zipf_i32": ([vp, vp, u64, u64, u64, u64, C.c_double, u64], i32),
     "pj_join_chained_reduce": ([vp, vp, vp, u64, vp, vp, u64, i32, u64, C.POINTER(JoinAgg)],
                                i32),
+    "pj_join_chained_reduce_noalloc": ([vp, vp, vp, u64, vp, vp, u64, i32, u64,
+                                        C.POINTER(JoinAgg)], i32),
     "pj_join_chained_pairs": ([vp, vp, u64, vp, u64, i32, u64, vp, vp, u64, C.POINTER(u64)],
                               i32),
     "pj_join_partitioned_reduce": ([vp, vp, vp, u64, vp, vp, u64, u64, u64, C.POINTER(RadixCfg),
@@ -305,6 +307,15 @@ class Context:
                                            np_, hash_bits,
                                            nb if max_build is None else max_build,
                                            C.byref(out)))
+        return out.as_dict()
+
+    def join_chained_reduce_noalloc(self, bk, bv, pk, pv, hash_bits, max_build=None):
+        out = JoinAgg()
+        nb, np_ = len(bk), len(pk)
+        _check(_lib.pj_join_chained_reduce_noalloc(self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk),
+                                                   _ptr(pv), np_, hash_bits,
+                                                   nb if max_build is None else max_build,
+                                                   C.byref(out)))
         return out.as_dict()
 
     def join_chained_pairs(self, bk, pk, hash_bits, cap, max_build=None):

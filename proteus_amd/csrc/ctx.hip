@@ -441,6 +441,9 @@ int pj_ctx_create(int device, uint64_t hbm_budget_bytes, pj_ctx **out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(PJ_E_UNSUPPORTED, "device %d is %s; this library is built for gfx950",
                 device, prop.gcnArchName);
+  // the allocation-free entry points' static scratch, resolved once per
+  // device while it is current
+  PJ_TRY(noalloc_scratch_init(device));
   pj_ctx *ctx = new pj_ctx();
   ctx->device = device;
   ctx->num_cus = prop.multiProcessorCount;

@@ -172,9 +172,89 @@ static uint64_t probe_grid(pj_ctx *ctx, uint64_t rows_per_thread, uint64_t np) {
   return g ? g : 1;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the table and the accumulator live in a
+// file-scope device array instead of the context pool.  Every device holds
+// its own copy of the array (at its own address) from module load until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device, never through the array's name.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocSlots = 1ULL << 22;  // 32 MiB of 64-bit slots
+constexpr uint64_t kNoAllocAccWords = 32;       // 256 B: the 7-word accumulator
+static __device__ __align__(256) unsigned long long scratch[kNoAllocSlots + kNoAllocAccWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_ptr[kMaxDevices];
+
+// Resolves the scratch array's address on `device`, which must be current.
+// Called once per device by pj_ctx_create.
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(PJ_E_ARG, "device %d outside [0, %d)", device, kMaxDevices);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PJ_OK;
+  void *p = nullptr;
+  PJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(PJ_E_HIP, "hipGetSymbolAddress(scratch) returned NULL on device %d", device);
+  g_scratch_ptr[device] = static_cast<unsigned long long *>(p);
+  return PJ_OK;
+}
+
+static unsigned long long *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 }  // namespace pj
 
 using namespace pj;
+
+extern "C" int pj_join_chained_reduce_noalloc(pj_ctx *ctx, const int32_t *d_bk,
+                                              const int32_t *d_bv, uint64_t nb,
+                                              const int32_t *d_pk, const int32_t *d_pv,
+                                              uint64_t np, int hash_bits,
+                                              uint64_t max_build, pj_join_agg *out) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (hash_bits < 0 || hash_bits > 32)
+    return fail(PJ_E_ARG, "hash_bits %d out of range [0, 32]", hash_bits);
+  if (nb > max_build)
+    return fail(PJ_E_CAPACITY,
+                "Build input size exceeded given parameter: %llu rows > "
+                "maxBuildInputSize %llu",
+                (unsigned long long)nb, (unsigned long long)max_build);
+  // the same geometry as chained_table()
+  uint64_t slots = 1ULL << hash_bits;
+  uint64_t need = 2ULL << ilog2_ceil(nb ? nb : 1);
+  if (slots < need) slots = need;
+  if (slots < 64) slots = 64;
+  if (slots > kNoAllocSlots)
+    return fail(PJ_E_CAPACITY,
+                "pj_join_chained_reduce_noalloc: a table of %llu slots (hash_bits %d, %llu "
+                "build rows) exceeds the static scratch of %llu slots (at most 2^21 build rows "
+                "and hash_bits <= 22)",
+                (unsigned long long)slots, hash_bits, (unsigned long long)nb,
+                (unsigned long long)kNoAllocSlots);
+  unsigned long long *base = noalloc_scratch(ctx->device);
+  if (!base)
+    return fail(PJ_E_HIP, "static scratch of device %d was not resolved", ctx->device);
+  unsigned long long *tab = base;
+  unsigned long long *acc = base + kNoAllocSlots;
+  const uint64_t mask = slots - 1;
+  // the previous call's contents stay behind: clear what this call reads
+  PJ_TRY(fill_u32(ctx, reinterpret_cast<uint32_t *>(tab), 0u, slots * 2));
+  PJ_TRY(join_acc_init(ctx, acc));
+  if (nb) {
+    uint64_t grid = ceil_div(nb, 256);
+    if (grid > 8192) grid = 8192;
+    PJ_LAUNCH(ctx, "join_chained_build", k_chained_build, grid, 256, 0, d_bk, nb, tab, mask);
+  }
+  if (np && nb)
+    PJ_LAUNCH(ctx, "join_chained_probe", k_chained_probe_reduce, probe_grid(ctx, 4, np), 256,
+              0, d_pk, d_pv, np, d_bv, (const unsigned long long *)tab, mask, acc);
+  return join_acc_read(ctx, acc, out);
+}
 
 extern "C" int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
                                       const int32_t *d_bv, uint64_t nb,

@@ -384,6 +384,9 @@ __device__ __forceinline__ void join_acc_flush(JoinAcc a, unsigned long long *g)
 }
 
 int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc);
+// Resolves the allocation-free entry points' static scratch on `device`
+// (which must be current); pj_ctx_create calls it.
+int noalloc_scratch_init(int device);
 int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
                   pj_join_agg *out);
 
