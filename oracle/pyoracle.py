@@ -187,6 +187,9 @@ def _i32(a):
 
 def join_chained(rk, rv, sk, sv, hash_bits, max_build=None, threads=1):
     rk, rv, sk, sv = _i32(rk), _i32(rv), _i32(sk), _i32(sv)
+    for k, v in ((rk, rv), (sk, sv)):
+        if v is not None and len(v) != len(k):
+            raise ValueError("value column has %d rows, key column %d" % (len(v), len(k)))
     if max_build is None:
         max_build = len(rk)
     out = JoinOut()

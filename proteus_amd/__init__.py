@@ -191,6 +191,13 @@ class DeviceArray:
         return self.n
 
 
+def _same_len(*pairs):
+    """Key and value columns of one side must have the same row count."""
+    for k, v in pairs:
+        if v is not None and len(v) != len(k):
+            raise PjError(PJ_E_ARG, "value column has %d rows, key column %d" % (len(v), len(k)))
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -301,6 +308,7 @@ class Context:
 
     # ---- operators -------------------------------------------------------
     def join_chained_reduce(self, bk, bv, pk, pv, hash_bits, max_build=None):
+        _same_len((bk, bv), (pk, pv))
         out = JoinAgg()
         nb, np_ = len(bk), len(pk)
         _check(_lib.pj_join_chained_reduce(self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk), _ptr(pv),
@@ -310,6 +318,7 @@ class Context:
         return out.as_dict()
 
     def join_chained_reduce_noalloc(self, bk, bv, pk, pv, hash_bits, max_build=None):
+        _same_len((bk, bv), (pk, pv))
         out = JoinAgg()
         nb, np_ = len(bk), len(pk)
         _check(_lib.pj_join_chained_reduce_noalloc(self.h, _ptr(bk), _ptr(bv), nb, _ptr(pk),
@@ -332,6 +341,7 @@ class Context:
         return rc, n.value, ob, op
 
     def join_partitioned_reduce(self, bk, bv, pk, pv, max_build=None, max_probe=None, cfg=None):
+        _same_len((bk, bv), (pk, pv))
         out = JoinAgg()
         c = None
         if cfg is not None:
@@ -345,6 +355,7 @@ class Context:
 
     def join_out_of_gpu_reduce(self, bk, bv, pk, pv, num_buckets=0, hbm_budget=0):
         """out-of-gpu-join -> reduce over host (numpy int32) columns."""
+        _same_len((bk, bv), (pk, pv))
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.int32)
                 for a in (bk, bv, pk, pv)]
         hp = [None if a is None else a.ctypes.data for a in arrs]

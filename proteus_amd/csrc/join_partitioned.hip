@@ -6,21 +6,24 @@
 // GpuHashRearrange (core/olap/lib/operators/gpu/gpu-hash-rearrange.cpp).
 //
 // Flow of one join (both sides go through the same steps):
-//   1. k_hist: joint histogram of the B = bits1 + bits2 top bits of
-//      fmix32(key) -- one read of the key column;
-//   2. host plan: 32-element-aligned region per pass-1 digit and per final
-//      partition, tile lists, join work units (<= probe_unit probe tuples each,
-//      the reference's decompose_chains idea, helpers.cpp:199-232);
-//   3. k_scatter pass 1 (fan-out 2^bits1) and pass 2 (2^bits2 inside every
-//      pass-1 region), (key, value) pairs, LDS write-combined full lines;
-//   4. k_pjoin: per work unit, the build partition goes into an LDS chained
-//      table (head[H] u32, entries (key, value) int2, next u16 -- the
-//      reference's LDS layout, :657-692, sized for up to NCAP rows instead of
-//      4096), then the unit's probe tuples walk it; matches fold into the
-//      join accumulator (the parent reduce).  A build partition larger than
-//      NCAP is processed in rounds of NCAP rows (each round re-reads the
-//      unit's probe tuples), so skew never overflows LDS.
-// Build partitions small enough for one LDS table skip steps 1-3 (B = 0).
+//   1. k_chunk_hist + column scans: per-chunk joint histograms of the
+//      B = bits1 + bits2 top bits of fmix32(key) -- one read of the key
+//      column -- give every (chunk, partition) its exact output offset, and
+//      k_pass2_items sizes the pass-2 work items from the digit totals;
+//   2. k_radix_pass pass 1 (fan-out 2^bits1, keys replaced by fmix32(key))
+//      and pass 2 (2^bits2 inside every pass-1 region): (key, value) pairs,
+//      LDS write-combined full lines, no global atomics;
+//   3. k_join_units: join work units (<= probe_unit probe tuples each, the
+//      reference's decompose_chains idea, helpers.cpp:199-232), largest
+//      first;
+//   4. k_pjoin_bitmap: per unit, the build partition goes into an LDS
+//      presence-bitmap table and the unit's probe tuples are looked up in it
+//      (one or two LDS reads, no key compare); units with duplicate build
+//      keys or oversized partitions are deferred to k_pjoin, the
+//      bucket-sorted LDS table, which takes any keys and processes
+//      partitions larger than its capacity in rounds.  Matches fold into the
+//      join accumulator (the parent reduce).
+// Build sides small enough for one LDS table skip steps 1-2 (B = 0).
 #include <algorithm>
 #include <cstdio>
 #include <vector>
@@ -107,6 +110,39 @@ __global__ __launch_bounds__(kScanBlock) void k_excl_scan(const uint32_t *__rest
   if (tid == 0) S[nb] = tot;
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_pass2_items(const uint32_t *__restrict__ T,
+                                                            int b1, int b2, int G,
+                                                            unsigned long long target,
+                                                            uint32_t *__restrict__ items) {
+  __shared__ uint32_t tmp[kScanBlock / 64];
+  const uint32_t F1 = 1u << b1, F2 = 1u << b2;
+  const uint32_t d1 = threadIdx.x;
+  unsigned long long t1 = 0;
+  if (d1 < F1) {
+    const uint32_t *row = T + ((uint64_t)d1 << b2);
+    for (uint32_t d2 = 0; d2 < F2; d2 += 8) {
+      uint32_t x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = d2 + j < F2 ? row[d2 + j] : 0u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t1 += x[j];
+    }
+  }
+  unsigned long long cpg = (unsigned long long)G;
+  if (t1) {
+    cpg = target * (unsigned long long)G / t1;
+    cpg = cpg < 1 ? 1 : (cpg > (unsigned long long)G ? (unsigned long long)G : cpg);
+  }
+  const uint32_t ngrp = d1 < F1 ? (uint32_t)(((unsigned long long)G + cpg - 1) / cpg) : 0u;
+  uint32_t tot;
+  const uint32_t first = block_excl_scan<kScanBlock>(ngrp, tmp, &tot);
+  if (d1 < F1) {
+    items[d1] = first;
+    items[F1 + 1 + d1] = (uint32_t)cpg;
+  }
+  if (d1 == 0) items[F1] = tot;
+}
+
 int colscan_slabs(const pj_ctx *ctx, int G, int nb, int *slab) {
   // about 2048 threads per CU over (bin, slab) pairs, slabs of >= 8 chunks
   const uint64_t want = (uint64_t)ctx->num_cus * 2048;
@@ -138,22 +174,260 @@ struct JoinUnit {
   uint32_t r_len, s_len;
 };
 
-// Per-unit LDS table + probe.  Workgroups claim units from a counter (the
-// host orders them largest first); a workgroup that claims another unit of
-// the build partition it has just built reuses the table.
+// Unit list header (device): hdr[0] = units, hdr[1] = claim counter of the
+// bitmap kernel, hdr[2] = units it deferred to the CSR kernel, hdr[3] =
+// claim counter of the CSR kernel.
+constexpr int kUnitHdrWords = 4;
+
+constexpr int ilog2_const(int x) { return x <= 1 ? 0 : 1 + ilog2_const(x >> 1); }
+
+// Probe tuples [s0, s1) of sk / sv: the 16-byte-aligned interior as int4
+// vectors (4 tuples per thread per step), the <= 3 tuples before and after
+// it one by one.  Three 4-tuple steps rotate through registers A, B, C: a
+// step's loads are issued two steps before it is probed.  The loads are
+// unconditional (indices clamped to the last vector, results past the end
+// unused): a load under a branch, or a register copy of a loaded value,
+// makes the compiler wait for outstanding loads there, which would serialise
+// the prefetch with the probe.  Without probe values the key column stands
+// in for them (probe4 ignores them).
+template <int BLOCK, class P4, class P1>
+__device__ __forceinline__ void probe_range(const int32_t *__restrict__ sk,
+                                            const int32_t *__restrict__ sv,
+                                            unsigned long long s0, unsigned long long s1,
+                                            P4 &&probe4, P1 &&probe1) {
+  const int tid = threadIdx.x;
+  const unsigned long long a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
+  if (a0 >= a1) {  // short unit: every tuple on its own
+    for (unsigned long long x = s0 + tid; x < s1; x += BLOCK) probe1(sk[x], sv ? sv[x] : 0);
+    return;
+  }
+  const int tail_lo = (int)(a0 - s0), tail_hi = (int)(s1 - a1);
+  if (tid < tail_lo + tail_hi) {
+    const unsigned long long x = tid < tail_lo ? s0 + tid : a1 + (tid - tail_lo);
+    probe1(sk[x], sv ? sv[x] : 0);
+  }
+  const int4 *kq = reinterpret_cast<const int4 *>(sk + a0);
+  const int4 *vq = reinterpret_cast<const int4 *>((sv ? sv : sk) + a0);
+  const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);  // >= 1 here
+  auto cl = [nvec](uint32_t x) { return x < nvec ? x : nvec - 1; };
+  uint32_t q = tid;
+  int4 kA = ld_nt4(kq + cl(q)), vA = ld_nt4(vq + cl(q));
+  int4 kB = ld_nt4(kq + cl(q + BLOCK)), vB = ld_nt4(vq + cl(q + BLOCK));
+  // whole triples of steps: a straight-line body, so the compiler's waits
+  // stay precise across the back edge
+  for (; q + 2 * BLOCK < nvec; q += 3 * BLOCK) {
+    const int4 kC = ld_nt4(kq + q + 2 * BLOCK), vC = ld_nt4(vq + q + 2 * BLOCK);
+    probe4(kA, vA);
+    kA = ld_nt4(kq + cl(q + 3 * BLOCK));
+    vA = ld_nt4(vq + cl(q + 3 * BLOCK));
+    probe4(kB, vB);
+    kB = ld_nt4(kq + cl(q + 4 * BLOCK));
+    vB = ld_nt4(vq + cl(q + 4 * BLOCK));
+    probe4(kC, vC);
+  }
+  if (q < nvec) probe4(kA, vA);
+  if (q + BLOCK < nvec) probe4(kB, vB);
+}
+
+// ---------------------------------------------------------------------------
+// Presence-bitmap join (unique build keys, hashed keys).
 //
-// The table is bucket-sorted ("CSR"): tab[b] packs the [start, end) range of
-// bucket b in ent[] (start in the high, end in the low 16 bits) and entries
-// are (key, value) pairs grouped by bucket.  Building it is count -> scan ->
-// scatter with LDS atomics (the scatter's cursor is the low half);
-// probing is one LDS read for the range plus the bucket's entries, which sit
-// next to each other -- the reference's chained LDS table (head / next,
-// gpu-partitioned-hash-join-chained.cpp:832-1037) needs one dependent read per
-// chain hop instead.  Every entry whose key equals the probe key is a match,
-// so duplicate build keys emit every pair.
+// The radix passes store h = fmix32(key), a bijection, and every key of a
+// final partition has the same top B bits.  With LOGH bucket bits taken from
+// the bottom of h and the RB = 32 - B - LOGH bits above them as the
+// "residue", (bucket, residue) determines the key.  So each bucket holds a
+// bitmap of the residues present in it and the index of its first value in
+// val[]; values are stored in residue order.  A probe reads the bucket: the
+// key is present iff bit r is set, and its value is
+// val[start + popcount(bitmap below r)].  No key is stored or compared and
+// no bucket is walked.  Two layouts:
+//   narrow (RB <= 4): tab[b] = 16-bit bitmap | start << 16, one LDS read;
+//   wide   (RB <= 6): bm[b] = 64-bit bitmap, off[b] = 16-bit start.
+//
+// A unit whose build partition has more than NCAPB rows, or a residue seen
+// twice (a duplicate build key), is appended to the deferred list for the
+// CSR kernel.  While a unit is probed, the next unit is claimed and its build
+// rows are loaded into registers, so a table is built without waiting for
+// HBM.  Workgroups claim units from a counter (largest first); a workgroup
+// that claims another unit of the partition it has just built reuses the
+// table.
+template <int BLOCK, int H, int NCAPB, int WAVES, bool WIDE>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_pjoin_bitmap(
+    const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
+    const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
+    const JoinUnit *__restrict__ units, uint32_t *__restrict__ hdr,
+    JoinUnit *__restrict__ deferred, unsigned long long *__restrict__ acc_out) {
+  constexpr int LOGH = ilog2_const(H);
+  static_assert((1 << LOGH) == H, "H is a power of two");
+  static_assert(NCAPB % BLOCK == 0 && H % BLOCK == 0, "whole rows per thread");
+  static_assert(NCAPB < 65536, "value starts are 16-bit");
+  constexpr uint32_t RMASK = WIDE ? 63u : 15u;
+  constexpr int RPT = NCAPB / BLOCK;  // build rows per thread
+  constexpr int BPT = H / BLOCK;      // buckets per thread in the scan
+  __shared__ uint32_t tab[WIDE ? 1 : H];
+  __shared__ unsigned long long bm[WIDE ? H : 1];
+  __shared__ uint16_t off[WIDE ? H : 1];
+  // a probe of an absent residue may index one past the last value (read,
+  // never used)
+  __shared__ int32_t val[NCAPB + 4];
+  __shared__ uint32_t scan_tmp[BLOCK / 64];
+  __shared__ uint32_t s_unit, s_dup;
+  const uint32_t nunits = hdr[0];
+  const int tid = threadIdx.x;
+  auto residue = [](int32_t h) -> uint32_t { return ((uint32_t)h >> LOGH) & RMASK; };
+  // (present, value index) of hashed key h in the built table
+  auto lookup = [&](int32_t h, uint32_t *idx) -> bool {
+    const uint32_t b = (uint32_t)h & (H - 1), r = residue(h);
+    if constexpr (WIDE) {
+      const unsigned long long m = bm[b];
+      *idx = (uint32_t)off[b] + (uint32_t)__popcll(m & ((1ULL << r) - 1ULL));
+      return (m >> r) & 1ULL;
+    } else {
+      const uint32_t w = tab[b];
+      *idx = (w >> 16) + (uint32_t)__popc(w & ((1u << r) - 1u));
+      return (w >> r) & 1u;
+    }
+  };
+  JoinAcc acc;
+  acc.init();
+  int32_t bk[RPT], bv[RPT];
+  auto fetch = [&](const JoinUnit &X) {
+    const int32_t *rkp = rk + X.r_begin;
+    const int32_t *rvp = rv ? rv + X.r_begin : nullptr;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const uint32_t i = (uint32_t)(q * BLOCK + tid);
+      bk[q] = i < X.r_len ? rkp[i] : 0;
+      bv[q] = (i < X.r_len && rvp) ? rvp[i] : 0;
+    }
+  };
+  if (tid == 0) s_unit = atomicAdd(&hdr[1], 1u);
+  __syncthreads();
+  uint32_t u = s_unit;
+  JoinUnit U{};
+  if (u < nunits) {
+    U = units[u];
+    if (U.r_len <= (uint32_t)NCAPB) fetch(U);
+  }
+  unsigned long long built_r = ~0ULL;
+  while (u < nunits) {
+    bool ok = U.r_len <= (uint32_t)NCAPB;
+    if (ok && U.r_begin != built_r) {
+      __syncthreads();  // the previous table is no longer probed
+      for (int i = tid; i < H; i += BLOCK) {
+        if constexpr (WIDE)
+          bm[i] = 0ULL;
+        else
+          tab[i] = 0u;
+      }
+      if (tid == 0) s_dup = 0;
+      __syncthreads();
+      bool dup = false;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q)
+        if ((uint32_t)(q * BLOCK + tid) < U.r_len) {
+          const uint32_t b = (uint32_t)bk[q] & (H - 1), r = residue(bk[q]);
+          if constexpr (WIDE) {
+            const unsigned long long bit = 1ULL << r;
+            dup |= (atomicOr(&bm[b], bit) & bit) != 0ULL;
+          } else {
+            const uint32_t bit = 1u << r;
+            dup |= (atomicOr(&tab[b], bit) & bit) != 0u;
+          }
+        }
+      if (dup) s_dup = 1;
+      __syncthreads();
+      if (s_dup) {
+        ok = false;
+        built_r = ~0ULL;
+      } else {
+        // value starts: exclusive scan of the bitmaps' popcounts
+        uint32_t c[BPT], sum = 0;
+#pragma unroll
+        for (int x = 0; x < BPT; ++x) {
+          if constexpr (WIDE)
+            c[x] = (uint32_t)__popcll(bm[tid * BPT + x]);
+          else
+            c[x] = (uint32_t)__popc(tab[tid * BPT + x]);
+          sum += c[x];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan<BLOCK>(sum, scan_tmp, &tot);
+#pragma unroll
+        for (int x = 0; x < BPT; ++x) {
+          if constexpr (WIDE)
+            off[tid * BPT + x] = (uint16_t)run;
+          else
+            tab[tid * BPT + x] |= run << 16;
+          run += c[x];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if ((uint32_t)(q * BLOCK + tid) < U.r_len) {
+            uint32_t idx;
+            (void)lookup(bk[q], &idx);
+            val[idx] = bv[q];
+          }
+        __syncthreads();
+        built_r = U.r_begin;
+      }
+    }
+    if (!ok && tid == 0) deferred[atomicAdd(&hdr[2], 1u)] = U;
+    // claim the next unit and start loading its build rows
+    if (tid == 0) s_unit = atomicAdd(&hdr[1], 1u);
+    __syncthreads();
+    const uint32_t un = s_unit;
+    __syncthreads();  // s_unit is written again only after every lane read it
+    JoinUnit Un{};
+    if (un < nunits) {
+      Un = units[un];
+      if (Un.r_begin != built_r && Un.r_len <= (uint32_t)NCAPB) fetch(Un);
+    }
+    if (ok) {
+      auto probe1 = [&](int32_t key, int32_t pv) {
+        uint32_t idx;
+        if (lookup(key, &idx)) acc.add1(val[idx], pv);
+      };
+      auto probe4 = [&](const int4 k4, const int4 v4) {
+        const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+        const int32_t vv[4] = {sv ? v4.x : 0, sv ? v4.y : 0, sv ? v4.z : 0, sv ? v4.w : 0};
+        bool hit[4];
+        uint32_t idx[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) hit[c] = lookup(kk[c], &idx[c]);
+        // in bounds also for an absent residue; the value is then unused
+        int32_t b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = val[idx[c]];
+        acc.add4_1(hit, b, vv);
+      };
+      probe_range<BLOCK>(sk, sv, U.s_begin, U.s_begin + U.s_len, probe4, probe1);
+    }
+    u = un;
+    U = Un;
+  }
+  join_acc_flush(acc, acc_out);
+}
+
+// ---------------------------------------------------------------------------
+// Bucket-sorted ("CSR") join, the general case: any keys, duplicates, build
+// partitions of any size.  tab[b] packs the [start, end) range of bucket b in
+// ent[] (start in the high, end in the low 16 bits) and entries are (key,
+// value) pairs grouped by bucket.  Building it is count -> scan -> scatter
+// with LDS atomics (the scatter's cursor is the low half); probing is one
+// LDS read for the range plus the bucket's entries, which sit next to each
+// other -- the reference's chained LDS table (head / next,
+// gpu-partitioned-hash-join-chained.cpp:832-1037) needs one dependent read
+// per chain hop instead.  Every entry whose key equals the probe key is a
+// match, so duplicate build keys emit every pair.  A build partition larger
+// than NCAP is processed in rounds of NCAP rows, each round re-reading the
+// unit's probe tuples, so skew never overflows LDS.
+//
 // HASHED: keys are fmix32(original key) (written by the radix passes); the
 // bucket is their low bits and equality of hashes is equality of keys.
 // Otherwise keys are raw and hashed here.
+// The units are units[0 .. *n_units), claimed with *claim.
 // Four waves per SIMD: two 512-thread workgroups per CU, or one of 1024 for
 // the largest table.
 template <int BLOCK, int H, int NCAP>
@@ -164,11 +438,9 @@ template <int BLOCK, int H, int NCAP, bool HASHED>
 __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) void k_pjoin(
     const int32_t *__restrict__ rk, const int32_t *__restrict__ rv,
     const int32_t *__restrict__ sk, const int32_t *__restrict__ sv,
-    const JoinUnit *__restrict__ units, uint32_t *__restrict__ unit_hdr,
-    unsigned long long *__restrict__ acc_out) {
-  // unit_hdr[0] = number of units, unit_hdr[1] = claim counter
-  const uint32_t nunits = unit_hdr[0];
-  uint32_t *next_unit = unit_hdr + 1;
+    const JoinUnit *__restrict__ units, const uint32_t *__restrict__ n_units,
+    uint32_t *__restrict__ claim, unsigned long long *__restrict__ acc_out) {
+  const uint32_t nunits = *n_units;
   static_assert(NCAP < 65536, "bucket ranges are 16-bit");
   static_assert(NCAP % BLOCK == 0 && H % BLOCK == 0, "whole rows per thread");
   constexpr int RPT = NCAP / BLOCK;  // build rows per thread
@@ -185,8 +457,7 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
   __shared__ uint32_t s_unit;
   unsigned long long built_r = ~0ULL;
   for (;;) {
-    // units are claimed from a counter (largest first: the host sorts them)
-    if (tid == 0) s_unit = atomicAdd(next_unit, 1u);
+    if (tid == 0) s_unit = atomicAdd(claim, 1u);
     __syncthreads();
     const uint32_t u = s_unit;
     __syncthreads();
@@ -225,9 +496,9 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
           }
           uint32_t tot;
           uint32_t run = block_excl_scan<BLOCK>(sum, scan_tmp, &tot);
-#pragma unroll
           // the start in both halves: the scatter's cursor increments the
           // low half, which ends at the bucket's end
+#pragma unroll
           for (int x = 0; x < BPT; ++x) {
             tab[tid * BPT + x] = run | (run << 16);
             run += c[x];
@@ -243,11 +514,6 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
         __syncthreads();
         built_r = rounds == 1 ? U.r_begin : ~0ULL;
       }
-      // Probe tuples [s_begin, s_begin + s_len): the 16-byte-aligned interior
-      // is read as int4 vectors (4 tuples per thread per step, the next step
-      // prefetched); the <= 3 tuples before and after it one by one.
-      const unsigned long long s0 = U.s_begin, s1 = U.s_begin + U.s_len;
-      const unsigned long long a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
       // all entries of a bucket of more than one entry
       auto probe_multi = [&](int32_t key, int32_t pv, uint32_t st, uint32_t en) {
         int64_t m = 0, sb = 0;
@@ -268,100 +534,63 @@ __global__ __launch_bounds__(BLOCK, (pjoin_waves_per_simd<BLOCK, H, NCAP>())) vo
         const uint32_t st = rg >> 16, en = rg & 0xffffu;
         if (st < en) probe_multi(key, pv, st, en);
       };
-      if (a0 >= a1) {  // short unit: every tuple on its own
-        for (unsigned long long x = s0 + tid; x < s1; x += BLOCK) probe1(sk[x], sv ? sv[x] : 0);
-      } else {
-        const int tail_lo = (int)(a0 - s0), tail_hi = (int)(s1 - a1);
-        if (tid < tail_lo + tail_hi) {
-          const unsigned long long x = tid < tail_lo ? s0 + tid : a1 + (tid - tail_lo);
-          probe1(sk[x], sv ? sv[x] : 0);
-        }
-        // Three 4-tuple steps rotate through registers A, B, C: a step's
-        // loads are issued two steps before it is probed.  The loads are
-        // unconditional (indices clamped to the last vector, results past
-        // the end unused): a load under a branch, or a register copy of a
-        // loaded value, makes the compiler wait for outstanding loads there,
-        // which would serialise the prefetch with the probe.  Without probe
-        // values the key column stands in for them.
-        const int4 *kq = reinterpret_cast<const int4 *>(sk + a0);
-        const int4 *vq = reinterpret_cast<const int4 *>((sv ? sv : sk) + a0);
-        const uint32_t nvec = (uint32_t)((a1 - a0) >> 2);  // >= 1 here
-        auto cl = [nvec](uint32_t x) { return x < nvec ? x : nvec - 1; };
-        auto probe4 = [&](const int4 k4, const int4 v4) {
-          const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-          const int32_t vv[4] = {sv ? v4.x : 0, sv ? v4.y : 0, sv ? v4.z : 0, sv ? v4.w : 0};
-          uint32_t rg[4];
+      auto probe4 = [&](const int4 k4, const int4 v4) {
+        const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+        const int32_t vv[4] = {sv ? v4.x : 0, sv ? v4.y : 0, sv ? v4.z : 0, sv ? v4.w : 0};
+        uint32_t rg[4];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
-          // entries are read only by the lanes whose bucket holds them
-          // (exec-masked LDS reads: most buckets hold zero or one entry)
-          int2 e0[4], e1[4], e2[4];
+        for (int c = 0; c < 4; ++c) rg[c] = tab[bucket(kk[c])];
+        // entries are read only by the lanes whose bucket holds them
+        // (exec-masked LDS reads: most buckets hold zero or one entry)
+        int2 e0[4], e1[4], e2[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
+          e0[c] = e1[c] = e2[c] = make_int2(0, 0);
+          if (en > st) e0[c] = ent[st];
+          if (en > st + 1) e1[c] = ent[st + 1];
+          if (en > st + 2) e2[c] = ent[st + 2];
+        }
+        // buckets of up to three entries without branches; longer ones
+        // finish in probe_multi.  A row with one match among them takes the
+        // single-match accumulate; rows matching twice or more the general
+        // one.
+        bool multi = false, dup = false;
+        bool h1m[4];
+        int32_t bs[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
+          const bool h0 = (en > st) & (e0[c].x == kk[c]);
+          const bool h1 = (en > st + 1) & (e1[c].x == kk[c]);
+          const bool h2 = (en > st + 2) & (e2[c].x == kk[c]);
+          h1m[c] = h0 | h1 | h2;
+          // the matching entry's value when there is one match
+          bs[c] = (e0[c].y & -(int32_t)h0) | (e1[c].y & -(int32_t)h1) |
+                  (e2[c].y & -(int32_t)h2);
+          dup |= (h0 & h1) | (h0 & h2) | (h1 & h2);
+          multi |= en > st + 3;
+        }
+        if (!dup) {
+          acc.add4_1(h1m, bs, vv);
+        } else {
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
-            e0[c] = e1[c] = e2[c] = make_int2(0, 0);
-            if (en > st) e0[c] = ent[st];
-            if (en > st + 1) e1[c] = ent[st + 1];
-            if (en > st + 2) e2[c] = ent[st + 2];
+            acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y,
+                        (en > st + 1) & (e1[c].x == kk[c]), e1[c].y,
+                        (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
           }
-          // buckets of up to three entries without branches (~98% of probes
-          // at the planner's load); longer ones finish in probe_multi.  A
-          // row with one match among them (always, for unique build keys)
-          // takes the single-match accumulate; rows matching twice or more
-          // take the general one.
-          bool multi = false, dup = false;
-          bool h1m[4];
-          int32_t bs[4];
+        }
+        if (multi) {
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
-            const bool h0 = (en > st) & (e0[c].x == kk[c]);
-            const bool h1 = (en > st + 1) & (e1[c].x == kk[c]);
-            const bool h2 = (en > st + 2) & (e2[c].x == kk[c]);
-            h1m[c] = h0 | h1 | h2;
-            // the matching entry's value when there is one match
-            bs[c] = (e0[c].y & -(int32_t)h0) | (e1[c].y & -(int32_t)h1) |
-                    (e2[c].y & -(int32_t)h2);
-            dup |= (h0 & h1) | (h0 & h2) | (h1 & h2);
-            multi |= en > st + 3;
+            if (en > st + 3) probe_multi(kk[c], vv[c], st + 3, en);
           }
-          if (!dup) {
-            acc.add4_1(h1m, bs, vv);
-          } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
-              acc.add3_if((en > st) & (e0[c].x == kk[c]), e0[c].y,
-                          (en > st + 1) & (e1[c].x == kk[c]), e1[c].y,
-                          (en > st + 2) & (e2[c].x == kk[c]), e2[c].y, vv[c]);
-            }
-          }
-          if (multi) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              const uint32_t st = rg[c] >> 16, en = rg[c] & 0xffffu;
-              if (en > st + 3) probe_multi(kk[c], vv[c], st + 3, en);
-            }
-          }
-        };
-        uint32_t q = tid;
-        int4 kA = ld_nt4(kq + cl(q)), vA = ld_nt4(vq + cl(q));
-        int4 kB = ld_nt4(kq + cl(q + BLOCK)), vB = ld_nt4(vq + cl(q + BLOCK));
-        // whole triples of steps: a straight-line body, so the compiler's
-        // waits stay precise across the back edge
-        for (; q + 2 * BLOCK < nvec; q += 3 * BLOCK) {
-          const int4 kC = ld_nt4(kq + q + 2 * BLOCK), vC = ld_nt4(vq + q + 2 * BLOCK);
-          probe4(kA, vA);
-          kA = ld_nt4(kq + cl(q + 3 * BLOCK));
-          vA = ld_nt4(vq + cl(q + 3 * BLOCK));
-          probe4(kB, vB);
-          kB = ld_nt4(kq + cl(q + 4 * BLOCK));
-          vB = ld_nt4(vq + cl(q + 4 * BLOCK));
-          probe4(kC, vC);
         }
-        if (q < nvec) probe4(kA, vA);
-        if (q + BLOCK < nvec) probe4(kB, vB);
-      }
+      };
+      probe_range<BLOCK>(sk, sv, U.s_begin, U.s_begin + U.s_len, probe4, probe1);
     }
   }
   join_acc_flush(acc, acc_out);
@@ -419,6 +648,8 @@ __global__ __launch_bounds__(kUnitBlock) void k_join_units(
     }
     unit_hdr[0] = run;
     unit_hdr[1] = 0;
+    unit_hdr[2] = 0;
+    unit_hdr[3] = 0;
   }
   __syncthreads();
   for (uint32_t q0 = tid; q0 < nb; q0 += K * kUnitBlock) {
@@ -576,35 +807,86 @@ static Geometry geometry_for(int lds_log) {
   }
 }
 
+// CSR join over units[0 .. *n_units) claimed with *claim.
 template <int BLOCK, int H, int NCAP>
-static int launch_pjoin_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
-                          const int32_t *sk, const int32_t *sv, const JoinUnit *units,
-                          uint32_t *hdr, uint32_t grid, unsigned long long *acc) {
+static int launch_pjoin_csr_g(pj_ctx *ctx, bool hashed, const int32_t *rk, const int32_t *rv,
+                              const int32_t *sk, const int32_t *sv, const JoinUnit *units,
+                              const uint32_t *n_units, uint32_t *claim, uint32_t grid,
+                              unsigned long long *acc) {
   if (hashed)
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<BLOCK, H, NCAP, true>), grid, BLOCK, 0,
-              rk, rv, sk, sv, units, hdr, acc);
+              rk, rv, sk, sv, units, n_units, claim, acc);
   else
     PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin<BLOCK, H, NCAP, false>), grid, BLOCK, 0,
-              rk, rv, sk, sv, units, hdr, acc);
+              rk, rv, sk, sv, units, n_units, claim, acc);
   return PJ_OK;
 }
 
-static int launch_pjoin(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk,
-                        const int32_t *rv, const int32_t *sk, const int32_t *sv,
-                        const JoinUnit *units, uint32_t *hdr, uint32_t grid,
-                        unsigned long long *acc) {
+static int launch_pjoin_csr(pj_ctx *ctx, int lds_log, bool hashed, const int32_t *rk,
+                            const int32_t *rv, const int32_t *sk, const int32_t *sv,
+                            const JoinUnit *units, const uint32_t *n_units, uint32_t *claim,
+                            uint32_t grid, unsigned long long *acc) {
   switch (lds_log) {
     case 11:
-      return launch_pjoin_g<512, 2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_csr_g<512, 2048, 3072>(ctx, hashed, rk, rv, sk, sv, units, n_units,
+                                                 claim, grid, acc);
     case 12:
-      return launch_pjoin_g<512, 4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_csr_g<512, 4096, 6144>(ctx, hashed, rk, rv, sk, sv, units, n_units,
+                                                 claim, grid, acc);
     case 13:
-      return launch_pjoin_g<512, 8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_csr_g<512, 8192, 5632>(ctx, hashed, rk, rv, sk, sv, units, n_units,
+                                                 claim, grid, acc);
     case 14:
-      return launch_pjoin_g<1024, 16384, 11264>(ctx, hashed, rk, rv, sk, sv, units, hdr, grid, acc);
+      return launch_pjoin_csr_g<1024, 16384, 11264>(ctx, hashed, rk, rv, sk, sv, units,
+                                                    n_units, claim, grid, acc);
     default:
       return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", lds_log);
   }
+}
+
+// The presence-bitmap join needs hashed keys (radix_bits B > 0) whose bits
+// between the bucket bits and the partition bits fit the bitmap:
+//   lds_log 13: narrow table, 8192 buckets, 32 - B - 13 <= 4 (B >= 15), up
+//               to 4608 rows in 50 KiB;
+//   lds_log 14: wide table, 4096 buckets, 32 - B - 12 <= 6 (B >= 14), up to
+//               9216 rows in 76 KiB.
+// Both run two 512-thread workgroups per CU.
+static bool bitmap_join_ok(int radix_bits, int lds_log) {
+  if (radix_bits <= 0) return false;
+  if (lds_log == 13) return 32 - radix_bits - 13 <= 4;
+  if (lds_log == 14) return 32 - radix_bits - 12 <= 6;
+  return false;
+}
+constexpr int kBitmapPerCu = 2;
+
+// Joins the units of the list at `du` (header, then the units at +256):
+// the bitmap kernel first when it applies, which defers the units it cannot
+// take to `deferred`, then the CSR kernel over what is left.
+static int launch_join(pj_ctx *ctx, const pj_radix_cfg &cfg, int radix_bits, const int32_t *rk,
+                       const int32_t *rv, const int32_t *sk, const int32_t *sv, uint8_t *du,
+                       JoinUnit *deferred, uint64_t max_units, unsigned long long *acc) {
+  const Geometry geo = geometry_for(cfg.lds_log);
+  const uint64_t mul = cfg.grid_mul > 0 ? cfg.grid_mul : 1;
+  uint32_t *hdr = reinterpret_cast<uint32_t *>(du);
+  const JoinUnit *units = reinterpret_cast<const JoinUnit *>(du + 256);
+  // workgroups beyond the number of units exit at their first claim
+  const uint32_t csr_grid =
+      (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * geo.per_cu * mul, max_units);
+  if (!bitmap_join_ok(radix_bits, cfg.lds_log))
+    return launch_pjoin_csr(ctx, cfg.lds_log, radix_bits > 0, rk, rv, sk, sv, units, hdr,
+                            hdr + 1, csr_grid, acc);
+  const uint32_t grid =
+      (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * kBitmapPerCu * mul, max_units);
+  if (cfg.lds_log == 13)
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin_bitmap<512, 8192, 4608, 4, false>), grid, 512, 0,
+              rk, rv, sk, sv, units, hdr, deferred, acc);
+  else
+    PJ_LAUNCH(ctx, "pjoin_probe", (k_pjoin_bitmap<512, 4096, 9216, 4, true>), grid, 512, 0,
+              rk, rv, sk, sv, units, hdr, deferred, acc);
+  // the deferred units (duplicate build keys, oversized partitions); with
+  // none, every workgroup exits at its first claim
+  return launch_pjoin_csr(ctx, cfg.lds_log, true, rk, rv, sk, sv, deferred, hdr + 2, hdr + 3,
+                          csr_grid, acc);
 }
 
 static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv,
@@ -616,7 +898,8 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
   const Geometry geo = geometry_for(cfg.lds_log);
   Side R, S;
-  Scratch dunits;  // unit header (2 x u32, 256 B), then the units
+  Scratch dunits;  // unit header (kUnitHdrWords x u32, 256 B), then the units
+  Scratch ddeferred;  // units the bitmap join left to the CSR join
   uint64_t max_units = 0;
   const int32_t *rk = d_bk, *rv = d_bv, *sk = d_pk, *sv = d_pv;
   if (nb && np) {
@@ -630,7 +913,7 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
         u.s_len = (uint32_t)std::min<uint64_t>(cfg.probe_unit, np - s0);
         units.push_back(u);
       }
-      std::vector<uint32_t> hdr{(uint32_t)units.size(), 0u};
+      std::vector<uint32_t> hdr{(uint32_t)units.size(), 0u, 0u, 0u};
       std::vector<uint8_t> buf;
       pack_add(buf, hdr);
       pack_add(buf, units);  // at offset 256
@@ -638,13 +921,16 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
       PJ_TRY(h2d(ctx, dunits.p, buf.data(), buf.size()));
       max_units = units.size();
     } else {
-      // The build side is partitioned on the auxiliary stream while the
-      // probe side is on the main one: the smaller side's passes alone
-      // leave HBM bandwidth unused (short grids, launch gaps).
+      // Schedule: the build side's histograms and pass 1 run on the
+      // auxiliary stream beside the probe side's histograms and scans on the
+      // main one (the smaller side's kernels alone leave HBM bandwidth
+      // unused); the probe side's pass 1 waits for the build side's pass 1
+      // and runs alone; the build side's pass 2 waits for the probe side's
+      // pass 1 and runs beside the probe side's pass 2.  Left free, the two
+      // sides' first passes shared the device in an order that changed from
+      // run to run, and the probe side's pass 1 took 4.1 to 6.6 ms.
       hipStream_t aux;
       PJ_TRY(aux_fork(ctx, &aux));
-      // (Starting the whole build side only after the probe side's
-      // histograms and scans measured the same.)
       const int b1 = cfg.bits1, b2 = cfg.bits2;
       int rc;
       {
@@ -653,14 +939,8 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
         if (rc == PJ_OK) rc = side_pass1(ctx, R, d_bk, d_bv, nb, b1, b2, "build");
       }
       if (rc == PJ_OK) rc = side_histograms(ctx, S, d_pk, np, b1, b2);
+      if (rc == PJ_OK) rc = aux_join(ctx);
       if (rc == PJ_OK) rc = side_pass1(ctx, S, d_pk, d_pv, np, b1, b2, "probe");
-      // The build side's last pass (one 1024-thread workgroup per CU) starts
-      // after the probe side's pass 1 and runs beside its pass 2.  Left
-      // free, it sometimes took the CUs that the build side's pass 1 left
-      // while the probe side's pass 1 still ran: that pass then took 4.9-6.9
-      // ms instead of 4.2 and the join 13.1 ms instead of 12.5.  (Making the
-      // probe side's pass 2 wait for the build side instead moves the
-      // contention into its pass 1 every time.)
       if (rc == PJ_OK) rc = aux_fork(ctx, &aux);
       if (rc == PJ_OK) {
         StreamSwap sw(ctx, aux);
@@ -690,14 +970,10 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
     }
   }
   if (max_units) {
-    // workgroups beyond the number of units exit at their first claim
-    const uint64_t g = (uint64_t)ctx->num_cus * geo.per_cu *
-                       (cfg.grid_mul > 0 ? cfg.grid_mul : 1);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(g, max_units);
-    uint8_t *du = dunits.as<uint8_t>();
-    PJ_TRY(launch_pjoin(ctx, geo.lds_log, B > 0, rk, rv, sk, sv,
-                        reinterpret_cast<const JoinUnit *>(du + 256),
-                        reinterpret_cast<uint32_t *>(du), grid, acc.as<unsigned long long>()));
+    if (bitmap_join_ok(B, cfg.lds_log))
+      PJ_TRY(ddeferred.alloc(ctx, max_units * sizeof(JoinUnit)));
+    PJ_TRY(launch_join(ctx, cfg, B, rk, rv, sk, sv, dunits.as<uint8_t>(),
+                       ddeferred.as<JoinUnit>(), max_units, acc.as<unsigned long long>()));
   }
   return join_acc_read(ctx, acc.as<unsigned long long>(), out);
 }
@@ -720,7 +996,9 @@ static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
 }
 
 int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
-  if (c.lds_log == 0) c.lds_log = 13;
+  // Large builds take the 8192-row tables (fan-out 2^7 in both passes and the
+  // wide bitmap join); smaller ones the 4096-row tables.
+  if (c.lds_log == 0) c.lds_log = nb > (1ULL << 26) ? 14 : 13;
   if (c.probe_unit == 0) c.probe_unit = 65536;
   if (c.lds_log < 11 || c.lds_log > 14) return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", c.lds_log);
   if (c.bits1 == 0 && c.bits2 == 0) derive_bits(c, nb);

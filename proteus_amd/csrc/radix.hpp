@@ -222,6 +222,16 @@ constexpr int kScanBlock = 256;
 __global__ void k_excl_scan(const uint32_t *__restrict__ T, int nb,
                             unsigned long long *__restrict__ S, uint32_t *__restrict__ ctr);
 
+// Pass-2 work items sized by the pass-1 digits' totals (one workgroup of
+// kScanBlock threads, 2^b1 <= kScanBlock): digit d1 of T1 tuples is cut into
+// ceil(G / cpg) items of cpg consecutive chunks, cpg = clamp(target * G /
+// T1, 1, G), so an item holds about `target` tuples however skewed the
+// digits are (a hot key's digit is cut into single chunks).
+// items[0 .. F1) = first item of each digit, items[F1] = the item count,
+// items[F1 + 1 + d1] = cpg of digit d1.
+__global__ void k_pass2_items(const uint32_t *__restrict__ T, int b1, int b2, int G,
+                              unsigned long long target, uint32_t *__restrict__ items);
+
 // One radix pass.  PASS2 = false: workgroup c partitions input chunk c on the
 // pass-1 digit.  PASS2 = true: workgroups take (d1, group of cpg chunks) items
 // of the pass-1 output from a counter and partition them on the pass-2 digit.
@@ -234,7 +244,8 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
     V2 *__restrict__ out_w, uint64_t n, uint64_t chunk, int G, int nb, int b2, uint32_t cpg,
     Joint joint,
     const uint32_t *__restrict__ H, const uint32_t *__restrict__ E,
-    const unsigned long long *__restrict__ S, uint32_t *__restrict__ seg_counter) {
+    const unsigned long long *__restrict__ S, uint32_t *__restrict__ seg_counter,
+    const uint32_t *__restrict__ items) {
   constexpr int IPT = T / BLOCK;
   static_assert(IPT % 4 == 0, "tile must give each thread a multiple of 4 items");
   static_assert(F <= BLOCK, "one thread per digit");
@@ -269,13 +280,19 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
   __shared__ uint32_t scan_tmp[BLOCK / 64];
   __shared__ uint32_t s_total_lines;
   __shared__ unsigned long long s_seg_lo, s_seg_hi;
-  __shared__ uint32_t s_seg;
+  __shared__ uint32_t s_d1, s_c0, s_c1, s_live;
+  // pass 2: the item table, read once (a claim then searches LDS, not HBM)
+  __shared__ uint32_t s_items[PASS2 ? 2 * 256 + 2 : 1];
 
   const int tid = threadIdx.x;
   const uint32_t F2 = 1u << b2;
   const uint32_t F1 = (uint32_t)nb >> b2;  // pass-1 digits
   // number of digits of this pass (<= F)
   const uint32_t ND = PASS2 ? F2 : F1;
+  if (PASS2 && items) {
+    for (uint32_t i = tid; i < 2 * F1 + 1; i += BLOCK) s_items[i] = items[i];
+    __syncthreads();
+  }
 
   for (;;) {
     // ---- pick the segment and set its cursors -------------------------------
@@ -299,15 +316,44 @@ __global__ __launch_bounds__(BLOCK) void k_radix_pass(
         hist[d] = 0;
       }
     } else {
-      // work item = (d1, group of `cpg` consecutive chunks): the runs those
-      // chunks wrote for d1 are adjacent, so the item is one contiguous range
-      if (tid == 0) s_seg = atomicAdd(seg_counter, 1u);
+      // work item = (d1, group of consecutive chunks): the runs those chunks
+      // wrote for d1 are adjacent, so the item is one contiguous range.  With
+      // `items` (k_pass2_items) digit d1 has its own group size, so items
+      // hold about the same number of tuples whatever the skew; without, every
+      // digit is cut into groups of `cpg` chunks.
+      if (tid == 0) {
+        const uint32_t seg = atomicAdd(seg_counter, 1u);
+        uint32_t d1 = 0, c0 = 0, c1 = 0, live = 0;
+        if (items) {
+          if (seg < s_items[F1]) {
+            uint32_t lo = 0, hi = F1 - 1;  // the last digit whose first item <= seg
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1) >> 1;
+              if (s_items[mid] <= seg) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t gs = s_items[F1 + 1 + lo];
+            d1 = lo;
+            c0 = (seg - s_items[lo]) * gs;
+            c1 = c0 + gs < (uint32_t)G ? c0 + gs : (uint32_t)G;
+            live = 1;
+          }
+        } else {
+          const uint32_t ngrp = ((uint32_t)G + cpg - 1) / cpg;
+          if (seg < ngrp * F1) {
+            d1 = seg / ngrp;
+            c0 = (seg % ngrp) * cpg;
+            c1 = c0 + cpg < (uint32_t)G ? c0 + cpg : (uint32_t)G;
+            live = 1;
+          }
+        }
+        s_d1 = d1;
+        s_c0 = c0;
+        s_c1 = c1;
+        s_live = live;
+      }
       __syncthreads();
-      const uint32_t seg = s_seg;
-      const uint32_t ngrp = ((uint32_t)G + cpg - 1) / cpg;
-      if (seg >= ngrp * F1) break;
-      const uint32_t d1 = seg / ngrp, g = seg % ngrp;
-      const uint32_t c0 = g * cpg, c1 = c0 + cpg < (uint32_t)G ? c0 + cpg : (uint32_t)G;
+      if (!s_live) break;
+      const uint32_t d1 = s_d1, c0 = s_c0, c1 = s_c1;
       const uint64_t row0 = (uint64_t)c0 * (uint64_t)nb + ((uint64_t)d1 << b2);
       const uint64_t row1 = (uint64_t)c1 * (uint64_t)nb + ((uint64_t)d1 << b2);
       if (tid < 64) {
@@ -559,7 +605,7 @@ RadixPlan radix_make_plan(pj_ctx *ctx, uint64_t n, int b1, int b2, int nb, int p
 
 // Device scratch of one side's histograms.
 struct RadixHist {
-  Scratch H, E, T, S, P, ctr;
+  Scratch H, E, T, S, P, ctr, items;
   std::vector<uint32_t> hT;            // partition sizes (host)
   std::vector<unsigned long long> hS;  // partition starts (host), nb + 1
   int slab = 0, nslab = 0;             // column-scan geometry
@@ -571,6 +617,7 @@ struct RadixHist {
     PJ_TRY(T.alloc(ctx, (uint64_t)p.nb * 4));
     PJ_TRY(S.alloc(ctx, (uint64_t)(p.nb + 1) * 8));
     PJ_TRY(ctr.alloc(ctx, 256));
+    PJ_TRY(items.alloc(ctx, (2 * 256 + 2) * 4));
     return PJ_OK;
   }
 };
@@ -604,6 +651,14 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
             h.slab, h.P.as<const uint32_t>(), h.E.as<uint32_t>(), h.T.as<uint32_t>());
   PJ_LAUNCH(ctx, "radix_scan", k_excl_scan, 1, kScanBlock, 0, h.T.as<const uint32_t>(), p.nb,
             h.S.as<unsigned long long>(), h.ctr.as<uint32_t>());
+  if (p.b2 > 0) {
+    // about 8 pass-2 items per CU (finer items measured slower: the per-item
+    // setup outweighs the shorter tail)
+    const unsigned long long target =
+        std::max<unsigned long long>(1, n / ((uint64_t)ctx->num_cus * 8));
+    PJ_LAUNCH(ctx, "radix_items", k_pass2_items, 1, kScanBlock, 0, h.T.as<const uint32_t>(),
+              p.b1, p.b2, p.G, target, h.items.as<uint32_t>());
+  }
   if (readback) PJ_TRY(radix_readback(ctx, p, h));
   return PJ_OK;
 }
@@ -617,24 +672,20 @@ int radix_launch_pass_v(pj_ctx *ctx, const char *name, bool pass2, const int32_t
   if (!pass2) {
     PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, ROWID, false, LINE, HASHKEYS, VEC>),
               p.G, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, 1u, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
-              h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
+              h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>(), nullptr);
   } else {
     if constexpr (!ROWID) {
       const uint32_t F1 = 1u << p.b1;
       constexpr uint64_t lds = (uint64_t)T * (4 + sizeof(V1) + (ValIO<V2>::present ? sizeof(V2) : 0)) +
                                (uint64_t)F * LINE * (4 + sizeof(V1) + (ValIO<V2>::present ? sizeof(V2) : 0));
+      (void)F1;
       const uint64_t per_cu = lds > 80000 ? 1 : 2;
-      uint64_t grid = (uint64_t)ctx->num_cus * per_cu;
-      // about 4 items per workgroup, each a group of consecutive chunks
-      uint64_t want = grid * 4;
-      uint32_t ngrp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(p.G, ceil_div(want, F1)));
-      uint32_t cpg = (uint32_t)ceil_div(p.G, ngrp);
-      ngrp = (uint32_t)ceil_div(p.G, cpg);
-      uint64_t nseg = (uint64_t)ngrp * F1;
-      if (grid > nseg) grid = nseg;
+      // items sized by k_pass2_items (radix_histograms); workgroups past the
+      // last item exit at their first claim
+      const uint64_t grid = (uint64_t)ctx->num_cus * per_cu;
       PJ_LAUNCH(ctx, name, (k_radix_pass<F, T, BLOCK, Joint, V1, V2, false, true, LINE, HASHKEYS, VEC>),
-                grid, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, cpg, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
-                h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>());
+                grid, BLOCK, 0, ik, iv, iw, ok, ov, ow, n, p.chunk, p.G, p.nb, p.b2, 1u, joint, h.H.as<const uint32_t>(), h.E.as<const uint32_t>(),
+                h.S.as<const unsigned long long>(), h.ctr.as<uint32_t>(), h.items.as<const uint32_t>());
     } else {
       return fail(PJ_E_ARG, "row-id values only in pass 1");
     }

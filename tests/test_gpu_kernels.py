@@ -51,7 +51,7 @@ def _join_inputs(seed, nr, ns, key_range, dup=False):
     if dup:
         rk = rs.integers(-key_range, key_range, nr).astype(np.int32)
     else:
-        rk = rs.permutation(np.arange(-nr // 2, nr - nr // 2)).astype(np.int32) * 3
+        rk = rs.permutation(np.arange(-(nr // 2), nr - nr // 2)).astype(np.int32) * 3
     rv = rs.integers(-2**31, 2**31, nr).astype(np.int32)
     sk = rs.integers(-key_range, key_range, ns).astype(np.int32)
     sv = rs.integers(-2**31, 2**31, ns).astype(np.int32)
