@@ -5,12 +5,11 @@
  * bench.py's cpu_baseline leg, to check results.  Never linked by the product.
  *
  * Reference files restated (paths relative to the reference repository):
- *   core/olap/lib/expressions/expressions-hasher.cpp:39-84,560-580
+ *   core/olap/lib/expressions/expressions-hasher.cpp:39-58
  *   core/olap/lib/operators/hash-join-chained.cpp:233-319,321-566,568-625
  *   core/olap/lib/operators/hash-join-chained-morsel.cpp:27-129
  *   core/olap/lib/operators/hash-group-by-chained.cpp:70-143,193-210,267-633
  *   core/olap/lib/operators/reduce-opt.cpp:52-430, agg-t.cpp:28
- *   core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained-helpers.cpp:49-56
  *   core/olap/lib/operators/gpu/gpu-partitioned-hash-join-chained.cpp:595-610
  *   core/olap/lib/operators/gpu/gpu-hash-rearrange.cpp:338-348
  *   core/olap/lib/operators/bloom-filter/bloom-filter.cpp:36-107,
@@ -39,47 +38,6 @@ uint64_t or_hash_i32(int32_t k) {
   h *= 0xc2b2ae35ULL;
   h ^= h >> 16;
   return h;
-}
-
-/* hashInt64 (expressions-hasher.cpp:66-84): murmur3 fmix64. */
-uint64_t or_hash_i64(int64_t k) {
-  uint64_t h = (uint64_t)k;
-  h ^= h >> 33;
-  h *= 0xff51afd7ed558ccdULL;
-  h ^= h >> 33;
-  h *= 0xc4ceb9fe1a85ec53ULL;
-  h ^= h >> 33;
-  return h;
-}
-
-/* RecordConstruction hash (expressions-hasher.cpp:560-580): boost-style
- * combine, seed starts at 0:  seed = (h + 0x9e3779b9 + (seed<<6) + (seed>>2))
- * ^ seed, per attribute in order. */
-uint64_t or_hash_combine(const uint64_t *hv, int n) {
-  uint64_t seed = 0;
-  for (int i = 0; i < n; ++i) {
-    uint64_t h = hv[i];
-    h += 0x9e3779b9ULL;
-    h += seed << 6;
-    h += seed >> 2;
-    h ^= seed;
-    seed = h;
-  }
-  return seed;
-}
-
-/* hashd (gpu-partitioned-hash-join-chained-helpers.cpp:49-56): 32-bit fmix on
- * a SIGNED int, so the shifts are arithmetic; products wrap mod 2^32. */
-int32_t or_hashd(int32_t v) {
-  uint32_t u;
-  v = (v >> 16) ^ v;
-  u = (uint32_t)v * 0x85ebca6bU;
-  v = (int32_t)u;
-  v = (v >> 13) ^ v;
-  u = (uint32_t)v * 0xc2b2ae35U;
-  v = (int32_t)u;
-  v = (v >> 16) ^ v;
-  return v;
 }
 
 static inline uint32_t fmix32u(uint32_t h) {

@@ -7,8 +7,14 @@
  *
  * Every function cites the reference file:line whose behaviour it restates.
  * Parity pins (see tests/test_oracle.py):
- *   - or_hash_i32 / or_hash_i64 / or_hashd: murmur3 fmix known answers,
- *     64-bit vs 32-bit low-bit divergence, sign-extension cases.
+ *   - or_phjc_bucket (the 32-bit murmur3 finaliser): scikit-learn's
+ *     murmurhash3_32 known answers.
+ *   - or_hash_i32 (hashInt32, 64-bit arithmetic): PARITY UNPINNED.  The
+ *     reference holds no vectors for it (its hashing test asserts only true,
+ *     tests/olap/deprecated/tests-hashing.cpp:78-110) and no library routine
+ *     computes it; only special cases (0, sign extension) are checked.  No
+ *     checked result depends on it: counts, checksums and per-group
+ *     aggregates do not depend on bucket layout.
  *   - or_join_chained(_mt): reference sailors goldens (74,383,10), closed
  *     forms (COUNT = |S| for a permutation build), dense direct-address
  *     oracle, brute force on tiny inputs.
@@ -31,9 +37,6 @@ extern "C" {
 
 /* ---- hashing ---------------------------------------------------------- */
 uint64_t or_hash_i32(int32_t k);
-uint64_t or_hash_i64(int64_t k);
-uint64_t or_hash_combine(const uint64_t *hv, int n);
-int32_t or_hashd(int32_t v);
 uint32_t or_phjc_bucket(int32_t k, int log_parts, int hash_bits);
 uint32_t or_rearrange_bucket(int32_t k, uint32_t buckets);
 

@@ -57,12 +57,6 @@ def _setup(L):
     L.or_bloom_build.argtypes = [vp, u64, vp, u64]
     L.or_bloom_probe.restype = u64
     L.or_bloom_probe.argtypes = [vp, u64, vp, u64, vp]
-    L.or_hash_i64.restype = u64
-    L.or_hash_i64.argtypes = [i64]
-    L.or_hash_combine.restype = u64
-    L.or_hash_combine.argtypes = [vp, C.c_int]
-    L.or_hashd.restype = i32
-    L.or_hashd.argtypes = [i32]
     L.or_phjc_bucket.restype = C.c_uint32
     L.or_phjc_bucket.argtypes = [i32, C.c_int, C.c_int]
     L.or_rearrange_bucket.restype = C.c_uint32
@@ -103,19 +97,6 @@ def _threads():
 # ---- hashing ---------------------------------------------------------------
 def hash_i32(k):
     return lib().or_hash_i32(int(np.int32(k)))
-
-
-def hash_i64(k):
-    return lib().or_hash_i64(int(np.int64(k)))
-
-
-def hash_combine(hv):
-    a = np.ascontiguousarray(hv, dtype=np.uint64)
-    return lib().or_hash_combine(_p(a), len(a))
-
-
-def hashd(v):
-    return lib().or_hashd(int(np.int32(v)))
 
 
 def phjc_bucket(k, log_parts, hash_bits):

@@ -896,7 +896,6 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
   Scratch acc;
   PJ_TRY(acc.alloc(ctx, 64));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
-  const Geometry geo = geometry_for(cfg.lds_log);
   Side R, S;
   Scratch dunits;  // unit header (kUnitHdrWords x u32, 256 B), then the units
   Scratch ddeferred;  // units the bitmap join left to the CSR join
@@ -913,7 +912,8 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
         u.s_len = (uint32_t)std::min<uint64_t>(cfg.probe_unit, np - s0);
         units.push_back(u);
       }
-      std::vector<uint32_t> hdr{(uint32_t)units.size(), 0u, 0u, 0u};
+      std::vector<uint32_t> hdr(kUnitHdrWords, 0u);
+      hdr[0] = (uint32_t)units.size();
       std::vector<uint8_t> buf;
       pack_add(buf, hdr);
       pack_add(buf, units);  // at offset 256

@@ -1119,6 +1119,17 @@ class Exec {
       }
       aggs.push_back(a);
     }
+    Col kout_lds;
+    std::vector<Col> aout_lds;
+    uint64_t ng_lds = 0;
+    if (groupby_lds(in, keys, max_groups, aggs, kout_lds, aout_lds, &ng_lds)) {
+      Rel out;
+      out.n = ng_lds;
+      out.cols.emplace_back(knames[0], kout_lds);
+      for (size_t i = 0; i < aggs.size(); ++i)
+        out.cols.emplace_back(aggs[i].name, finish_agg(aout_lds[i], aggs[i].mult, aggs[i].t, ng_lds));
+      return out;
+    }
     if (keys.size() > (size_t)R::kGbMaxKeys) raise(PJ_E_UNSUPPORTED, "group-by over more than 4 keys");
     if (aggs.size() > (size_t)R::kGbMaxAggs) raise(PJ_E_UNSUPPORTED, "group-by with more than 16 aggregates");
     R::GbKey gk[R::kGbMaxKeys];
@@ -1150,31 +1161,112 @@ class Exec {
     Rel out;
     out.n = ng;
     for (size_t i = 0; i < keys.size(); ++i) out.cols.emplace_back(knames[i], kout[i]);
-    for (size_t i = 0; i < aggs.size(); ++i) {
-      // accumulators are int64 / FP64: scale counts, narrow to the expression type
-      Col c = aout[i];
-      const AggIn &a = aggs[i];
-      if (a.mult != 1 || a.t != c.t) {
-        Rel tmp;
-        tmp.n = ng;
-        tmp.cols.emplace_back(Attr{"", "v"}, c);
-        Compiler cc(*this, tmp);
-        cc.push(R::OP_COL, c.t, cc.bind(c), 0);
-        if (a.mult != 1) {
-          cc.push(R::OP_CONST, R::I64, 0, a.mult);
-          cc.push(R::OP_MUL, R::I64, 0, 0);
-        }
-        if (a.t != c.t) cc.push(R::OP_CAST, a.t, c.t, 0);
-        Col o;
-        o.t = a.t;
-        o.owner = alloc(std::max<uint64_t>(ng, 1) * R::col_size(a.t));
-        o.dev = o.owner->p;
-        if (ng) ck(R::eval(ctx_, cc.p, ng, o.owner->p, a.t));
-        c = o;
-      }
-      out.cols.emplace_back(a.name, c);
-    }
+    for (size_t i = 0; i < aggs.size(); ++i)
+      out.cols.emplace_back(aggs[i].name, finish_agg(aout[i], aggs[i].mult, aggs[i].t, ng));
     return out;
+  }
+
+  // Accumulators are int64 / FP64: scale counts by `mult` (a sum of a
+  // constant), narrow to the expression type t.
+  Col finish_agg(Col c, int64_t mult, int t, uint64_t ng) {
+    if (mult == 1 && t == c.t) return c;
+    Rel tmp;
+    tmp.n = ng;
+    tmp.cols.emplace_back(Attr{"", "v"}, c);
+    Compiler cc(*this, tmp);
+    cc.push(R::OP_COL, c.t, cc.bind(c), 0);
+    if (mult != 1) {
+      cc.push(R::OP_CONST, R::I64, 0, mult);
+      cc.push(R::OP_MUL, R::I64, 0, 0);
+    }
+    if (t != c.t) cc.push(R::OP_CAST, t, c.t, 0);
+    Col o;
+    o.t = t;
+    o.owner = alloc(std::max<uint64_t>(ng, 1) * R::col_size(t));
+    o.dev = o.owner->p;
+    if (ng) ck(R::eval(ctx_, cc.p, ng, o.owner->p, t));
+    return o;
+  }
+
+  // The LDS group-by (pj_groupby_i32: per-workgroup LDS tables, radix
+  // partitioning above one table's groups, no global atomics) for the shape
+  // it computes: one int32 key; COUNT; SUM / MIN / MAX of one integer column;
+  // SUM of one float32 column.  Returns false (nothing run) for any other
+  // group-by, which the general kernel takes.
+  template <class AggIn>
+  bool groupby_lds(const Rel &in, const std::vector<Col> &keys, uint64_t max_groups,
+                   const std::vector<AggIn> &aggs, Col &kout, std::vector<Col> &aout,
+                   uint64_t *n_groups) {
+    if (getenv("PROTEUS_AMD_NO_LDS_GROUPBY")) return false;
+    if (keys.size() != 1 || keys[0].t != R::I32 || aggs.empty()) return false;
+    const void *vcol = nullptr, *wcol = nullptr;
+    int vt = -1;
+    enum { K_CNT, K_SUM, K_MIN, K_MAX, K_FSUM };
+    std::vector<int> kind;
+    for (const AggIn &a : aggs) {
+      if (a.m == R::M_CNT) {
+        kind.push_back(K_CNT);
+        continue;
+      }
+      const bool integer = a.col.t == R::I32 || a.col.t == R::I64;
+      if (integer && (a.m == R::M_SUM || a.m == R::M_MIN || a.m == R::M_MAX)) {
+        if (vcol && vcol != a.col.dev) return false;  // one value column only
+        vcol = a.col.dev;
+        vt = a.col.t;
+        kind.push_back(a.m == R::M_SUM ? K_SUM : a.m == R::M_MIN ? K_MIN : K_MAX);
+      } else if (a.col.t == R::F32 && a.m == R::M_SUM) {
+        if (wcol && wcol != a.col.dev) return false;
+        wcol = a.col.dev;
+        kind.push_back(K_FSUM);
+      } else {
+        return false;
+      }
+    }
+    const uint64_t cap = std::min(max_groups, std::max<uint64_t>(in.n, 1));
+    // int32 values are widened to int64 (the kernel's value column)
+    Col v64;
+    if (vcol && vt == R::I32) {
+      Rel tmp;
+      tmp.n = in.n;
+      Col c;
+      c.t = R::I32;
+      c.dev = vcol;
+      tmp.cols.emplace_back(Attr{"", "v"}, c);
+      Compiler cc(*this, tmp);
+      cc.push(R::OP_COL, R::I32, cc.bind(c), 0);
+      cc.push(R::OP_CAST, R::I64, R::I32, 0);
+      v64.t = R::I64;
+      v64.owner = alloc(std::max<uint64_t>(in.n, 1) * 8);
+      v64.dev = v64.owner->p;
+      if (in.n) ck(R::eval(ctx_, cc.p, in.n, v64.owner->p, R::I64));
+      vcol = v64.dev;
+    }
+    kout.t = R::I32;
+    kout.dict = keys[0].dict;
+    kout.owner = alloc(cap * 4);
+    kout.dev = kout.owner->p;
+    BufPtr outs[5];
+    void *ptr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int k : kind)
+      if (!outs[k]) {
+        outs[k] = alloc(cap * 8);
+        ptr[k] = outs[k]->p;
+      }
+    *n_groups = 0;
+    if (in.n)
+      ck(pj_groupby_i32(ctx_, (const int32_t *)keys[0].dev, (const int64_t *)vcol,
+                        (const float *)wcol, in.n, 0, cap, (int32_t *)kout.owner->p,
+                        (int64_t *)ptr[K_CNT], (int64_t *)ptr[K_SUM], (int64_t *)ptr[K_MIN],
+                        (int64_t *)ptr[K_MAX], (double *)ptr[K_FSUM], n_groups));
+    aout.clear();
+    for (int k : kind) {
+      Col c;
+      c.t = k == K_FSUM ? R::F64 : R::I64;
+      c.owner = outs[k];
+      c.dev = outs[k]->p;
+      aout.push_back(c);
+    }
+    return true;
   }
 
   // ---- joins ---------------------------------------------------------

@@ -101,21 +101,6 @@ class Star:
     def plan(self, sort=True, bloom_size=0):
         return plan(sort, bloom_size)
 
-    def oracle(self):
-        """numpy result (host copies of the fact columns)."""
-        f = {k: v.numpy().astype(np.int64) for k, v in self.fact.items()}
-        keep = ((1992 + f["fk_date"] // 365) == YEAR) & (f["fk_supp"] % 5 == REGION) & \
-               (f["fk_part"] % 25 == CATEGORY)
-        nation = f["fk_supp"][keep] % 25
-        city = f["fk_cust"][keep] % 160
-        rev = f["price"][keep] * (100 - f["discount"][keep])
-        key = nation * 160 + city
-        tot = np.zeros(25 * 160, np.int64)
-        cnt = np.zeros(25 * 160, np.int64)
-        np.add.at(tot, key, rev)
-        np.add.at(cnt, key, 1)
-        return [[int(k // 160), int(k % 160), int(tot[k])] for k in np.nonzero(cnt)[0]]
-
     def close(self):
         for v in list(self.fact.values()) + self.dim_dev:
             v.free()

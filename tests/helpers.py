@@ -65,3 +65,28 @@ def brute_groupby(k, v, w):
             e[3] = max(e[3], vi)
             e[4] += wi
     return out
+
+
+def star_expected(star_obj, slice_rows=1 << 26):
+    """The star-schema query's result (rows [s_nation, c_city, revenue]) from
+    the fact columns of a proteus_amd.star.Star, computed with numpy in
+    slices of slice_rows rows copied from the device, so 6e8-row facts fit
+    in host memory.  Dimension attributes are the closed forms of their row
+    ids (star.dims())."""
+    import proteus_amd as pa
+    from proteus_amd import star as st
+    tot = np.zeros(25 * 160, np.int64)
+    cnt = np.zeros(25 * 160, np.int64)
+    n = star_obj.n
+    for lo in range(0, n, slice_rows):
+        m = min(slice_rows, n - lo)
+        f = {name: pa.DeviceArray(star_obj.ctx, d.ptr + lo * 4, m, np.int32).numpy()
+             for name, d in star_obj.fact.items()}
+        keep = ((1992 + f["fk_date"] // 365) == st.YEAR) & (f["fk_supp"] % 5 == st.REGION) & \
+               (f["fk_part"] % 25 == st.CATEGORY)
+        g = {name: c[keep].astype(np.int64) for name, c in f.items()}
+        key = (g["fk_supp"] % 25) * 160 + g["fk_cust"] % 160
+        rev = g["price"] * (100 - g["discount"])
+        np.add.at(tot, key, rev)
+        np.add.at(cnt, key, 1)
+    return [[int(k // 160), int(k % 160), int(tot[k])] for k in np.nonzero(cnt)[0]]

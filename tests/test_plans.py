@@ -311,3 +311,89 @@ def test_reduce_with_predicate(ctx):
     hit = (sk < nr) & (sv > (1 << 30))
     assert got == [int(hit.sum()), int(rv[pos[sk[hit]]].astype(np.int64).sum())]
     ctx.clear_catalog()
+
+
+def _gb_plan(ctx, n, groups, dist, seed):
+    """A groupby plan over registered device columns C3(k int32, v int64,
+    w float32): COUNT, SUM/MIN/MAX(v), SUM(w) per k; returns the plan text,
+    the host columns and the device arrays."""
+    if dist == "uniform":
+        k = ctx.gen_uniform_i32(n, seed, 4, 0, groups)
+    else:
+        k = ctx.gen_zipf_i32(n, seed, 4, groups, 0.8, seed)
+    v = ctx.gen_uniform_i64(n, seed, 5, 0, 1 << 20)
+    w = ctx.gen_unit_f32(n, seed, 6)
+    ctx.clear_catalog()
+    ctx.register_column("C3", "k", "int", k)
+    ctx.register_column("C3", "v", "int64", v)
+    ctx.register_column("C3", "w", "float32", w)
+    b = rb.RelBuilder(ctx)
+    q = b.scan("C3", ["k", "v", "w"]).to_gpu().unpack() \
+         .groupby([rb.col("C3", "k")],
+                  [(rb.as_(rb.int64(1), "g", "cnt"), "sum"),
+                   (rb.as_(rb.col("C3", "v"), "g", "s"), "sum"),
+                   (rb.as_(rb.col("C3", "v"), "g", "mn"), "min"),
+                   (rb.as_(rb.col("C3", "v"), "g", "mx"), "max"),
+                   (rb.as_(rb.col("C3", "w"), "g", "fs"), "sum")],
+                  max(1, (groups - 1).bit_length()), groups)
+    return json.dumps(q.plan()), (k, v, w)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("groups,dist", [(1 << 10, "uniform"), (1 << 16, "zipf")])
+def test_groupby_plan_runs_lds_kernel(ctx, groups, dist):
+    """A one-int32-key groupby plan with COUNT / SUM / MIN / MAX of an int64
+    column and the FP64 SUM of a float32 column runs the LDS group-by (no
+    global-atomic kernel) and equals the dense oracle: integers bit-exact,
+    FP64 sums exact for the u/2^24 inputs (1e-9 relative written as the
+    bound)."""
+    n = 1 << 20
+    text, (k, v, w) = _gb_plan(ctx, n, groups, dist, 31)
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    out = ctx.execute_plan(text)
+    lds = ctx.prof_get("gb_agg")[1]
+    generic = ctx.prof_get("rel_groupby")[1]
+    ctx.prof_enable(False)
+    assert lds > 0 and generic == 0
+    rows = [ln.split(",") for ln in out.splitlines() if ln]
+    rc, d = O.groupby_dense(k.numpy(), v.numpy(), w.numpy(), groups)
+    assert rc == O.OR_OK
+    present = np.nonzero(d["cnt"])[0]
+    assert len(rows) == len(present)
+    got = {int(r[0]): r[1:] for r in rows}
+    for g in present:
+        r = got[int(g)]
+        assert [int(x) for x in r[:4]] == [int(d["cnt"][g]), int(d["sum"][g]), int(d["min"][g]),
+                                           int(d["max"][g])]
+        assert abs(float(r[4]) - d["fsum"][g]) <= 1e-9 * abs(d["fsum"][g])
+    for x in (k, v, w):
+        x.free()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_groupby_plan_falls_back_to_general_kernel(ctx):
+    """Shapes the LDS group-by does not compute (two integer value columns,
+    an FP64 value, two keys) run the general kernel with the same results as
+    the LDS path on the shared aggregates."""
+    n, groups = 1 << 16, 1 << 8
+    text, (k, v, w) = _gb_plan(ctx, n, groups, "uniform", 7)
+    lds_rows = sorted(ln for ln in ctx.execute_plan(text).splitlines() if ln)
+    import os as _os
+    _os.environ["PROTEUS_AMD_NO_LDS_GROUPBY"] = "1"
+    try:
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        gen_rows = sorted(ln for ln in ctx.execute_plan(text).splitlines() if ln)
+        assert ctx.prof_get("rel_groupby")[1] > 0 and ctx.prof_get("gb_agg")[1] == 0
+        ctx.prof_enable(False)
+    finally:
+        del _os.environ["PROTEUS_AMD_NO_LDS_GROUPBY"]
+    assert [r.split(",")[:5] for r in gen_rows] == [r.split(",")[:5] for r in lds_rows]
+    for a, b in zip(gen_rows, lds_rows):
+        fa, fb = float(a.split(",")[5]), float(b.split(",")[5])
+        assert abs(fa - fb) <= 1e-9 * abs(fb)
+    for x in (k, v, w):
+        x.free()
+    ctx.clear_catalog()

@@ -9,6 +9,7 @@ import pytest
 
 from proteus_amd import relbuilder as rb
 from proteus_amd import star
+from tests import helpers as H
 
 
 def _rows(text):
@@ -20,7 +21,7 @@ def _rows(text):
 def test_star_plan_matches_oracle(ctx, pipeline):
     ctx.clear_catalog()
     s = star.Star(ctx, 1 << 23, seed=3)
-    want = s.oracle()
+    want = H.star_expected(s)
     assert len(want) == 800
     old = os.environ.get("PROTEUS_AMD_NO_PIPELINE")
     try:
