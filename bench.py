@@ -12,6 +12,7 @@ Prints ONE JSON line.  Only one GPU is used; --gpus N > 1 exits with a message.
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -79,7 +80,7 @@ def main():
 
     for _ in range(a.warmup):
         res = ctx.join_partitioned_reduce(rk, rv, sk, sv)
-    sec, kt, res = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
+    sec, kt, res, per = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
     # per-launch algorithmic bytes: what an ideal kernel must move through HBM
     algo = {
         "radix_scatter1_probe": 16.0 * NS, "radix_scatter2_probe": 16.0 * NS,
@@ -121,6 +122,9 @@ def main():
         "e2e_roofline": {"algorithmic_bytes": 40.0 * (NR + NS), "achieved_gbs": e2e_gbs,
                          "frac": e2e_gbs / HBM_PEAK_GBS},
         "input_gtuples_per_s": (NR + NS) / sec / 1e9,
+        "step_ms": {"median": statistics.median(per) * 1e3, "min": min(per) * 1e3,
+                    "max": max(per) * 1e3},
+        "protocol": protocol(ctx, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv)),
         "result": res,
         "kernels_ms_per_step": {n: round(v[0] / a.steps, 4) for n, v in kt.items()},
         "device": info,
@@ -140,17 +144,26 @@ def main():
         sk.free()
         sk = ctx.gen_zipf_fk_i32(NS, SEED, 2, NR, RANGE, 1.0, SEED)
         ctx.join_partitioned_reduce(rk, rv, sk, sv)
-        zsec, _, zres = timed(ctx, a.steps, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
-        line["variants"] = {"zipf_theta1": {"ms_per_step": zsec * 1e3,
-                                            "value": NS / zsec / 1e9, "result": zres}}
+        zsec, zkt, zres, _ = timed(ctx, a.steps,
+                                   lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv))
+        line["variants"] = {"zipf_theta1": {
+            "ms_per_step": zsec * 1e3, "value": NS / zsec / 1e9, "vs_uniform": zsec / sec,
+            "protocol": protocol(ctx, lambda: ctx.join_partitioned_reduce(rk, rv, sk, sv)),
+            "kernels_ms_per_step": {n: round(v[0] / a.steps, 4) for n, v in zkt.items()},
+            "result": zres}}
     if not a.no_extra:
+        for x in (rk, rv, sk, sv):
+            x.free()
+        rk = rv = sk = sv = None
+        line["c1"] = chained_c1(ctx, a.steps)
         line["groupby"] = groupby_c3(ctx, a.steps)
         line["star"] = star_c4(ctx, a.steps)
         line["sort"] = sort_plan(ctx, a.steps)
         line["out_of_gpu_join"] = out_of_gpu(ctx, 2)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
-            x.free()
+            if x is not None:
+                x.free()
         line["batched"] = batched_mix(ctx, a.batched_queries, a.qps)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.variant)
@@ -158,15 +171,56 @@ def main():
     ctx.close()
 
 
+def chained_c1(ctx, steps):
+    """The plain equi-join on the chained path (hashjoin-chained -> reduce,
+    pj_join_chained_reduce): R = 2^20 permuted int32 keys with int32 payload,
+    S = 2^24 fk uniform over R's keys; COUNT = 2^24 (closed form) and the
+    int64 checksum.  hash_bits = 21 and maxBuildInputSize = 1.1 * |R| (the
+    RelBuilder defaults, relbuilder.cpp:653-668).  Algorithmic bytes: 8 per
+    input tuple (each column read once), 142.6 MB."""
+    nr, ns = 1 << 20, 1 << 24
+    rk = ctx.gen_perm_i32(nr, SEED)
+    rv = ctx.gen_uniform_i32(nr, SEED, 1, 0, 1 << 31)
+    sk = ctx.gen_uniform_i32(ns, SEED, 2, 0, nr)
+    sv = ctx.gen_uniform_i32(ns, SEED, 3, 0, 1 << 31)
+    maxb = int(1.1 * nr)
+
+    def run():
+        return ctx.join_chained_reduce(rk, rv, sk, sv, 21, maxb)
+    for _ in range(3):
+        res = run()
+    names = ["join_chained_build", "join_chained_probe", "fill_u32", "join_acc_init"]
+    sec, kt, res, per = timed(ctx, max(steps, 10), run, names)
+    nsteps = max(steps, 10)
+    algo = 8.0 * (nr + ns)
+    probe_ms = kt["join_chained_probe"][0] / kt["join_chained_probe"][1]
+    out = {"build_rows": nr, "probe_rows": ns, "hash_bits": 21, "ms": sec * 1e3,
+           "step_ms_median": statistics.median(per) * 1e3,
+           "input_gtuples_per_s": (nr + ns) / sec / 1e9, "probe_gtuples_per_s": ns / sec / 1e9,
+           "roofline": {"bound": "hbm", "algorithmic_bytes": algo, "achieved_gbs": algo / sec / 1e9,
+                        "frac": algo / sec / 1e9 / HBM_PEAK_GBS,
+                        "probe_kernel_ms": probe_ms},
+           "kernels_ms_per_step": {n: round(v[0] / nsteps, 4) for n, v in kt.items()},
+           "protocol": protocol(ctx, run), "result": res, "count_is_closed_form": res["count"] == ns}
+    for x in (rk, rv, sk, sv):
+        x.free()
+    return out
+
+
 def groupby_c3(ctx, steps):
     """Hash group-by of 2^29 rows (int32 key, int64 v uniform [0, 2^20),
     float32 w = u/2^24): COUNT, SUM, MIN, MAX of v and FP64 SUM of w per
-    group, for 2^10 / 2^16 / 2^22 groups, uniform and Zipf(0.8) keys.
-    Algorithmic bytes: 16 per row read + 44 per group written.  Checked by
-    conservation: the counts sum to the row count and the SUMs to SUM(v)."""
+    group, for 2^10 / 2^16 / 2^22 groups, uniform and Zipf(0.8) keys; each
+    timed through the C ABI (pj_groupby_i32) and as a plan through
+    pj_plan_execute (scan -> groupby -> reduce(COUNT(*), SUM(cnt)), the
+    group-by lowered to the same LDS kernels; the reduce keeps the printed
+    result small).  Algorithmic bytes: 16 per row read + 44 per group
+    written.  Checked by conservation: the counts sum to the row count and
+    the SUMs to SUM(v); the plan's row and count totals must agree."""
     import ctypes as C
     import numpy as np
     import proteus_amd as pa
+    from proteus_amd import relbuilder as rb
     lib = pa.lib()
     n = 1 << 29
     v = ctx.gen_uniform_i64(n, SEED, 5, 0, 1 << 20)
@@ -189,21 +243,41 @@ def groupby_c3(ctx, steps):
                                              outs[1].ptr, outs[2].ptr, outs[3].ptr, outs[4].ptr,
                                              outs[5].ptr, C.byref(ng)))
             run()
-            ctx.sync()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                run()
-            ctx.sync()
-            sec = (time.perf_counter() - t0) / steps
+            names = ["radix_hist", "radix_colsum", "radix_colscan", "radix_scan", "radix_items",
+                     "gb_scatter1", "gb_scatter2", "gb_agg", "gb_merge", "gb_compact"]
+            sec, kt, _, _ = timed(ctx, steps, run, names)
             m = ng.value
             cnt = outs[1].numpy()[:m]
             sums = outs[2].numpy()[:m]
             ok = int(cnt.sum()) == n and int(sums.sum()) == vsum
+            # the same group-by as a plan
+            ctx.clear_catalog()
+            ctx.register_column("C3", "k", "int", k)
+            ctx.register_column("C3", "v", "int64", v)
+            ctx.register_column("C3", "w", "float32", w)
+            q = rb.RelBuilder(ctx).scan("C3", ["k", "v", "w"]).to_gpu().unpack() \
+                .groupby([rb.col("C3", "k")],
+                         [(rb.as_(rb.int64(1), "g", "cnt"), "sum"),
+                          (rb.as_(rb.col("C3", "v"), "g", "s"), "sum"),
+                          (rb.as_(rb.col("C3", "v"), "g", "mn"), "min"),
+                          (rb.as_(rb.col("C3", "v"), "g", "mx"), "max"),
+                          (rb.as_(rb.col("C3", "w"), "g", "fs"), "sum")], hb, g) \
+                .reduce([rb.int64(1), rb.col("g", "cnt"), rb.col("g", "s")],
+                        ["sum", "sum", "sum"])
+            text = json.dumps(q.plan())
+            res = [int(x) for x in ctx.execute_plan(text).strip().split(",")]
+            psec, pkt, _, _ = timed(ctx, steps, lambda: ctx.execute_plan(text),
+                                    ["gb_agg", "rel_groupby"])
+            ctx.clear_catalog()
             gbs = (16.0 * n + 44.0 * m) / sec / 1e9
-            out["configs"].append({"groups": g, "keys": dist, "ms": sec * 1e3,
-                                   "rows_per_s": n / sec, "groups_out": m,
-                                   "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
-                                   "conservation_ok": ok})
+            out["configs"].append({
+                "groups": g, "keys": dist, "ms": sec * 1e3, "rows_per_s": n / sec,
+                "groups_out": m, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                "conservation_ok": ok,
+                "kernels_ms_per_call": {x: round(t / steps, 4) for x, (t, c) in kt.items()},
+                "plan_ms": psec * 1e3, "plan_vs_direct": psec / sec,
+                "plan_kernels": {x: c for x, (t, c) in pkt.items()},
+                "plan_result_ok": res == [m, n, vsum]})
             k.free()
         for o in outs:
             o.free()
@@ -343,6 +417,9 @@ def batched_mix(ctx, nq, qps):
         res[mode] = mix.results()
         out[name + "_qps"] = nq / st["wall_s"]
         out[name + "_probe_rows_per_s"] = mix.probe_rows() / st["wall_s"]
+        # every probe row is generated in the query's service time (12 B
+        # written) and read by the probe (12 B): 24 B of HBM traffic per row
+        out[name + "_hbm_gbs"] = 24.0 * mix.probe_rows() / st["wall_s"] / 1e9
         if mode == 0:
             out["batches"] = st["batches"]
     out["speedup"] = out["batched_qps"] / out["plain_qps"]
@@ -378,23 +455,45 @@ def batched_mix(ctx, nq, qps):
     return out
 
 
-def timed(ctx, steps, fn):
-    """Runs fn `steps` times between two device synchronisations; returns the
-    mean seconds per step, the per-kernel (ms, launches) recorded with HIP
-    events on the context's stream, and the last result."""
-    names = ["radix_hist", "radix_colscan", "radix_scan", "radix_scatter1_build",
-             "radix_scatter1_probe", "radix_scatter2_build", "radix_scatter2_probe",
-             "pjoin_probe", "join_acc_init"]
+JOIN_KERNELS = ["radix_hist", "radix_colsum", "radix_colscan", "radix_scan", "radix_items",
+                "radix_scatter1_build", "radix_scatter1_probe", "radix_scatter2_build",
+                "radix_scatter2_probe", "pjoin_units", "pjoin_probe", "join_acc_init"]
+
+
+def timed(ctx, steps, fn, names=JOIN_KERNELS):
+    """Runs fn `steps` times between two device synchronisations (every call
+    returns after its results are on the host); returns the mean seconds per
+    step, the per-kernel (ms, launches) recorded with HIP events on the
+    streams the kernels run on, the last result and the per-step seconds."""
     ctx.prof_enable(True)
     ctx.prof_reset()
     ctx.sync()
+    per = []
     t0 = time.perf_counter()
     for _ in range(steps):
+        a = time.perf_counter()
         res = fn()
+        per.append(time.perf_counter() - a)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.prof_enable(False)
-    return (t1 - t0) / steps, kernel_table(ctx, names), res
+    return (t1 - t0) / steps, kernel_table(ctx, names), res, per
+
+
+def protocol(ctx, fn, runs=5):
+    """The reference's perftest protocol (benchmark-aliases.hpp:60-67,
+    query-perf.cpp:32-45): one warm-up run, then `runs` timed runs; the
+    median milliseconds."""
+    fn()
+    ctx.sync()
+    ts = []
+    for _ in range(runs):
+        a = time.perf_counter()
+        fn()
+        ctx.sync()
+        ts.append((time.perf_counter() - a) * 1e3)
+    return {"warmup": 1, "runs": runs, "median_ms": statistics.median(ts),
+            "min_ms": min(ts), "max_ms": max(ts)}
 
 
 def pmc_traffic(kernel, variant):
@@ -411,11 +510,58 @@ def pmc_traffic(kernel, variant):
     return None
 
 
+def cpu_info():
+    """CPU model and core counts of this host."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads():
+    """Worker threads for the CPU baseline: this process's CPU share on the
+    GPU box (OMP_NUM_THREADS, set to the share there), else every core it
+    may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_protocol(fn, runs=5):
+    """1 warm-up + `runs` timed runs, median seconds (the reference's perftest
+    protocol, benchmark-aliases.hpp:60-67)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        a = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - a)
+    return statistics.median(ts)
+
+
 def cpu_baseline(variant):
-    """The oracle's morsel-parallel chained join (restating the reference's CPU
-    HashJoinChainedMorsel) on a 1/4-scale sample of the same workload."""
+    """The oracle's restatement of the reference's CPU operators, timed on
+    this host's cores with the same seeded inputs as the GPU runs:
+      - C2 (the headline's workload) at 1/4 scale, |R| = 2^25, |S| = 2^28, on
+        `threads` workers: the morsel-parallel chained hash join
+        (HashJoinChainedMorsel: one shared table, 64-bit hashInt32,
+        hash_bits = 26 as RelBuilder picks it) -> COUNT, SUM;
+      - C1 at full size (2^20 x 2^24) on `threads` workers and on 1;
+      - C3 at 1/16 scale (2^25 rows, 2^16 uniform groups) on `threads`
+        workers: per-worker chained group-by tables merged per key.
+    Each: 1 warm-up + 5 runs, median; input generation excluded."""
     from oracle import pyoracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
+    out = {"kind": "port", "cores": threads, "unit": "Gtuples/s"}
+    out.update(cpu_info())
+    # C2 at 1/4 scale
     nr, ns = NR >> 2, NS >> 2
     rng = int(1.33 * nr)
     rk = O.gen_perm_i32(nr, SEED)
@@ -425,15 +571,49 @@ def cpu_baseline(variant):
     else:
         sk = O.gen_zipf_fk_i32(ns, SEED, 2, nr, rng, 1.0, SEED)
     sv = O.gen_uniform_i32(ns, SEED, 3, 0, 1 << 31)
-    hb = 26  # ceil(log2(2^25)) + 1, the RelBuilder default
-    t0 = time.perf_counter()
-    rc, out = O.join_chained(rk, rv, sk, sv, hb, max_build=nr, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": ns / dt / 1e9, "unit": "Gtuples/s", "cores": threads, "kind": "port",
-            "sample": "|R|=2^25, |S|=2^28 (1/4 scale, same distributions), chained hash join "
-                      "(morsel-parallel shared table, hash_bits=26), build + probe + reduce, "
-                      "%.2f s" % dt,
-            "result_count": out["count"]}
+    res = {}
+
+    def c2():
+        res["c2"] = O.join_chained(rk, rv, sk, sv, 26, max_build=nr, threads=threads)[1]
+    t = cpu_protocol(c2)
+    out["value"] = ns / t / 1e9
+    out["sample"] = ("C2 at 1/4 scale: |R| = 2^25, |S| = 2^28 (same distributions), chained hash "
+                     "join (morsel-parallel shared table, hash_bits = 26) -> COUNT, SUM on %d "
+                     "threads; median of 5 runs after 1 warm-up, %.2f s per run" % (threads, t))
+    out["c2_quarter"] = {"ms": t * 1e3, "probe_gtuples_per_s": ns / t / 1e9,
+                         "input_gtuples_per_s": (nr + ns) / t / 1e9,
+                         "count": res["c2"]["count"]}
+    del rk, rv, sk, sv
+    # C1 at full size, all threads and one
+    nr, ns = 1 << 20, 1 << 24
+    rk = O.gen_perm_i32(nr, SEED)
+    rv = O.gen_uniform_i32(nr, SEED, 1, 0, 1 << 31)
+    sk = O.gen_uniform_i32(ns, SEED, 2, 0, nr)
+    sv = O.gen_uniform_i32(ns, SEED, 3, 0, 1 << 31)
+    c1 = {}
+    for th in (threads, 1):
+        def run(th=th):
+            res["c1"] = O.join_chained(rk, rv, sk, sv, 21, max_build=int(1.1 * nr),
+                                       threads=th)[1]
+        t = cpu_protocol(run)
+        c1["threads_%d" % th] = {"ms": t * 1e3, "probe_gtuples_per_s": ns / t / 1e9,
+                                 "input_gtuples_per_s": (nr + ns) / t / 1e9}
+    c1["count"] = res["c1"]["count"]
+    out["c1_full"] = c1
+    out["threads_1"] = c1["threads_1"]
+    del rk, rv, sk, sv
+    # C3 at 1/16 scale, 2^16 uniform groups
+    n, g = 1 << 25, 1 << 16
+    k = O.gen_uniform_i32(n, SEED, 4, 0, g)
+    v = O.gen_uniform_i64(n, SEED, 5, 0, 1 << 20)
+    w = O.gen_unit_f32(n, SEED, 6)
+
+    def c3():
+        res["c3"] = O.groupby_chained(k, v, w, 17, g, threads=threads)[1]
+    t = cpu_protocol(c3)
+    out["c3_sixteenth"] = {"rows": n, "groups": g, "ms": t * 1e3, "rows_per_s": n / t,
+                           "groups_out": len(res["c3"]["keys"])}
+    return out
 
 
 if __name__ == "__main__":

@@ -123,7 +123,7 @@ int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
  * in the reference: it is for callers that cannot allocate device memory
  * (code under hipGraph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, results and errors, but the table and the
- * accumulator live in a static 32 MiB + 256 B device array of this library
+ * accumulator live in a static 32 MiB + 8 KiB device array of this library
  * (one per GPU, held until the process ends) instead of the context pool:
  * the table (max(2^hash_bits, 2 * next_pow2(nb)) 8-byte slots) must fit
  * 2^22 slots, i.e. nb <= 2^21 and hash_bits <= 22, else PJ_E_CAPACITY.

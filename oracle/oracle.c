@@ -655,6 +655,123 @@ int or_groupby_chained(const int32_t *k, const int64_t *v, const float *w,
   return rc;
 }
 
+/* The same group-by with nthreads workers, as the reference runs a CPU
+ * group-by under a router: every worker builds its own chained table over a
+ * contiguous slice of the rows (or_groupby_chained), then one pass merges the
+ * partial groups in worker order into the output table with the monoids'
+ * combine (COUNT and SUM add, MIN / MAX compare, FP64 SUM adds).  Output
+ * order: first appearance in the concatenation of the partials. */
+typedef struct {
+  const int32_t *k;
+  const int64_t *v;
+  const float *w;
+  uint64_t lo, hi, max_groups;
+  int hash_bits, rc;
+  or_groups part;
+  uint64_t ng;
+} gbmt_arg;
+static void *gbmt_worker(void *p) {
+  gbmt_arg *a = (gbmt_arg *)p;
+  a->rc = or_groupby_chained(a->k + a->lo, a->v ? a->v + a->lo : NULL, a->w ? a->w + a->lo : NULL,
+                             a->hi - a->lo, a->hash_bits, a->max_groups, &a->part, &a->ng);
+  return NULL;
+}
+int or_groupby_chained_mt(const int32_t *k, const int64_t *v, const float *w, uint64_t n,
+                          int hash_bits, uint64_t max_groups, int nthreads, or_groups *out,
+                          uint64_t *ngroups) {
+  *ngroups = 0;
+  if (hash_bits < 0 || hash_bits > 32) return OR_E_ARG;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 64) nthreads = 64;
+  const uint64_t cap = max_groups ? max_groups : 1;
+  gbmt_arg args[64];
+  pthread_t th[64];
+  int rc = OR_OK;
+  const uint64_t per = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    gbmt_arg *a = &args[t];
+    memset(a, 0, sizeof(*a));
+    a->k = k;
+    a->v = v;
+    a->w = w;
+    a->lo = per * t < n ? per * t : n;
+    a->hi = per * (t + 1) < n ? per * (t + 1) : n;
+    a->max_groups = max_groups;
+    a->hash_bits = hash_bits;
+    a->part.keys = (int32_t *)malloc(cap * 4);
+    a->part.cnt = (int64_t *)malloc(cap * 8);
+    a->part.sum = (int64_t *)malloc(cap * 8);
+    a->part.min = (int64_t *)malloc(cap * 8);
+    a->part.max = (int64_t *)malloc(cap * 8);
+    a->part.fsum = (double *)malloc(cap * 8);
+    if (!a->part.keys || !a->part.cnt || !a->part.sum || !a->part.min || !a->part.max ||
+        !a->part.fsum)
+      rc = OR_E_OOM;
+  }
+  if (rc == OR_OK) {
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, gbmt_worker, &args[t]);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    for (int t = 0; t < nthreads; ++t)
+      if (args[t].rc != OR_OK) rc = args[t].rc;
+  }
+  if (rc == OR_OK) {  // merge the partial tables
+    const uint64_t hsz = 1ULL << hash_bits, mask = hsz - 1;
+    uint32_t *head = (uint32_t *)malloc(hsz * 4);
+    uint32_t *next = (uint32_t *)malloc(cap * 4);
+    if (!head || !next) rc = OR_E_OOM;
+    if (rc == OR_OK) memset(head, 0xff, hsz * 4);
+    uint64_t cnt = 0;
+    for (int t = 0; t < nthreads && rc == OR_OK; ++t) {
+      const or_groups *p = &args[t].part;
+      for (uint64_t i = 0; i < args[t].ng; ++i) {
+        const int32_t key = p->keys[i];
+        const uint64_t h = or_hash_i32(key) & mask;
+        uint32_t cur = head[h], last = 0xffffffffU;
+        while (cur != 0xffffffffU && out->keys[cur] != key) {
+          last = cur;
+          cur = next[cur];
+        }
+        if (cur == 0xffffffffU) {
+          if (cnt >= max_groups) {
+            rc = OR_E_CAPACITY;
+            break;
+          }
+          cur = (uint32_t)cnt++;
+          next[cur] = 0xffffffffU;
+          if (last == 0xffffffffU)
+            head[h] = cur;
+          else
+            next[last] = cur;
+          out->keys[cur] = key;
+          out->cnt[cur] = p->cnt[i];
+          out->sum[cur] = p->sum[i];
+          out->min[cur] = p->min[i];
+          out->max[cur] = p->max[i];
+          out->fsum[cur] = p->fsum[i];
+        } else {
+          out->cnt[cur] += p->cnt[i];
+          out->sum[cur] = (int64_t)((uint64_t)out->sum[cur] + (uint64_t)p->sum[i]);
+          if (p->min[i] < out->min[cur]) out->min[cur] = p->min[i];
+          if (p->max[i] > out->max[cur]) out->max[cur] = p->max[i];
+          out->fsum[cur] += p->fsum[i];
+        }
+      }
+    }
+    *ngroups = cnt;
+    free(head);
+    free(next);
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    free(args[t].part.keys);
+    free(args[t].part.cnt);
+    free(args[t].part.sum);
+    free(args[t].part.min);
+    free(args[t].part.max);
+    free(args[t].part.fsum);
+  }
+  return rc;
+}
+
 /* Dense-array oracle for keys in [0, g): per-key COUNT/SUM/MIN/MAX exact and
  * the float sum accumulated as an exact int64 count of 2^-24 units when every
  * w is a multiple of 2^-24 below 1 (the generator's contract); otherwise a

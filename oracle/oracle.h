@@ -102,6 +102,11 @@ typedef struct {
 int or_groupby_chained(const int32_t *k, const int64_t *v, const float *w,
                        uint64_t n, int hash_bits, uint64_t max_groups,
                        or_groups *out, uint64_t *ngroups);
+/* or_groupby_chained over nthreads slices, partial tables merged per key
+ * (the reference's per-worker group-by and final merge). */
+int or_groupby_chained_mt(const int32_t *k, const int64_t *v, const float *w, uint64_t n,
+                          int hash_bits, uint64_t max_groups, int nthreads, or_groups *out,
+                          uint64_t *ngroups);
 int or_groupby_dense(const int32_t *k, const int64_t *v, const float *w,
                      uint64_t n, uint64_t g, int nthreads, or_groups *out);
 

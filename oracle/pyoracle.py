@@ -85,6 +85,8 @@ def _setup(L):
                                 C.POINTER(u64)]
     L.or_groupby_chained.argtypes = [vp, vp, vp, u64, C.c_int, u64, C.POINTER(Groups),
                                      C.POINTER(u64)]
+    L.or_groupby_chained_mt.argtypes = [vp, vp, vp, u64, C.c_int, u64, C.c_int,
+                                        C.POINTER(Groups), C.POINTER(u64)]
     L.or_groupby_dense.argtypes = [vp, vp, vp, u64, u64, C.c_int, C.POINTER(Groups)]
     L.or_reduce_i64.argtypes = [vp, u64, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
     L.or_histogram.argtypes = [vp, u64, C.c_int, C.c_int, vp]
@@ -215,14 +217,18 @@ def _alloc_groups(g):
     return arrs, st
 
 
-def groupby_chained(k, v, w, hash_bits, max_groups):
+def groupby_chained(k, v, w, hash_bits, max_groups, threads=1):
     k = _i32(k)
     v = None if v is None else np.ascontiguousarray(v, np.int64)
     w = None if w is None else np.ascontiguousarray(w, np.float32)
     arrs, st = _alloc_groups(max(max_groups, 1))
     ng = C.c_uint64(0)
-    rc = lib().or_groupby_chained(_p(k), _p(v), _p(w), len(k), hash_bits, max_groups,
-                                  C.byref(st), C.byref(ng))
+    if threads == 1:
+        rc = lib().or_groupby_chained(_p(k), _p(v), _p(w), len(k), hash_bits, max_groups,
+                                      C.byref(st), C.byref(ng))
+    else:
+        rc = lib().or_groupby_chained_mt(_p(k), _p(v), _p(w), len(k), hash_bits, max_groups,
+                                         threads, C.byref(st), C.byref(ng))
     n = ng.value
     return rc, {name: a[:n] for name, a in arrs.items()}
 

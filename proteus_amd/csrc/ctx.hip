@@ -238,30 +238,41 @@ __global__ void k_copy1(const uint8_t *__restrict__ src, uint8_t *__restrict__ d
 
 // join accumulator: 7 x u64 in device memory -------------------------------
 __global__ void k_join_acc_init(unsigned long long *g) {
-  if (threadIdx.x < 7) {
+  for (int i = threadIdx.x; i < kJoinAccWords; i += blockDim.x) {
+    const int s = i % kJoinAccStride;
     unsigned long long v = 0;
-    if (threadIdx.x == 3 || threadIdx.x == 5) v = (unsigned long long)INT64_MAX;
-    if (threadIdx.x == 4 || threadIdx.x == 6) v = (unsigned long long)INT64_MIN;
-    g[threadIdx.x] = v;
+    if (s == 3 || s == 5) v = (unsigned long long)INT64_MAX;
+    if (s == 4 || s == 6) v = (unsigned long long)INT64_MIN;
+    g[i] = v;
   }
 }
 
 int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc) {
-  PJ_LAUNCH(ctx, "join_acc_init", k_join_acc_init, 1, 64, 0, d_acc);
+  PJ_LAUNCH(ctx, "join_acc_init", k_join_acc_init, 1, 256, 0, d_acc);
   return PJ_OK;
 }
 
 int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
                   pj_join_agg *out) {
-  int64_t h[7];
+  int64_t h[kJoinAccWords];
   PJ_TRY(d2h(ctx, h, d_acc, sizeof(h)));
-  out->count = h[0];
-  out->sum_b = h[1];
-  out->sum_p = h[2];
-  out->min_b = h[3];
-  out->max_b = h[4];
-  out->min_p = h[5];
-  out->max_p = h[6];
+  out->count = 0;
+  out->sum_b = 0;
+  out->sum_p = 0;
+  out->min_b = INT64_MAX;
+  out->max_b = INT64_MIN;
+  out->min_p = INT64_MAX;
+  out->max_p = INT64_MIN;
+  for (int c = 0; c < kJoinAccCopies; ++c) {
+    const int64_t *x = h + c * kJoinAccStride;
+    out->count = (int64_t)((uint64_t)out->count + (uint64_t)x[0]);
+    out->sum_b = (int64_t)((uint64_t)out->sum_b + (uint64_t)x[1]);
+    out->sum_p = (int64_t)((uint64_t)out->sum_p + (uint64_t)x[2]);
+    out->min_b = x[3] < out->min_b ? x[3] : out->min_b;
+    out->max_b = x[4] > out->max_b ? x[4] : out->max_b;
+    out->min_p = x[5] < out->min_p ? x[5] : out->min_p;
+    out->max_p = x[6] > out->max_p ? x[6] : out->max_p;
+  }
   return PJ_OK;
 }
 

@@ -61,14 +61,19 @@ __global__ __launch_bounds__(256) void k_chained_probe_reduce(
       v[u] = (live[u] && pv) ? pv[j] : 0;
       h[u] = hash_i32(k[u]) & mask;
     }
+    // the four rows' first slots are loaded together, then each row walks
+    // its run of occupied slots
+    unsigned long long first[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) first[u] = slots[h[u]];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (!live[u]) continue;
       int64_t m = 0, sb = 0;
       int32_t mnb = INT32_MAX, mxb = INT32_MIN;
       uint64_t p = h[u];
+      unsigned long long w = first[u];
       while (true) {
-        unsigned long long w = slots[p];
         if (w == 0ULL) break;
         if ((int32_t)(uint32_t)(w >> 32) == k[u]) {
           uint64_t row = (w & 0xffffffffULL) - 1;
@@ -79,6 +84,7 @@ __global__ __launch_bounds__(256) void k_chained_probe_reduce(
           mxb = max(mxb, b);
         }
         p = (p + 1) & mask;
+        w = slots[p];
       }
       acc.add(m, sb, mnb, mxb, v[u]);
     }
@@ -180,7 +186,7 @@ static uint64_t probe_grid(pj_ctx *ctx, uint64_t rows_per_thread, uint64_t np) {
 // hipGetSymbolAddress returns on that device, never through the array's name.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kNoAllocSlots = 1ULL << 22;  // 32 MiB of 64-bit slots
-constexpr uint64_t kNoAllocAccWords = 32;       // 256 B: the 7-word accumulator
+constexpr uint64_t kNoAllocAccWords = kJoinAccWords;  // 8 KiB: the accumulator copies
 static __device__ __align__(256) unsigned long long scratch[kNoAllocSlots + kNoAllocAccWords];
 
 constexpr int kMaxDevices = 64;
@@ -265,7 +271,7 @@ extern "C" int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
   Scratch tab, acc;
   uint64_t mask = 0;
   PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask));
-  PJ_TRY(acc.alloc(ctx, 64));
+  PJ_TRY(acc.alloc(ctx, kJoinAccWords * 8));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
   if (np && nb)
     PJ_LAUNCH(ctx, "join_chained_probe", k_chained_probe_reduce, probe_grid(ctx, 4, np), 256,

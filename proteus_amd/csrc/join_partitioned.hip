@@ -894,7 +894,7 @@ static int join_partitioned(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_b
                             uint64_t np, pj_radix_cfg cfg, pj_join_agg *out) {
   const int B = cfg.bits1 + cfg.bits2;
   Scratch acc;
-  PJ_TRY(acc.alloc(ctx, 64));
+  PJ_TRY(acc.alloc(ctx, kJoinAccWords * 8));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
   Side R, S;
   Scratch dunits;  // unit header (kUnitHdrWords x u32, 256 B), then the units

@@ -359,30 +359,61 @@ struct JoinAcc {
   }
 };
 
-// Wave-reduce a JoinAcc and fold lane 0's result into the 7-slot global
-// accumulator (count, sum_b, sum_p as u64 adds; min/max as signed atomics).
-// With no match at all the identities are INT64_MAX / INT64_MIN.
+// The global join accumulator: kJoinAccCopies copies of 7 slots (count,
+// sum_b, sum_p as u64 adds; min_b, max_b, min_p, max_p as signed min/max),
+// each copy on its own 128-byte line.  Workgroup b folds into copy
+// b % kJoinAccCopies and join_acc_read combines the copies: same-address
+// global atomics serialise (about 160 ns apart on MI355X), so a single copy
+// made the flush of a large grid cost milliseconds.
+constexpr int kJoinAccCopies = 64;
+constexpr int kJoinAccStride = 16;  // u64 words per copy
+constexpr int kJoinAccWords = kJoinAccCopies * kJoinAccStride;
+
+// Reduce a JoinAcc over the workgroup (every thread of the block calls it,
+// at most 1024 threads) and fold the result into the block's accumulator
+// copy with one set of atomics.  With no match at all the identities are
+// INT64_MAX / INT64_MIN.
 __device__ __forceinline__ void join_acc_flush(JoinAcc a, unsigned long long *g) {
+  __shared__ long long red[16][7];
   const int64_t cnt = a.count + (int64_t)a.n1;
-  int64_t c = wave_sum_i64(cnt);
-  int64_t sb = wave_sum_i64(a.sum_b);
-  int64_t sp = wave_sum_i64(a.sum_p);
   const bool any = cnt != 0;
-  int64_t mnb = wave_min_i64(any ? (int64_t)a.min_b : INT64_MAX);
-  int64_t mxb = wave_max_i64(any ? (int64_t)a.max_b : INT64_MIN);
-  int64_t mnp = wave_min_i64(any ? (int64_t)a.min_p : INT64_MAX);
-  int64_t mxp = wave_max_i64(any ? (int64_t)a.max_p : INT64_MIN);
-  if ((threadIdx.x & 63) == 0 && c != 0) {
-    atomicAdd(&g[0], (unsigned long long)c);
-    atomicAdd(&g[1], (unsigned long long)sb);
-    atomicAdd(&g[2], (unsigned long long)sp);
-    atomicMin((long long *)&g[3], (long long)mnb);
-    atomicMax((long long *)&g[4], (long long)mxb);
-    atomicMin((long long *)&g[5], (long long)mnp);
-    atomicMax((long long *)&g[6], (long long)mxp);
+  int64_t v[7];
+  v[0] = wave_sum_i64(cnt);
+  v[1] = wave_sum_i64(a.sum_b);
+  v[2] = wave_sum_i64(a.sum_p);
+  v[3] = wave_min_i64(any ? (int64_t)a.min_b : INT64_MAX);
+  v[4] = wave_max_i64(any ? (int64_t)a.max_b : INT64_MIN);
+  v[5] = wave_min_i64(any ? (int64_t)a.min_p : INT64_MAX);
+  v[6] = wave_max_i64(any ? (int64_t)a.max_p : INT64_MIN);
+  const int wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) red[wid][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      v[0] += red[w][0];
+      v[1] += red[w][1];
+      v[2] += red[w][2];
+      v[3] = red[w][3] < v[3] ? red[w][3] : v[3];
+      v[4] = red[w][4] > v[4] ? red[w][4] : v[4];
+      v[5] = red[w][5] < v[5] ? red[w][5] : v[5];
+      v[6] = red[w][6] > v[6] ? red[w][6] : v[6];
+    }
+    if (v[0] != 0) {
+      unsigned long long *c = g + (blockIdx.x % kJoinAccCopies) * kJoinAccStride;
+      atomicAdd(&c[0], (unsigned long long)v[0]);
+      atomicAdd(&c[1], (unsigned long long)v[1]);
+      atomicAdd(&c[2], (unsigned long long)v[2]);
+      atomicMin((long long *)&c[3], (long long)v[3]);
+      atomicMax((long long *)&c[4], (long long)v[4]);
+      atomicMin((long long *)&c[5], (long long)v[5]);
+      atomicMax((long long *)&c[6], (long long)v[6]);
+    }
   }
 }
 
+// d_acc holds kJoinAccWords u64 (kJoinAccWords * 8 bytes).
 int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc);
 // Resolves the allocation-free entry points' static scratch on `device`
 // (which must be current); pj_ctx_create calls it.

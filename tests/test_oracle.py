@@ -234,3 +234,24 @@ def test_histogram_conservation():
     k = O.gen_uniform_i32(1 << 15, 1, 1, -2**31, 1 << 32)
     h = O.histogram(k, 32 - 7, 7)
     assert h.sum() == len(k)
+
+
+def test_groupby_chained_mt_equals_single_thread():
+    """The per-worker chained group-by with its merge equals the
+    single-threaded one (as a set of groups) and the dense oracle."""
+    n, g = 200003, 1 << 12
+    k = O.gen_zipf_i32(n, 5, 1, g, 0.8, 2)
+    v = O.gen_uniform_i64(n, 5, 2, -(1 << 40), 1 << 41)
+    w = O.gen_unit_f32(n, 5, 3)
+    rc1, one = O.groupby_chained(k, v, w, 12, g)
+    rc8, mt = O.groupby_chained(k, v, w, 12, g, threads=8)
+    rcd, dense = O.groupby_dense(k, v, w, g)
+    assert rc1 == rc8 == rcd == O.OR_OK
+    o1, o8 = np.argsort(one["keys"]), np.argsort(mt["keys"])
+    for name in ("keys", "cnt", "sum", "min", "max"):
+        assert np.array_equal(one[name][o1], mt[name][o8]), name
+    keys = mt["keys"][o8]
+    for name in ("cnt", "sum", "min", "max", "fsum"):
+        assert np.array_equal(mt[name][o8], dense[name][keys]), name
+    rc, _ = O.groupby_chained(k, v, w, 12, 100, threads=4)
+    assert rc == O.OR_E_CAPACITY
