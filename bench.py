@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary measurements (Zipf fk variant)")
+    ap.add_argument("--blocks", default="c1,groupby,star,sort,oog",
+                    help="secondary measurements to run (comma-separated subset of "
+                         "c1,groupby,star,sort,oog)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched / served query mix")
     ap.add_argument("--batched-queries", type=int, default=4096)
@@ -155,11 +158,17 @@ def main():
         for x in (rk, rv, sk, sv):
             x.free()
         rk = rv = sk = sv = None
-        line["c1"] = chained_c1(ctx, a.steps)
-        line["groupby"] = groupby_c3(ctx, a.steps)
-        line["star"] = star_c4(ctx, a.steps)
-        line["sort"] = sort_plan(ctx, a.steps)
-        line["out_of_gpu_join"] = out_of_gpu(ctx, 2)
+        blocks = set(a.blocks.split(","))
+        if "c1" in blocks:
+            line["c1"] = chained_c1(ctx, a.steps)
+        if "groupby" in blocks:
+            line["groupby"] = groupby_c3(ctx, a.steps)
+        if "star" in blocks:
+            line["star"] = star_c4(ctx, a.steps)
+        if "sort" in blocks:
+            line["sort"] = sort_plan(ctx, a.steps)
+        if "oog" in blocks:
+            line["out_of_gpu_join"] = out_of_gpu(ctx, 2)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
             if x is not None:

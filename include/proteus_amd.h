@@ -175,8 +175,10 @@ int pj_join_partitioned_reduce(pj_ctx *ctx, const int32_t *d_bk,
  * fit hbm_budget bytes (0: half of HBM), through a pinned host arena, and
  * each bucket pair is joined by the radix-partitioned join; the aggregates
  * are those of pj_join_partitioned_reduce over the whole inputs.
- * num_buckets = 0 picks the fewest buckets that fit the budget; a bucket
- * that still does not fit (skewed keys) returns PJ_E_CAPACITY.
+ * num_buckets = 0 picks the fewest buckets that fit the budget.  At most 256
+ * buckets (one device bucketing pass): num_buckets > 256 returns PJ_E_ARG,
+ * and a budget that would need more than 256 buckets returns PJ_E_CAPACITY,
+ * as does a bucket that still does not fit (skewed keys).
  * h_bv / h_pv may be NULL (value 0). */
 int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk,
                               const int32_t *h_bv, uint64_t nb,
@@ -292,8 +294,10 @@ int pj_batch_autotune(pj_ctx *ctx, const pj_query *qs, uint32_t nq, pj_batch_cfg
  * takes only light queries so they never wait behind a heavy batch); mode
  * 1 runs one query at a time with pj_query_run in arrival order.  Probe
  * inputs of queries with gen_seed != 0 and d_fk == NULL are placed in
- * arenas allocated once per call.  done_s[i] = completion time of query i
- * (seconds after the start). */
+ * arenas allocated once per call; their pointers are reset to NULL when the
+ * call returns (caller-supplied columns are left as given).  done_s[i] =
+ * completion time of query i (seconds after the start), -1 for a query not
+ * run.  The reference has no serving loop: this is the added pattern. */
 typedef struct {
   int mode;                /* 0 batched, 1 one query per launch              */
   int slots;               /* streams in flight (mode 0), 1..8               */
@@ -305,6 +309,10 @@ typedef struct {
   uint64_t max_batch_rows; /* probe rows per batch: a batch takes no further  */
                            /* query that would pass it (0: 2^26); its first   */
                            /* query always fits                              */
+  const volatile int32_t *cancel; /* NULL, or a host flag: once non-zero, no  */
+                           /* query is admitted, batches on the device finish, */
+                           /* queries not run keep done_s = -1 and the call  */
+                           /* returns PJ_E_CANCELLED                         */
 } pj_serve_cfg;
 typedef struct {
   uint64_t batches;

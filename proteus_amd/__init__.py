@@ -72,7 +72,7 @@ class BatchCfg(C.Structure):
 class ServeCfg(C.Structure):
     _fields_ = [("mode", C.c_int), ("slots", C.c_int), ("max_batch", C.c_uint32),
                 ("hbm_budget", C.c_uint64), ("heavy_rows", C.c_uint64), ("batch", BatchCfg),
-                ("max_batch_rows", C.c_uint64)]
+                ("max_batch_rows", C.c_uint64), ("cancel", C.c_void_p)]
 
 
 class ServeStats(C.Structure):
@@ -493,15 +493,21 @@ class Context:
         return best.as_dict()
 
     def serve_run(self, queries, arrivals, mode=0, slots=3, max_batch=4096, hbm_budget=0,
-                  tile_rows=0, wg_per_cu=0, heavy_rows=0, lds_dim=0, max_batch_rows=0):
-        """Runs the serving loop; returns (completion seconds per query, stats)."""
+                  tile_rows=0, wg_per_cu=0, heavy_rows=0, lds_dim=0, max_batch_rows=0,
+                  cancel=None):
+        """Runs the serving loop; returns (completion seconds per query, -1 for a
+        query not run, and stats).  cancel: a ctypes.c_int32 the caller may set
+        from another thread (the call then raises PJ_E_CANCELLED after the
+        batches on the device finish)."""
         n = len(queries)
         arr = (Query * n)(*queries)
         arrv = (C.c_double * n)(*arrivals)
         done = (C.c_double * n)()
         cfg = ServeCfg(mode, slots, max_batch, hbm_budget, heavy_rows,
-                       BatchCfg(tile_rows, wg_per_cu, lds_dim), max_batch_rows)
+                       BatchCfg(tile_rows, wg_per_cu, lds_dim), max_batch_rows,
+                       C.addressof(cancel) if cancel is not None else None)
         st = ServeStats()
+        self.last_done, self.last_queries = done, arr
         _check(_lib.pj_serve_run(self.h, arr, arrv, n, C.byref(cfg), done, C.byref(st)))
         return list(done), {"batches": st.batches, "max_in_flight_bytes": st.max_in_flight_bytes,
                             "wall_s": st.wall_s}

@@ -49,6 +49,7 @@ int dalloc(pj_ctx *ctx, uint64_t bytes, void **p) {
   if (it != pool.free_blocks.end() && it->first <= sz * 2) {
     *p = it->second.p;
     uint64_t got = it->first;
+    pool.lru.erase(it->second.stamp);
     pool.free_blocks.erase(it);
     pool.cached -= got;
     pool.live[*p] = got;
@@ -64,6 +65,7 @@ int dalloc(pj_ctx *ctx, uint64_t bytes, void **p) {
     // release cached blocks and retry once
     for (auto &kv : pool.free_blocks) (void)hipFree(kv.second.p);
     pool.free_blocks.clear();
+    pool.lru.clear();
     pool.cached = 0;
     (void)hipGetLastError();
     e = hipMalloc(p, sz);
@@ -88,21 +90,23 @@ int dfree(pj_ctx *ctx, void *p) {
   uint64_t sz = it->second;
   pool.live.erase(it);
   pool.in_use -= sz;
-  pool.free_blocks.emplace(sz, Pool::Cached{p, ++pool.clock});
+  const uint64_t stamp = ++pool.clock;
+  auto ins = pool.free_blocks.emplace(sz, Pool::Cached{p, stamp});
+  pool.lru.emplace(stamp, ins);
   pool.cached += sz;
   // Keep the cache within a quarter of HBM by dropping the least recently
   // freed blocks: the blocks of an earlier workload go first, and a workload
-  // whose own blocks fit stops evicting (every hipFree waits for the whole
-  // device, which would serialise overlapping batches).
+  // whose own blocks fit stops evicting.  A cached block may still be read
+  // by work queued on any of the context's streams (main, auxiliary, batch
+  // slots, the out-of-GPU copy stream); hipFree waits for all work on the
+  // device before it returns the memory, which is what makes the release
+  // safe -- and what makes every eviction a device-wide wait.
   const uint64_t cap = std::max<uint64_t>(8ULL << 30, ctx->hbm_bytes / 4);
-  while (pool.cached > cap && !pool.free_blocks.empty()) {
-    auto old = pool.free_blocks.begin();
-    for (auto j = pool.free_blocks.begin(); j != pool.free_blocks.end(); ++j)
-      if (j->second.stamp < old->second.stamp) old = j;
-    // all work on this context is stream-ordered; free after the stream drains
-    (void)hipStreamSynchronize(ctx->stream);
+  while (pool.cached > cap && !pool.lru.empty()) {
+    auto old = pool.lru.begin()->second;
     (void)hipFree(old->second.p);
     pool.cached -= old->first;
+    pool.lru.erase(pool.lru.begin());
     pool.free_blocks.erase(old);
     ++pool.evictions;
   }

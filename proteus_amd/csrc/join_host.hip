@@ -140,12 +140,23 @@ extern "C" int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk, const
   // a bucket pair in HBM takes its 8 B per tuple plus the partitioned join's
   // scratch, about 3x in all; buckets sized for half that, for skew
   constexpr uint64_t kJoinBytes = 24;
+  // The device bucketing is one radix pass (pj_hash_rearrange_i32), which
+  // takes at most kMaxBuckets buckets.
+  constexpr uint32_t kMaxBuckets = 256;
+  if (num_buckets > kMaxBuckets)
+    return fail(PJ_E_ARG, "out-of-GPU join with %u buckets: at most %u", num_buckets, kMaxBuckets);
   uint32_t buckets = num_buckets;
   if (!buckets) {
     const uint64_t need = (nb + np) * kJoinBytes * 2;
-    buckets = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, ceil_div(need, hbm_budget)), 1u << 16);
+    const uint64_t want = std::max<uint64_t>(1, ceil_div(need, hbm_budget));
+    if (want > kMaxBuckets)
+      return fail(PJ_E_CAPACITY,
+                  "out-of-GPU join: %llu + %llu tuples need %llu buckets under an HBM budget of "
+                  "%llu bytes, more than the %u one bucketing pass makes (raise the budget)",
+                  (unsigned long long)nb, (unsigned long long)np, (unsigned long long)want,
+                  (unsigned long long)hbm_budget, kMaxBuckets);
+    buckets = (uint32_t)want;
   }
-  if (buckets > (1u << 16)) return fail(PJ_E_ARG, "out-of-GPU join with %u buckets (limit 65536)", buckets);
   // chunks of the bucketing phase: 24 B per tuple in HBM (input and two
   // output sets), several chunks per side so that copies overlap
   const uint64_t chunk = std::max<uint64_t>(1 << 16, hbm_budget / 64);

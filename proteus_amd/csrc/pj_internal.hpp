@@ -129,6 +129,8 @@ struct Pool {
     uint64_t stamp;  // clock when the block was freed
   };
   std::multimap<uint64_t, Cached> free_blocks;  // size -> block
+  // the cached blocks by free stamp, oldest first (the eviction order)
+  std::map<uint64_t, std::multimap<uint64_t, Cached>::iterator> lru;
   std::unordered_map<void *, uint64_t> live;    // ptr -> size
 };
 

@@ -145,12 +145,12 @@ struct ValIO<NoVal> {
 
 // Per-chunk joint histogram: chunk c = [c*chunk, min(n, (c+1)*chunk)) into
 // row c of H (nb u32 bins, dynamic LDS).  chunk is a multiple of 4.
-template <class Joint>
-__global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__ keys,
+template <class Joint, int HB>
+__global__ __launch_bounds__(HB) void k_chunk_hist(const int32_t *__restrict__ keys,
                                                      uint64_t n, uint64_t chunk, int nb,
                                                      Joint joint, uint32_t *__restrict__ H) {
   extern __shared__ uint32_t hsh[];
-  for (int i = threadIdx.x; i < nb; i += 1024) hsh[i] = 0;
+  for (int i = threadIdx.x; i < nb; i += HB) hsh[i] = 0;
   __syncthreads();
   const uint64_t lo = (uint64_t)blockIdx.x * chunk;
   const uint64_t hi = lo + chunk < n ? lo + chunk : n;
@@ -163,9 +163,9 @@ __global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__
     constexpr int U = 4;
     const int4 *kv = reinterpret_cast<const int4 *>(keys + lo);
     const uint64_t last = nvec - 1;
-    const uint64_t nsets = (nvec + U * 1024 - 1) / (U * 1024);
+    const uint64_t nsets = (nvec + U * HB - 1) / (U * HB);
     auto vix = [&](uint64_t set, int u) {
-      return (set * U + u) * 1024 + threadIdx.x;
+      return (set * U + u) * HB + threadIdx.x;
     };
     auto load = [&](uint64_t set, int4 (&k)[U]) {
 #pragma unroll
@@ -197,11 +197,11 @@ __global__ __launch_bounds__(1024) void k_chunk_hist(const int32_t *__restrict__
     const uint64_t j = lo + nvec * 4 + threadIdx.x;  // < 4 tail keys
     if (j < hi) atomicAdd(&hsh[joint(keys[j])], 1u);
   } else if (lo < hi) {
-    for (uint64_t j = lo + threadIdx.x; j < hi; j += 1024) atomicAdd(&hsh[joint(keys[j])], 1u);
+    for (uint64_t j = lo + threadIdx.x; j < hi; j += HB) atomicAdd(&hsh[joint(keys[j])], 1u);
   }
   __syncthreads();
   uint32_t *row = H + (uint64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += 1024) row[i] = hsh[i];
+  for (int i = threadIdx.x; i < nb; i += HB) row[i] = hsh[i];
 }
 
 // Column-wise exclusive scan over chunks, E[c][d] = sum_{c'<c} H[c'][d] and
@@ -638,11 +638,13 @@ int radix_histograms(pj_ctx *ctx, const int32_t *d_k, uint64_t n, const RadixPla
                      Joint joint, RadixHist &h, bool readback) {
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k_chunk_hist<Joint>,
+    (void)hipFuncSetAttribute((const void *)k_chunk_hist<Joint, 1024>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 131072);
     attr_set = true;
   }
-  PJ_LAUNCH(ctx, "radix_hist", k_chunk_hist<Joint>, p.G, 1024, (size_t)p.nb * 4, d_k, n,
+  // (512-thread workgroups, which would leave wave slots for the other
+  // side's scans, measured no faster)
+  PJ_LAUNCH(ctx, "radix_hist", (k_chunk_hist<Joint, 1024>), p.G, 1024, (size_t)p.nb * 4, d_k, n,
             p.chunk, p.nb, joint, h.H.as<uint32_t>());
   const dim3 cgrid((uint32_t)ceil_div(p.nb, 256), (uint32_t)h.nslab);
   PJ_LAUNCH(ctx, "radix_colsum", k_colsum, cgrid, 256, 0, h.H.as<const uint32_t>(), p.G, p.nb,

@@ -38,18 +38,31 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   // Probe inputs are carved from preallocated arenas (one per slot), so
   // admission never calls the allocator: 16-byte aligned columns.
-  auto place = [&](pj_query &q, char *base) {
+  // Queries whose probe columns this call placed in an arena: only their
+  // pointers are reset afterwards; caller-supplied columns are left alone.
+  std::vector<char> placed(nq, 0);
+  auto place = [&](uint32_t i, char *base) {
+    pj_query &q = qs[i];
     const uint64_t c = (q.n * 4 + 15) & ~15ULL;
     q.d_fk = (const int32_t *)base;
     q.d_val = (const int32_t *)(base + c);
     q.d_grp = (const int32_t *)(base + 2 * c);
+    placed[i] = 1;
   };
+  // The host may set *cfg->cancel at any time: no further query is admitted,
+  // batches on the device finish, the queries not run get done_s = -1 and
+  // the call returns PJ_E_CANCELLED.
+  auto cancelled = [&]() { return cfg->cancel && *cfg->cancel != 0; };
+  for (uint32_t i = 0; i < nq; ++i) done_s[i] = -1.0;
   auto padded = [&](const pj_query &q) -> uint64_t {
     return input_bytes(q) ? 3 * ((q.n * 4 + 15) & ~15ULL) : 0;
   };
   auto release = [&](const std::vector<uint32_t> &ids) {
     for (uint32_t i : ids)
-      if (qs[i].gen_seed) qs[i].d_fk = qs[i].d_val = qs[i].d_grp = nullptr;
+      if (placed[i]) {
+        qs[i].d_fk = qs[i].d_val = qs[i].d_grp = nullptr;
+        placed[i] = 0;
+      }
   };
   uint64_t arena_bytes = 0;
   for (uint32_t i = 0; i < nq; ++i) arena_bytes = std::max(arena_bytes, padded(qs[i]));
@@ -65,8 +78,17 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
     }
     for (uint32_t i = 0; i < nq; ++i) {
       double wait = arrival_s[i] - now();
-      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-      if (input_bytes(qs[i])) place(qs[i], (char *)arena);
+      while (wait > 0 && !cancelled()) {
+        std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+        wait = arrival_s[i] - now();
+      }
+      if (cancelled()) {
+        if (arena) pj_free(ctx, arena);
+        st.wall_s = now();
+        if (stats) *stats = st;
+        return pj_fail(PJ_E_CANCELLED, "serving loop cancelled after %u of %u queries", i, nq);
+      }
+      if (input_bytes(qs[i])) place(i, (char *)arena);
       int rc = pj_query_run(ctx, &qs[i], &cfg->batch);
       release({i});
       if (rc != PJ_OK) {
@@ -136,8 +158,15 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   std::vector<pj_query> batch;
   int err = PJ_OK;
   auto is_light = [&](uint32_t i) { return lanes && qs[i].n < heavy_rows; };
+  bool stop = false;  // cancelled: admit nothing more, drain
   while (completed < nq && err == PJ_OK) {
     bool progress = false;
+    if (!stop && cancelled()) stop = true;
+    if (stop) {
+      bool any_busy = false;
+      for (auto &f : fl) any_busy |= f.busy;
+      if (!any_busy) break;
+    }
     const double t = now();
     while (next < nq && arrival_s[next] <= t) {
       (is_light(next) ? light : heavy).push_back(next);
@@ -157,7 +186,7 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
       progress = true;
     }
     if (err != PJ_OK) break;
-    for (int s = 0; s < nslots; ++s) {
+    for (int s = 0; s < nslots && !stop; ++s) {
       if (fl[s].busy) continue;
       InFlight &f = fl[s];
       batch.clear();
@@ -177,7 +206,7 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
         const uint64_t b = padded(q);
         if (f.bytes + b > slot_bytes[s]) break;
         if (!batch.empty() && f.rows + q.n > max_rows) break;
-        if (b) place(q, (char *)arenas[s] + f.bytes);
+        if (b) place(src->front(), (char *)arenas[s] + f.bytes);
         f.bytes += b;
         f.rows += q.n;
         f.qs.push_back(src->front());
@@ -215,5 +244,7 @@ extern "C" int pj_serve_run(pj_ctx *ctx, pj_query *qs, const double *arrival_s, 
   if (err != PJ_OK) return err;
   st.wall_s = now();
   if (stats) *stats = st;
+  if (completed < nq)
+    return pj_fail(PJ_E_CANCELLED, "serving loop cancelled after %u of %u queries", completed, nq);
   return PJ_OK;
 }

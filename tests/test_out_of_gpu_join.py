@@ -85,3 +85,22 @@ def test_plan_operator(ctx):
     rc, want = O.join_dense(rk, rv, sk, sv)
     assert got == [want["count"], want["sum_b"] + want["sum_p"]]
     ctx.clear_catalog()
+
+
+@pytest.mark.gpu
+def test_bucket_limits(ctx):
+    """One device bucketing pass makes at most 256 buckets: 300 explicit
+    buckets are an argument error, and a budget that would need more than
+    256 buckets a capacity error, both before any work; 256 buckets run."""
+    nr, ns = 1 << 14, 1 << 16
+    rk, rv, sk, sv = _inputs(nr, ns, 4)
+    with pytest.raises(pa.PjError) as e:
+        ctx.join_out_of_gpu_reduce(rk, rv, sk, sv, num_buckets=300)
+    assert e.value.code == pa.PJ_E_ARG and "at most 256" in str(e.value)
+    # (2^14 + 2^16) tuples x 48 B / 4 KiB budget: 960 buckets
+    with pytest.raises(pa.PjError) as e:
+        ctx.join_out_of_gpu_reduce(rk, rv, sk, sv, hbm_budget=4096)
+    assert e.value.code == pa.PJ_E_CAPACITY and "buckets" in str(e.value)
+    got = ctx.join_out_of_gpu_reduce(rk, rv, sk, sv, num_buckets=256)
+    rc, want = O.join_dense(rk, rv, sk, sv)
+    assert got == want

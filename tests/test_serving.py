@@ -258,3 +258,44 @@ def test_direct_and_hashed_dimensions(ctx, lo, step, groups):
     for a in cols + [cnt, sm]:
         a.free()
     ctx.dim_destroy(dim)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_serving_cancel_and_caller_columns(ctx, mode):
+    """A host flag set while queries are still arriving cancels the serving
+    loop: batches on the device finish, the queries run keep results equal to
+    running them alone, the rest report done = -1, and the call raises
+    PJ_E_CANCELLED.  Probe columns the caller supplied are left as given; the
+    ones the loop placed in its arena are reset."""
+    import ctypes
+    import threading
+    mix = workload.Mix(ctx, 64, seed=17, u_lo=10, u_hi=13, qps=200)
+    qs, bufs = _alloc_inputs(ctx, mix)
+    for q in qs:
+        ctx.query_run(q)
+    alone = mix.results()
+    # queries 0..7 with caller columns (regenerated from gen_seed), the rest placed
+    served = [qs[i] if i < 8 else mix.queries()[i] for i in range(mix.nq)]
+    flag = ctypes.c_int32(0)
+    timer = threading.Timer(0.1, lambda: setattr(flag, "value", 1))
+    timer.start()
+    with pytest.raises(pa.PjError) as e:
+        ctx.serve_run(served, mix.arrivals, mode=mode, slots=3, hbm_budget=64 << 20, cancel=flag)
+    timer.join()
+    assert e.value.code == pa.PJ_E_CANCELLED
+    done = list(ctx.last_done)
+    ran = [i for i, d in enumerate(done) if d >= 0]
+    assert 0 < len(ran) < mix.nq
+    got = mix.results()
+    for i in ran:
+        assert done[i] >= mix.arrivals[i]
+        assert np.array_equal(got[i][0], alone[i][0]) and np.array_equal(got[i][1], alone[i][1])
+    arr = ctx.last_queries
+    for i in range(8):
+        assert arr[i].d_fk == qs[i].d_fk
+    for i in range(8, mix.nq):
+        assert not arr[i].d_fk
+    for b in bufs:
+        b.free()
+    mix.close()
