@@ -346,6 +346,32 @@ int pj_bloom_repack(pj_ctx *ctx, uint64_t bloom_id, const int32_t *d_keys,
                     const int *esize, void *const *d_out, uint64_t *n_out);
 int pj_bloom_release(pj_ctx *ctx, uint64_t bloom_id);
 
+/* ---- host -> HBM staging -----------------------------------------------------
+ * The reference's cpu-to-gpu operator (core/olap/lib/operators/cpu-to-gpu.cpp:
+ * 69-239) moves host-resident blocks of a relation to the device, and its
+ * storage manager loads the files of prepared relations
+ * (core/storage/lib/storage-manager.cpp:300-330; binary-block-plugin.cpp:67-100
+ * reads column <rel>.<attr> as a headerless array).  Here a copy of `bytes`
+ * bytes from a file at `offset` (path != NULL) or from pageable host memory
+ * (h_src != NULL; exactly one of the two) to d_dst is cut into 8 MiB chunks
+ * that worker threads fill into pinned buffers, each chunk DMA'd on a copy
+ * stream of the context as soon as it is filled.
+ *   pj_stage_begin queues the copy and returns at once with a ticket; the
+ *     host memory must stay valid until pj_stage_wait returns.  A file shorter
+ *     than offset + bytes is PJ_E_ARG, before anything is queued.
+ *   pj_stage_wait(ticket) waits on the host until every chunk of the copy is
+ *     issued, then makes the context's stream wait for the copy on the device:
+ *     kernels launched on the context afterwards see the data, and kernels
+ *     launched before it overlap with the DMA.  Each ticket is waited for once;
+ *     a read error is reported here.
+ *   pj_stage_file is begin + wait + a context synchronisation. */
+int pj_stage_begin(pj_ctx *ctx, const char *path, const void *h_src,
+                   uint64_t offset, uint64_t bytes, void *d_dst,
+                   uint64_t *ticket);
+int pj_stage_wait(pj_ctx *ctx, uint64_t ticket);
+int pj_stage_file(pj_ctx *ctx, const char *path, uint64_t offset,
+                  uint64_t bytes, void *d_dst);
+
 /* ---- plan API ----------------------------------------------------------------
  * PlanExecutor::parseOperator (core/olap/lib/plan/plan-parser.cpp:238-1675)
  * + PreparedStatement::execute (prepared-statement.cpp:41-140): parse a

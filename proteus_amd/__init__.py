@@ -138,6 +138,9 @@ _SIGS = {
     "pj_batch_autotune": ([vp, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
     "pj_serve_run": ([vp, C.POINTER(Query), C.POINTER(C.c_double), u32, C.POINTER(ServeCfg),
                       C.POINTER(C.c_double), C.POINTER(ServeStats)], i32),
+    "pj_stage_begin": ([vp, C.c_char_p, vp, u64, u64, vp, C.POINTER(u64)], i32),
+    "pj_stage_wait": ([vp, u64], i32),
+    "pj_stage_file": ([vp, C.c_char_p, u64, u64, vp], i32),
     "pj_register_column": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, i32], i32),
     "pj_clear_catalog": ([vp], i32),
     "pj_plan_execute": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, u64], i32),
@@ -246,6 +249,23 @@ class Context:
         if len(a):
             _check(_lib.pj_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
         return d
+
+    # ---- host -> HBM staging through the pinned ring ------------------------
+    def stage_file(self, path, dst, nbytes, offset=0):
+        """Copy nbytes of file `path` from `offset` into device memory dst."""
+        _check(_lib.pj_stage_file(self.h, os.fsencode(path), offset, nbytes, _ptr(dst)))
+
+    def stage_begin(self, dst, nbytes, path=None, host=None, offset=0):
+        """Queue a copy from a file or a host numpy array; returns the ticket.
+        A host array must stay alive until stage_wait returns."""
+        t = u64()
+        hp = None if host is None else host.ctypes.data
+        _check(_lib.pj_stage_begin(self.h, None if path is None else os.fsencode(path), hp,
+                                   offset, nbytes, _ptr(dst), C.byref(t)))
+        return t.value
+
+    def stage_wait(self, ticket):
+        _check(_lib.pj_stage_wait(self.h, ticket))
 
     def copy(self, dst, src, nbytes):
         _check(_lib.pj_copy(self.h, _ptr(dst), _ptr(src), nbytes))

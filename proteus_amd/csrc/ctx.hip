@@ -487,6 +487,7 @@ void pj_bloom_release_all(pj_ctx *ctx);
 int pj_ctx_destroy(pj_ctx *ctx) {
   if (!ctx) return PJ_OK;
   (void)hipSetDevice(ctx->device);
+  pj::stage_release(ctx);
   (void)hipStreamSynchronize(ctx->stream);
   pj_batch_release(ctx);
   pj_bloom_release_all(ctx);

@@ -174,9 +174,14 @@ struct pj_ctx {
   // use), with a fork and a join event
   hipStream_t aux = nullptr;
   hipEvent_t aux_fork = nullptr, aux_join = nullptr;
+  // host -> HBM staging workers, pinned buffers and copy stream (stage.hip)
+  void *stager = nullptr;
 };
 
 namespace pj {
+
+// stops the staging workers after the queued copies and frees their buffers
+void stage_release(pj_ctx *ctx);
 
 // scratch allocation from the context pool (rounded, cached, budgeted)
 int dalloc(pj_ctx *ctx, uint64_t bytes, void **p);

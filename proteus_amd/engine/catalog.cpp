@@ -47,14 +47,23 @@ int type_size(Type t) {
   return 0;
 }
 
+// Frees a column the engine staged; a copy still in flight is waited for
+// first, so the memory is not reused under it.
+void release_column(pj_ctx *ctx, Column &c) {
+  if (c.ticket) {
+    pj_stage_wait(ctx, c.ticket);
+    c.ticket = 0;
+  }
+  if (c.staged && c.dev) pj_free(ctx, c.dev);
+  c.dev = nullptr;
+  c.staged = false;
+}
+
 void release_staged(pj_ctx *ctx, Catalog &cat) {
   for (auto &r : cat.rels)
-    for (auto &c : r.second.cols)
-      if (c.second.staged && c.second.dev) {
-        pj_free(ctx, c.second.dev);
-        c.second.dev = nullptr;
-        c.second.staged = false;
-      }
+    for (auto &c : r.second.cols) release_column(ctx, c.second);
+  for (auto &f : cat.files) release_column(ctx, f.second.col);
+  cat.files.clear();
 }
 
 }  // namespace engine
@@ -83,7 +92,7 @@ extern "C" int pj_register_column(pj_ctx *ctx, const char *rel, const char *attr
     return pj_fail(PJ_E_UNSUPPORTED, "pj_register_column: unsupported type '%s'", type);
   Catalog &cat = catalog_of(ctx);
   Column &old = cat.rels[rel].cols[attr];
-  if (old.staged && old.dev) pj_free(ctx, old.dev);
+  release_column(ctx, old);
   Column c;
   c.type = t;
   c.n = n;

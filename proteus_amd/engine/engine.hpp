@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -32,18 +33,30 @@ struct Column {
   void *dev = nullptr;         // device pointer (caller's or staged)
   const void *host = nullptr;  // caller's host pointer (staged on first use)
   bool staged = false;         // dev was allocated by the engine
+  uint64_t ticket = 0;         // staging copy not yet waited for (pj_stage_wait)
 };
 
 struct Relation {
   std::map<std::string, Column> cols;
 };
 
+// A column file of a prepared relation (<rel>.<attr>, headerless), staged to
+// HBM on first use and kept there while the file is unchanged.
+struct FileColumn {
+  Column col;
+  uint64_t size = 0;
+  int64_t mtime_ns = 0;
+  std::shared_ptr<std::vector<std::string>> dict;  // dstring: code -> string
+};
+
 struct Catalog {
   std::map<std::string, Relation> rels;
+  std::map<std::string, FileColumn> files;  // by path
 };
 
 Catalog &catalog_of(pj_ctx *ctx);
 void release_staged(pj_ctx *ctx, Catalog &cat);
+void release_column(pj_ctx *ctx, Column &c);
 
 }  // namespace engine
 }  // namespace pj

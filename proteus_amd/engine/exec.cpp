@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <sys/stat.h>
 #include <map>
 #include <memory>
 #include <set>
@@ -186,6 +187,7 @@ class Exec {
   std::string run(const Json &plan) {
     Need top;
     top.all = true;
+    prefetch(plan);
     Rel r = exec(plan, top);
     if (!printed_) print_rel(r, nullptr, "pm-csv");
     return out_;
@@ -417,11 +419,9 @@ class Exec {
 
   int64_t dict_lookup(const std::string &v, const Json &dict) {
     if (!dict["path"].is_string()) raise(PJ_E_PARSE, "dstring constant without dict.path");
-    std::ifstream is(path(dict["path"].as_string()));
-    if (!is) raise(PJ_E_PARSE, "cannot open dictionary " + dict["path"].as_string());
-    std::string line, pre = v + ":";
-    while (std::getline(is, line))
-      if (line.compare(0, pre.size(), pre) == 0) return std::stoll(line.substr(pre.size()));
+    std::shared_ptr<Dict> d = read_dict(path(dict["path"].as_string()));
+    for (size_t i = 0; i < d->size(); ++i)
+      if ((*d)[i] == v) return (int64_t)i;
     raise(PJ_E_PARSE, "value '" + v + "' not in dictionary " + dict["path"].as_string());
   }
 
@@ -531,10 +531,11 @@ class Exec {
     const Json &projs = pl["projections"];
     if (type == "csv" || type == "pm-csv") return scan_csv(pl, name);
     if (type == "bincol") return scan_bincol(pl, name);
-    // every other plugin type resolves to the caller's registered columns
+    // every other plugin type resolves to the caller's registered columns,
+    // else to the column files of a prepared relation
     Catalog &cat = catalog_of(ctx_);
     auto it = cat.rels.find(name);
-    if (it == cat.rels.end()) raise(PJ_E_PARSE, "relation '" + name + "' is not registered");
+    if (it == cat.rels.end()) return scan_block(pl, name);
     Rel out;
     bool have_n = false;
     for (auto &c : it->second.cols) {
@@ -548,12 +549,11 @@ class Exec {
       auto ci = it->second.cols.find(attr);
       if (ci == it->second.cols.end()) raise(PJ_E_PARSE, "column " + name + "." + attr + " is not registered");
       Column &cc = ci->second;
-      if (!cc.dev && cc.n) {
-        void *d = nullptr;
-        ck(pj_malloc(ctx_, cc.n * type_size(cc.type), &d));
-        ck(pj_memcpy_h2d(ctx_, d, cc.host, cc.n * type_size(cc.type)));
-        cc.dev = d;
-        cc.staged = true;
+      stage_registered(cc);
+      if (cc.ticket) {
+        const uint64_t t = cc.ticket;
+        cc.ticket = 0;
+        ck(pj_stage_wait(ctx_, t));
       }
       Col col;
       col.t = cc.type == Type::I32 ? R::I32 : cc.type == Type::I64 ? R::I64
@@ -565,6 +565,161 @@ class Exec {
     for (auto &c : it->second.cols)
       if (c.second.n != out.n) raise(PJ_E_ARG, "columns of relation '" + name + "' differ in length");
     return out;
+  }
+
+  // ---- host -> HBM staging ---------------------------------------------
+  // A registered host column is copied to HBM on first use through the
+  // pinned staging ring (pj_stage_begin); the scan that reads it orders the
+  // context's stream after the copy.
+  void stage_registered(Column &cc) {
+    if (cc.dev || !cc.n) return;
+    void *d = nullptr;
+    ck(pj_malloc(ctx_, cc.n * type_size(cc.type), &d));
+    cc.dev = d;
+    cc.staged = true;
+    ck(pj_stage_begin(ctx_, nullptr, cc.host, 0, cc.n * type_size(cc.type), d, &cc.ticket));
+  }
+
+  // Column file <name>.<attr> of a prepared relation (the layout of
+  // tools/prepare_data.py:374-443: native little-endian int32 for "int",
+  // int64 for "int64" / "date" / "datetime", FP64 for "float", int32 codes
+  // for "dstring" with the dictionary <file>.dict of "value:code" lines).
+  // Staged to HBM on first use and kept while the file's size and mtime are
+  // unchanged.
+  FileColumn &file_column(const std::string &name, const Json &proj) {
+    const std::string attr = proj["attrName"].as_string();
+    bool is_str;
+    const int t = parse_col_type(proj_type(proj, name), &is_str);
+    if (t == R::BOOL) raise(PJ_E_UNSUPPORTED, "block column " + name + "." + attr + " of type bool");
+    const std::string file = path(name + "." + attr);
+    struct stat sb;
+    if (stat(file.c_str(), &sb) != 0)
+      raise(PJ_E_PARSE, "relation '" + name + "' is not registered and has no column file " + file);
+    const int64_t mt = (int64_t)sb.st_mtim.tv_sec * 1000000000LL + sb.st_mtim.tv_nsec;
+    const uint64_t esz = R::col_size(t);
+    if ((uint64_t)sb.st_size % esz)
+      raise(PJ_E_PARSE, "column file " + file + " is not a whole number of " + std::to_string(esz) + "-byte values");
+    Catalog &cat = catalog_of(ctx_);
+    auto it = cat.files.find(file);
+    if (it != cat.files.end()) {
+      FileColumn &f = it->second;
+      if (f.size == (uint64_t)sb.st_size && f.mtime_ns == mt && f.col.type == col_type(t)) return f;
+      release_column(ctx_, f.col);
+      cat.files.erase(it);
+    }
+    FileColumn f;
+    f.size = (uint64_t)sb.st_size;
+    f.mtime_ns = mt;
+    f.col.type = col_type(t);
+    f.col.n = f.size / esz;
+    if (is_str) f.dict = read_dict(file + ".dict");
+    void *d = nullptr;
+    ck(pj_malloc(ctx_, std::max<uint64_t>(f.size, 1), &d));
+    uint64_t ticket = 0;
+    if (pj_stage_begin(ctx_, file.c_str(), nullptr, 0, f.size, d, &ticket) != PJ_OK) {
+      const std::string msg = pj_last_error();
+      pj_free(ctx_, d);
+      raise(PJ_E_PARSE, msg);
+    }
+    f.col.dev = d;
+    f.col.staged = true;
+    f.col.ticket = ticket;
+    return cat.files[file] = f;
+  }
+
+  // Orders the context's stream after a file column's copy; a failed copy
+  // drops the column from the cache.
+  void wait_file(const std::string &file, FileColumn &f) {
+    if (!f.col.ticket) return;
+    const uint64_t t = f.col.ticket;
+    f.col.ticket = 0;
+    if (pj_stage_wait(ctx_, t) != PJ_OK) {
+      const std::string msg = pj_last_error();
+      release_column(ctx_, f.col);
+      catalog_of(ctx_).files.erase(file);
+      raise(PJ_E_PARSE, msg);
+    }
+  }
+
+  static Type col_type(int t) {
+    return t == R::I64 ? Type::I64 : t == R::F64 ? Type::F64 : t == R::F32 ? Type::F32 : Type::I32;
+  }
+
+  // "value:code" lines, the value ending at the last ':' (storage-manager.cpp:314-330)
+  std::shared_ptr<Dict> read_dict(const std::string &file) {
+    std::ifstream is(file);
+    if (!is) raise(PJ_E_PARSE, "cannot open dictionary " + file);
+    auto d = std::make_shared<Dict>();
+    std::string line;
+    while (std::getline(is, line)) {
+      if (!line.empty() && line.back() == '\r') line.pop_back();
+      if (line.empty()) continue;
+      const size_t k = line.find_last_of(':');
+      if (k == std::string::npos) raise(PJ_E_PARSE, "dictionary line without ':' in " + file);
+      char *end = nullptr;
+      const long long code = strtoll(line.c_str() + k + 1, &end, 10);
+      if (code < 0 || code > INT32_MAX || *end) raise(PJ_E_PARSE, "bad dictionary code in " + file);
+      if ((size_t)code >= d->size()) d->resize((size_t)code + 1);
+      (*d)[(size_t)code] = line.substr(0, k);
+    }
+    return d;
+  }
+
+  Rel scan_block(const Json &pl, const std::string &name) {
+    Rel out;
+    bool first = true;
+    for (auto &p : pl["projections"].items()) {
+      FileColumn &f = file_column(name, p);
+      wait_file(path(name + "." + p["attrName"].as_string()), f);
+      if (!first && f.col.n != out.n) raise(PJ_E_PARSE, "column files of " + name + " differ in length");
+      out.n = f.col.n;
+      first = false;
+      Col c;
+      c.t = f.col.type == Type::I64 ? R::I64 : f.col.type == Type::F64 ? R::F64 : R::I32;
+      c.dev = f.col.dev;
+      c.dict = f.dict;
+      out.cols.emplace_back(Attr{name, p["attrName"].as_string()}, c);
+    }
+    if (first) raise(PJ_E_PARSE, "relation '" + name + "' is not registered, and a scan of its column files needs projections");
+    return out;
+  }
+
+  // Starts the staging of every host-resident column the plan scans, in the
+  // order the operators run (a join's build side before its probe side), so
+  // the later scans' copies overlap the earlier operators' kernels.
+  void prefetch(const Json &op) {
+    if (!op.is_object() || !op["operator"].is_string()) return;
+    const std::string &k = op["operator"].as_string();
+    if (k == "scan") {
+      const Json &pl = op["plugin"];
+      if (!pl.is_object() || !pl["name"].is_string()) return;
+      const std::string type = pl["type"].is_string() ? pl["type"].as_string() : "block";
+      if (type == "csv" || type == "pm-csv" || type == "bincol") return;
+      const std::string name = pl["name"].as_string();
+      Catalog &cat = catalog_of(ctx_);
+      auto it = cat.rels.find(name);
+      for (auto &p : pl["projections"].items()) {
+        if (!p["attrName"].is_string()) continue;
+        if (it != cat.rels.end()) {
+          auto ci = it->second.cols.find(p["attrName"].as_string());
+          if (ci != it->second.cols.end()) stage_registered(ci->second);
+        } else {
+          file_column(name, p);
+        }
+      }
+      return;
+    }
+    if (k == "join") {
+      prefetch(op["leftInput"]);
+      prefetch(op["rightInput"]);
+      return;
+    }
+    if (is_join_op(k)) {
+      prefetch(op["build_input"]);
+      prefetch(op["probe_input"]);
+      return;
+    }
+    prefetch(op["input"]);
   }
 
   const std::vector<std::string> &lines_of(const std::string &file) {

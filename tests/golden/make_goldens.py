@@ -23,6 +23,10 @@ PLANS = ["reduce-scan", "reduce-twofields-scan", "reduce-join", "reduce-select",
          "reduce-join-record", "reduce-join-record-nonselective", "reduce-scan-bin",
          "reduce-scan-gpu", "gpu-group-by", "reduce-join-parts", "reduce-scan-part",
          "reduce-scan-partsupp"]
+# the Star Schema Benchmark's queries as bare physical plans over prepared
+# ("block") relations: tests/inputs/plans/proteus_bare_plans/q*.sql_bare.json
+SSB_BARE = ["q1_1", "q1_2", "q1_3", "q2_1", "q2_2", "q2_3", "q3_1", "q3_2", "q3_3", "q3_4",
+            "q4_1", "q4_2", "q4_3"]
 OUTPUTS = {
     # RelBuilder operator tests, core/olap/tests/operator-tests/tests-join-sailors.cpp
     "sailorsScan": "tests/olap/outputs/tests-sailors/sailorsScan.csv",
@@ -52,6 +56,13 @@ def main():
         dst = os.path.join(HERE, "inputs", "plans", pl + ".json")
         os.makedirs(os.path.dirname(dst), exist_ok=True)
         shutil.copy(os.path.join(REF, "tests/inputs/plans", pl + ".json"), dst)
+    for q in SSB_BARE:
+        dst = os.path.join(HERE, "inputs", "plans", "ssb", q + ".json")
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        with open(os.path.join(REF, "tests/inputs/plans/proteus_bare_plans", q + ".sql_bare.json")) as fh:
+            plan = json.load(fh)
+        with open(dst, "w") as fh:  # the same document without its indentation
+            json.dump(plan, fh, separators=(",", ":"))
     gold = {}
     for name, f in OUTPUTS.items():
         with open(os.path.join(REF, f)) as fh:

@@ -44,9 +44,13 @@ def test_relbuilder_builds_reference_shapes():
 
 
 def test_reference_plan_fixtures_parse_as_json():
-    for f in os.listdir(PLANS):
-        with open(os.path.join(PLANS, f)) as fh:
-            json.load(fh)
+    n = 0
+    for d, _, files in os.walk(PLANS):
+        for f in files:
+            with open(os.path.join(d, f)) as fh:
+                json.load(fh)
+            n += 1
+    assert n >= 25
 
 
 # ---- GPU: the reference's plan fixtures -----------------------------------
