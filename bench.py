@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PCIE_PEAK_GBS = 64.0  # PCIe 5.0 x16, one direction
 NR, NS = 1 << 27, 1 << 30
 RANGE = int(1.33 * NR)  # 178,509,578: ~24.8% probe misses
 SEED = 20251203
@@ -34,9 +35,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary measurements (Zipf fk variant)")
-    ap.add_argument("--blocks", default="c1,groupby,star,sort,oog",
+    ap.add_argument("--blocks", default="c1,groupby,star,sort,oog,staging",
                     help="secondary measurements to run (comma-separated subset of "
-                         "c1,groupby,star,sort,oog)")
+                         "c1,groupby,star,sort,oog,staging)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched / served query mix")
     ap.add_argument("--batched-queries", type=int, default=4096)
@@ -169,6 +170,8 @@ def main():
             line["sort"] = sort_plan(ctx, a.steps)
         if "oog" in blocks:
             line["out_of_gpu_join"] = out_of_gpu(ctx, 2)
+        if "staging" in blocks:
+            line["staging"] = staging(ctx, 3)
     if not a.no_batched:
         for x in (rk, rv, sk, sv):
             if x is not None:
@@ -319,6 +322,88 @@ def out_of_gpu(ctx, steps):
     return {"build_rows": nr, "probe_rows": ns, "hbm_budget_bytes": budget, "ms": sec * 1e3,
             "input_gtuples_per_s": (nr + ns) / sec / 1e9,
             "pcie_gbs": 24.0 * (nr + ns) / sec / 1e9, "identical": got == ref}
+
+
+def staging(ctx, steps):
+    """Host -> HBM staging through the pinned ring (pj_stage_file /
+    pj_stage_begin), against one hipMemcpy from the same pageable array
+    (pj_memcpy_h2d), over 2 GiB; then a prepared relation of 2^27 rows
+    (three int32 column files, 1.5 GiB) scanned by the plan
+    reduce(SUM(a * b)) where c < 25: the first run stages the files, later
+    runs read the columns kept in HBM.  Files are written to a temporary
+    directory (and so sit in the page cache); rates are bytes moved over
+    PCIe per second."""
+    import json as _json
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    from proteus_amd import relbuilder as rb
+    nbytes = 1 << 31
+    host = np.random.default_rng(SEED).integers(0, 1 << 31, nbytes // 4, dtype=np.int32)
+    tmp = tempfile.mkdtemp(prefix="pj_stage_")
+    out = {"bytes": nbytes, "pcie_peak_gbs": PCIE_PEAK_GBS}
+    try:
+        f = os.path.join(tmp, "blob.bin")
+        host.tofile(f)
+        d = ctx.empty(nbytes // 4, "int32")
+
+        def rate(fn):
+            fn()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            ctx.sync()
+            return nbytes / ((time.perf_counter() - t0) / steps) / 1e9
+
+        out["file_gbs"] = rate(lambda: ctx.stage_file(f, d, nbytes))
+        out["file_identical"] = bool(np.array_equal(d.numpy()[:1 << 20], host[:1 << 20]))
+        out["host_gbs"] = rate(lambda: ctx.stage_wait(ctx.stage_begin(d, nbytes, host=host)))
+        out["hipmemcpy_pageable_gbs"] = rate(lambda: ctx.copy_h2d(d, host))
+        d.free()
+        os.remove(f)
+        # a prepared relation: <rel>.<attr> files and a catalog next to them
+        n = 1 << 27
+        rs = np.random.default_rng(SEED + 1)
+        cols = {"a": rs.integers(0, 1000, n, dtype=np.int32),
+                "b": rs.integers(0, 1000, n, dtype=np.int32),
+                "c": rs.integers(0, 50, n, dtype=np.int32)}
+        rel = "inputs/blk/fact.csv"
+        os.makedirs(os.path.join(tmp, "inputs", "blk"))
+        attrs = []
+        for no, (name, v) in enumerate(cols.items(), 1):
+            v.tofile(os.path.join(tmp, rel + "." + name))
+            attrs.append({"type": {"type": "int"}, "relName": rel, "attrName": name, "attrNo": no})
+        with open(os.path.join(tmp, "inputs", "blk", "catalog.json"), "w") as fh:
+            _json.dump({"blk_fact": {"path": rel, "type": {"type": "bag", "inner": {
+                "type": "record", "attributes": attrs}}, "plugin": {"type": "block"}}}, fh)
+        m = cols["c"] < 25
+        want = int((cols["a"][m].astype(np.int64) * cols["b"][m]).sum())
+        col = lambda a: rb.col(rel, a)  # noqa: E731
+        q = rb.RelBuilder(ctx, tmp).scan(rel, ["a", "b", "c"]).to_gpu().unpack() \
+            .filter(rb.lt(col("c"), rb.lit(25))) \
+            .reduce([rb.mul(rb.cast(col("a"), "int64"), rb.cast(col("b"), "int64"))], ["sum"])
+        text = _json.dumps(q.plan())
+        ctx.clear_catalog()
+        ctx.sync()
+        t0 = time.perf_counter()
+        got = int(ctx.execute_plan(text, base_dir=tmp).strip())
+        cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.execute_plan(text, base_dir=tmp)
+        warm = (time.perf_counter() - t0) / steps
+        ctx.clear_catalog()
+        fbytes = 12 * n
+        out["block_plan"] = {"rows": n, "file_bytes": fbytes, "first_run_ms": cold * 1e3,
+                             "first_run_gbs": fbytes / cold / 1e9, "resident_run_ms": warm * 1e3,
+                             "resident_hbm_gbs": fbytes / warm / 1e9, "identical": got == want}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out["frac"] = max(out.get("file_gbs", 0), out.get("host_gbs", 0)) / PCIE_PEAK_GBS
+    return out
 
 
 def sort_plan(ctx, steps):

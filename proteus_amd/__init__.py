@@ -250,6 +250,11 @@ class Context:
             _check(_lib.pj_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
         return d
 
+    def copy_h2d(self, dst, a):
+        """One hipMemcpy of host array a into device memory dst."""
+        a = np.ascontiguousarray(a)
+        _check(_lib.pj_memcpy_h2d(self.h, _ptr(dst), a.ctypes.data, a.nbytes))
+
     # ---- host -> HBM staging through the pinned ring ------------------------
     def stage_file(self, path, dst, nbytes, offset=0):
         """Copy nbytes of file `path` from `offset` into device memory dst."""
