@@ -346,6 +346,18 @@ int pj_bloom_repack(pj_ctx *ctx, uint64_t bloom_id, const int32_t *d_keys,
                     const int *esize, void *const *d_out, uint64_t *n_out);
 int pj_bloom_release(pj_ctx *ctx, uint64_t bloom_id);
 
+/* ---- device sort ---------------------------------------------------------------
+ * qsort_<fields>(ptr, N) of core/platform/lib/util/sort/gpu/sort-util.hpp:27-58
+ * (the records gpu-sort.cpp packs and sorts on the device): sorts n records in
+ * place in device memory, ascending and lexicographic with the first field
+ * most significant; equal records keep their order.  `fields` names the
+ * fields in order, 'i' = int32 and 'l' = int64 (1 to 8 of them); each field
+ * sits at its natural alignment and the record is padded to its widest field
+ * (a C struct of those members).  d_rows must be aligned to the widest field.
+ * A stable LSD radix sort (8-bit digits; digit positions every record shares
+ * are skipped).  Waits for the context. */
+int pj_sort_records(pj_ctx *ctx, void *d_rows, uint64_t n, const char *fields);
+
 /* ---- host -> HBM staging -----------------------------------------------------
  * The reference's cpu-to-gpu operator (core/olap/lib/operators/cpu-to-gpu.cpp:
  * 69-239) moves host-resident blocks of a relation to the device, and its

@@ -138,6 +138,7 @@ _SIGS = {
     "pj_batch_autotune": ([vp, C.POINTER(Query), u32, C.POINTER(BatchCfg)], i32),
     "pj_serve_run": ([vp, C.POINTER(Query), C.POINTER(C.c_double), u32, C.POINTER(ServeCfg),
                       C.POINTER(C.c_double), C.POINTER(ServeStats)], i32),
+    "pj_sort_records": ([vp, vp, u64, C.c_char_p], i32),
     "pj_stage_begin": ([vp, C.c_char_p, vp, u64, u64, vp, C.POINTER(u64)], i32),
     "pj_stage_wait": ([vp, u64], i32),
     "pj_stage_file": ([vp, C.c_char_p, u64, u64, vp], i32),
@@ -249,6 +250,11 @@ class Context:
         if len(a):
             _check(_lib.pj_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
         return d
+
+    def sort_records(self, rows, n, fields):
+        """Sort n device records (fields 'i' int32 / 'l' int64, C-struct
+        layout) in place, ascending, first field most significant, stable."""
+        _check(_lib.pj_sort_records(self.h, _ptr(rows), n, fields.encode()))
 
     def copy_h2d(self, dst, a):
         """One hipMemcpy of host array a into device memory dst."""

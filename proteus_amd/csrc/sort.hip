@@ -1,56 +1,310 @@
 // Device sort of the plan operator `sort` (the reference's GPU sort,
 // core/olap/lib/operators/gpu/gpu-sort.cpp with
-// core/platform/lib/util/sort/gpu/sort-util.cu): a stable LSD sort of a row
-// permutation.  For each sort expression, from the last to the first, every
-// row's value (read through the current permutation) is turned into an
-// unsigned key whose order is the column's order, and the permutation is
-// stably radix-sorted by it (hipCUB's DeviceRadixSort on the context
-// stream).  Stability makes the result lexicographic over the expressions,
-// with ties kept in input order.
+// core/platform/lib/util/sort/gpu/sort-util.cu, which sorts packed records
+// with thrust): a stable LSD radix sort of a row permutation.  For each sort
+// expression, from the last to the first, every row's value (read through the
+// current permutation) is turned into an unsigned key whose order is the
+// column's order, and the (key, row) pairs are stably radix-sorted by it.
+// Stability makes the result lexicographic over the expressions, with ties
+// kept in input order.
 //
 // Keys: 32-bit types (int32, bool, dictionary ranks) flip the sign bit into
 // a 32-bit key; int64 flips it into a 64-bit key; FP64 (and float32, widened)
 // takes its bit pattern, flipped whole when negative and with the sign bit
 // set otherwise, after mapping -0.0 to +0.0 so that the two compare equal.
 // DESC inverts the key; NONE directions are skipped.
+//
+// The radix sort (8-bit digits, least significant first):
+//   * one read of the keys counts every digit position; a position where all
+//     keys share one digit is skipped (narrow key ranges sort in fewer
+//     passes);
+//   * per pass, the input is cut into G chunks, one workgroup each (as many
+//     as fit the CUs at once); k_rs_chunk_hist counts each chunk's digits,
+//     column scans give every (chunk, digit) its exact output offset, and
+//     k_rs_scatter walks its chunk tile by tile in order, ranking each tile
+//     stably and writing it out digit by digit from LDS.
+//   * stable ranking of an 8192-key tile: each of the 8 waves owns a
+//     contiguous eighth of it, 16 rows of 64 keys.  Row by row, lanes holding the same digit
+//     find each other with eight ballots; a key's rank is its wave's running
+//     count of the digit plus the matching lanes before it, and the lowest
+//     such lane advances the count.  Wave offsets per digit and the tile's
+//     digit starts follow from the counts.  No atomics: the order is fixed by
+//     the input order alone.
 #include <algorithm>
-
-#include <hipcub/hipcub.hpp>
+#include <vector>
 
 #include "pj_internal.hpp"
+#include "radix.hpp"
 #include "rel_api.hpp"
 
 namespace pj {
 namespace rel {
 
-__global__ __launch_bounds__(256) void k_sort_key(const void *__restrict__ col, int t, int desc,
-                                                  const int32_t *__restrict__ rank, uint32_t nrank,
-                                                  const uint32_t *__restrict__ perm, uint64_t n,
-                                                  unsigned long long *__restrict__ key) {
+// Scatter geometry per key type: BLOCK threads with IPT keys each make a
+// tile; PER_CU workgroups fit a CU's LDS at once.
+template <class K>
+struct RsCfg;
+template <>
+struct RsCfg<uint32_t> {
+  static constexpr int BLOCK = 512, IPT = 16, PER_CU = 2;
+};
+template <>
+struct RsCfg<unsigned long long> {
+  static constexpr int BLOCK = 512, IPT = 16, PER_CU = 1;
+};
+constexpr int kRsHistWaves = 4;  // k_rs_chunk_hist: 256 threads
+
+template <class K>
+__device__ __forceinline__ uint32_t rs_digit(K k, int shift) {
+  return (uint32_t)(k >> shift) & 255u;
+}
+
+// Counts of every 8-bit digit position of the keys, hist[p * 256 + d].
+template <class K>
+__global__ __launch_bounds__(256) void k_rs_digits(const K *__restrict__ keys, uint64_t n,
+                                                   uint32_t *__restrict__ hist) {
+  constexpr int P = sizeof(K);
+  __shared__ uint32_t h[P * 256];
+  for (int i = threadIdx.x; i < P * 256; i += 256) h[i] = 0;
+  __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * 256;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    const uint32_t r = perm[i];
+    const K k = keys[i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&h[p * 256 + rs_digit(k, 8 * p)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P * 256; i += 256)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Digit counts of chunk c = [c * chunk, min(n, (c + 1) * chunk)) into row c
+// of H[G][256].  One LDS copy per wave spreads same-digit atomics.
+template <class K>
+__global__ __launch_bounds__(256) void k_rs_chunk_hist(const K *__restrict__ keys, uint64_t n,
+                                                       uint64_t chunk, int shift,
+                                                       uint32_t *__restrict__ H) {
+  __shared__ uint32_t h[kRsHistWaves][256];
+  const int tid = threadIdx.x, w = tid >> 6;
+  for (int i = tid; i < kRsHistWaves * 256; i += 256) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+  // four keys per thread in flight before their atomics
+  uint64_t j = lo + tid;
+  for (; j + 3 * 256 < hi; j += 4 * 256) {
+    const K a = keys[j], b = keys[j + 256], c = keys[j + 512], d = keys[j + 768];
+    atomicAdd(&h[w][rs_digit(a, shift)], 1u);
+    atomicAdd(&h[w][rs_digit(b, shift)], 1u);
+    atomicAdd(&h[w][rs_digit(c, shift)], 1u);
+    atomicAdd(&h[w][rs_digit(d, shift)], 1u);
+  }
+  for (; j < hi; j += 256) atomicAdd(&h[w][rs_digit(keys[j], shift)], 1u);
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int x = 0; x < kRsHistWaves; ++x) s += h[x][tid];
+  H[(uint64_t)blockIdx.x * 256 + tid] = s;
+}
+
+// One stable pass: chunk c of (ik, iv) to its (digit, chunk) ranges of
+// (ok, ov): digit d of chunk c starts at S[d] + E[c][d].
+template <class K, int BLOCK = RsCfg<K>::BLOCK, int IPT = RsCfg<K>::IPT>
+__global__ __launch_bounds__(BLOCK) void k_rs_scatter(
+    const K *__restrict__ ik, const uint32_t *__restrict__ iv, K *__restrict__ ok,
+    uint32_t *__restrict__ ov, uint64_t n, uint64_t chunk, int shift,
+    const uint32_t *__restrict__ E, const unsigned long long *__restrict__ S) {
+  constexpr int kRsTile = BLOCK * IPT, kRsWaves = BLOCK / 64, kRsIpt = IPT;
+  static_assert(BLOCK >= 256, "one thread per digit");
+  __shared__ __align__(16) K st_k[kRsTile];
+  __shared__ uint32_t st_v[kRsTile];
+  __shared__ uint32_t cntw[kRsWaves][256];  // per wave: running digit counts
+  __shared__ unsigned long long cursor[256];
+  __shared__ uint32_t start[256];
+  __shared__ uint32_t scan_tmp[kRsWaves];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool dthread = tid < 256;  // threads 0..255 own one digit each
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const uint64_t c = blockIdx.x;
+  const uint64_t lo = c * chunk;
+  const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+  if (dthread) cursor[tid] = S[tid] + E[c * 256 + tid];
+  const uint64_t span = hi > lo ? hi - lo : 0;
+  const uint32_t ntiles = (uint32_t)((span + kRsTile - 1) / kRsTile);
+  // item i of a thread is tile element w * 64 * kRsIpt + i * 64 + lane: each
+  // wave owns a contiguous quarter of the tile, so input order is (wave,
+  // item, lane) and a wave's running counts rank its keys
+  auto elem = [&](int i) { return (uint32_t)(w * 64 * kRsIpt + i * 64 + lane); };
+
+  K kA[kRsIpt], kB[kRsIpt];
+  uint32_t vA[kRsIpt], vB[kRsIpt];
+  auto load = [&](uint32_t t, K *k, uint32_t *v) {
+    const uint64_t base = lo + (uint64_t)t * kRsTile;
+    const uint32_t len = (uint32_t)(hi - base < (uint64_t)kRsTile ? hi - base : kRsTile);
+#pragma unroll
+    for (int i = 0; i < kRsIpt; ++i) {
+      const uint32_t e = elem(i);
+      k[i] = e < len ? ik[base + e] : K(0);
+      v[i] = e < len ? iv[base + e] : 0u;
+    }
+  };
+  auto process = [&](uint32_t t, const K *k, const uint32_t *v) {
+    const uint64_t base = lo + (uint64_t)t * kRsTile;
+    const uint32_t len = (uint32_t)(hi - base < (uint64_t)kRsTile ? hi - base : kRsTile);
+    for (int x = tid; x < kRsWaves * 256; x += BLOCK) (&cntw[0][0])[x] = 0;
+    __syncthreads();
+    uint32_t dg[kRsIpt], rk[kRsIpt];
+#pragma unroll
+    for (int i = 0; i < kRsIpt; ++i) {
+      const bool valid = elem(i) < len;
+      const uint32_t d = valid ? rs_digit(k[i], shift) : 0u;
+      unsigned long long peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      const uint32_t before = (uint32_t)__popcll(peers & lt);
+      const uint32_t run = cntw[w][d];  // every lane reads before the leader writes
+      dg[i] = d;
+      rk[i] = run + before;
+      if (valid && before == 0) cntw[w][d] = run + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // per digit (one per thread): wave counts -> wave offsets, tile total
+    uint32_t tot_d = 0;
+    if (dthread) {
+#pragma unroll
+      for (int x = 0; x < kRsWaves; ++x) {
+        const uint32_t cx = cntw[x][tid];
+        cntw[x][tid] = tot_d;
+        tot_d += cx;
+      }
+    }
+    uint32_t tot;
+    const uint32_t st = block_excl_scan<BLOCK, uint32_t>(tot_d, scan_tmp, &tot);
+    if (dthread) start[tid] = st;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRsIpt; ++i) {
+      if (elem(i) < len) {
+        const uint32_t p = start[dg[i]] + cntw[w][dg[i]] + rk[i];
+        st_k[p] = k[i];
+        st_v[p] = v[i];
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < len; p += BLOCK) {
+      const K key = st_k[p];
+      const uint32_t d = rs_digit(key, shift);
+      const unsigned long long o = cursor[d] + (p - start[d]);
+      ok[o] = key;
+      ov[o] = st_v[p];
+    }
+    __syncthreads();
+    if (dthread) cursor[tid] += tot_d;
+    // the next tile's first barrier orders this update before its reads
+  };
+  __syncthreads();
+  if (ntiles) load(0, kA, vA);
+  uint32_t t = 0;
+  for (; t + 1 < ntiles; t += 2) {
+    load(t + 1, kB, vB);
+    process(t, kA, vA);
+    if (t + 2 < ntiles) load(t + 2, kA, vA);
+    process(t + 1, kB, vB);
+  }
+  if (t < ntiles) process(t, kA, vA);
+}
+
+// Sorts (keys, vals) by keys, stably.  On return *keys / *vals point at the
+// sorted arrays, which are either the inputs or the alternates.
+template <class K>
+int radix_sort_pairs(pj_ctx *ctx, K **keys, K **keys_alt, uint32_t **vals, uint32_t **vals_alt,
+                     uint64_t n) {
+  constexpr int P = sizeof(K);
+  if (n < 2) return PJ_OK;
+  Scratch dh;
+  PJ_TRY(dh.alloc(ctx, P * 256 * 4));
+  PJ_TRY(zero(ctx, dh.p, P * 256 * 4));
+  const uint64_t dgrid = std::min<uint64_t>(ceil_div(n, (uint64_t)256 * 16), (uint64_t)ctx->num_cus * 4);
+  PJ_LAUNCH(ctx, "sort_digits", k_rs_digits<K>, dgrid, 256, 0, *keys, n, dh.as<uint32_t>());
+  std::vector<uint32_t> hh(P * 256);
+  PJ_TRY(d2h(ctx, hh.data(), dh.p, P * 256 * 4));
+  std::vector<int> passes;
+  for (int p = 0; p < P; ++p) {
+    uint32_t mx = 0;
+    for (int d = 0; d < 256; ++d) mx = std::max(mx, hh[p * 256 + d]);
+    if (mx != n) passes.push_back(p);
+  }
+  if (passes.empty()) return PJ_OK;
+  // workgroups that fit the CUs at once (LDS-bound), chunks whole tiles
+  constexpr uint64_t kTile = (uint64_t)RsCfg<K>::BLOCK * RsCfg<K>::IPT;
+  uint64_t G = std::min<uint64_t>((uint64_t)ctx->num_cus * RsCfg<K>::PER_CU, ceil_div(n, kTile));
+  const uint64_t chunk = ceil_div(ceil_div(n, G), kTile) * kTile;
+  G = ceil_div(n, chunk);
+  int slab = 0;
+  const int nslab = colscan_slabs(ctx, (int)G, 256, &slab);
+  Scratch H, E, T, S, Pp, ctr;
+  PJ_TRY(H.alloc(ctx, G * 256 * 4));
+  PJ_TRY(E.alloc(ctx, G * 256 * 4));
+  PJ_TRY(T.alloc(ctx, 256 * 4));
+  PJ_TRY(S.alloc(ctx, 257 * 8));
+  PJ_TRY(Pp.alloc(ctx, (uint64_t)nslab * 256 * 4));
+  PJ_TRY(ctr.alloc(ctx, 256));
+  for (int p : passes) {
+    const int shift = 8 * p;
+    PJ_LAUNCH(ctx, "sort_hist", k_rs_chunk_hist<K>, G, 256, 0, *keys, n, chunk, shift,
+              H.as<uint32_t>());
+    const dim3 cgrid(1, (uint32_t)nslab);
+    PJ_LAUNCH(ctx, "sort_colsum", k_colsum, cgrid, 256, 0, H.as<const uint32_t>(), (int)G, 256,
+              slab, Pp.as<uint32_t>());
+    PJ_LAUNCH(ctx, "sort_colscan", k_colscan, cgrid, 256, 0, H.as<const uint32_t>(), (int)G,
+              256, slab, Pp.as<const uint32_t>(), E.as<uint32_t>(), T.as<uint32_t>());
+    PJ_LAUNCH(ctx, "sort_scan", k_excl_scan, 1, kScanBlock, 0, T.as<const uint32_t>(), 256,
+              S.as<unsigned long long>(), ctr.as<uint32_t>());
+    PJ_LAUNCH(ctx, "sort_scatter", k_rs_scatter<K>, G, RsCfg<K>::BLOCK, 0, *keys, *vals, *keys_alt,
+              *vals_alt, n, chunk, shift, E.as<const uint32_t>(), S.as<const unsigned long long>());
+    std::swap(*keys, *keys_alt);
+    std::swap(*vals, *vals_alt);
+  }
+  return PJ_OK;
+}
+
+// The sort key of row perm[i] of a column whose rows are `stride` bytes
+// apart (K = uint32_t for 32-bit types, uint64_t otherwise).
+template <class K>
+__global__ __launch_bounds__(256) void k_sort_key(const char *__restrict__ col, uint64_t stride,
+                                                  int t, int desc,
+                                                  const int32_t *__restrict__ rank, uint32_t nrank,
+                                                  const uint32_t *__restrict__ perm, uint64_t n,
+                                                  K *__restrict__ key) {
+  const uint64_t gs = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += gs) {
+    const char *p = col + (uint64_t)perm[i] * stride;
     unsigned long long u;
     switch (t) {
       case I32: {
-        int32_t v = ((const int32_t *)col)[r];
+        int32_t v = *(const int32_t *)p;
         if (rank) v = (uint32_t)v < nrank ? rank[v] : (int32_t)nrank;  // unknown codes last
         u = (uint32_t)v ^ 0x80000000u;
         if (desc) u = ~u & 0xffffffffULL;
         break;
       }
       case BOOL: {
-        u = ((const uint8_t *)col)[r] != 0;
+        u = *(const uint8_t *)p != 0;
         if (desc) u = ~u & 0xffffffffULL;
         break;
       }
       case I64: {
-        u = (unsigned long long)((const int64_t *)col)[r] ^ 0x8000000000000000ULL;
+        u = (unsigned long long)*(const int64_t *)p ^ 0x8000000000000000ULL;
         if (desc) u = ~u;
         break;
       }
       default: {  // F64, F32
-        double d = t == F32 ? (double)((const float *)col)[r] : ((const double *)col)[r];
+        double d = t == F32 ? (double)*(const float *)p : *(const double *)p;
         if (d == 0.0) d = 0.0;
         const unsigned long long b = __builtin_bit_cast(unsigned long long, d);
         u = (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
@@ -58,7 +312,7 @@ __global__ __launch_bounds__(256) void k_sort_key(const void *__restrict__ col, 
         break;
       }
     }
-    key[i] = u;
+    key[i] = (K)u;
   }
 }
 
@@ -67,10 +321,18 @@ __global__ __launch_bounds__(256) void k_sort_iota(uint32_t *__restrict__ p, uin
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) p[i] = (uint32_t)i;
 }
 
-int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm) {
+static int type_bytes(int t) {
+  return t == I64 || t == F64 ? 8 : t == BOOL ? 1 : 4;
+}
+
+// Sorts the row permutation perm (n entries, initialised here) by the keys;
+// strides[k] (bytes) defaults to the column's element size.
+static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *strides, int nk,
+                             uint64_t n, uint32_t *perm) {
   if (nk < 0 || (nk && !keys) || (n && !perm)) return fail(PJ_E_ARG, "sort with invalid arguments");
   if (!n) return PJ_OK;
   if (n > (uint64_t)INT32_MAX) return fail(PJ_E_CAPACITY, "sort of more than 2^31-1 rows");
+  bool wide = false;
   for (int k = 0; k < nk; ++k) {
     if (!keys[k].col) return fail(PJ_E_ARG, "sort key %d without a column", k);
     if (keys[k].t != I32 && keys[k].t != I64 && keys[k].t != F64 && keys[k].t != F32 &&
@@ -78,43 +340,95 @@ int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *pe
       return fail(PJ_E_ARG, "sort key %d of unknown type %d", k, keys[k].t);
     if (keys[k].rank && (keys[k].t != I32 || keys[k].nrank > (uint32_t)INT32_MAX))
       return fail(PJ_E_ARG, "sort key %d: ranks need int32 codes", k);
+    wide |= keys[k].dir && !(keys[k].t == I32 || keys[k].t == BOOL);
   }
   uint64_t grid = ceil_div(n, (uint64_t)256);
   if (grid > (uint64_t)ctx->num_cus * 16) grid = (uint64_t)ctx->num_cus * 16;
-  Scratch p1, k0, k1, tmp;
+  Scratch p1, k0, k1;
   PJ_TRY(p1.alloc(ctx, n * 4));
-  PJ_TRY(k0.alloc(ctx, n * 8));
-  PJ_TRY(k1.alloc(ctx, n * 8));
+  PJ_TRY(k0.alloc(ctx, n * (wide ? 8 : 4)));
+  PJ_TRY(k1.alloc(ctx, n * (wide ? 8 : 4)));
   PJ_LAUNCH(ctx, "sort_iota", k_sort_iota, grid, 256, 0, perm, n);
-  hipcub::DoubleBuffer<uint32_t> vals(perm, p1.as<uint32_t>());
-  size_t tmp_bytes = 0;
-  for (int bits : {32, 64}) {
-    hipcub::DoubleBuffer<unsigned long long> kb(k0.as<unsigned long long>(),
-                                                 k1.as<unsigned long long>());
-    size_t b = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, b, kb, vals, (int)n, 0, bits,
-                                           ctx->stream) != hipSuccess)
-      return fail(PJ_E_HIP, "radix sort sizing failed");
-    tmp_bytes = std::max(tmp_bytes, b);
-  }
-  PJ_TRY(tmp.alloc(ctx, tmp_bytes ? tmp_bytes : 1));
+  uint32_t *va = perm, *vb = p1.as<uint32_t>();
   for (int k = nk - 1; k >= 0; --k) {
     if (!keys[k].dir) continue;
-    const int bits = (keys[k].t == I32 || keys[k].t == BOOL) ? 32 : 64;
-    PJ_LAUNCH(ctx, "sort_key", k_sort_key, grid, 256, 0, keys[k].col, keys[k].t,
-              keys[k].dir < 0 ? 1 : 0, keys[k].rank, keys[k].nrank, vals.Current(), n,
-              k0.as<unsigned long long>());
-    hipcub::DoubleBuffer<unsigned long long> kb(k0.as<unsigned long long>(),
-                                                 k1.as<unsigned long long>());
-    size_t bytes = tmp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kb, vals, (int)n, 0, bits,
-                                           ctx->stream) != hipSuccess)
-      return fail(PJ_E_HIP, "radix sort failed");
+    const uint64_t stride = strides ? strides[k] : (uint64_t)type_bytes(keys[k].t);
+    const int desc = keys[k].dir < 0 ? 1 : 0;
+    if (keys[k].t == I32 || keys[k].t == BOOL) {
+      uint32_t *ka = k0.as<uint32_t>(), *kb = k1.as<uint32_t>();
+      PJ_LAUNCH(ctx, "sort_key", k_sort_key<uint32_t>, grid, 256, 0, (const char *)keys[k].col,
+                stride, keys[k].t, desc, keys[k].rank, keys[k].nrank, va, n, ka);
+      PJ_TRY(radix_sort_pairs<uint32_t>(ctx, &ka, &kb, &va, &vb, n));
+    } else {
+      unsigned long long *ka = k0.as<unsigned long long>(), *kb = k1.as<unsigned long long>();
+      PJ_LAUNCH(ctx, "sort_key", k_sort_key<unsigned long long>, grid, 256, 0,
+                (const char *)keys[k].col, stride, keys[k].t, desc, keys[k].rank, keys[k].nrank,
+                va, n, ka);
+      PJ_TRY(radix_sort_pairs<unsigned long long>(ctx, &ka, &kb, &va, &vb, n));
+    }
   }
-  if (vals.Current() != perm)
-    PJ_HIP(hipMemcpyAsync(perm, vals.Current(), n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  if (va != perm) PJ_HIP(hipMemcpyAsync(perm, va, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
   return sync(ctx);
+}
+
+int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm) {
+  return sort_perm_strided(ctx, keys, nullptr, nk, n, perm);
+}
+
+// out row i = in row perm[i], rows of `words` 4-byte words
+__global__ __launch_bounds__(256) void k_gather_rows(const uint32_t *__restrict__ in,
+                                                     const uint32_t *__restrict__ perm,
+                                                     uint64_t n, uint32_t words,
+                                                     uint32_t *__restrict__ out) {
+  const uint64_t total = n * words;
+  const uint64_t gs = (uint64_t)gridDim.x * 256;
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < total; x += gs) {
+    const uint64_t r = x / words, w = x - r * words;
+    out[x] = in[(uint64_t)perm[r] * words + w];
+  }
 }
 
 }  // namespace rel
 }  // namespace pj
+
+using namespace pj;
+using namespace pj::rel;
+
+// qsort_<fields> of core/platform/lib/util/sort/gpu/sort-util.hpp:27-58.
+extern "C" int pj_sort_records(pj_ctx *ctx, void *d_rows, uint64_t n, const char *fields) {
+  if (!ctx || !fields || (n && !d_rows)) return fail(PJ_E_ARG, "pj_sort_records: null argument");
+  const int nf = (int)strlen(fields);
+  if (nf < 1 || nf > 8) return fail(PJ_E_ARG, "pj_sort_records: 1 to 8 fields, got '%s'", fields);
+  // record layout: fields in order at their natural alignment, the record
+  // padded to its widest field
+  std::vector<SortKey> keys(nf);
+  std::vector<uint64_t> offs(nf), strides(nf);
+  uint64_t off = 0, align = 4;
+  for (int i = 0; i < nf; ++i) {
+    const char f = fields[i];
+    if (f != 'i' && f != 'l')
+      return fail(PJ_E_ARG, "pj_sort_records: field type '%c' is neither 'i' (int32) nor 'l' (int64)", f);
+    const uint64_t sz = f == 'i' ? 4 : 8;
+    off = (off + sz - 1) / sz * sz;
+    offs[i] = off;
+    off += sz;
+    align = std::max(align, sz);
+  }
+  const uint64_t rec = (off + align - 1) / align * align;
+  if (!n) return PJ_OK;
+  if (((uintptr_t)d_rows & (align - 1)) != 0)
+    return fail(PJ_E_ARG, "pj_sort_records: rows must be %llu-byte aligned", (unsigned long long)align);
+  for (int i = 0; i < nf; ++i) {
+    keys[i] = SortKey{(const char *)d_rows + offs[i], fields[i] == 'i' ? I32 : I64, 1, nullptr, 0};
+    strides[i] = rec;
+  }
+  Scratch perm, tmp;
+  PJ_TRY(perm.alloc(ctx, n * 4));
+  PJ_TRY(tmp.alloc(ctx, n * rec));
+  PJ_TRY(sort_perm_strided(ctx, keys.data(), strides.data(), nf, n, perm.as<uint32_t>()));
+  uint64_t grid = std::min<uint64_t>(ceil_div(n * (rec / 4), (uint64_t)256), (uint64_t)ctx->num_cus * 16);
+  PJ_LAUNCH(ctx, "sort_gather", k_gather_rows, grid, 256, 0, (const uint32_t *)d_rows,
+            perm.as<const uint32_t>(), n, (uint32_t)(rec / 4), tmp.as<uint32_t>());
+  PJ_HIP(hipMemcpyAsync(d_rows, tmp.p, n * rec, hipMemcpyDeviceToDevice, ctx->stream));
+  return sync(ctx);
+}
