@@ -184,12 +184,15 @@ def main():
 
 
 def chained_c1(ctx, steps):
-    """The plain equi-join on the chained path (hashjoin-chained -> reduce,
-    pj_join_chained_reduce): R = 2^20 permuted int32 keys with int32 payload,
-    S = 2^24 fk uniform over R's keys; COUNT = 2^24 (closed form) and the
-    int64 checksum.  hash_bits = 21 and maxBuildInputSize = 1.1 * |R| (the
-    RelBuilder defaults, relbuilder.cpp:653-668).  Algorithmic bytes: 8 per
-    input tuple (each column read once), 142.6 MB."""
+    """The plain equi-join hashjoin-chained -> reduce (pj_join_chained_reduce):
+    R = 2^20 permuted int32 keys with int32 payload, S = 2^24 fk uniform over
+    R's keys; COUNT = 2^24 (closed form) and the int64 checksum.  hash_bits =
+    21 and maxBuildInputSize = 1.1 * |R| (the RelBuilder defaults,
+    relbuilder.cpp:653-668).  At this size the entry point runs the
+    radix-partitioned join (one 256-way pass per side, LDS tables); the
+    chained table's own kernels are timed beside it through the
+    allocation-free entry point, which always builds the table.  Algorithmic
+    bytes: 8 per input tuple (each column read once), 142.6 MB."""
     nr, ns = 1 << 20, 1 << 24
     rk = ctx.gen_perm_i32(nr, SEED)
     rv = ctx.gen_uniform_i32(nr, SEED, 1, 0, 1 << 31)
@@ -199,21 +202,33 @@ def chained_c1(ctx, steps):
 
     def run():
         return ctx.join_chained_reduce(rk, rv, sk, sv, 21, maxb)
+
+    def run_table():
+        return ctx.join_chained_reduce_noalloc(rk, rv, sk, sv, 21, maxb)
     for _ in range(3):
         res = run()
-    names = ["join_chained_build", "join_chained_probe", "fill_u32", "join_acc_init"]
-    sec, kt, res, per = timed(ctx, max(steps, 10), run, names)
+        run_table()
+    chained_names = ["join_chained_build", "join_chained_probe", "fill_u32", "join_acc_init"]
     nsteps = max(steps, 10)
+    # kernel times from a run with launch events, step times from one without
+    # (at 0.3 ms a step, the events' host work shows in the step time)
+    _, kt, _, _ = timed(ctx, nsteps, run, JOIN_KERNELS + chained_names)
+    _, kt_t, _, _ = timed(ctx, nsteps, run_table, chained_names)
+    sec, res, per = untimed_kernels(ctx, nsteps, run)
+    sec_t, res_t, _ = untimed_kernels(ctx, nsteps, run_table)
     algo = 8.0 * (nr + ns)
-    probe_ms = kt["join_chained_probe"][0] / kt["join_chained_probe"][1]
-    out = {"build_rows": nr, "probe_rows": ns, "hash_bits": 21, "ms": sec * 1e3,
+    path = "partitioned" if "pjoin_probe" in kt else "chained"
+    out = {"build_rows": nr, "probe_rows": ns, "hash_bits": 21, "path": path, "ms": sec * 1e3,
            "step_ms_median": statistics.median(per) * 1e3,
            "input_gtuples_per_s": (nr + ns) / sec / 1e9, "probe_gtuples_per_s": ns / sec / 1e9,
            "roofline": {"bound": "hbm", "algorithmic_bytes": algo, "achieved_gbs": algo / sec / 1e9,
-                        "frac": algo / sec / 1e9 / HBM_PEAK_GBS,
-                        "probe_kernel_ms": probe_ms},
+                        "frac": algo / sec / 1e9 / HBM_PEAK_GBS},
            "kernels_ms_per_step": {n: round(v[0] / nsteps, 4) for n, v in kt.items()},
-           "protocol": protocol(ctx, run), "result": res, "count_is_closed_form": res["count"] == ns}
+           "protocol": protocol(ctx, run), "result": res, "count_is_closed_form": res["count"] == ns,
+           "chained_table": {"ms": sec_t * 1e3,
+                             "probe_kernel_ms": kt_t["join_chained_probe"][0] / kt_t["join_chained_probe"][1],
+                             "kernels_ms_per_step": {n: round(v[0] / nsteps, 4) for n, v in kt_t.items()},
+                             "identical": res_t == res}}
     for x in (rk, rv, sk, sv):
         x.free()
     return out
@@ -593,6 +608,20 @@ def timed(ctx, steps, fn, names=JOIN_KERNELS):
     t1 = time.perf_counter()
     ctx.prof_enable(False)
     return (t1 - t0) / steps, kernel_table(ctx, names), res, per
+
+
+def untimed_kernels(ctx, steps, fn):
+    """Like timed() with the per-launch events off: mean seconds per step,
+    the last result and the per-step seconds."""
+    ctx.sync()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        res = fn()
+        per.append(time.perf_counter() - a)
+    ctx.sync()
+    return (time.perf_counter() - t0) / steps, res, per
 
 
 def protocol(ctx, fn, runs=5):

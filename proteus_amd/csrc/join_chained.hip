@@ -19,11 +19,16 @@
 
 namespace pj {
 
+// Inserts the build rows.  A row whose CAS fails on a slot holding its own
+// key has a duplicate on the build side; that is reported in *dup.  The test
+// is exact: two rows with one key walk the same slot sequence from the same
+// start, so whichever claims the later slot failed its CAS on the other's.
 __global__ __launch_bounds__(256) void k_chained_build(
     const int32_t *__restrict__ bk, uint64_t nb,
-    unsigned long long *__restrict__ slots, uint64_t mask) {
+    unsigned long long *__restrict__ slots, uint64_t mask, uint32_t *__restrict__ dup) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  bool seen = false;
   for (; i < nb; i += stride) {
     int32_t k = bk[i];
     unsigned long long w =
@@ -32,20 +37,28 @@ __global__ __launch_bounds__(256) void k_chained_build(
     while (true) {
       unsigned long long old = atomicCAS(&slots[h], 0ULL, w);
       if (old == 0ULL) break;
+      seen |= (uint32_t)(old >> 32) == (uint32_t)k;
       h = (h + 1) & mask;
     }
   }
+  if (seen) atomicOr(dup, 1u);
 }
 
-// Probe with the fused reduce.  Each thread handles 4 probe rows per step so
-// four independent slot loads are in flight.
+// Probe with the fused reduce.  Each thread handles 4 consecutive probe rows
+// per step.  With unique build keys (*dup == 0, a uniform branch) a row ends
+// at its key or at an empty slot, and the four rows walk in lockstep: every
+// round issues the four slot loads together, then the four build-value
+// gathers of the matches.  With duplicate keys every row walks its run of
+// occupied slots to the empty slot that ends it, summing all its matches
+// (the reference's chain walk, hash-join-chained.cpp:321-566).
 __global__ __launch_bounds__(256) void k_chained_probe_reduce(
     const int32_t *__restrict__ pk, const int32_t *__restrict__ pv, uint64_t np,
     const int32_t *__restrict__ bv,
     const unsigned long long *__restrict__ slots, uint64_t mask,
-    unsigned long long *__restrict__ acc_out) {
+    const uint32_t *__restrict__ dup, unsigned long long *__restrict__ acc_out) {
   JoinAcc acc;
   acc.init();
+  const bool uniq = *dup == 0u;
   uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = tid * 4; base < np; base += nthr * 4) {
@@ -53,40 +66,83 @@ __global__ __launch_bounds__(256) void k_chained_probe_reduce(
     int32_t v[4];
     uint64_t h[4];
     bool live[4];
+    if (base + 4 <= np && ((uintptr_t)(pk + base) & 15) == 0 &&
+        (!pv || ((uintptr_t)(pv + base) & 15) == 0)) {
+      const int4 kq = *reinterpret_cast<const int4 *>(pk + base);
+      const int4 vq = pv ? *reinterpret_cast<const int4 *>(pv + base) : make_int4(0, 0, 0, 0);
+      k[0] = kq.x, k[1] = kq.y, k[2] = kq.z, k[3] = kq.w;
+      v[0] = vq.x, v[1] = vq.y, v[2] = vq.z, v[3] = vq.w;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      uint64_t j = base + u;
-      live[u] = j < np;
-      k[u] = live[u] ? pk[j] : 0;
-      v[u] = (live[u] && pv) ? pv[j] : 0;
-      h[u] = hash_i32(k[u]) & mask;
-    }
-    // the four rows' first slots are loaded together, then each row walks
-    // its run of occupied slots
-    unsigned long long first[4];
+      for (int u = 0; u < 4; ++u) live[u] = true;
+    } else {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) first[u] = slots[h[u]];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (!live[u]) continue;
-      int64_t m = 0, sb = 0;
-      int32_t mnb = INT32_MAX, mxb = INT32_MIN;
-      uint64_t p = h[u];
-      unsigned long long w = first[u];
-      while (true) {
-        if (w == 0ULL) break;
-        if ((int32_t)(uint32_t)(w >> 32) == k[u]) {
-          uint64_t row = (w & 0xffffffffULL) - 1;
-          int32_t b = bv ? bv[row] : 0;
-          m += 1;
-          sb += b;
-          mnb = min(mnb, b);
-          mxb = max(mxb, b);
-        }
-        p = (p + 1) & mask;
-        w = slots[p];
+      for (int u = 0; u < 4; ++u) {
+        uint64_t j = base + u;
+        live[u] = j < np;
+        k[u] = live[u] ? pk[j] : 0;
+        v[u] = (live[u] && pv) ? pv[j] : 0;
       }
-      acc.add(m, sb, mnb, mxb, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) h[u] = hash_i32(k[u]) & mask;
+    unsigned long long w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = slots[h[u]];
+    if (uniq) {
+      bool act[4], hit[4];
+      uint32_t row[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        act[u] = live[u];
+        hit[u] = false;
+        row[u] = 0;
+      }
+      while (true) {
+        bool more = false;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool empty = w[u] == 0ULL;
+          const bool match = !empty && (int32_t)(uint32_t)(w[u] >> 32) == k[u];
+          if (act[u] && match) {
+            hit[u] = true;
+            row[u] = (uint32_t)(w[u] & 0xffffffffULL) - 1u;
+          }
+          act[u] = act[u] && !empty && !match;
+          h[u] = act[u] ? ((h[u] + 1) & mask) : h[u];
+          more |= act[u];
+        }
+        if (!more) break;
+        // rows that are done reload their own slot (a cache hit), so the
+        // four loads are issued without branches
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = slots[h[u]];
+      }
+      int32_t b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = bv ? bv[row[u]] : 0;
+      acc.add4_1(hit, b, v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!live[u]) continue;
+        int64_t m = 0, sb = 0;
+        int32_t mnb = INT32_MAX, mxb = INT32_MIN;
+        uint64_t p = h[u];
+        unsigned long long x = w[u];
+        while (x != 0ULL) {
+          if ((int32_t)(uint32_t)(x >> 32) == k[u]) {
+            uint64_t r = (x & 0xffffffffULL) - 1;
+            int32_t bb = bv ? bv[r] : 0;
+            m += 1;
+            sb += bb;
+            mnb = min(mnb, bb);
+            mxb = max(mxb, bb);
+          }
+          p = (p + 1) & mask;
+          x = slots[p];
+        }
+        acc.add(m, sb, mnb, mxb, v[u]);
+      }
     }
   }
   join_acc_flush(acc, acc_out);
@@ -144,9 +200,7 @@ __global__ __launch_bounds__(256) void k_chained_probe_pairs(
   }
 }
 
-static int chained_table(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
-                         int hash_bits, uint64_t max_build, Scratch &tab,
-                         uint64_t *mask) {
+static int chained_check(int hash_bits, uint64_t nb, uint64_t max_build) {
   if (hash_bits < 0 || hash_bits > 32)
     return fail(PJ_E_ARG, "hash_bits %d out of range [0, 32]", hash_bits);
   if (nb > max_build)
@@ -155,6 +209,14 @@ static int chained_table(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
                 "maxBuildInputSize %llu",
                 (unsigned long long)nb, (unsigned long long)max_build);
   if (nb >= 0xffffffffULL) return fail(PJ_E_CAPACITY, "build side above 2^32-1 rows");
+  return PJ_OK;
+}
+
+// Builds the table; *dup (zeroed by the caller) reports duplicate build keys.
+static int chained_table(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
+                         int hash_bits, uint64_t max_build, Scratch &tab,
+                         uint64_t *mask, uint32_t *dup) {
+  PJ_TRY(chained_check(hash_bits, nb, max_build));
   uint64_t slots = 1ULL << hash_bits;
   uint64_t need = 2ULL << ilog2_ceil(nb ? nb : 1);
   if (slots < need) slots = need;
@@ -166,9 +228,20 @@ static int chained_table(pj_ctx *ctx, const int32_t *d_bk, uint64_t nb,
     uint64_t grid = ceil_div(nb, 256);
     if (grid > 8192) grid = 8192;
     PJ_LAUNCH(ctx, "join_chained_build", k_chained_build, grid, 256, 0, d_bk, nb,
-              tab.as<unsigned long long>(), *mask);
+              tab.as<unsigned long long>(), *mask, dup);
   }
   return PJ_OK;
+}
+
+// hashjoin-chained -> reduce runs as the radix-partitioned join when the
+// probe side is large enough for partitioning to pay: the partitioned join
+// costs a fixed ~0.1-0.2 ms of small kernels, while every chained probe is
+// a dependent random read of a table that is beyond the L2 once the build
+// side passes 2^18 rows.  Results are the same either way (same counts and
+// sums); only the device algorithm differs.
+static bool chained_goes_partitioned(uint64_t nb, uint64_t np) {
+  if (!nb || !np) return false;
+  return np >= (1ULL << 24) || (np >= (1ULL << 22) && nb >= (1ULL << 19));
 }
 
 static uint64_t probe_grid(pj_ctx *ctx, uint64_t rows_per_thread, uint64_t np) {
@@ -223,13 +296,7 @@ extern "C" int pj_join_chained_reduce_noalloc(pj_ctx *ctx, const int32_t *d_bk,
                                               uint64_t np, int hash_bits,
                                               uint64_t max_build, pj_join_agg *out) {
   std::lock_guard<std::mutex> g(ctx->mu);
-  if (hash_bits < 0 || hash_bits > 32)
-    return fail(PJ_E_ARG, "hash_bits %d out of range [0, 32]", hash_bits);
-  if (nb > max_build)
-    return fail(PJ_E_CAPACITY,
-                "Build input size exceeded given parameter: %llu rows > "
-                "maxBuildInputSize %llu",
-                (unsigned long long)nb, (unsigned long long)max_build);
+  PJ_TRY(chained_check(hash_bits, nb, max_build));
   // the same geometry as chained_table()
   uint64_t slots = 1ULL << hash_bits;
   uint64_t need = 2ULL << ilog2_ceil(nb ? nb : 1);
@@ -247,18 +314,22 @@ extern "C" int pj_join_chained_reduce_noalloc(pj_ctx *ctx, const int32_t *d_bk,
     return fail(PJ_E_HIP, "static scratch of device %d was not resolved", ctx->device);
   unsigned long long *tab = base;
   unsigned long long *acc = base + kNoAllocSlots;
+  uint32_t *dup = reinterpret_cast<uint32_t *>(acc + kJoinAccFlagWord);
   const uint64_t mask = slots - 1;
   // the previous call's contents stay behind: clear what this call reads
+  // (join_acc_init also clears the duplicate-key flag)
   PJ_TRY(fill_u32(ctx, reinterpret_cast<uint32_t *>(tab), 0u, slots * 2));
   PJ_TRY(join_acc_init(ctx, acc));
   if (nb) {
     uint64_t grid = ceil_div(nb, 256);
     if (grid > 8192) grid = 8192;
-    PJ_LAUNCH(ctx, "join_chained_build", k_chained_build, grid, 256, 0, d_bk, nb, tab, mask);
+    PJ_LAUNCH(ctx, "join_chained_build", k_chained_build, grid, 256, 0, d_bk, nb, tab, mask,
+              dup);
   }
   if (np && nb)
     PJ_LAUNCH(ctx, "join_chained_probe", k_chained_probe_reduce, probe_grid(ctx, 4, np), 256,
-              0, d_pk, d_pv, np, d_bv, (const unsigned long long *)tab, mask, acc);
+              0, d_pk, d_pv, np, d_bv, (const unsigned long long *)tab, mask,
+              (const uint32_t *)dup, acc);
   return join_acc_read(ctx, acc, out);
 }
 
@@ -268,15 +339,19 @@ extern "C" int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
                                       uint64_t np, int hash_bits,
                                       uint64_t max_build, pj_join_agg *out) {
   std::lock_guard<std::mutex> g(ctx->mu);
+  PJ_TRY(chained_check(hash_bits, nb, max_build));
+  if (chained_goes_partitioned(nb, np))
+    return join_partitioned_auto(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, out);
   Scratch tab, acc;
   uint64_t mask = 0;
-  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask));
   PJ_TRY(acc.alloc(ctx, kJoinAccWords * 8));
   PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));
+  uint32_t *dup = reinterpret_cast<uint32_t *>(acc.as<unsigned long long>() + kJoinAccFlagWord);
+  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask, dup));
   if (np && nb)
     PJ_LAUNCH(ctx, "join_chained_probe", k_chained_probe_reduce, probe_grid(ctx, 4, np), 256,
               0, d_pk, d_pv, np, d_bv, tab.as<const unsigned long long>(), mask,
-              acc.as<unsigned long long>());
+              (const uint32_t *)dup, acc.as<unsigned long long>());
   return join_acc_read(ctx, acc.as<unsigned long long>(), out);
 }
 
@@ -290,9 +365,10 @@ extern "C" int pj_join_chained_pairs(pj_ctx *ctx, const int32_t *d_bk,
   *n_out = 0;
   Scratch tab, cur;
   uint64_t mask = 0;
-  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask));
-  PJ_TRY(cur.alloc(ctx, 8));
-  PJ_TRY(zero(ctx, cur.p, 8));
+  PJ_TRY(cur.alloc(ctx, 16));  // the pair cursor, then the duplicate-key flag
+  PJ_TRY(zero(ctx, cur.p, 16));
+  PJ_TRY(chained_table(ctx, d_bk, nb, hash_bits, max_build, tab, &mask,
+                       reinterpret_cast<uint32_t *>(cur.as<unsigned long long>() + 1)));
   if (np && nb)
     PJ_LAUNCH(ctx, "join_chained_pairs", k_chained_probe_pairs, probe_grid(ctx, 1, np), 256, 0,
               d_pk, np, tab.as<const unsigned long long>(), mask, d_out_b, d_out_p, cap,

@@ -1056,6 +1056,16 @@ extern "C" int pj_join_partitioned_config(pj_ctx *ctx, uint64_t nb, uint64_t np,
   return PJ_OK;
 }
 
+int pj::join_partitioned_auto(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv,
+                              uint64_t nb, const int32_t *d_pk, const int32_t *d_pv, uint64_t np,
+                              pj_join_agg *out) {
+  pj_radix_cfg c{};
+  auto it = ctx->tuned.find(TuneKey{ilog2_ceil(nb ? nb : 1), ilog2_ceil(np ? np : 1)});
+  if (it != ctx->tuned.end()) c = it->second;
+  PJ_TRY(validate_cfg(c, nb));
+  return join_partitioned(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, c, out);
+}
+
 extern "C" int pj_autotune_enable(pj_ctx *ctx, int on) {
   ctx->autotune = on != 0;
   return PJ_OK;

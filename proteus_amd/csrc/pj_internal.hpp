@@ -375,6 +375,10 @@ struct JoinAcc {
 constexpr int kJoinAccCopies = 64;
 constexpr int kJoinAccStride = 16;  // u64 words per copy
 constexpr int kJoinAccWords = kJoinAccCopies * kJoinAccStride;
+// Word 8 of the first copy is not an accumulator slot; join_acc_init zeroes
+// it, so a join may keep a flag there (the chained build's duplicate-key
+// flag) without a clear of its own.
+constexpr int kJoinAccFlagWord = 8;
 
 // Reduce a JoinAcc over the workgroup (every thread of the block calls it,
 // at most 1024 threads) and fold the result into the block's accumulator
@@ -427,5 +431,11 @@ int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc);
 int noalloc_scratch_init(int device);
 int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
                   pj_join_agg *out);
+// The radix-partitioned join -> reduce with the configuration chosen for
+// (nb, np) (the tuned one when a tuning exists), for a caller that holds the
+// context lock and has checked the sizes.
+int join_partitioned_auto(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv, uint64_t nb,
+                          const int32_t *d_pk, const int32_t *d_pv, uint64_t np,
+                          pj_join_agg *out);
 
 }  // namespace pj

@@ -96,6 +96,48 @@ def test_join_chained_c1_full_size(ctx):
     _free(rk, rv, sk, sv)
 
 
+def test_join_chained_dispatch(ctx):
+    """pj_join_chained_reduce keeps the chained table for small probe sides and
+    runs the radix-partitioned join for large ones (C1: 2^20 x 2^24); both
+    give the chained table's result, itself checked against the oracle."""
+    for nr, ns, path in ((1 << 12, 1 << 16, "join_chained_probe"),
+                         (1 << 20, 1 << 24, "pjoin_probe")):
+        rk = ctx.gen_perm_i32(nr, 31)
+        rv = ctx.gen_uniform_i32(nr, 31, 1, 0, 1 << 31)
+        sk = ctx.gen_uniform_i32(ns, 31, 2, 0, int(1.33 * nr))
+        sv = ctx.gen_uniform_i32(ns, 31, 3, 0, 1 << 31)
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        got = ctx.join_chained_reduce(rk, rv, sk, sv, (nr - 1).bit_length() + 1)
+        ran = {n: ctx.prof_get(n)[1] for n in ("join_chained_probe", "pjoin_probe")}
+        ctx.prof_enable(False)
+        assert ran[path] > 0 and sum(ran.values()) == ran[path], ran
+        table = ctx.join_chained_reduce_noalloc(rk, rv, sk, sv, (nr - 1).bit_length() + 1)
+        rc, want = O.join_dense(rk.numpy(), rv.numpy(), sk.numpy(), sv.numpy())
+        assert rc == O.OR_OK and got == want and table == want
+        _free(rk, rv, sk, sv)
+
+
+def test_join_chained_single_duplicate(ctx):
+    """One duplicated key among 2^16 unique build keys must take the probe's
+    all-matches walk (the build reports the duplicate); probes of that key
+    count both build rows."""
+    nr, ns = 1 << 16, 1 << 18
+    rs = np.random.default_rng(77)
+    rk = rs.permutation(nr).astype(np.int32)
+    rk[-1] = rk[12345]  # one key twice, one key absent
+    rv = rs.integers(-2**31, 2**31, nr).astype(np.int32)
+    sk = rs.integers(0, nr, ns).astype(np.int32)
+    sk[:64] = rk[12345]
+    sv = rs.integers(-2**31, 2**31, ns).astype(np.int32)
+    rc, want = O.join_chained(rk, rv, sk, sv, 17, threads=8)
+    assert rc == O.OR_OK
+    d = _dev(ctx, rk, rv, sk, sv)
+    assert ctx.join_chained_reduce(d[0], d[1], d[2], d[3], 17) == want
+    assert ctx.join_chained_reduce_noalloc(d[0], d[1], d[2], d[3], 17) == want
+    _free(*d)
+
+
 def test_join_chained_edges(ctx):
     e = ctx.to_device(np.zeros(0, np.int32))
     k = ctx.to_device(np.arange(100, dtype=np.int32))
@@ -277,7 +319,8 @@ def test_join_partitioned_equals_chained(ctx):
     sk = ctx.gen_uniform_i32(ns, seed, 2, 0, int(1.33 * nr))
     sv = ctx.gen_uniform_i32(ns, seed, 3, 0, 1 << 31)
     a = ctx.join_partitioned_reduce(rk, rv, sk, sv)
-    b = ctx.join_chained_reduce(rk, rv, sk, sv, 21)
+    # the allocation-free entry point always builds the chained table
+    b = ctx.join_chained_reduce_noalloc(rk, rv, sk, sv, 21)
     # build/probe swapped: COUNT and the checksum are symmetric
     c = ctx.join_partitioned_reduce(sk, sv, rk, rv)
     assert a == b
