@@ -179,9 +179,40 @@ def test_fmix32_matches_library_murmur3():
         assert fm == murmurhash3_32(int(np.uint32(key).view(np.int32)), seed=0, positive=True)
 
 
+def _unxorshift64(y, s):
+    """Inverse of x -> x ^ (x >> s) on 64 bits (fixed-point iteration: each
+    round fixes s more high bits)."""
+    x = y
+    for _ in range(64 // s + 1):
+        x = y ^ (x >> s)
+    return x
+
+
+def test_hash_i32_inverts_to_sign_extended_key():
+    """hashInt32 (or_hash_i32, expressions-hasher.cpp:39-58) is MurmurHash3's
+    fmix32 step sequence (shifts 16, 13, 16; multipliers 0x85ebca6b,
+    0xc2b2ae35, as published with MurmurHash3) evaluated in 64-bit arithmetic
+    on the sign-extended key.  Each step is a bijection of 64-bit words, so
+    undoing them (xorshift inverses, multiplicative inverses mod 2^64 computed
+    here) must give back the sign-extended key exactly.  A wrong shift, a
+    wrong constant, a 32-bit evaluation or a zero extension all fail this."""
+    M = (1 << 64) - 1
+    inv1 = pow(0x85ebca6b, -1, 1 << 64)
+    inv2 = pow(0xc2b2ae35, -1, 1 << 64)
+    rs = np.random.default_rng(11)
+    keys = [0, 1, -1, 2, -2, 0x7fffffff, -0x80000000] + rs.integers(-2**31, 2**31, 300).tolist()
+    for k in keys:
+        h = int(O.hash_i32(int(k))) & M
+        x = _unxorshift64(h, 16)
+        x = (x * inv2) & M
+        x = _unxorshift64(x, 13)
+        x = (x * inv1) & M
+        x = _unxorshift64(x, 16)
+        assert x == (int(k) & M), k
+
+
 def test_hash_special_cases():
-    """hashInt32 (or_hash_i32) is parity unpinned: no reference vectors exist,
-    so only its special cases are checked here."""
+    """Special cases of hashInt32 (or_hash_i32) and the rearrange bucket."""
     assert O.hash_i32(0) == 0
     # bucket of a 1-bucket rearrange is 0; buckets cover [0, b)
     assert all(O.rearrange_bucket(k, 1) == 0 for k in range(-5, 5))
