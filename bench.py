@@ -237,8 +237,10 @@ def chained_c1(ctx, steps):
 def groupby_c3(ctx, steps):
     """Hash group-by of 2^29 rows (int32 key, int64 v uniform [0, 2^20),
     float32 w = u/2^24): COUNT, SUM, MIN, MAX of v and FP64 SUM of w per
-    group, for 2^10 / 2^16 / 2^22 groups, uniform and Zipf(0.8) keys; each
-    timed through the C ABI (pj_groupby_i32) and as a plan through
+    group, for 2^10 / 2^16 / 2^22 groups, uniform and Zipf(0.8) keys (dense:
+    direct LDS tables) and the uniform keys spread over all of int32 by a
+    bijection (sparse: hashed LDS tables); each timed through the C ABI
+    (pj_groupby_i32), and the dense ones also as a plan through
     pj_plan_execute (scan -> groupby -> reduce(COUNT(*), SUM(cnt)), the
     group-by lowered to the same LDS kernels; the reduce keeps the printed
     result small).  Algorithmic bytes: 16 per row read + 44 per group
@@ -257,11 +259,20 @@ def groupby_c3(ctx, steps):
     for g in (1 << 10, 1 << 16, 1 << 22):
         outs = [ctx.empty(g, t) for t in (np.int32, np.int64, np.int64, np.int64, np.int64,
                                           np.float64)]
-        for dist in ("uniform", "zipf0.8"):
+        for dist in ("uniform", "zipf0.8", "sparse"):
             if dist == "uniform":
                 k = ctx.gen_uniform_i32(n, SEED, 4, 0, g)
-            else:
+            elif dist == "zipf0.8":
                 k = ctx.gen_zipf_i32(n, SEED, 4, g, 0.8, SEED)
+            else:
+                # the uniform keys times an odd constant (a bijection of
+                # int32): the same groups spread over all of int32, so the
+                # hashed tables run instead of the direct ones
+                k0 = ctx.gen_uniform_i32(n, SEED, 4, 0, g)
+                kh = (k0.numpy().view(np.uint32) * np.uint32(2654435761)).view(np.int32)
+                k0.free()
+                k = ctx.to_device(kh)
+                del kh
             ng = C.c_uint64()
             hb = max(1, (g - 1).bit_length())
 
@@ -270,13 +281,25 @@ def groupby_c3(ctx, steps):
                                              outs[1].ptr, outs[2].ptr, outs[3].ptr, outs[4].ptr,
                                              outs[5].ptr, C.byref(ng)))
             run()
-            names = ["radix_hist", "radix_colsum", "radix_colscan", "radix_scan", "radix_items",
-                     "gb_scatter1", "gb_scatter2", "gb_agg", "gb_merge", "gb_compact"]
+            names = ["gb_key_minmax", "radix_hist", "radix_colsum", "radix_colscan",
+                     "radix_scan", "radix_items", "gb_scatter1", "gb_scatter2", "gb_agg",
+                     "gb_merge", "gb_compact"]
             sec, kt, _, _ = timed(ctx, steps, run, names)
             m = ng.value
             cnt = outs[1].numpy()[:m]
             sums = outs[2].numpy()[:m]
             ok = int(cnt.sum()) == n and int(sums.sum()) == vsum
+            gbs = (16.0 * n + 44.0 * m) / sec / 1e9
+            entry = {
+                "groups": g, "keys": dist, "ms": sec * 1e3, "rows_per_s": n / sec,
+                "groups_out": m, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                "tables": "hashed" if dist == "sparse" else "direct",
+                "conservation_ok": ok,
+                "kernels_ms_per_call": {x: round(t / steps, 4) for x, (t, c) in kt.items()}}
+            out["configs"].append(entry)
+            if dist == "sparse":
+                k.free()
+                continue
             # the same group-by as a plan
             ctx.clear_catalog()
             ctx.register_column("C3", "k", "int", k)
@@ -296,15 +319,9 @@ def groupby_c3(ctx, steps):
             psec, pkt, _, _ = timed(ctx, steps, lambda: ctx.execute_plan(text),
                                     ["gb_agg", "rel_groupby"])
             ctx.clear_catalog()
-            gbs = (16.0 * n + 44.0 * m) / sec / 1e9
-            out["configs"].append({
-                "groups": g, "keys": dist, "ms": sec * 1e3, "rows_per_s": n / sec,
-                "groups_out": m, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
-                "conservation_ok": ok,
-                "kernels_ms_per_call": {x: round(t / steps, 4) for x, (t, c) in kt.items()},
-                "plan_ms": psec * 1e3, "plan_vs_direct": psec / sec,
-                "plan_kernels": {x: c for x, (t, c) in pkt.items()},
-                "plan_result_ok": res == [m, n, vsum]})
+            entry.update({"plan_ms": psec * 1e3, "plan_vs_direct": psec / sec,
+                          "plan_kernels": {x: c for x, (t, c) in pkt.items()},
+                          "plan_result_ok": res == [m, n, vsum]})
             k.free()
         for o in outs:
             o.free()

@@ -18,7 +18,19 @@
 //     radix-partitioned on the top bits of fmix32(key) (the join's scatter
 //     kernel, 16-byte rows), so each partition's groups fit one table;
 //   * units of the same partition are merged by the same kernel (entries in,
-//     entries out) until one unit per partition remains -- no global atomics.
+//     entries out) until one unit per partition remains -- no global atomics;
+//   * dense keys skip the hashing.  Partition p holds the keys whose bits
+//     [11, 11 + pd) equal p, and its table is direct: slot = key & 2047.  Two
+//     keys of one partition with the same slot agree on their low 11 + pd
+//     bits, so they differ by a multiple of 2^(11+pd); when the keys' span
+//     (max - min + 1) is at most 2^(11+pd) that cannot happen, and the key of
+//     slot s is kmin + ((p << 11 | s) - kmin) mod 2^(11+pd).  The histogram
+//     pass records min and max on the way (pd > 0); with no partitions the
+//     aggregation kernel checks each unit's span itself and abandons the
+//     attempt when one exceeds 2048.  A direct table has no probe loop and
+//     no CAS, and a slot is live when its count is non-zero.  The probe loop
+//     of the hashed table, not the LDS atomics, bounds the hashed kernel: at
+//     2^10 groups a direct table aggregates 2^29 rows in 1.5 ms against 2.8.
 // Exact int64 arithmetic for COUNT/SUM/MIN/MAX; FSUM accumulates in FP64
 // (Proteus float is FP64, core/olap/lib/values/expressionTypes.cpp:87-89).
 #include <algorithm>
@@ -35,6 +47,15 @@ constexpr int kGbCap = 1024;     // groups per table the planner aims for
 constexpr int kGbBlock = 1024;  // one workgroup per CU (96 KiB of LDS), 16 waves
 constexpr int kGbMergeFanIn = 64;     // merge fan-in with many partitions
 constexpr int kGbMergeFanInFew = 8;   // with at most 64 partitions
+constexpr int kGbDirectLog = 11;      // a direct table covers 2^11 = kGbSlots keys
+static_assert((1 << kGbDirectLog) == kGbSlots, "direct range = table size");
+
+// Table modes of k_gb_agg.
+enum GbMode : int {
+  kGbHashed = 0,  // fmix32 slots, CAS insert
+  kGbDirect = 1,  // slot = key & 2047, key = kbase + ((part << 11 | s) - kbase) & wmask
+  kGbProbe = 2,   // level 0 without partitions: direct, each unit checks its own span
+};
 
 struct GbEntries {  // SoA entry arrays of one level
   int32_t *k;
@@ -70,13 +91,24 @@ __device__ __forceinline__ bool gb_insert(unsigned long long *tk, int32_t key,
   return false;
 }
 
+// The slot of `key` in the unit's table: direct modes take its low 11
+// bits; hashed finds or claims it with gb_insert (false: the table is full).
+template <int MODE>
+__device__ __forceinline__ bool gb_slot(unsigned long long *tk, int32_t key, uint32_t &s) {
+  if (MODE != kGbHashed) {
+    s = (uint32_t)key & (kGbSlots - 1);
+    return true;
+  }
+  return gb_insert(tk, key, s);
+}
+
 // Adds row (key, v, w) to the LDS table; false when the table is full.
-template <bool HV, bool HW>
+template <bool HV, bool HW, int MODE>
 __device__ __forceinline__ bool gb_add_row(unsigned long long *tk, unsigned long long *tcnt,
                                            long long *tsum, long long *tmin, long long *tmax,
                                            double *tfs, int32_t key, int64_t v, float w) {
   uint32_t s;
-  if (!gb_insert(tk, key, s)) return false;
+  if (!gb_slot<MODE>(tk, key, s)) return false;
   atomicAdd(&tcnt[s], 1ULL);
   if (HV) {
     atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)v);
@@ -89,32 +121,54 @@ __device__ __forceinline__ bool gb_add_row(unsigned long long *tk, unsigned long
 
 // One unit per workgroup iteration.  MERGE = inputs are entries of the
 // previous level's units; otherwise raw rows (k, v if HV, w if HW).
-template <bool MERGE, bool HV, bool HW>
+// MODE kGbDirect: the key of slot s is kbase + (((part << 11) | s) - kbase)
+// & wmask.  MODE kGbProbe (level 0, no partitions): each unit's key span is
+// checked; a span above 2048 sets *abort and every workgroup stops at its
+// next unit, otherwise the unit's own minimum is kbase with wmask 2047; the
+// keys' min / max also go to range[0] / range[1].
+template <bool MERGE, bool HV, bool HW, int MODE>
 __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
     const GbUnit *__restrict__ units, uint32_t nunits,
     const int32_t *__restrict__ rk, const int64_t *__restrict__ rv,
     const float *__restrict__ rw, GbEntries in, const uint32_t *__restrict__ in_count,
     GbEntries out, uint32_t *__restrict__ out_count, uint32_t *__restrict__ overflow,
-    bool vec) {
-  __shared__ unsigned long long tk[kGbSlots];
+    bool vec, uint32_t kbase, uint32_t wmask, int *__restrict__ range,
+    uint32_t *__restrict__ abort) {
+  static_assert(MODE != kGbProbe || !MERGE, "the span check is for level 0");
+  __shared__ unsigned long long tk[MODE == kGbHashed ? kGbSlots : 1];
   __shared__ unsigned long long tcnt[kGbSlots];
   __shared__ long long tsum[kGbSlots], tmin[kGbSlots], tmax[kGbSlots];
   __shared__ double tfs[kGbSlots];
   __shared__ uint32_t wave_off[kGbBlock / 64 + 1];
+  __shared__ int s_lo, s_hi;
+  __shared__ uint32_t s_abort;
   const int tid = threadIdx.x;
   // units round-robin over the grid (the host makes many similar units)
   for (uint32_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     __syncthreads();
     for (int i = tid; i < kGbSlots; i += kGbBlock) {
-      tk[i] = 0ULL;
+      if (MODE == kGbHashed) tk[i] = 0ULL;
       tcnt[i] = 0ULL;
       tsum[i] = 0;
       tmin[i] = INT64_MAX;
       tmax[i] = INT64_MIN;
       tfs[i] = 0.0;
     }
+    if (MODE == kGbProbe && tid == 0) {
+      s_lo = INT32_MAX;
+      s_hi = INT32_MIN;
+      s_abort = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
+    if (MODE == kGbProbe && s_abort) break;
     const GbUnit U = units[u];
+    int lo = INT32_MAX, hi = INT32_MIN;  // kGbProbe: this thread's keys
+    auto note = [&](int32_t k) {
+      if (MODE == kGbProbe) {
+        lo = min(lo, k);
+        hi = max(hi, k);
+      }
+    };
     bool ovf = false;
     if (!MERGE) {
       // Rows [s0, s1): with `vec` (every column 16-byte aligned) the
@@ -127,8 +181,10 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
       const uint64_t s0 = U.begin, s1 = U.begin + U.len;
       const uint64_t a0 = (s0 + 3) & ~3ULL, a1 = s1 & ~3ULL;
       auto row = [&](uint64_t x) {
-        ovf |= !gb_add_row<HV, HW>(tk, tcnt, tsum, tmin, tmax, tfs, rk[x], HV ? rv[x] : 0,
-                                   HW ? rw[x] : 0.f);
+        const int32_t k = rk[x];
+        note(k);
+        ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmin, tmax, tfs, k, HV ? rv[x] : 0,
+                                         HW ? rw[x] : 0.f);
       };
       if (!vec || a0 >= a1) {
         for (uint64_t x = s0 + tid; x < s1; x += kGbBlock) row(x);
@@ -169,8 +225,11 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
           const float ww[4] = {HW ? r.w.x : 0.f, HW ? r.w.y : 0.f, HW ? r.w.z : 0.f,
                                HW ? r.w.w : 0.f};
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
-            ovf |= !gb_add_row<HV, HW>(tk, tcnt, tsum, tmin, tmax, tfs, kk[c], vv[c], ww[c]);
+          for (int c = 0; c < 4; ++c) {
+            note(kk[c]);
+            ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmin, tmax, tfs, kk[c], vv[c],
+                                             ww[c]);
+          }
         };
         uint32_t q = tid;
         Vec A = load(q), B = load(q + kGbBlock);
@@ -192,7 +251,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
         const uint64_t b = src * kGbSlots;
         for (uint32_t j = tid; j < n; j += kGbBlock) {
           uint32_t s;
-          if (!gb_insert(tk, in.k[b + j], s)) {
+          if (!gb_slot<MODE>(tk, in.k[b + j], s)) {
             ovf = true;
             continue;
           }
@@ -205,13 +264,40 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
       }
     }
     if (ovf) atomicOr(overflow, 1u);
+    uint32_t kb = kbase, wm = wmask, part = U.part;
+    if (MODE == kGbProbe) {
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+      }
+      if ((tid & 63) == 0 && lo <= hi) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+      }
+    }
     __syncthreads();
+    if (MODE == kGbProbe) {
+      if (s_lo <= s_hi) {
+        if ((int64_t)s_hi - (int64_t)s_lo >= kGbSlots) {
+          if (tid == 0) __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;  // uniform: every thread read the same s_lo / s_hi
+        }
+        if (tid == 0) {
+          atomicMin(&range[0], s_lo);
+          atomicMax(&range[1], s_hi);
+        }
+      }
+      kb = (uint32_t)s_lo;
+      wm = kGbSlots - 1;
+      part = 0;
+    }
     // scan: compact live slots into this unit's output area (wave ballots)
     const int lane = tid & 63, wid = tid >> 6;
     uint32_t running = 0;
     for (int base = 0; base < kGbSlots; base += kGbBlock) {
       int i = base + tid;
-      bool live = tk[i] != 0ULL;
+      bool live = MODE == kGbHashed ? tk[i] != 0ULL : tcnt[i] != 0ULL;
       unsigned long long bal = __ballot(live);
       uint32_t before = __popcll(bal & ((1ULL << lane) - 1ULL));
       if (lane == 0) wave_off[wid] = (uint32_t)__popcll(bal);
@@ -223,7 +309,9 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
       }
       if (live) {
         uint64_t o = (uint64_t)u * kGbSlots + running + woff + before;
-        out.k[o] = (int32_t)(uint32_t)tk[i];
+        out.k[o] = MODE == kGbHashed
+                       ? (int32_t)(uint32_t)tk[i]
+                       : (int32_t)(kb + ((((part << kGbDirectLog) | (uint32_t)i) - kb) & wm));
         out.cnt[o] = (int64_t)tcnt[i];
         out.sum[o] = tsum[i];
         out.mn[o] = tmin[i];
@@ -274,17 +362,18 @@ struct GbLevel {
   }
 };
 
-template <int F>
+template <int F, class Joint>
 static int gb_pass(pj_ctx *ctx, const char *name, bool pass2, const int32_t *ik,
                    const int64_t *iv, const float *iw, int32_t *ok, int64_t *ov, float *ow,
-                   uint64_t n, const RadixPlan &p, JointRadix j, RadixHist &h) {
-  return radix_launch_pass<F, 2048, 256, JointRadix, int64_t, float, false>(
+                   uint64_t n, const RadixPlan &p, Joint j, RadixHist &h) {
+  return radix_launch_pass<F, 2048, 256, Joint, int64_t, float, false>(
       ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
 }
 
+template <class Joint>
 static int gb_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
                         const int32_t *ik, const int64_t *iv, const float *iw, int32_t *ok,
-                        int64_t *ov, float *ow, uint64_t n, const RadixPlan &p, JointRadix j,
+                        int64_t *ov, float *ow, uint64_t n, const RadixPlan &p, Joint j,
                         RadixHist &h) {
   switch (bits) {
     case 1: return gb_pass<2>(ctx, name, pass2, ik, iv, iw, ok, ov, ow, n, p, j, h);
@@ -298,21 +387,114 @@ static int gb_pass_bits(pj_ctx *ctx, const char *name, int bits, bool pass2,
   }
 }
 
+// How the group-by partitions its rows and which tables it uses.
+struct GbPlan {
+  int pbits = 0;         // partition bits (0: one partition)
+  int mode = kGbHashed;  // kGbHashed: fmix32 digits; kGbDirect: JointLow digits
+  uint32_t kbase = 0;    // kGbDirect: the key of slot s of partition p is
+  uint32_t wmask = 0;    //   kbase + (((p << 11) | s) - kbase) & wmask
+};
+
+// Histogram of the partition digits (read back to the host).  Direct plans
+// partition on key bits [11, 11 + pbits) and record min / max in `range`.
+static int gb_hist(pj_ctx *ctx, const int32_t *d_k, uint64_t n, int pbits, bool direct,
+                   int *range, RadixPlan &plan, RadixHist &rh) {
+  const int b1 = pbits <= 7 ? pbits : (pbits + 1) / 2;
+  plan = radix_make_plan(ctx, n, b1, pbits - b1, 1 << pbits, 2);
+  PJ_TRY(rh.alloc(ctx, plan));
+  if (direct)
+    return radix_histograms(ctx, d_k, n, plan,
+                            JointLow{kGbDirectLog, (1u << pbits) - 1u, range}, rh, true);
+  return radix_histograms(ctx, d_k, n, plan, JointRadix{32 - pbits}, rh, true);
+}
+
+// The one or two radix passes after gb_hist, with its digit function `j`.
+template <class Joint>
+static int gb_scatter(pj_ctx *ctx, const RadixPlan &plan, Joint j, RadixHist &rh, uint64_t n,
+                      const int32_t *&pk, const int64_t *&pv, const float *&pw,
+                      Scratch (&bufs)[6]) {
+  // The six partition arrays start at different odd multiples of 4 KiB
+  // inside their allocations: pass 2 reads and writes the same offsets of
+  // equally sized arrays, and bases that coincide modulo the HBM channel
+  // interleave send its read and write streams to the same channels.
+  constexpr uint64_t kSkew[6] = {0, 17 * 4096, 33 * 4096, 49 * 4096, 65 * 4096, 81 * 4096};
+  auto at = [&](int i) { return (char *)bufs[i].p + kSkew[i]; };
+  PJ_TRY(bufs[0].alloc(ctx, n * 4 + 256 + kSkew[0]));
+  PJ_TRY(bufs[1].alloc(ctx, n * 8 + 256 + kSkew[1]));
+  PJ_TRY(bufs[2].alloc(ctx, n * 4 + 256 + kSkew[2]));
+  int32_t *ok1 = (int32_t *)at(0);
+  int64_t *ov1 = (int64_t *)at(1);
+  float *ow1 = (float *)at(2);
+  PJ_TRY(gb_pass_bits(ctx, "gb_scatter1", plan.b1, false, pk, pv, pw, ok1, ov1, ow1, n, plan, j,
+                      rh));
+  pk = ok1;
+  pv = ov1;
+  pw = ow1;
+  if (plan.b2) {
+    PJ_TRY(bufs[3].alloc(ctx, n * 4 + 256 + kSkew[3]));
+    PJ_TRY(bufs[4].alloc(ctx, n * 8 + 256 + kSkew[4]));
+    PJ_TRY(bufs[5].alloc(ctx, n * 4 + 256 + kSkew[5]));
+    int32_t *ok2 = (int32_t *)at(3);
+    int64_t *ov2 = (int64_t *)at(4);
+    float *ow2 = (float *)at(5);
+    PJ_TRY(gb_pass_bits(ctx, "gb_scatter2", plan.b2, true, pk, pv, pw, ok2, ov2, ow2, n, plan,
+                        j, rh));
+    pk = ok2;
+    pv = ov2;
+    pw = ow2;
+  }
+  return PJ_OK;
+}
+
+// Launches k_gb_agg in mode `mode` (a runtime value) with HV / HW fixed.
+template <bool MERGE, bool HV, bool HW, class... A>
+static int gb_launch(pj_ctx *ctx, const char *name, int mode, uint32_t grid, A... args) {
+  switch (mode) {
+    case kGbHashed:
+      PJ_LAUNCH(ctx, name, (k_gb_agg<MERGE, HV, HW, kGbHashed>), grid, kGbBlock, 0, args...);
+      return PJ_OK;
+    case kGbDirect:
+      PJ_LAUNCH(ctx, name, (k_gb_agg<MERGE, HV, HW, kGbDirect>), grid, kGbBlock, 0, args...);
+      return PJ_OK;
+    default:
+      if constexpr (!MERGE) {
+        PJ_LAUNCH(ctx, name, (k_gb_agg<false, HV, HW, kGbProbe>), grid, kGbBlock, 0, args...);
+        return PJ_OK;
+      }
+      return fail(PJ_E_ARG, "group-by merge in probe mode");
+  }
+}
+
+// One group-by under plan `pl`.  pl.pbits > 0 needs gb_hist's plan and
+// histogram for the same digits (rh != nullptr) or, for hashed plans, makes
+// them.  *overflowed: a hashed table filled up (more partitions needed);
+// *aborted: a kGbProbe unit spanned more than 2048 keys (no result).
 static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, const float *d_w,
-                       uint64_t n, int pbits, uint64_t max_groups, int32_t *d_ok,
-                       int64_t *d_cnt, int64_t *d_sum, int64_t *d_min, int64_t *d_max,
-                       double *d_fs, uint64_t *n_groups, bool *overflowed) {
+                       uint64_t n, const GbPlan &pl, RadixPlan *plan_in, RadixHist *rh_in,
+                       uint64_t max_groups, int32_t *d_ok, int64_t *d_cnt, int64_t *d_sum,
+                       int64_t *d_min, int64_t *d_max, double *d_fs, uint64_t *n_groups,
+                       bool *overflowed, bool *aborted) {
   *overflowed = false;
+  *aborted = false;
   *n_groups = 0;
   // ---- partition (optional) ---------------------------------------------
   const int32_t *pk = d_k;
   const int64_t *pv = d_v;
   const float *pw = d_w;
   std::vector<uint64_t> part_start{0}, part_count{n};
-  Scratch k1, v1, w1, k2, v2, w2;
-  RadixHist rh;
+  Scratch bufs[6];
+  RadixPlan own_plan;
+  RadixHist own_rh;
   Scratch zv, zw;  // zero columns when v / w are absent but partitioning runs
-  if (pbits > 0 && n) {
+  if (pl.pbits > 0 && n) {
+    RadixPlan *plan = plan_in;
+    RadixHist *rh = rh_in;
+    if (!rh) {
+      if (pl.mode != kGbHashed) return fail(PJ_E_ARG, "direct group-by without its histogram");
+      PJ_TRY(gb_hist(ctx, d_k, n, pl.pbits, false, nullptr, own_plan, own_rh));
+      plan = &own_plan;
+      rh = &own_rh;
+    }
     if (!pv) {
       PJ_TRY(zv.alloc(ctx, n * 8));
       PJ_TRY(zero(ctx, zv.p, n * 8));
@@ -323,45 +505,14 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
       PJ_TRY(zero(ctx, zw.p, n * 4));
       pw = zw.as<float>();
     }
-    const int b1 = pbits <= 7 ? pbits : (pbits + 1) / 2;
-    const int b2 = pbits - b1;
-    RadixPlan plan = radix_make_plan(ctx, n, b1, b2, 1 << pbits, 2);
-    PJ_TRY(rh.alloc(ctx, plan));
-    JointRadix j{32 - pbits};
-    PJ_TRY(radix_histograms(ctx, d_k, n, plan, j, rh, true));
-    // The six partition arrays start at different odd multiples of 4 KiB
-    // inside their allocations: pass 2 reads and writes the same offsets of
-    // equally sized arrays, and bases that coincide modulo the HBM channel
-    // interleave send its read and write streams to the same channels.
-    constexpr uint64_t kSkew[6] = {0, 17 * 4096, 33 * 4096, 49 * 4096, 65 * 4096, 81 * 4096};
-    auto at = [](Scratch &b, int i) { return (char *)b.p + kSkew[i]; };
-    PJ_TRY(k1.alloc(ctx, n * 4 + 256 + kSkew[0]));
-    PJ_TRY(v1.alloc(ctx, n * 8 + 256 + kSkew[1]));
-    PJ_TRY(w1.alloc(ctx, n * 4 + 256 + kSkew[2]));
-    int32_t *ok1 = (int32_t *)at(k1, 0);
-    int64_t *ov1 = (int64_t *)at(v1, 1);
-    float *ow1 = (float *)at(w1, 2);
-    PJ_TRY(gb_pass_bits(ctx, "gb_scatter1", b1, false, pk, pv, pw, ok1, ov1, ow1, n, plan, j,
-                        rh));
-    pk = ok1;
-    pv = ov1;
-    pw = ow1;
-    if (b2) {
-      PJ_TRY(k2.alloc(ctx, n * 4 + 256 + kSkew[3]));
-      PJ_TRY(v2.alloc(ctx, n * 8 + 256 + kSkew[4]));
-      PJ_TRY(w2.alloc(ctx, n * 4 + 256 + kSkew[5]));
-      int32_t *ok2 = (int32_t *)at(k2, 3);
-      int64_t *ov2 = (int64_t *)at(v2, 4);
-      float *ow2 = (float *)at(w2, 5);
-      PJ_TRY(gb_pass_bits(ctx, "gb_scatter2", b2, true, pk, pv, pw, ok2, ov2, ow2, n, plan, j,
-                          rh));
-      pk = ok2;
-      pv = ov2;
-      pw = ow2;
-    }
-    const int NB = 1 << pbits;
-    part_start.assign(rh.hS.begin(), rh.hS.begin() + NB);
-    part_count.assign(rh.hT.begin(), rh.hT.end());
+    if (pl.mode == kGbDirect)
+      PJ_TRY(gb_scatter(ctx, *plan, JointLow{kGbDirectLog, (1u << pl.pbits) - 1u, nullptr}, *rh,
+                        n, pk, pv, pw, bufs));
+    else
+      PJ_TRY(gb_scatter(ctx, *plan, JointRadix{32 - pl.pbits}, *rh, n, pk, pv, pw, bufs));
+    const int NB = 1 << pl.pbits;
+    part_start.assign(rh->hS.begin(), rh->hS.begin() + NB);
+    part_count.assign(rh->hT.begin(), rh->hT.end());
     if (!d_v) pv = nullptr;
     if (!d_w) pw = nullptr;
   }
@@ -371,7 +522,6 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
   const uint64_t target_units = (uint64_t)ctx->num_cus * 8;
   uint64_t unit_rows = std::max<uint64_t>(ceil_div(n ? n : 1, target_units), 16384);
   std::vector<GbUnit> units;
-  std::vector<uint32_t> unit_part;
   for (size_t p = 0; p < part_start.size(); ++p) {
     for (uint64_t r = 0; r < part_count[p]; r += unit_rows) {
       GbUnit u;
@@ -382,9 +532,14 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     }
   }
   if (units.empty()) return PJ_OK;
-  Scratch ovf, dunits;
-  PJ_TRY(ovf.alloc(ctx, 4));
-  PJ_TRY(zero(ctx, ovf.p, 4));
+  // flags: [0] table overflow, [1] probe abort, [2..3] probe min / max
+  Scratch flags, dunits;
+  PJ_TRY(flags.alloc(ctx, 16));
+  const int32_t init[4] = {0, 0, INT32_MAX, INT32_MIN};
+  PJ_TRY(h2d(ctx, flags.p, init, 16));
+  uint32_t *ovf = flags.as<uint32_t>();
+  uint32_t *abort = ovf + 1;
+  int *range = flags.as<int>() + 2;
   auto level = std::make_unique<GbLevel>();
   PJ_TRY(level->alloc(ctx, units.size()));
   PJ_TRY(dunits.alloc(ctx, units.size() * sizeof(GbUnit)));
@@ -398,11 +553,10 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
     const bool vec = a16(pk) && (!pv || a16(pv)) && (!pw || a16(pw));
     auto launch = [&](auto hv, auto hw) -> int {
-      PJ_LAUNCH(ctx, "gb_agg", (k_gb_agg<false, decltype(hv)::value, decltype(hw)::value>), grid,
-                kGbBlock, 0, dunits.as<const GbUnit>(), (uint32_t)units.size(), pk, pv, pw,
-                none, (const uint32_t *)nullptr, level->e, level->count.as<uint32_t>(),
-                ovf.as<uint32_t>(), vec);
-      return PJ_OK;
+      return gb_launch<false, decltype(hv)::value, decltype(hw)::value>(
+          ctx, "gb_agg", pl.mode, grid, dunits.as<const GbUnit>(), (uint32_t)units.size(), pk,
+          pv, pw, none, (const uint32_t *)nullptr, level->e, level->count.as<uint32_t>(), ovf,
+          vec, pl.kbase, pl.wmask, range, abort);
     };
     using T_ = std::true_type;
     using F_ = std::false_type;
@@ -410,6 +564,22 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     else if (pv) PJ_TRY(launch(T_{}, F_{}));
     else if (pw) PJ_TRY(launch(F_{}, T_{}));
     else PJ_TRY(launch(F_{}, F_{}));
+  }
+  // The merges' tables: a probe run whose units all fit their spans merges
+  // directly when the whole key span fits one table (kbase = min), hashed
+  // otherwise (each unit's entries are correct either way).
+  GbPlan mp = pl;
+  if (pl.mode == kGbProbe) {
+    int32_t f[4];
+    PJ_TRY(d2h(ctx, f, flags.p, 16));
+    if (f[1]) {
+      *aborted = true;
+      return PJ_OK;
+    }
+    const bool narrow = f[2] <= f[3] && (int64_t)f[3] - (int64_t)f[2] < kGbSlots;
+    mp.mode = narrow ? kGbDirect : kGbHashed;
+    mp.kbase = (uint32_t)f[2];
+    mp.wmask = kGbSlots - 1;
   }
   // ---- merge levels: fold units of the same partition ------------------
   // Units are ordered by partition.  Each level merges up to `fan` units of
@@ -441,16 +611,17 @@ static int groupby_run(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_v, cons
     PJ_TRY(dn.alloc(ctx, next.size() * sizeof(GbUnit)));
     PJ_TRY(h2d(ctx, dn.p, next.data(), next.size() * sizeof(GbUnit)));
     uint32_t grid = (uint32_t)std::min<uint64_t>(next.size(), (uint64_t)ctx->num_cus);
-    PJ_LAUNCH(ctx, "gb_merge", (k_gb_agg<true, true, true>), grid, kGbBlock, 0, dn.as<const GbUnit>(),
-              (uint32_t)next.size(), (const int32_t *)nullptr, (const int64_t *)nullptr,
-              (const float *)nullptr, level->e, level->count.as<const uint32_t>(), nl->e,
-              nl->count.as<uint32_t>(), ovf.as<uint32_t>(), false);
+    PJ_TRY((gb_launch<true, true, true>(
+        ctx, "gb_merge", mp.mode, grid, dn.as<const GbUnit>(), (uint32_t)next.size(),
+        (const int32_t *)nullptr, (const int64_t *)nullptr, (const float *)nullptr, level->e,
+        level->count.as<const uint32_t>(), nl->e, nl->count.as<uint32_t>(), ovf, false,
+        mp.kbase, mp.wmask, range, abort)));
     PJ_TRY(sync(ctx));
     units.swap(next);
     level.swap(nl);
   }
   uint32_t of = 0;
-  PJ_TRY(d2h(ctx, &of, ovf.p, 4));
+  PJ_TRY(d2h(ctx, &of, ovf, 4));
   if (of) {
     *overflowed = true;
     return PJ_OK;
@@ -526,10 +697,49 @@ extern "C" int pj_groupby_i32(pj_ctx *ctx, const int32_t *d_k, const int64_t *d_
   if (n >= (1ULL << 40)) return fail(PJ_E_CAPACITY, "input above the supported row count");
   int pbits = 0;
   while (pbits < 14 && ((uint64_t)kGbCap << pbits) < max_groups) ++pbits;
+  if (!n) return PJ_OK;
+  bool ovf = false, aborted = false;
+  // Direct tables first: one table covers 2048 keys, so the direct plan
+  // needs one partition bit fewer than the hashed one for the same bound.
+  int pd = pbits > 0 ? pbits - 1 : 0;
+  if (pd > 0) {
+    Scratch range;
+    PJ_TRY(range.alloc(ctx, 8));
+    const int32_t init[2] = {INT32_MAX, INT32_MIN};
+    PJ_TRY(h2d(ctx, range.p, init, 8));
+    RadixPlan plan;
+    RadixHist rh;
+    PJ_TRY(gb_hist(ctx, d_k, n, pd, true, range.as<int>(), plan, rh));
+    int32_t mm[2];
+    PJ_TRY(d2h(ctx, mm, range.p, 8));
+    const uint64_t span = (uint64_t)((int64_t)mm[1] - (int64_t)mm[0]) + 1;
+    if (span <= (uint64_t)kGbSlots) {
+      pd = 0;  // the bound overstates the groups: one direct table, no partitions
+    } else {
+      const int dn = ilog2_ceil(ceil_div(span, (uint64_t)kGbSlots));
+      if (dn <= pd) {
+        if (dn <= 7 && pd > 7) {
+          // fewer digits that still cover the span save the second pass
+          pd = dn;
+          PJ_TRY(gb_hist(ctx, d_k, n, pd, true, range.as<int>(), plan, rh));
+        }
+        GbPlan pl{pd, kGbDirect, (uint32_t)mm[0], (uint32_t)((1ULL << (kGbDirectLog + pd)) - 1)};
+        return groupby_run(ctx, d_k, d_v, d_w, n, pl, &plan, &rh, max_groups, d_out_k, d_cnt,
+                           d_sum, d_min, d_max, d_fsum, n_groups, &ovf, &aborted);
+      }
+      pd = -1;  // sparse keys: hashed tables
+    }
+  }
+  if (pd == 0) {
+    GbPlan pl{0, kGbProbe, 0u, 0u};
+    PJ_TRY(groupby_run(ctx, d_k, d_v, d_w, n, pl, nullptr, nullptr, max_groups, d_out_k, d_cnt,
+                       d_sum, d_min, d_max, d_fsum, n_groups, &ovf, &aborted));
+    if (!aborted) return PJ_OK;
+  }
   while (true) {
-    bool ovf = false;
-    PJ_TRY(groupby_run(ctx, d_k, d_v, d_w, n, pbits, max_groups, d_out_k, d_cnt, d_sum, d_min,
-                       d_max, d_fsum, n_groups, &ovf));
+    GbPlan pl{pbits, kGbHashed, 0u, 0u};
+    PJ_TRY(groupby_run(ctx, d_k, d_v, d_w, n, pl, nullptr, nullptr, max_groups, d_out_k, d_cnt,
+                       d_sum, d_min, d_max, d_fsum, n_groups, &ovf, &aborted));
     if (!ovf) return PJ_OK;
     if (pbits >= 14)
       return fail(PJ_E_CAPACITY,

@@ -36,20 +36,34 @@ constexpr int kLine = 32;  // tuples per write-combining line (128 B of int32)
 
 // Joint digit functors: bin in [0, NB) of a key.  d1 = bin >> b2,
 // d2 = bin & (2^b2 - 1).
+// A Joint with kRange = true also has k_chunk_hist record the keys' min and
+// max in range[0] / range[1] (initialised to INT32_MAX / INT32_MIN).
 struct JointRadix {  // top B bits of fmix32(key)
+  static constexpr bool kRange = false;
   int shift;         // 32 - B, B >= 1
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
     return fmix32((uint32_t)k) >> shift;
   }
 };
 struct JointHashed {  // top B bits of a key that already is fmix32(original key)
+  static constexpr bool kRange = false;
   int shift;
   __device__ __forceinline__ uint32_t operator()(int32_t h) const { return (uint32_t)h >> shift; }
 };
 struct JointMod {  // hashInt32(key) URem buckets (gpu-hash-rearrange.cpp:338-348)
+  static constexpr bool kRange = false;
   uint32_t buckets;
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
     return (uint32_t)(hash_i32(k) % (uint64_t)buckets);
+  }
+};
+struct JointLow {  // key bits [shift, shift + B): B = bits above the low `shift`
+  static constexpr bool kRange = true;
+  int shift;
+  uint32_t mask;  // 2^B - 1
+  int *range;     // min / max of the keys, recorded by k_chunk_hist
+  __device__ __forceinline__ uint32_t operator()(int32_t k) const {
+    return ((uint32_t)k >> shift) & mask;
   }
 };
 
@@ -152,6 +166,11 @@ __global__ __launch_bounds__(HB) void k_chunk_hist(const int32_t *__restrict__ k
   extern __shared__ uint32_t hsh[];
   for (int i = threadIdx.x; i < nb; i += HB) hsh[i] = 0;
   __syncthreads();
+  int kmin = INT32_MAX, kmax = INT32_MIN;
+  auto note = [&](int32_t k) {
+    kmin = min(kmin, k);
+    kmax = max(kmax, k);
+  };
   const uint64_t lo = (uint64_t)blockIdx.x * chunk;
   const uint64_t hi = lo + chunk < n ? lo + chunk : n;
   const uint64_t nvec = lo < hi ? (hi - lo) >> 2 : 0;
@@ -182,6 +201,12 @@ __global__ __launch_bounds__(HB) void k_chunk_hist(const int32_t *__restrict__ k
         atomicAdd(&hsh[joint(k[u].y)], inc);
         atomicAdd(&hsh[joint(k[u].z)], inc);
         atomicAdd(&hsh[joint(k[u].w)], inc);
+        if (Joint::kRange) {  // clamped duplicates are keys of the chunk too
+          note(k[u].x);
+          note(k[u].y);
+          note(k[u].z);
+          note(k[u].w);
+        }
       }
     };
     int4 A[U], B[U];
@@ -195,9 +220,26 @@ __global__ __launch_bounds__(HB) void k_chunk_hist(const int32_t *__restrict__ k
     }
     if (s < nsets) count(s, A);
     const uint64_t j = lo + nvec * 4 + threadIdx.x;  // < 4 tail keys
-    if (j < hi) atomicAdd(&hsh[joint(keys[j])], 1u);
+    if (j < hi) {
+      atomicAdd(&hsh[joint(keys[j])], 1u);
+      if (Joint::kRange) note(keys[j]);
+    }
   } else if (lo < hi) {
-    for (uint64_t j = lo + threadIdx.x; j < hi; j += HB) atomicAdd(&hsh[joint(keys[j])], 1u);
+    for (uint64_t j = lo + threadIdx.x; j < hi; j += HB) {
+      atomicAdd(&hsh[joint(keys[j])], 1u);
+      if (Joint::kRange) note(keys[j]);
+    }
+  }
+  if constexpr (Joint::kRange) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      kmin = min(kmin, __shfl_xor(kmin, o, 64));
+      kmax = max(kmax, __shfl_xor(kmax, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && kmin <= kmax) {
+      atomicMin(&joint.range[0], kmin);
+      atomicMax(&joint.range[1], kmax);
+    }
   }
   __syncthreads();
   uint32_t *row = H + (uint64_t)blockIdx.x * nb;

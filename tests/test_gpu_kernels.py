@@ -361,6 +361,61 @@ def test_groupby_parity(ctx, case):
     _free(*d)
 
 
+def _key_set(kind, n, seed):
+    """Group keys whose min / max steer pj_groupby_i32 to its direct tables
+    (dense key ranges) or to its hashed tables (sparse keys)."""
+    rs = np.random.default_rng(seed)
+    lo, hi = np.iinfo(np.int32).min, np.iinfo(np.int32).max
+    if kind == "sparse_full_range":  # 2^16 keys spread over all of int32: hashed
+        pool = rs.integers(lo, hi, 1 << 16, endpoint=True).astype(np.int32)
+        pool[:2] = (lo, hi)
+        return pool[rs.integers(0, len(pool), n)], 1 << 16
+    if kind == "sparse_strided":  # multiples of 2^20: span beyond 2^14 ranges, hashed
+        return (rs.integers(-1000, 1000, n) * (1 << 20)).astype(np.int32), 2048
+    if kind == "near_max":  # [INT32_MAX - 3000, INT32_MAX]: direct, two ranges
+        return (hi - rs.integers(0, 3001, n)).astype(np.int32), 4096
+    if kind == "near_min":  # [INT32_MIN, INT32_MIN + 5000]: direct, three ranges
+        return (lo + rs.integers(0, 5001, n)).astype(np.int32), 8192
+    if kind == "span_2048":  # exactly one range: direct, no partitioning
+        k = rs.integers(100, 100 + 2048, n).astype(np.int32)
+        k[:2] = (100, 100 + 2047)
+        return k, 2048
+    if kind == "span_2049":  # one key past one range: direct, two ranges
+        k = rs.integers(-7, -7 + 2049, n).astype(np.int32)
+        k[:2] = (-7, -7 + 2048)
+        return k, 4096
+    if kind == "holes":  # multiples of 7 over 11 ranges: direct with empty slots
+        return (rs.integers(0, 3000, n) * 7).astype(np.int32), 1 << 15
+    if kind == "small_span_large_bound":  # maxInputSize 2^22, 1000 keys: direct, one range
+        return rs.integers(0, 1000, n).astype(np.int32), 1 << 22
+    if kind == "narrow_span_large_bound":  # 2^15 keys under a 2^22 bound: fewer digits, one pass
+        return rs.integers(0, 1 << 15, n).astype(np.int32), 1 << 22
+    if kind == "cross_aligned":  # [1000, 66536) crosses 2^16: direct, keys wrap in the window
+        return rs.integers(1000, 66536, n).astype(np.int32), 1 << 16
+    if kind == "sorted_spaced":  # sorted multiples of 10: each unit fits a table, the whole does not
+        return np.sort(rs.integers(0, 1000, n) * 10).astype(np.int32), 1024
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["sparse_full_range", "sparse_strided", "near_max", "near_min",
+                                  "span_2048", "span_2049", "holes", "small_span_large_bound",
+                                  "narrow_span_large_bound", "cross_aligned", "sorted_spaced"])
+def test_groupby_direct_and_hashed_tables(ctx, kind):
+    """Every key layout against the chained-table oracle: dense key ranges
+    take the direct tables (including ranges at both ends of int32 and a
+    range that crosses an aligned boundary), sparse keys the hashed tables,
+    and sorted sparse keys direct tables per unit merged through hashed
+    ones."""
+    n, seed = 150000, 41
+    k, maxg = _key_set(kind, n, seed)
+    v = O.gen_uniform_i64(n, seed, 2, -(1 << 40), 1 << 41)
+    w = O.gen_unit_f32(n, seed, 3)
+    d = _dev(ctx, k, v, w)
+    got = ctx.groupby_i32(d[0], d[1], d[2], maxg)
+    _check_groups(got, k, v, w)
+    _free(*d)
+
+
 def test_groupby_optional_columns_and_capacity(ctx):
     n = 20000
     k = O.gen_uniform_i32(n, 1, 1, 0, 500)
