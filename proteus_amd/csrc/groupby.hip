@@ -102,18 +102,28 @@ __device__ __forceinline__ bool gb_slot(unsigned long long *tk, int32_t key, uin
   return gb_insert(tk, key, s);
 }
 
+// MIN / MAX of slot s from v: both bounds are read with one 16-byte load
+// (tmm[2s] = min, tmm[2s + 1] = max) and an atomic runs only where v would
+// change a bound.  A bound only moves outwards, so a stale read can only
+// cause an atomic that changes nothing, never skip one that is needed; on
+// hot slots (skewed keys) most rows then need no MIN / MAX atomic at all.
+__device__ __forceinline__ void gb_minmax(long long *tmm, uint32_t s, long long v) {
+  const longlong2 mm = *reinterpret_cast<const longlong2 *>(&tmm[2 * s]);
+  if (v < mm.x) atomicMin(&tmm[2 * s], v);
+  if (v > mm.y) atomicMax(&tmm[2 * s + 1], v);
+}
+
 // Adds row (key, v, w) to the LDS table; false when the table is full.
 template <bool HV, bool HW, int MODE>
 __device__ __forceinline__ bool gb_add_row(unsigned long long *tk, unsigned long long *tcnt,
-                                           long long *tsum, long long *tmin, long long *tmax,
-                                           double *tfs, int32_t key, int64_t v, float w) {
+                                           long long *tsum, long long *tmm, double *tfs,
+                                           int32_t key, int64_t v, float w) {
   uint32_t s;
   if (!gb_slot<MODE>(tk, key, s)) return false;
   atomicAdd(&tcnt[s], 1ULL);
   if (HV) {
     atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)v);
-    atomicMin(&tmin[s], (long long)v);
-    atomicMax(&tmax[s], (long long)v);
+    gb_minmax(tmm, s, (long long)v);
   }
   if (HW) atomicAdd(&tfs[s], (double)w);
   return true;
@@ -137,7 +147,8 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
   static_assert(MODE != kGbProbe || !MERGE, "the span check is for level 0");
   __shared__ unsigned long long tk[MODE == kGbHashed ? kGbSlots : 1];
   __shared__ unsigned long long tcnt[kGbSlots];
-  __shared__ long long tsum[kGbSlots], tmin[kGbSlots], tmax[kGbSlots];
+  __shared__ long long tsum[kGbSlots];
+  __shared__ __align__(16) long long tmm[2 * kGbSlots];  // MIN, MAX interleaved
   __shared__ double tfs[kGbSlots];
   __shared__ uint32_t wave_off[kGbBlock / 64 + 1];
   __shared__ int s_lo, s_hi;
@@ -150,8 +161,8 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
       if (MODE == kGbHashed) tk[i] = 0ULL;
       tcnt[i] = 0ULL;
       tsum[i] = 0;
-      tmin[i] = INT64_MAX;
-      tmax[i] = INT64_MIN;
+      tmm[2 * i] = INT64_MAX;
+      tmm[2 * i + 1] = INT64_MIN;
       tfs[i] = 0.0;
     }
     if (MODE == kGbProbe && tid == 0) {
@@ -183,7 +194,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
       auto row = [&](uint64_t x) {
         const int32_t k = rk[x];
         note(k);
-        ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmin, tmax, tfs, k, HV ? rv[x] : 0,
+        ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmm, tfs, k, HV ? rv[x] : 0,
                                          HW ? rw[x] : 0.f);
       };
       if (!vec || a0 >= a1) {
@@ -227,8 +238,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             note(kk[c]);
-            ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmin, tmax, tfs, kk[c], vv[c],
-                                             ww[c]);
+            ovf |= !gb_add_row<HV, HW, MODE>(tk, tcnt, tsum, tmm, tfs, kk[c], vv[c], ww[c]);
           }
         };
         uint32_t q = tid;
@@ -257,8 +267,10 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
           }
           atomicAdd(&tcnt[s], (unsigned long long)in.cnt[b + j]);
           atomicAdd((unsigned long long *)&tsum[s], (unsigned long long)in.sum[b + j]);
-          atomicMin(&tmin[s], (long long)in.mn[b + j]);
-          atomicMax(&tmax[s], (long long)in.mx[b + j]);
+          const long long mn = in.mn[b + j], mx = in.mx[b + j];
+          const longlong2 mm = *reinterpret_cast<const longlong2 *>(&tmm[2 * s]);
+          if (mn < mm.x) atomicMin(&tmm[2 * s], mn);
+          if (mx > mm.y) atomicMax(&tmm[2 * s + 1], mx);
           atomicAdd(&tfs[s], in.fs[b + j]);
         }
       }
@@ -314,8 +326,8 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_agg(
                        : (int32_t)(kb + ((((part << kGbDirectLog) | (uint32_t)i) - kb) & wm));
         out.cnt[o] = (int64_t)tcnt[i];
         out.sum[o] = tsum[i];
-        out.mn[o] = tmin[i];
-        out.mx[o] = tmax[i];
+        out.mn[o] = tmm[2 * i];
+        out.mx[o] = tmm[2 * i + 1];
         out.fs[o] = tfs[i];
       }
       running += tot;
