@@ -267,6 +267,18 @@ def test_histogram_conservation():
     assert h.sum() == len(k)
 
 
+def test_histogram_matches_bucket_by_bucket_count():
+    """or_histogram's bins equal a count of each key's partition digit, the
+    digit taken from or_phjc_bucket (fmix32, pinned against scikit-learn's
+    murmur3 above), for the top bits and for a middle bit field."""
+    k = O.gen_uniform_i32(1 << 12, 2, 1, -2**31, 1 << 32)
+    for shift, bits in ((32 - 7, 7), (9, 5)):
+        want = np.zeros(1 << bits, np.uint64)
+        for x in k:
+            want[O.phjc_bucket(int(x), shift, bits)] += 1
+        assert np.array_equal(O.histogram(k, shift, bits), want), (shift, bits)
+
+
 def test_groupby_chained_mt_equals_single_thread():
     """The per-worker chained group-by with its merge equals the
     single-threaded one (as a set of groups) and the dense oracle."""
