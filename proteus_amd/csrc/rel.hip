@@ -763,11 +763,14 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 
 // Probe chains without interpretation.  The common pipeline shape -- int32
 // key column -> probe, column cmp constant -> filter, then plain columns out
-// -- runs row-at-a-time: each lane carries kChainRows independent rows
-// through the stages with no LDS and no barriers (a row ends at its first
-// miss; a wave stops when all its rows have), which keeps far more probes in
-// flight than a tile-synchronous interpreter.
-constexpr int kChainRows = 8;
+// -- runs row-at-a-time: each lane carries independent rows through the
+// stages with no barriers (a row ends at its first miss; a wave stops when
+// all its rows have), which keeps far more probes in flight than a
+// tile-synchronous interpreter.  Stage 0 runs on kChainRowsA rows per lane;
+// its survivors are compacted and the later stages run on kChainRowsB rows
+// per lane (k_chain_q).
+constexpr int kChainRowsA = 4;
+constexpr int kChainRowsB = 1;
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
@@ -800,6 +803,7 @@ struct ChainArgs {
   uint32_t *counts;  // survivors per workgroup
   uint64_t region;   // output slots per workgroup: block b writes from b * region
   int32_t nst, nout;
+  int32_t ntab;      // tables (LDS ridl slots per row)
 };
 
 // Loads of R values of one column at row indices ix (branch-free: the type
@@ -972,71 +976,132 @@ __device__ __forceinline__ void gather_out(const void *c, int ct, const uint32_t
   }
 }
 
-template <int K, int R>
-__global__ __launch_bounds__(256) void k_chain(const ChainArgs *__restrict__ ap) {
-  extern __shared__ uint32_t ridl[];
+// Writes the alive rows of cr to the workgroup's output segment: positions
+// from one LDS atomic per wave, the output columns gathered four at a time
+// (all their loads in flight together; past the last output the last one is
+// loaded again and not stored).
+template <int R>
+__device__ __forceinline__ void chain_emit(const ChainArgs &a, const ChainRows<R> &cr,
+                                           uint32_t *s_cnt, int lane) {
+  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+  uint32_t cnt = 0;
+  unsigned long long m[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    m[k] = __ballot(cr.alive[k]);
+    cnt += __popcll(m[k]);
+  }
+  if (!cnt) return;
+  unsigned long long p0 = 0;
+  if (lane == 0) p0 = (unsigned long long)blockIdx.x * a.region + atomicAdd(s_cnt, cnt);
+  p0 = __shfl(p0, 0, 64);
+  uint64_t pos[R];
+  uint32_t off = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    pos[k] = p0 + off + __popcll(m[k] & lt);
+    off += __popcll(m[k]);
+  }
+  for (int j0 = 0; j0 < a.nout; j0 += 4) {
+    int64_t v[4][R];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = min(j0 + u, a.nout - 1);
+      const int col = a.outs[j].col;
+      uint32_t ix[R];
+      cr.at(a.clevel[col], ix);
+      gather_out<R>(a.cols[col], a.ctype[col], ix, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (j0 + u < a.nout) {
+        const ChainOut o = a.outs[j0 + u];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (cr.alive[k]) st_slot(o.out, o.out_t, pos[k], v[u][k]);
+      }
+    }
+  }
+}
+
+// The probe chain with the survivors of stage 0 compacted.  Each wave takes
+// tiles of 64 * RA consecutive rows: stage 0 runs on RA rows per lane, then
+// the rows still alive (with the build row stage 0 found, when it probes a
+// table) go to a wave-private LDS queue, and the remaining stages and the
+// outputs run over the queue with RB rows per lane.  A selective first
+// stage (the star plan's date join keeps 1/7 of the rows) would otherwise
+// leave every later stage computing RA rows per lane, most of them dead:
+// the kernel is bound by instruction issue, not by its bytes.  The waves of
+// a workgroup share only the output counter; a wave's LDS queue and the
+// ridl slots of its own lanes are written and read by that wave alone, in
+// program order.
+template <int K, int RA, int RB>
+__global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ ap) {
+  static_assert(RB <= RA, "phase B reuses phase A's ridl slots");
+  extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_cnt;
   const ChainArgs &a = *ap;
+  const int ntab = a.ntab;
+  uint32_t *ridl = lds;                                   // ntab * RA * 256
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * 64 * RA);  // per wave: rows, then ids
+  uint32_t *qrid = qrow + 64 * RA;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
-  const uint32_t n32 = (uint32_t)a.n;
-  const uint32_t stride = gridDim.x * 256u * R;
-  ChainRows<R> cr;
-  cr.ridl = ridl;
-  for (uint32_t base = blockIdx.x * 256u * R; base < n32; base += stride) {
+  const ChainStage s0 = a.st[0];
+  const int t0 = s0.kind == 0 ? s0.tab : -1;  // the table stage 0 probes, if any
+  const uint64_t stride = (uint64_t)gridDim.x * 256u * RA;  // 64-bit: no wrap near n = 2^32
+  for (uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA; base < a.n;
+       base += stride) {
+    // ---- phase A: stage 0 on RA rows per lane ----------------------------
+    ChainRows<RA> ca;
+    ca.ridl = ridl;
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      cr.row[k] = base + k * 256u + threadIdx.x;
-      cr.alive[k] = cr.row[k] < n32;
+    for (int k = 0; k < RA; ++k) {
+      const uint64_t r = base + k * 64u + lane;
+      ca.alive[k] = r < a.n;
+      ca.row[k] = ca.alive[k] ? (uint32_t)r : 0u;
     }
-    bool live = true;
-    int32_t pre[R];
-    chain_prefetch<0, K, R>(a, cr, pre);
-    chain_stages<0, K, R>(a, cr, live, pre);
-    if (!live) continue;
-    // outputs: positions from one LDS atomic per wave
-    uint32_t cnt = 0;
-    unsigned long long m[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      m[k] = __ballot(cr.alive[k]);
-      cnt += __popcll(m[k]);
+    {
+      bool live = true;
+      int32_t pre[RA];
+      chain_prefetch<0, 1, RA>(a, ca, pre);
+      chain_stages<0, 1, RA>(a, ca, live, pre);
     }
-    if (!cnt) continue;
-    unsigned long long p0 = 0;
-    if (lane == 0) p0 = (unsigned long long)blockIdx.x * a.region + atomicAdd(&s_cnt, cnt);
-    p0 = __shfl(p0, 0, 64);
-    uint64_t pos[R];
-    uint32_t off = 0;
+    // ---- compaction of the survivors into the wave's queue ------------------
+    const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+    uint32_t q = 0;
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      pos[k] = p0 + off + __popcll(m[k] & lt);
-      off += __popcll(m[k]);
-    }
-    // outputs four at a time: all their loads in flight together (past the
-    // last output the last one is loaded again and not stored)
-    for (int j0 = 0; j0 < a.nout; j0 += 4) {
-      int64_t v[4][R];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = min(j0 + u, a.nout - 1);
-        const int col = a.outs[j].col;
-        uint32_t ix[R];
-        cr.at(a.clevel[col], ix);
-        gather_out<R>(a.cols[col], a.ctype[col], ix, v[u]);
+    for (int k = 0; k < RA; ++k) {
+      const unsigned long long m = __ballot(ca.alive[k]);
+      if (ca.alive[k]) {
+        const uint32_t at = q + __popcll(m & lt);
+        qrow[at] = ca.row[k];
+        if (t0 >= 0) qrid[at] = ridl[(t0 * RA + k) * 256 + threadIdx.x];
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (j0 + u < a.nout) {
-          const ChainOut o = a.outs[j0 + u];
-#pragma unroll
-          for (int k = 0; k < R; ++k)
-            if (cr.alive[k]) st_slot(o.out, o.out_t, pos[k], v[u][k]);
-        }
-      }
+      q += __popcll(m);
     }
+    __builtin_amdgcn_wave_barrier();
+    // ---- phase B: the other stages and the outputs over the queue ------------
+    for (uint32_t q0 = 0; q0 < q; q0 += 64u * RB) {
+      ChainRows<RB> cb;
+      cb.ridl = ridl;
+#pragma unroll
+      for (int k = 0; k < RB; ++k) {
+        const uint32_t e = q0 + k * 64u + lane;
+        cb.alive[k] = e < q;
+        const uint32_t ec = cb.alive[k] ? e : 0u;
+        cb.row[k] = qrow[ec];
+        if (t0 >= 0) ridl[(t0 * RB + k) * 256 + threadIdx.x] = cb.alive[k] ? qrid[ec] : 0u;
+      }
+      bool live = true;
+      int32_t pre[RB];
+      chain_prefetch<1, K, RB>(a, cb, pre);
+      chain_stages<1, K, RB>(a, cb, live, pre);
+      if (live) chain_emit<RB>(a, cb, &s_cnt, lane);
+      __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
   if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
@@ -1346,9 +1411,10 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
     std::vector<void *> sparse(p.out_t.size());
     if (as_chain(p, sparse.data(), &ca)) {
       for (size_t j = 0; j < p.tables.size(); ++j) ca.tables[j] = p.tables[j];
-      const uint64_t tile = 256 * kChainRows;
+      const uint64_t tile = 256 * kChainRowsA;
       const uint64_t grid = std::min<uint64_t>(ceil_div(n, tile), (uint64_t)ctx->num_cus * 8);
       ca.n = n;
+      ca.ntab = (int32_t)std::max<size_t>(1, p.tables.size());
       ca.region = ceil_div(ceil_div(n, tile), grid) * tile;
       Segments seg;
       PJ_TRY(seg.init(ctx, p.out_t, grid, ca.region));
@@ -1368,17 +1434,20 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       std::vector<uint8_t> blob(sizeof(ChainArgs) + 256, 0);
       std::memcpy(blob.data(), &ca, sizeof(ChainArgs));
       PJ_TRY(h2d(ctx, dargs.p, blob.data(), blob.size()));
-      const size_t lds = std::max<size_t>(1, p.tables.size()) * kChainRows * 256 * 4;
       const ChainArgs *da = dargs.as<const ChainArgs>();
+      // ridl slots of every table for phase A's rows, and per wave a queue
+      // of (row, stage-0 build row) pairs
+      constexpr int RA = kChainRowsA, RB = kChainRowsB;
+      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * 2 * 64 * RA * 4;
       switch (ca.nst) {
-        case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<1, kChainRows>), grid, 256, lds, da); break;
-        case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<2, kChainRows>), grid, 256, lds, da); break;
-        case 3: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<3, kChainRows>), grid, 256, lds, da); break;
-        case 4: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<4, kChainRows>), grid, 256, lds, da); break;
-        case 5: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<5, kChainRows>), grid, 256, lds, da); break;
-        case 6: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<6, kChainRows>), grid, 256, lds, da); break;
-        case 7: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<7, kChainRows>), grid, 256, lds, da); break;
-        default: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain<8, kChainRows>), grid, 256, lds, da); break;
+        case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<1, RA, RB>), grid, 256, lds, da); break;
+        case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<2, RA, RB>), grid, 256, lds, da); break;
+        case 3: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<3, RA, RB>), grid, 256, lds, da); break;
+        case 4: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<4, RA, RB>), grid, 256, lds, da); break;
+        case 5: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<5, RA, RB>), grid, 256, lds, da); break;
+        case 6: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<6, RA, RB>), grid, 256, lds, da); break;
+        case 7: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<7, RA, RB>), grid, 256, lds, da); break;
+        default: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<8, RA, RB>), grid, 256, lds, da); break;
       }
       return seg.compact(ctx, p.out_t, outs, n_out);
     }
