@@ -89,3 +89,80 @@ def test_star_plan_shape():
     p = star.plan()
     s = json.dumps(p)
     assert s.count('"hashjoin-chained"') == 4 and '"groupby"' in s and '"sort"' in s
+
+
+def _chain_tables(ctx, n, seed):
+    """Fact F(a, b, v) and dimensions A (keys 0..999, all present: a join on
+    A keeps every fact row) and B (multiples of 3 below 5000: a join on B
+    keeps a third), registered on the device."""
+    rs = np.random.default_rng(seed)
+    cols = {
+        ("F", "a"): rs.integers(0, 1000, n).astype(np.int32),
+        ("F", "b"): rs.integers(0, 5000, n).astype(np.int32),
+        ("F", "v"): rs.integers(-100, 100, n).astype(np.int32),
+        ("A", "k"): rs.permutation(1000).astype(np.int32),
+        ("B", "k"): rs.permutation(np.arange(0, 5000, 3)).astype(np.int32),
+    }
+    cols[("A", "w")] = (cols[("A", "k")] % 17).astype(np.int32)
+    cols[("B", "w")] = (cols[("B", "k")] % 11).astype(np.int32)
+    devs = []
+    for (rel, name), v in cols.items():
+        d = ctx.to_device(v)
+        devs.append(d)
+        ctx.register_column(rel, name, "int", d)
+    return cols, devs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["all_survive_first", "filter_first", "single_join"])
+def test_probe_chain_stage0_compaction(ctx, case):
+    """The fused probe chain (the rel_pipeline_chain kernel) when stage 0
+    keeps every row (each wave's queue holds its whole tile and the later
+    stages run in several rounds), when stage 0 is a filter on a fact column
+    (no build row to carry), and with a single join; 300,001 fact rows (a
+    partial last tile).  Grouped results equal numpy's."""
+    ctx.clear_catalog()
+    n = 300_001
+    c, devs = _chain_tables(ctx, n, 21)
+    b = rb.RelBuilder(ctx)
+    col = rb.col
+    F = b.scan("F", ["a", "b", "v"]).to_gpu().unpack()
+    A = b.scan("A", ["k", "w"]).to_gpu().unpack()
+    B = b.scan("B", ["k", "w"]).to_gpu().unpack()
+    fa, fb, fv = c[("F", "a")], c[("F", "b")], c[("F", "v")]
+    aw = np.zeros(1000, np.int64)
+    aw[c[("A", "k")]] = c[("A", "w")]
+    bw = np.full(5000, -1, np.int64)
+    bw[c[("B", "k")]] = c[("B", "w")]
+    if case == "all_survive_first":
+        j = F.join(A, col("A", "k"), col("F", "a"), 11, 1000) \
+             .join(B, col("B", "k"), col("F", "b"), 13, 2000)
+        keep = bw[fb] >= 0
+        g, x = aw[fa][keep], (fv.astype(np.int64) + bw[fb])[keep]
+        gkey, xval = col("A", "w"), rb.add(col("F", "v"), col("B", "w"))
+    elif case == "filter_first":
+        j = F.filter(rb.gt(col("F", "v"), rb.lit(50))) \
+             .join(B, col("B", "k"), col("F", "b"), 13, 2000) \
+             .join(A, col("A", "k"), col("F", "a"), 11, 1000)
+        keep = (fv > 50) & (bw[fb] >= 0)
+        g, x = aw[fa][keep], (fv.astype(np.int64) + bw[fb])[keep]
+        gkey, xval = col("A", "w"), rb.add(col("F", "v"), col("B", "w"))
+    else:
+        j = F.join(B, col("B", "k"), col("F", "b"), 13, 2000)
+        keep = bw[fb] >= 0
+        g, x = bw[fb][keep], fv[keep].astype(np.int64)
+        gkey, xval = col("B", "w"), col("F", "v")
+    q = j.groupby([gkey], [(rb.as_(rb.cast(xval, "int64"), "o", "s"), "sum"),
+                           (rb.as_(rb.int64(1), "o", "c"), "sum")], 6, 64) \
+         .sort([gkey, col("o", "s"), col("o", "c")], ["ASC", "NONE", "NONE"])
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    got = _rows(str(q.prepare().execute()))
+    chain_launches = ctx.prof_get("rel_pipeline_chain")[1]
+    ctx.prof_enable(False)
+    want = [[int(k), int(x[g == k].sum()), int((g == k).sum())] for k in np.unique(g)]
+    assert got == want
+    assert chain_launches >= 1
+    ctx.clear_catalog()
+    for d in devs:
+        d.free()
