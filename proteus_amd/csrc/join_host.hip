@@ -13,7 +13,11 @@
 // joined with the radix-partitioned join; the per-bucket aggregates combine
 // exactly (int64 counts and sums add, minima and maxima combine), since
 // equal keys fall in the same bucket.  PCIe carries 24 B per tuple (8 in,
-// 8 back, 8 in again); a chunk's copy back overlaps the next chunk's upload.
+// 8 back, 8 in again).  The caller's (pageable) chunks come in through the
+// context's pinned staging ring, the next chunk while the current one is
+// rearranged; a chunk's copy back to the arena overlaps the next chunk's
+// upload; a bucket's pieces go up from the pinned arena as one queue of
+// asynchronous copies that the bucket's join follows on the same stream.
 #include <algorithm>
 #include <vector>
 
@@ -68,34 +72,75 @@ struct CopyStream {
   }
 };
 
+// Staging tickets not yet waited for; the destructor waits for them and for
+// the context stream, so no copy still writes into buffers that are freed
+// after an error.  Declared after the buffers it protects.
+struct Tickets {
+  pj_ctx *ctx;
+  std::vector<uint64_t> open;
+  explicit Tickets(pj_ctx *c) : ctx(c) {}
+  int begin(const void *h, uint64_t bytes, void *d, uint64_t *t) {
+    PJ_TRY(pj_stage_begin(ctx, nullptr, h, 0, bytes, d, t));
+    open.push_back(*t);
+    return PJ_OK;
+  }
+  int wait(uint64_t t) {
+    open.erase(std::find(open.begin(), open.end(), t));
+    return pj_stage_wait(ctx, t);
+  }
+  ~Tickets() {
+    for (uint64_t t : open) (void)pj_stage_wait(ctx, t);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+};
+
 int bucket_side(pj_ctx *ctx, const int32_t *hk, const int32_t *hv, uint64_t n, uint32_t buckets,
                 uint64_t chunk, int32_t *ak, int32_t *av, std::vector<std::vector<Piece>> &pieces) {
   pieces.assign(buckets, {});
   if (!n) return PJ_OK;
   const uint64_t c = std::min(chunk, n);
-  DevBuf ik(ctx), iv(ctx), ok0(ctx), ov0(ctx), ok1(ctx), ov1(ctx);
-  PJ_TRY(ik.alloc(c * 4));
+  const uint64_t nchunks = ceil_div(n, c);
+  // two input and two output buffer sets: chunk i + 1 comes in while chunk
+  // i is rearranged, chunk i goes back while chunk i + 1 is rearranged
+  DevBuf ik0(ctx), iv0(ctx), ik1(ctx), iv1(ctx), ok0(ctx), ov0(ctx), ok1(ctx), ov1(ctx);
+  PJ_TRY(ik0.alloc(c * 4));
   PJ_TRY(ok0.alloc(c * 4));
   PJ_TRY(ok1.alloc(c * 4));
+  if (nchunks > 1) PJ_TRY(ik1.alloc(c * 4));
   if (hv) {
-    PJ_TRY(iv.alloc(c * 4));
+    PJ_TRY(iv0.alloc(c * 4));
     PJ_TRY(ov0.alloc(c * 4));
     PJ_TRY(ov1.alloc(c * 4));
+    if (nchunks > 1) PJ_TRY(iv1.alloc(c * 4));
   }
+  DevBuf *ik[2] = {&ik0, &ik1}, *iv[2] = {&iv0, &iv1};
   DevBuf *ok[2] = {&ok0, &ok1}, *ov[2] = {&ov0, &ov1};
   CopyStream cs;
   PJ_TRY(cs.init());
+  Tickets tk(ctx);
+  uint64_t tkk[2] = {0, 0}, tkv[2] = {0, 0};
+  auto stage = [&](uint64_t i) -> int {
+    const uint64_t base = i * c, len = std::min(c, n - base);
+    const int s = (int)(i & 1);
+    PJ_TRY(tk.begin(hk + base, len * 4, ik[s]->p, &tkk[s]));
+    if (hv) PJ_TRY(tk.begin(hv + base, len * 4, iv[s]->p, &tkv[s]));
+    return PJ_OK;
+  };
   std::vector<uint64_t> off(buckets + 1);
-  uint32_t idx = 0;
-  for (uint64_t base = 0; base < n; base += c, ++idx) {
-    const int b = idx & 1;
-    const uint64_t len = std::min(c, n - base);
-    if (idx >= 2) PJ_HIP(hipEventSynchronize(cs.done[b]));  // buffer set b is free
-    PJ_TRY(pj_memcpy_h2d(ctx, ik.p, hk + base, len * 4));
-    if (hv) PJ_TRY(pj_memcpy_h2d(ctx, iv.p, hv + base, len * 4));
-    PJ_TRY(pj_hash_rearrange_i32(ctx, (const int32_t *)ik.p, hv ? (const int32_t *)iv.p : nullptr,
-                                 len, buckets, (int32_t *)ok[b]->p,
-                                 hv ? (int32_t *)ov[b]->p : nullptr, off.data()));
+  PJ_TRY(stage(0));
+  for (uint64_t i = 0; i < nchunks; ++i) {
+    const int b = (int)(i & 1);
+    const uint64_t base = i * c, len = std::min(c, n - base);
+    // input set b ^ 1 was last read by chunk i - 1's rearrange, which has
+    // completed (pj_hash_rearrange_i32 synchronises)
+    if (i + 1 < nchunks) PJ_TRY(stage(i + 1));
+    PJ_TRY(tk.wait(tkk[b]));  // the context stream now waits for chunk i's DMA
+    if (hv) PJ_TRY(tk.wait(tkv[b]));
+    if (i >= 2) PJ_HIP(hipEventSynchronize(cs.done[b]));  // output set b is free
+    PJ_TRY(pj_hash_rearrange_i32(ctx, (const int32_t *)ik[b]->p,
+                                 hv ? (const int32_t *)iv[b]->p : nullptr, len, buckets,
+                                 (int32_t *)ok[b]->p, hv ? (int32_t *)ov[b]->p : nullptr,
+                                 off.data()));
     // the rearrange has synchronised: its output is complete
     PJ_HIP(hipMemcpyAsync(ak + base, ok[b]->p, len * 4, hipMemcpyDeviceToHost, cs.s));
     if (hv) PJ_HIP(hipMemcpyAsync(av + base, ov[b]->p, len * 4, hipMemcpyDeviceToHost, cs.s));
@@ -113,11 +158,14 @@ uint64_t total(const std::vector<Piece> &ps) {
   return t;
 }
 
-// The pieces of one bucket, concatenated into device memory.
+// The pieces of one bucket, concatenated into device memory: asynchronous
+// copies from the pinned arena on the context stream, which the bucket's
+// join then follows.
 int gather_pieces(pj_ctx *ctx, const int32_t *arena, const std::vector<Piece> &ps, int32_t *dst) {
   uint64_t at = 0;
   for (const Piece &p : ps) {
-    PJ_TRY(pj_memcpy_h2d(ctx, dst + at, arena + p.off, p.len * 4));
+    PJ_HIP(hipMemcpyAsync(dst + at, arena + p.off, p.len * 4, hipMemcpyHostToDevice,
+                          ctx->stream));
     at += p.len;
   }
   return PJ_OK;
@@ -180,6 +228,11 @@ extern "C" int pj_join_out_of_gpu_reduce(pj_ctx *ctx, const int32_t *h_bk, const
     max_np = std::max(max_np, sb);
   }
   DevBuf drk(ctx), drv(ctx), dsk(ctx), dsv(ctx);
+  // after the buffers: no queued piece copy outlives them on an error return
+  struct StreamDrain {
+    pj_ctx *ctx;
+    ~StreamDrain() { (void)hipStreamSynchronize(ctx->stream); }
+  } drain{ctx};
   PJ_TRY(drk.alloc(max_nb * 4));
   PJ_TRY(dsk.alloc(max_np * 4));
   if (h_bv) PJ_TRY(drv.alloc(max_nb * 4));
