@@ -233,6 +233,16 @@ def test_hash_rearrange_plan_keeps_rows(ctx):
                  ["sum", "sum", "max"])
     got = [int(x) for x in str(q.prepare().execute()).strip().split(",")]
     assert got == [ns, int(sv.astype(np.int64).sum()), int(sk.max())]
+    # the rows themselves: the input rows, grouped by bucket = hashInt32(k)
+    # mod 7 in ascending bucket order (the oracle's bucket function)
+    q2 = S.pack([rb.col("S", "k"), rb.col("S", "v")], rb.col("S", "k"), 7).print()
+    rows = [[int(x) for x in ln.split(",")] for ln in
+            str(q2.prepare().execute()).strip().splitlines()]
+    assert len(rows) == ns
+    assert sorted(map(tuple, rows)) == sorted(zip(sk.tolist(), sv.tolist()))
+    buckets = [O.rearrange_bucket(k, 7) for k, _ in rows]
+    assert buckets == sorted(buckets)
+    assert len(set(buckets)) == 7
     ctx.clear_catalog()
 
 
