@@ -7,6 +7,7 @@
 // conflict-free).  This plays the role of the reference's expression code
 // generation (core/olap/lib/expressions/expressions-generator.cpp) without a
 // JIT.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -1107,13 +1108,20 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
   if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
 }
 
-// Moves every workgroup's output segment to its place in the dense output.
-__global__ void k_seg_compact(const void *src, int esize, const uint32_t *counts,
-                              const unsigned long long *offs, uint64_t region, void *dst) {
-  const uint32_t b = blockIdx.y;
+// Moves every workgroup's output segment to its place in the dense output,
+// all columns in one launch: blockIdx.y = segment, blockIdx.z = column.
+struct SegCols {
+  const void *src[kPipeMaxOut];
+  void *dst[kPipeMaxOut];
+  int32_t esize[kPipeMaxOut];
+};
+__global__ void k_seg_compact(SegCols sc, const uint32_t *counts,
+                              const unsigned long long *offs, uint64_t region) {
+  const uint32_t b = blockIdx.y, col = blockIdx.z;
+  const int esize = sc.esize[col];
   const uint32_t c = counts[b];
-  const char *s = (const char *)src + (uint64_t)b * region * esize;
-  char *d = (char *)dst + offs[b] * esize;
+  const char *s = (const char *)sc.src[col] + (uint64_t)b * region * esize;
+  char *d = (char *)sc.dst[col] + offs[b] * esize;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
     if (esize == 8) ((uint64_t *)d)[i] = ((const uint64_t *)s)[i];
     else if (esize == 4) ((uint32_t *)d)[i] = ((const uint32_t *)s)[i];
@@ -1359,13 +1367,26 @@ struct Segments {
   int compact(pj_ctx *ctx, const std::vector<int32_t> &types, void *const *outs, uint64_t *n_out) {
     PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, counts.as<const uint32_t>(), grid,
               offs.as<unsigned long long>());
+    // the segments' sizes and the total, read back together: the largest
+    // segment sizes the copy grid (most segments hold a few hundred rows)
+    std::vector<uint32_t> cnt(grid);
     unsigned long long total = 0;
+    PJ_TRY(d2h(ctx, cnt.data(), counts.p, grid * 4));
     PJ_TRY(d2h(ctx, &total, offs.as<unsigned long long>() + grid, 8));
-    for (size_t j = 0; j < types.size() && total; ++j) {
-      const uint32_t gx = (uint32_t)std::min<uint64_t>(ceil_div(region, 256), 16);
-      hipLaunchKernelGGL(k_seg_compact, dim3(gx, (uint32_t)grid), dim3(256), 0, ctx->stream, bufs[j]->p,
-                         col_size(types[j]), counts.as<const uint32_t>(),
-                         offs.as<const unsigned long long>(), region, outs[j]);
+    const uint32_t mx = cnt.empty() ? 0u : *std::max_element(cnt.begin(), cnt.end());
+    if (total && !types.empty()) {
+      if (types.size() > (size_t)kPipeMaxOut)
+        return fail(PJ_E_UNSUPPORTED, "pipeline with more than %d outputs", kPipeMaxOut);
+      SegCols sc{};
+      for (size_t j = 0; j < types.size(); ++j) {
+        sc.src[j] = bufs[j]->p;
+        sc.dst[j] = outs[j];
+        sc.esize[j] = (int32_t)col_size(types[j]);
+      }
+      const uint32_t gx = (uint32_t)std::min<uint64_t>(ceil_div(mx, 256), 16);
+      hipLaunchKernelGGL(k_seg_compact, dim3(gx ? gx : 1, (uint32_t)grid, (uint32_t)types.size()),
+                         dim3(256), 0, ctx->stream, sc, counts.as<const uint32_t>(),
+                         offs.as<const unsigned long long>(), region);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return fail(PJ_E_HIP, "pipeline compaction: %s", hipGetErrorString(e));
     }

@@ -325,6 +325,36 @@ static int type_bytes(int t) {
   return t == I64 || t == F64 ? 8 : t == BOOL ? 1 : 4;
 }
 
+// Small sorts (n <= kRankSortMax rows): one workgroup ranks every row by
+// comparing it with every other row on the order-preserving keys of all sort
+// expressions at once (key 0 most significant), ties by row index, so the
+// result is the stable lexicographic order; perm[rank] = row.  The keys
+// (nk x n u64) sit in LDS and every lane of a wave reads the same row j at
+// a time (broadcast).  A handful of launches replace the radix passes' ten
+// per key, which at hundreds of rows are launch latency.
+constexpr uint64_t kRankSortMax = 2048;
+constexpr uint64_t kRankSortLds = 65536;  // bytes of keys in LDS
+__global__ __launch_bounds__(1024) void k_rank_sort(const unsigned long long *__restrict__ keys,
+                                                    int nk, uint32_t n,
+                                                    uint32_t *__restrict__ perm) {
+  extern __shared__ unsigned long long sk[];
+  for (uint32_t x = threadIdx.x; x < (uint32_t)nk * n; x += blockDim.x) sk[x] = keys[x];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      // row j before row i: first differing key smaller, or all equal and j < i
+      int c = 0;
+      for (int k = 0; k < nk && c == 0; ++k) {
+        const unsigned long long a = sk[(uint64_t)k * n + j], b = sk[(uint64_t)k * n + i];
+        c = a < b ? -1 : (a > b ? 1 : 0);
+      }
+      rank += (c < 0 || (c == 0 && j < i)) ? 1u : 0u;
+    }
+    perm[rank] = i;
+  }
+}
+
 // Sorts the row permutation perm (n entries, initialised here) by the keys;
 // strides[k] (bytes) defaults to the column's element size.
 static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *strides, int nk,
@@ -344,6 +374,31 @@ static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *s
   }
   uint64_t grid = ceil_div(n, (uint64_t)256);
   if (grid > (uint64_t)ctx->num_cus * 16) grid = (uint64_t)ctx->num_cus * 16;
+  int nka = 0;  // sort expressions with a direction
+  for (int k = 0; k < nk; ++k) nka += keys[k].dir != 0;
+  if (n <= kRankSortMax && (uint64_t)nka * n * 8 <= kRankSortLds) {
+    if (!nka) {
+      PJ_LAUNCH(ctx, "sort_iota", k_sort_iota, grid, 256, 0, perm, n);
+      return sync(ctx);
+    }
+    Scratch iota, kk;
+    PJ_TRY(iota.alloc(ctx, n * 4));
+    PJ_TRY(kk.alloc(ctx, (uint64_t)nka * n * 8));
+    PJ_LAUNCH(ctx, "sort_iota", k_sort_iota, grid, 256, 0, iota.as<uint32_t>(), n);
+    int at = 0;
+    for (int k = 0; k < nk; ++k) {
+      if (!keys[k].dir) continue;
+      const uint64_t stride = strides ? strides[k] : (uint64_t)type_bytes(keys[k].t);
+      PJ_LAUNCH(ctx, "sort_key", k_sort_key<unsigned long long>, grid, 256, 0,
+                (const char *)keys[k].col, stride, keys[k].t, keys[k].dir < 0 ? 1 : 0,
+                keys[k].rank, keys[k].nrank, iota.as<const uint32_t>(), n,
+                kk.as<unsigned long long>() + (uint64_t)at * n);
+      ++at;
+    }
+    PJ_LAUNCH(ctx, "sort_rank", k_rank_sort, 1, 1024, (size_t)nka * n * 8,
+              kk.as<const unsigned long long>(), nka, (uint32_t)n, perm);
+    return sync(ctx);
+  }
   Scratch p1, k0, k1;
   PJ_TRY(p1.alloc(ctx, n * 4));
   PJ_TRY(k0.alloc(ctx, n * (wide ? 8 : 4)));

@@ -262,11 +262,12 @@ def test_plan_errors(ctx):
 
 # ---- GPU: the device sort --------------------------------------------------
 @pytest.mark.gpu
-def test_device_sort_matches_stable_lexsort(ctx):
+@pytest.mark.parametrize("n", [1500, 50000])
+def test_device_sort_matches_stable_lexsort(ctx, n):
     """sort over int32 (ties, negatives), FP64 (-0.0 and +0.0 equal,
-    negatives) and int64 (NONE: carried, ties keep input order)."""
+    negatives) and int64 (NONE: carried, ties keep input order); 1500 rows
+    take the one-workgroup rank sort, 50000 the radix passes."""
     rs = np.random.default_rng(12)
-    n = 50000
     a = rs.integers(-50, 50, n).astype(np.int32)
     c = rs.choice(np.array([-2.5, -0.0, 0.0, 1.0, 3.25, -1e300, 7e-300]), n)
     b = np.arange(n, dtype=np.int64) * 7 - 3

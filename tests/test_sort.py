@@ -44,7 +44,7 @@ def test_record_layout_is_a_c_struct():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("fields", ["i", "l", "il", "li", "iil", "lil", "iiii"])
-@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 100_003])
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 2048, 2049, 4095, 4096, 4097, 100_003])
 @pytest.mark.parametrize("narrow", [False, True])
 def test_sort_records_matches_numpy(ctx, fields, n, narrow):
     rs = np.random.default_rng(n * 31 + len(fields) + narrow)
