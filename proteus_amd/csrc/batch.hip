@@ -763,9 +763,7 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   const uint64_t hash_slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
   if (n) {
     int64_t kmin = 0, kmax = 0;
-    double unused = 0;
-    int rc = rel::reduce(ctx, d_keys, rel::I32, n, rel::M_MIN, &kmin, &unused);
-    if (rc == PJ_OK) rc = rel::reduce(ctx, d_keys, rel::I32, n, rel::M_MAX, &kmax, &unused);
+    int rc = rel::reduce_minmax(ctx, d_keys, rel::I32, n, &kmin, &kmax);
     if (rc != PJ_OK) {
       delete dm;
       return rc;

@@ -1281,6 +1281,28 @@ int reduce(pj_ctx *ctx, const void *col, int t, uint64_t n, int monoid, int64_t 
   return PJ_OK;
 }
 
+int reduce_minmax(pj_ctx *ctx, const void *col, int t, uint64_t n, int64_t *mn, int64_t *mx) {
+  if (t == F64 || t == F32) return fail(PJ_E_ARG, "reduce_minmax over an integer column only");
+  if (!n) return fail(PJ_E_ARG, "reduce_minmax of no rows");
+  Scratch part, res;
+  const int np = (int)std::min<uint64_t>(1024, grid_for(ctx, n, 256));
+  PJ_TRY(part.alloc(ctx, (uint64_t)np * 16));
+  PJ_TRY(res.alloc(ctx, 16));
+  // both reductions queued, then one read-back of the two results
+  PJ_LAUNCH(ctx, "rel_reduce", k_reduce_part, np, 256, 0, col, t, n, (int)M_MIN, part.as<int64_t>());
+  PJ_LAUNCH(ctx, "rel_reduce_final", k_reduce_final, 1, 1024, 0, part.as<const int64_t>(), np, t,
+            (int)M_MIN, res.as<int64_t>());
+  PJ_LAUNCH(ctx, "rel_reduce", k_reduce_part, np, 256, 0, col, t, n, (int)M_MAX,
+            part.as<int64_t>() + np);
+  PJ_LAUNCH(ctx, "rel_reduce_final", k_reduce_final, 1, 1024, 0, part.as<const int64_t>() + np, np,
+            t, (int)M_MAX, res.as<int64_t>() + 1);
+  int64_t v[2] = {0, 0};
+  PJ_TRY(d2h(ctx, v, res.p, 16));
+  *mn = v[0];
+  *mx = v[1];
+  return PJ_OK;
+}
+
 int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, uint64_t n,
             uint64_t max_groups, const GbOut &out, uint64_t *n_groups) {
   *n_groups = 0;

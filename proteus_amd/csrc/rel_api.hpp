@@ -76,6 +76,8 @@ enum Monoid : int32_t { M_SUM = 0, M_MIN = 1, M_MAX = 2, M_AND = 3, M_OR = 4, M_
 // in FP64 (fixed order: per-workgroup partials, then one workgroup).
 int reduce(pj_ctx *ctx, const void *col, int t, uint64_t n, int monoid, int64_t *out_i,
            double *out_f);
+// MIN and MAX of n >= 1 rows of an integer column, with one read-back.
+int reduce_minmax(pj_ctx *ctx, const void *col, int t, uint64_t n, int64_t *mn, int64_t *mx);
 
 // General hash group-by over up to 4 key columns and up to 16 aggregates.
 struct GbKey { const void *col; int32_t t; };
