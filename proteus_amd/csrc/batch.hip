@@ -29,7 +29,9 @@
 struct pj_dim {
   uint64_t n = 0;
   uint64_t slots = 0;  // table entries: a power of two (hash) or `range` (direct)
-  unsigned long long *table = nullptr;
+  unsigned long long *table = nullptr;  // 64-bit slots key << 32 | value
+  uint32_t *ids = nullptr;  // instead, for a direct table of build row ids: the id, or
+                            // kNoRow (pipelines of the plan executor)
   int32_t lo = 0;      // direct tables: smallest key
   uint32_t range = 0;  // direct tables: max key - lo + 1; 0 = hash table
 };
@@ -96,10 +98,27 @@ __global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
     t[i] = kEmptySlot;
 }
 
-// flags: bit 0 duplicate key, bit 1 reserved key INT32_MIN
+// A direct table of build row ids: slot key - lo holds the row (a row id is
+// below 2^31, so kNoRow marks an empty slot).  flags as k_dim_build.
+constexpr uint32_t kNoRow = 0xffffffffu;
+__global__ void k_dim_build_ids(const int32_t *__restrict__ k, uint64_t n, uint32_t *ids, int32_t lo,
+                                uint32_t range, unsigned int *flags) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t off = (uint32_t)k[i] - (uint32_t)lo;
+    if (off >= range) {  // cannot happen: lo and range cover every key
+      atomicOr(flags, 4u);
+      continue;
+    }
+    if (atomicCAS(&ids[off], kNoRow, (uint32_t)i) != kNoRow) atomicOr(flags, 1u);
+  }
+}
+
+// flags: bit 0 duplicate key, bit 1 reserved key INT32_MIN.  The value of
+// row i is v[i], or i itself when rows is set (v is then unused), or 0.
 __global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__restrict__ v, uint64_t n,
                             unsigned long long *t, uint64_t mask, int32_t lo, uint32_t range,
-                            unsigned int *flags) {
+                            unsigned int *flags, bool rows) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int32_t key = k[i];
@@ -107,7 +126,8 @@ __global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__rest
       atomicOr(flags, 2u);
       continue;
     }
-    const unsigned long long w = (unsigned long long)(uint32_t)key << 32 | (uint32_t)(v ? v[i] : 0);
+    const uint32_t val = rows ? (uint32_t)i : (uint32_t)(v ? v[i] : 0);
+    const unsigned long long w = (unsigned long long)(uint32_t)key << 32 | val;
     bool in;
     uint64_t h = dim_start(key, mask, lo, range, &in);
     if (!in) {  // cannot happen: lo and range cover every key
@@ -727,6 +747,7 @@ static int check_bad(const void *h_buf) {
 int pj::rel::dim_table(const pj_dim *dim, PipeTable *t) {
   if (!dim) return fail(PJ_E_ARG, "null dimension");
   t->table = dim->table;
+  t->ids = dim->ids;
   t->mask = dim->range ? 0 : dim->slots - 1;
   t->lo = dim->lo;
   t->range = dim->range;
@@ -743,8 +764,10 @@ extern "C" void pj_batch_release(pj_ctx *ctx) {
   ctx->batch = nullptr;
 }
 
-extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
-                             pj_dim **out) {
+// pj_dim_create; row_ids: the values are the build row ids (d_vals unused),
+// and a direct table stores 4-byte ids instead of 8-byte slots.
+static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
+                      bool row_ids, pj_dim **out) {
   std::lock_guard<std::mutex> g(ctx->mu);
   *out = nullptr;
   if (n && !d_keys) return fail(PJ_E_ARG, "dimension without keys");
@@ -775,7 +798,9 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
     }
   }
   dm->slots = dm->range ? dm->range : hash_slots;
-  int rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table);
+  const bool ids = row_ids && dm->range;
+  int rc = ids ? dalloc(ctx, dm->slots * 4, (void **)&dm->ids)
+               : dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;
     return rc;
@@ -783,19 +808,36 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   Scratch flags;
   auto fin = [&](int r) {
     if (r != PJ_OK) {
-      dfree(ctx, dm->table);
+      if (dm->table) dfree(ctx, dm->table);
+      if (dm->ids) dfree(ctx, dm->ids);
       delete dm;
     }
     return r;
   };
   if ((rc = flags.alloc(ctx, 4)) != PJ_OK) return fin(rc);
   if ((rc = zero(ctx, flags.p, 4)) != PJ_OK) return fin(rc);
+  if (ids) {
+    if ((rc = fill_u32(ctx, dm->ids, kNoRow, dm->slots)) != PJ_OK) return fin(rc);
+    if (n)
+      hipLaunchKernelGGL(k_dim_build_ids,
+                         dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
+                         dim3(256), 0, ctx->stream, d_keys, n, dm->ids, dm->lo, dm->range,
+                         flags.as<unsigned int>());
+    if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
+    unsigned int f = 0;
+    if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
+    if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
+    if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
+    *out = dm;
+    return PJ_OK;
+  }
   const uint64_t gs = std::min<uint64_t>(ceil_div(dm->slots, 256), (uint64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
   if (n)
     hipLaunchKernelGGL(k_dim_build, dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
                        dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->table,
-                       dm->range ? 0 : dm->slots - 1, dm->lo, dm->range, flags.as<unsigned int>());
+                       dm->range ? 0 : dm->slots - 1, dm->lo, dm->range, flags.as<unsigned int>(),
+                       row_ids);
   if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
   unsigned int f = 0;
   if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
@@ -806,11 +848,21 @@ extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *
   return PJ_OK;
 }
 
+extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
+                             pj_dim **out) {
+  return dim_create(ctx, d_keys, d_vals, n, false, out);
+}
+
+int pj::rel::dim_create_rows(pj_ctx *ctx, const int32_t *d_keys, uint64_t n, pj_dim **out) {
+  return dim_create(ctx, d_keys, nullptr, n, true, out);
+}
+
 extern "C" int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim) {
   if (!dim) return PJ_OK;
   std::lock_guard<std::mutex> g(ctx->mu);
   PJ_TRY(sync(ctx));
-  dfree(ctx, dim->table);
+  if (dim->table) dfree(ctx, dim->table);
+  if (dim->ids) dfree(ctx, dim->ids);
   delete dim;
   return PJ_OK;
 }
@@ -819,7 +871,7 @@ extern "C" int pj_dim_info(pj_ctx *ctx, const pj_dim *dim, uint64_t *rows, uint6
   (void)ctx;
   if (!dim) return fail(PJ_E_ARG, "null dimension");
   if (rows) *rows = dim->n;
-  if (table_bytes) *table_bytes = dim->slots * 8;
+  if (table_bytes) *table_bytes = dim->slots * (dim->ids ? 4 : 8);
   return PJ_OK;
 }
 

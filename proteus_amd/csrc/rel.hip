@@ -706,8 +706,17 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
               h[j] = dim_start(key[j], tb.mask, tb.lo, tb.range, &in);
               act[j] = i < n && in;
             }
-            // all of the lane's keys advance one slot per round
-            for (;;) {
+            if (tb.ids) {  // direct table of 4-byte build rows: no probe sequence
+#pragma unroll
+              for (int j = 0; j < PER; ++j) {
+                const uint32_t x = act[j] ? tb.ids[h[j]] : 0xffffffffu;
+                if (x != 0xffffffffu) {
+                  rl[rr[j]] = x;
+                  hit[j] = true;
+                }
+              }
+            } else for (;;) {
+              // all of the lane's keys advance one slot per round
               bool any = false;
               unsigned long long sv[PER];
 #pragma unroll
@@ -920,7 +929,16 @@ __device__ __forceinline__ void chain_stages(const ChainArgs &a, ChainRows<R> &c
           hit[k] = false;
           found[k] = 0;
         }
-        for (;;) {
+        if (tb.ids) {  // direct table of 4-byte build rows: one load, no key compare
+          uint32_t x[R];
+#pragma unroll
+          for (int k = 0; k < R; ++k) x[k] = tb.ids[act[k] ? h[k] : 0u];
+#pragma unroll
+          for (int k = 0; k < R; ++k) {
+            hit[k] = act[k] && x[k] != 0xffffffffu;
+            found[k] = x[k];
+          }
+        } else for (;;) {
           unsigned long long sv[R];
 #pragma unroll
           for (int k = 0; k < R; ++k) sv[k] = tb.table[act[k] ? h[k] : 0u];

@@ -1771,12 +1771,8 @@ class Exec {
           break;
         }
         Col bk = eval_as(*js.build_k, b, R::I32);
-        Col ids;
-        ids.owner = alloc(std::max<uint64_t>(b.n, 1) * 4);
-        ids.dev = ids.owner->p;
-        ck(R::iota(ctx_, (uint32_t *)ids.owner->p, b.n));
-        pj_dim *dim = nullptr;
-        int rc = pj_dim_create(ctx_, (const int32_t *)bk.dev, (const int32_t *)ids.dev, b.n, &dim);
+        pj_dim *dim = nullptr;  // the values are the build rows
+        int rc = R::dim_create_rows(ctx_, (const int32_t *)bk.dev, b.n, &dim);
         if (rc == PJ_E_ARG) {  // duplicate or reserved build keys
           ok = false;
           break;
