@@ -239,8 +239,9 @@ typedef struct pj_dim pj_dim;
 /* Resident HBM table of (key, value); keys unique and != INT32_MIN
  * (PJ_E_ARG otherwise).  d_vals may be NULL (value 0).  The input columns
  * are not kept.  Keys whose span fits the open-addressing table they would
- * take get a direct table (one slot per key of the span), others open
- * addressing; pj_dim_info reports the table's bytes. */
+ * take get a direct table (one slot per key of the span: 8 bytes, or 4 --
+ * the value alone -- when the span is dense and above 2048 keys), others
+ * open addressing; pj_dim_info reports the table's bytes. */
 int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
                   uint64_t n, pj_dim **out);
 int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim);

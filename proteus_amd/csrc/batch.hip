@@ -32,6 +32,8 @@ struct pj_dim {
   unsigned long long *table = nullptr;  // 64-bit slots key << 32 | value
   uint32_t *ids = nullptr;  // instead, for a direct table of build row ids: the id, or
                             // kNoRow (pipelines of the plan executor)
+  int32_t *vals = nullptr;  // instead, for a dense direct table (range = n: every key
+                            // of the span present): the value of key lo + i at i
   int32_t lo = 0;      // direct tables: smallest key
   uint32_t range = 0;  // direct tables: max key - lo + 1; 0 = hash table
 };
@@ -47,6 +49,7 @@ constexpr uint32_t kMaxBatch = 4096;    // queries per batch
 
 struct QDesc {
   const unsigned long long *table;
+  const int32_t *dval; // dense direct table (table null): the value of key dlo + i
   uint64_t mask;       // hash table: slots - 1
   int32_t dlo;         // direct table: smallest key
   uint32_t drange;     // direct table: slots; 0 = hash table
@@ -111,6 +114,25 @@ __global__ void k_dim_build_ids(const int32_t *__restrict__ k, uint64_t n, uint3
       continue;
     }
     if (atomicCAS(&ids[off], kNoRow, (uint32_t)i) != kNoRow) atomicOr(flags, 1u);
+  }
+}
+
+// A dense direct table (range = n keys, so every key of the span is present
+// once they are unique): vals[key - lo] = v[i] (or 0), 4 bytes a key; bits
+// (range bits, zeroed) catch a duplicate.  flags as k_dim_build.
+__global__ void k_dim_build_dense(const int32_t *__restrict__ k, const int32_t *__restrict__ v,
+                                  uint64_t n, int32_t *vals, uint32_t *bits, int32_t lo,
+                                  uint32_t range, unsigned int *flags) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t off = (uint32_t)k[i] - (uint32_t)lo;
+    if (off >= range) {  // cannot happen: lo and range cover every key
+      atomicOr(flags, 4u);
+      continue;
+    }
+    const uint32_t bit = 1u << (off & 31);
+    if (atomicOr(&bits[off >> 5], bit) & bit) atomicOr(flags, 1u);
+    vals[off] = v ? v[i] : 0;
   }
 }
 
@@ -204,19 +226,25 @@ template <bool LDS_TAB, bool LDS_GRP>
 __device__ __forceinline__ void probe_row(const QDesc &q, const unsigned long long *ltab,
                                           uint32_t *gcnt, unsigned long long *gsum, int32_t k,
                                           int32_t v, int32_t g, uint32_t &bad) {
-  bool in;
-  uint64_t h = dim_start(k, q.mask, q.dlo, q.drange, &in);
-  if (!in) return;
   int32_t b = 0;
-  for (;;) {
-    const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];
-    if (s == kEmptySlot) return;  // before the key test: a probe key INT32_MIN is a miss
-    if ((int32_t)(s >> 32) == k) {
-      b = (int32_t)(uint32_t)s;
-      break;
+  if (!LDS_TAB && q.dval) {
+    const uint32_t off = (uint32_t)k - (uint32_t)q.dlo;
+    if (off >= q.drange) return;
+    b = q.dval[off];
+  } else {
+    bool in;
+    uint64_t h = dim_start(k, q.mask, q.dlo, q.drange, &in);
+    if (!in) return;
+    for (;;) {
+      const unsigned long long s = LDS_TAB ? ltab[h] : q.table[h];
+      if (s == kEmptySlot) return;  // before the key test: a probe key INT32_MIN is a miss
+      if ((int32_t)(s >> 32) == k) {
+        b = (int32_t)(uint32_t)s;
+        break;
+      }
+      if (q.drange) return;  // a direct slot holds only its own key
+      h = (h + 1) & q.mask;
     }
-    if (q.drange) return;  // a direct slot holds only its own key
-    h = (h + 1) & q.mask;
   }
   if ((uint32_t)g >= q.groups) {
     ++bad;
@@ -242,36 +270,46 @@ __device__ __forceinline__ void probe8(const QDesc &q, const unsigned long long 
                                        const int32_t (&g)[8], uint32_t &bad) {
   const uint32_t mask = (uint32_t)q.mask;
   const bool direct = q.drange != 0;
-  uint32_t h[8];
   int32_t b[8];
-  bool act[8], hit[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &act[j]);
-    hit[j] = false;
-    b[j] = 0;
-  }
-  for (;;) {
-    unsigned long long s[8];
+  bool hit[8];
+  if (!LDS_TAB && q.dval) {  // dense: one 4-byte load, no key test
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const uint32_t at = act[j] ? h[j] : 0u;
-      s[j] = LDS_TAB ? ltab[at] : q.table[at];
+      const uint32_t off = (uint32_t)k[j] - (uint32_t)q.dlo;
+      hit[j] = off < q.drange;
+      b[j] = q.dval[hit[j] ? off : 0u];
     }
-    bool any = false;
+  } else {
+    uint32_t h[8];
+    bool act[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const bool empty = s[j] == kEmptySlot;
-      const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
-      const bool got = act[j] && eq;
-      const bool more = act[j] && !empty && !eq && !direct;
-      hit[j] = hit[j] || got;
-      b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
-      h[j] = more ? ((h[j] + 1) & mask) : h[j];
-      act[j] = more;
-      any = any || act[j];
+      h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &act[j]);
+      hit[j] = false;
+      b[j] = 0;
     }
-    if (!any) break;
+    for (;;) {
+      unsigned long long s[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t at = act[j] ? h[j] : 0u;
+        s[j] = LDS_TAB ? ltab[at] : q.table[at];
+      }
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool empty = s[j] == kEmptySlot;
+        const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
+        const bool got = act[j] && eq;
+        const bool more = act[j] && !empty && !eq && !direct;
+        hit[j] = hit[j] || got;
+        b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
+        h[j] = more ? ((h[j] + 1) & mask) : h[j];
+        act[j] = more;
+        any = any || act[j];
+      }
+      if (!any) break;
+    }
   }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -302,35 +340,45 @@ __device__ __forceinline__ void stage8(const QDesc &q, bool valid, const int32_t
                                        unsigned long long *s_pbase) {
   const uint32_t mask = (uint32_t)q.mask;
   const bool direct = q.drange != 0;
-  uint32_t h[8];
   int32_t b[8];
-  bool act[8], hit[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    bool in;
-    h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &in);
-    act[j] = valid && in;
-    hit[j] = false;
-    b[j] = 0;
-  }
-  for (;;) {
-    unsigned long long s[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = q.table[act[j] ? h[j] : 0u];
-    bool any = false;
+  bool hit[8];
+  if (q.dval) {  // dense: one 4-byte load, no key test
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const bool empty = s[j] == kEmptySlot;
-      const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
-      const bool got = act[j] && eq;
-      const bool more = act[j] && !empty && !eq && !direct;
-      hit[j] = hit[j] || got;
-      b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
-      h[j] = more ? ((h[j] + 1) & mask) : h[j];
-      act[j] = more;
-      any = any || act[j];
+      const uint32_t off = (uint32_t)k[j] - (uint32_t)q.dlo;
+      hit[j] = valid && off < q.drange;
+      b[j] = q.dval[hit[j] ? off : 0u];
     }
-    if (!any) break;
+  } else {
+    uint32_t h[8];
+    bool act[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      bool in;
+      h[j] = dim_start(k[j], q.mask, q.dlo, q.drange, &in);
+      act[j] = valid && in;
+      hit[j] = false;
+      b[j] = 0;
+    }
+    for (;;) {
+      unsigned long long s[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] = q.table[act[j] ? h[j] : 0u];
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool empty = s[j] == kEmptySlot;
+        const bool eq = !empty && (int32_t)(s[j] >> 32) == k[j];
+        const bool got = act[j] && eq;
+        const bool more = act[j] && !empty && !eq && !direct;
+        hit[j] = hit[j] || got;
+        b[j] = got ? (int32_t)(uint32_t)s[j] : b[j];
+        h[j] = more ? ((h[j] + 1) & mask) : h[j];
+        act[j] = more;
+        any = any || act[j];
+      }
+      if (!any) break;
+    }
   }
   uint32_t rank[8];
 #pragma unroll
@@ -579,6 +627,7 @@ static int fill_desc(const pj_query &q, QDesc *d) {
     return fail(PJ_E_ARG, "generated query needs fk_range in [1, 2^31]");
   if (q.n >= (1ULL << 40)) return fail(PJ_E_ARG, "query probe side too large");
   d->table = q.dim->table;
+  d->dval = q.dim->vals;
   d->mask = q.dim->range ? 0 : q.dim->slots - 1;
   d->dlo = q.dim->lo;
   d->drange = q.dim->range;
@@ -664,7 +713,7 @@ static int pack(const pj_query *qs, uint32_t nq, uint32_t T, uint32_t lds_max, v
   for (int pass = 0; pass < 2; ++pass)
     for (uint32_t i = 0; i < nq; ++i) {
       if (!qs[i].dim) return fail(PJ_E_ARG, "query without a dimension");
-      const bool lds = lds_max && qs[i].dim->n <= lds_max;
+      const bool lds = lds_max && qs[i].dim->table && qs[i].dim->n <= lds_max;
       if (lds != (pass == 0)) continue;
       PJ_TRY(fill_desc(qs[i], &d[j]));
       d[j].tile0 = tiles;
@@ -746,6 +795,7 @@ static int check_bad(const void *h_buf) {
 
 int pj::rel::dim_table(const pj_dim *dim, PipeTable *t) {
   if (!dim) return fail(PJ_E_ARG, "null dimension");
+  if (dim->vals) return fail(PJ_E_ARG, "dimension of values, not of build rows");
   t->table = dim->table;
   t->ids = dim->ids;
   t->mask = dim->range ? 0 : dim->slots - 1;
@@ -780,7 +830,9 @@ static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
   // kLdsSlots slots.  Keys whose span is no larger than that table (dense
   // keys, as primary keys usually are, or a filtered subset of them) get a
   // direct table indexed by key - min instead: one load per probe, and keys
-  // outside the span miss without one.
+  // outside the span miss without one.  A dense span (as many keys as slots)
+  // above the LDS size holds just the 4-byte values: every slot is a key,
+  // and half the bytes per slot put more of a table in each cache level.
   const int lg = ilog2_ceil(n ? n : 1);
   const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
   const uint64_t hash_slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
@@ -799,8 +851,10 @@ static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
   }
   dm->slots = dm->range ? dm->range : hash_slots;
   const bool ids = row_ids && dm->range;
-  int rc = ids ? dalloc(ctx, dm->slots * 4, (void **)&dm->ids)
-               : dalloc(ctx, dm->slots * 8, (void **)&dm->table);
+  const bool dense = !row_ids && dm->range && dm->range == n && n > kLdsDim;
+  int rc = ids     ? dalloc(ctx, dm->slots * 4, (void **)&dm->ids)
+           : dense ? dalloc(ctx, dm->slots * 4, (void **)&dm->vals)
+                   : dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;
     return rc;
@@ -810,6 +864,7 @@ static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
     if (r != PJ_OK) {
       if (dm->table) dfree(ctx, dm->table);
       if (dm->ids) dfree(ctx, dm->ids);
+      if (dm->vals) dfree(ctx, dm->vals);
       delete dm;
     }
     return r;
@@ -823,6 +878,23 @@ static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals,
                          dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
                          dim3(256), 0, ctx->stream, d_keys, n, dm->ids, dm->lo, dm->range,
                          flags.as<unsigned int>());
+    if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
+    unsigned int f = 0;
+    if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
+    if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
+    if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
+    *out = dm;
+    return PJ_OK;
+  }
+  if (dense) {
+    Scratch bits;
+    const uint64_t words = ceil_div(dm->slots, 32);
+    if ((rc = bits.alloc(ctx, words * 4)) != PJ_OK) return fin(rc);
+    if ((rc = zero(ctx, bits.p, words * 4)) != PJ_OK) return fin(rc);
+    hipLaunchKernelGGL(k_dim_build_dense,
+                       dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
+                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->vals,
+                       bits.as<uint32_t>(), dm->lo, dm->range, flags.as<unsigned int>());
     if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
     unsigned int f = 0;
     if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
@@ -863,6 +935,7 @@ extern "C" int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim) {
   PJ_TRY(sync(ctx));
   if (dim->table) dfree(ctx, dim->table);
   if (dim->ids) dfree(ctx, dim->ids);
+  if (dim->vals) dfree(ctx, dim->vals);
   delete dim;
   return PJ_OK;
 }
@@ -871,7 +944,7 @@ extern "C" int pj_dim_info(pj_ctx *ctx, const pj_dim *dim, uint64_t *rows, uint6
   (void)ctx;
   if (!dim) return fail(PJ_E_ARG, "null dimension");
   if (rows) *rows = dim->n;
-  if (table_bytes) *table_bytes = dim->slots * (dim->ids ? 4 : 8);
+  if (table_bytes) *table_bytes = dim->slots * (dim->table ? 8 : 4);
   return PJ_OK;
 }
 

@@ -50,6 +50,16 @@ def test_dimension_errors(ctx):
     with pytest.raises(pa.PjError) as e:
         ctx.dim_create(ctx.to_device(np.array([5, 6, 7, 6], np.int32)))
     assert e.value.code == pa.PJ_E_ARG
+    # above 2048 keys a dense span keeps the 4-byte values alone; a duplicate
+    # that leaves the span as wide as the key count is still caught
+    d = ctx.dim_create(ctx.to_device(np.arange(-7, 4993, dtype=np.int32)))
+    assert ctx.dim_info(d) == (5000, 5000 * 4)
+    ctx.dim_destroy(d)
+    k = np.arange(5000, dtype=np.int32)
+    k[10] = k[20]
+    with pytest.raises(pa.PjError) as e:
+        ctx.dim_create(ctx.to_device(k))
+    assert e.value.code == pa.PJ_E_ARG
 
 
 @pytest.mark.gpu
@@ -230,7 +240,8 @@ def test_direct_and_hashed_dimensions(ctx, lo, step, groups):
     dim = ctx.dim_create(ctx.to_device(keys.astype(np.int32)), ctx.to_device(vals))
     span = step * nb
     direct = span - step + 1 <= (1 << 17)  # the hash table of 5000 keys: 2^17 slots
-    assert ctx.dim_info(dim)[1] == ((span - step + 1) * 8 if direct else (1 << 17) * 8)
+    slot = 4 if step == 1 else 8  # a dense span above 2048 keys: 4-byte values alone
+    assert ctx.dim_info(dim)[1] == ((span - step + 1) * slot if direct else (1 << 17) * 8)
     n = 200003
     fk = np.clip(lo - 1000 + rs.integers(0, span + 2000, n), -(1 << 31), (1 << 31) - 1)
     fk[:3] = [-(1 << 31), (1 << 31) - 1, lo]
