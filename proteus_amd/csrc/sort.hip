@@ -325,24 +325,33 @@ static int type_bytes(int t) {
   return t == I64 || t == F64 ? 8 : t == BOOL ? 1 : 4;
 }
 
-// Small sorts (n <= kRankSortMax rows): one workgroup ranks every row by
-// comparing it with every other row on the order-preserving keys of all sort
-// expressions at once (key 0 most significant), ties by row index, so the
-// result is the stable lexicographic order; perm[rank] = row.  The keys
-// (nk x n u64) sit in LDS and every lane of a wave reads the same row j at
-// a time (broadcast).  A handful of launches replace the radix passes' ten
-// per key, which at hundreds of rows are launch latency.
+// Small sorts (n <= kRankSortMax rows): every row is ranked by comparing it
+// with every other row on the order-preserving keys of all sort expressions
+// at once (key 0 most significant), ties by row index, so the result is the
+// stable lexicographic order; perm[rank] = row.  Workgroup b ranks rows
+// [64 b, 64 b + 64), one per lane; its 16 waves split the rows compared
+// against into sixteenths and add their partial ranks in LDS, so a lane's
+// chain of dependent LDS compares is n / 16 long.  The keys (nk x n u64)
+// sit in LDS of every workgroup and every lane of a wave reads the same row
+// j at a time (broadcast).  A handful of launches replace the radix passes'
+// ten per key, which at hundreds of rows are launch latency.
 constexpr uint64_t kRankSortMax = 2048;
 constexpr uint64_t kRankSortLds = 65536;  // bytes of keys in LDS
+constexpr int kRankWaves = 16;
 __global__ __launch_bounds__(1024) void k_rank_sort(const unsigned long long *__restrict__ keys,
                                                     int nk, uint32_t n,
                                                     uint32_t *__restrict__ perm) {
   extern __shared__ unsigned long long sk[];
+  __shared__ uint32_t part[kRankWaves][64];
   for (uint32_t x = threadIdx.x; x < (uint32_t)nk * n; x += blockDim.x) sk[x] = keys[x];
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < n; ++j) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 64 + lane;
+  const uint32_t per = (n + kRankWaves - 1) / kRankWaves;
+  const uint32_t j0 = w * per, j1 = min(n, j0 + per);
+  uint32_t rank = 0;
+  if (i < n)
+    for (uint32_t j = j0; j < j1; ++j) {
       // row j before row i: first differing key smaller, or all equal and j < i
       int c = 0;
       for (int k = 0; k < nk && c == 0; ++k) {
@@ -351,7 +360,13 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const unsigned long long *__
       }
       rank += (c < 0 || (c == 0 && j < i)) ? 1u : 0u;
     }
-    perm[rank] = i;
+  part[w][lane] = rank;
+  __syncthreads();
+  if (w == 0 && i < n) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int q = 0; q < kRankWaves; ++q) r += part[q][lane];
+    perm[r] = i;
   }
 }
 
@@ -395,7 +410,8 @@ static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *s
                 kk.as<unsigned long long>() + (uint64_t)at * n);
       ++at;
     }
-    PJ_LAUNCH(ctx, "sort_rank", k_rank_sort, 1, 1024, (size_t)nka * n * 8,
+    PJ_LAUNCH(ctx, "sort_rank", k_rank_sort, (uint32_t)ceil_div(n, (uint64_t)64), 1024,
+              (size_t)nka * n * 8,
               kk.as<const unsigned long long>(), nka, (uint32_t)n, perm);
     return sync(ctx);
   }
