@@ -15,8 +15,9 @@
 // DESC inverts the key; NONE directions are skipped.
 //
 // The radix sort (8-bit digits, least significant first):
-//   * one read of the keys counts every digit position; a position where all
-//     keys share one digit is skipped (narrow key ranges sort in fewer
+//   * one read of the keys ORs them and their complements together; a digit
+//     position where both ORs leave a byte with no bit set in common is one
+//     every key shares, and is skipped (narrow key ranges sort in fewer
 //     passes);
 //   * per pass, the input is cut into G chunks, one workgroup each (as many
 //     as fit the CUs at once); k_rs_chunk_hist counts each chunk's digits,
@@ -59,23 +60,40 @@ __device__ __forceinline__ uint32_t rs_digit(K k, int shift) {
   return (uint32_t)(k >> shift) & 255u;
 }
 
-// Counts of every 8-bit digit position of the keys, hist[p * 256 + d].
+// Workgroup b writes the OR of the complements of its keys to ors[2 b] and
+// the OR of its keys to ors[2 b + 1]: bit i varies among the keys iff it is
+// set in both ORs over all workgroups (folded on the host: same-address
+// atomics from every wave serialise in L2).
 template <class K>
 __global__ __launch_bounds__(256) void k_rs_digits(const K *__restrict__ keys, uint64_t n,
-                                                   uint32_t *__restrict__ hist) {
-  constexpr int P = sizeof(K);
-  __shared__ uint32_t h[P * 256];
-  for (int i = threadIdx.x; i < P * 256; i += 256) h[i] = 0;
-  __syncthreads();
-  const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    const K k = keys[i];
+                                                   unsigned long long *__restrict__ ors) {
+  __shared__ unsigned long long s0[4], s1[4];
+  constexpr int R = 8;  // loads in flight per thread
+  unsigned long long o0 = 0, o1 = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256 * R;
+  for (uint64_t b = (uint64_t)blockIdx.x * 256 * R + threadIdx.x; b < n; b += stride) {
+    K k[R];
 #pragma unroll
-    for (int p = 0; p < P; ++p) atomicAdd(&h[p * 256 + rs_digit(k, 8 * p)], 1u);
+    for (int j = 0; j < R; ++j) k[j] = b + j * 256 < n ? keys[b + j * 256] : keys[b];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      o0 |= (unsigned long long)(K)~k[j];
+      o1 |= (unsigned long long)k[j];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    o0 |= __shfl_xor(o0, off);
+    o1 |= __shfl_xor(o1, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s0[threadIdx.x >> 6] = o0;
+    s1[threadIdx.x >> 6] = o1;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < P * 256; i += 256)
-    if (h[i]) atomicAdd(&hist[i], h[i]);
+  if (threadIdx.x == 0) {
+    ors[2 * blockIdx.x] = s0[0] | s0[1] | s0[2] | s0[3];
+    ors[2 * blockIdx.x + 1] = s1[0] | s1[1] | s1[2] | s1[3];
+  }
 }
 
 // Digit counts of chunk c = [c * chunk, min(n, (c + 1) * chunk)) into row c
@@ -226,19 +244,22 @@ int radix_sort_pairs(pj_ctx *ctx, K **keys, K **keys_alt, uint32_t **vals, uint3
                      uint64_t n) {
   constexpr int P = sizeof(K);
   if (n < 2) return PJ_OK;
+  const uint64_t dgrid = std::min<uint64_t>(ceil_div(n, (uint64_t)256 * 8), (uint64_t)ctx->num_cus * 4);
   Scratch dh;
-  PJ_TRY(dh.alloc(ctx, P * 256 * 4));
-  PJ_TRY(zero(ctx, dh.p, P * 256 * 4));
-  const uint64_t dgrid = std::min<uint64_t>(ceil_div(n, (uint64_t)256 * 16), (uint64_t)ctx->num_cus * 4);
-  PJ_LAUNCH(ctx, "sort_digits", k_rs_digits<K>, dgrid, 256, 0, *keys, n, dh.as<uint32_t>());
-  std::vector<uint32_t> hh(P * 256);
-  PJ_TRY(d2h(ctx, hh.data(), dh.p, P * 256 * 4));
-  std::vector<int> passes;
-  for (int p = 0; p < P; ++p) {
-    uint32_t mx = 0;
-    for (int d = 0; d < 256; ++d) mx = std::max(mx, hh[p * 256 + d]);
-    if (mx != n) passes.push_back(p);
+  PJ_TRY(dh.alloc(ctx, dgrid * 16));
+  PJ_LAUNCH(ctx, "sort_digits", k_rs_digits<K>, dgrid, 256, 0, *keys, n,
+            dh.as<unsigned long long>());
+  std::vector<unsigned long long> ors(dgrid * 2);
+  PJ_TRY(d2h(ctx, ors.data(), dh.p, dgrid * 16));
+  unsigned long long o0 = 0, o1 = 0;
+  for (uint64_t b = 0; b < dgrid; ++b) {
+    o0 |= ors[2 * b];
+    o1 |= ors[2 * b + 1];
   }
+  const unsigned long long varying = o0 & o1;
+  std::vector<int> passes;
+  for (int p = 0; p < P; ++p)
+    if ((varying >> (8 * p)) & 255u) passes.push_back(p);
   if (passes.empty()) return PJ_OK;
   // workgroups that fit the CUs at once (LDS-bound), chunks whole tiles
   constexpr uint64_t kTile = (uint64_t)RsCfg<K>::BLOCK * RsCfg<K>::IPT;
@@ -281,38 +302,64 @@ __global__ __launch_bounds__(256) void k_sort_key(const char *__restrict__ col, 
                                                   const int32_t *__restrict__ rank, uint32_t nrank,
                                                   const uint32_t *__restrict__ perm, uint64_t n,
                                                   K *__restrict__ key) {
-  const uint64_t gs = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += gs) {
-    const char *p = col + (uint64_t)perm[i] * stride;
-    unsigned long long u;
-    switch (t) {
-      case I32: {
-        int32_t v = *(const int32_t *)p;
-        if (rank) v = (uint32_t)v < nrank ? rank[v] : (int32_t)nrank;  // unknown codes last
-        u = (uint32_t)v ^ 0x80000000u;
-        if (desc) u = ~u & 0xffffffffULL;
-        break;
-      }
-      case BOOL: {
-        u = *(const uint8_t *)p != 0;
-        if (desc) u = ~u & 0xffffffffULL;
-        break;
-      }
-      case I64: {
-        u = (unsigned long long)*(const int64_t *)p ^ 0x8000000000000000ULL;
-        if (desc) u = ~u;
-        break;
-      }
-      default: {  // F64, F32
-        double d = t == F32 ? (double)*(const float *)p : *(const double *)p;
-        if (d == 0.0) d = 0.0;
-        const unsigned long long b = __builtin_bit_cast(unsigned long long, d);
-        u = (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
-        if (desc) u = ~u;
-        break;
-      }
+  // four rows per thread: their row ids, then their values, are loaded
+  // together so the gathers' latencies overlap
+  constexpr int R = 4;
+  const uint64_t gs = (uint64_t)gridDim.x * 256 * R;
+  const bool wide = t == I64 || t == F64;
+  for (uint64_t b = (uint64_t)blockIdx.x * 256 * R + threadIdx.x; b < n; b += gs) {
+    bool ok[R];
+    uint32_t pr[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      ok[j] = b + j * 256 < n;
+      pr[j] = perm[ok[j] ? b + j * 256 : b];
     }
-    key[i] = (K)u;
+    unsigned long long raw[R];
+    if (wide) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) raw[j] = *(const unsigned long long *)(col + (uint64_t)pr[j] * stride);
+    } else if (t == BOOL) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) raw[j] = *(const uint8_t *)(col + (uint64_t)pr[j] * stride);
+    } else {  // I32, F32
+#pragma unroll
+      for (int j = 0; j < R; ++j) raw[j] = *(const uint32_t *)(col + (uint64_t)pr[j] * stride);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (!ok[j]) continue;
+      unsigned long long u;
+      switch (t) {
+        case I32: {
+          int32_t v = (int32_t)(uint32_t)raw[j];
+          if (rank) v = (uint32_t)v < nrank ? rank[v] : (int32_t)nrank;  // unknown codes last
+          u = (uint32_t)v ^ 0x80000000u;
+          if (desc) u = ~u & 0xffffffffULL;
+          break;
+        }
+        case BOOL: {
+          u = raw[j] != 0;
+          if (desc) u = ~u & 0xffffffffULL;
+          break;
+        }
+        case I64: {
+          u = raw[j] ^ 0x8000000000000000ULL;
+          if (desc) u = ~u;
+          break;
+        }
+        default: {  // F64, F32
+          double d = t == F32 ? (double)__builtin_bit_cast(float, (uint32_t)raw[j])
+                              : __builtin_bit_cast(double, raw[j]);
+          if (d == 0.0) d = 0.0;
+          const unsigned long long bb = __builtin_bit_cast(unsigned long long, d);
+          u = (bb >> 63) ? ~bb : (bb | 0x8000000000000000ULL);
+          if (desc) u = ~u;
+          break;
+        }
+      }
+      key[b + j * 256] = (K)u;
+    }
   }
 }
 
