@@ -102,12 +102,15 @@ __global__ void k_dim_fill(unsigned long long *t, uint64_t n) {
 }
 
 // A direct table of build row ids: slot key - lo holds the row (a row id is
-// below 2^31, so kNoRow marks an empty slot).  flags as k_dim_build.
+// below 2^31, so kNoRow marks an empty slot).  Rows whose mask byte is 0
+// (mask may be null) are left out.  flags as k_dim_build.
 constexpr uint32_t kNoRow = 0xffffffffu;
-__global__ void k_dim_build_ids(const int32_t *__restrict__ k, uint64_t n, uint32_t *ids, int32_t lo,
-                                uint32_t range, unsigned int *flags) {
+__global__ void k_dim_build_ids(const int32_t *__restrict__ k, const uint8_t *__restrict__ mask,
+                                uint64_t n, uint32_t *ids, int32_t lo, uint32_t range,
+                                unsigned int *flags) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (mask && !mask[i]) continue;
     const uint32_t off = (uint32_t)k[i] - (uint32_t)lo;
     if (off >= range) {  // cannot happen: lo and range cover every key
       atomicOr(flags, 4u);
@@ -138,11 +141,14 @@ __global__ void k_dim_build_dense(const int32_t *__restrict__ k, const int32_t *
 
 // flags: bit 0 duplicate key, bit 1 reserved key INT32_MIN.  The value of
 // row i is v[i], or i itself when rows is set (v is then unused), or 0.
-__global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__restrict__ v, uint64_t n,
-                            unsigned long long *t, uint64_t mask, int32_t lo, uint32_t range,
-                            unsigned int *flags, bool rows) {
+// Rows whose keep byte is 0 (keep may be null) are left out.
+__global__ void k_dim_build(const int32_t *__restrict__ k, const int32_t *__restrict__ v,
+                            const uint8_t *__restrict__ keep, uint64_t n, unsigned long long *t,
+                            uint64_t mask, int32_t lo, uint32_t range, unsigned int *flags,
+                            bool rows) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (keep && !keep[i]) continue;
     const int32_t key = k[i];
     if (key == INT32_MIN) {
       atomicOr(flags, 2u);
@@ -814,129 +820,249 @@ extern "C" void pj_batch_release(pj_ctx *ctx) {
   ctx->batch = nullptr;
 }
 
-// pj_dim_create; row_ids: the values are the build row ids (d_vals unused),
-// and a direct table stores 4-byte ids instead of 8-byte slots.
+// Hash-table slots of a dimension of n keys: open addressing at load <= 1/2;
+// tables of mid-sized dimensions (that still fit the L2s) at load <= 1/16,
+// which shortens the longest probe sequence a wave waits for.  Dimensions
+// probed from LDS keep <= kLdsSlots slots.
+static uint64_t dim_hash_slots(uint64_t n) {
+  const int lg = ilog2_ceil(n ? n : 1);
+  const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
+  return std::max<uint64_t>(64, 1ULL << (lg + spread));
+}
+
+// Keys whose span is no larger than the hash table they would take (dense
+// keys, as primary keys usually are, or a filtered subset of them) -- or, for
+// tables of build rows, than twice the build side's rows -- get a direct
+// table indexed by key - min: one load per probe, and keys outside the span
+// miss without one.  kmin > kmax: no keys.
+static void dim_layout(pj_dim *dm, uint64_t keys, uint64_t rows, int64_t kmin, int64_t kmax) {
+  const uint64_t hs = dim_hash_slots(keys);
+  if (keys && kmin > INT32_MIN) {
+    const uint64_t span = (uint64_t)(kmax - kmin) + 1;
+    if (span <= std::max<uint64_t>(2 * rows, hs)) {
+      dm->lo = (int32_t)kmin;
+      dm->range = (uint32_t)span;
+    }
+  }
+  dm->slots = dm->range ? dm->range : hs;
+}
+
+static void dim_free(pj_ctx *ctx, pj_dim *dm) {
+  if (!dm) return;
+  if (dm->table) dfree(ctx, dm->table);
+  if (dm->ids) dfree(ctx, dm->ids);
+  if (dm->vals) dfree(ctx, dm->vals);
+  delete dm;
+}
+
+static int dim_flags_error(unsigned int f) {
+  if (f & 2) return fail(PJ_E_ARG, "dimension key INT32_MIN is reserved");
+  if (f & 1) return fail(PJ_E_ARG, "dimension keys must be unique");
+  if (f & 4) return fail(PJ_E_HIP, "dimension key outside its direct table");
+  return PJ_OK;
+}
+
+// pj_dim_create.  A dense span (as many keys as slots) above the LDS size
+// holds just the 4-byte values: every slot is a key, and half the bytes per
+// slot put more of a table in each cache level.
 static int dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
-                      bool row_ids, pj_dim **out) {
+                      pj_dim **out) {
   std::lock_guard<std::mutex> g(ctx->mu);
   *out = nullptr;
   if (n && !d_keys) return fail(PJ_E_ARG, "dimension without keys");
   if (n >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
   pj_dim *dm = new pj_dim();
   dm->n = n;
-  // Open addressing at load <= 1/2; tables of mid-sized dimensions (that
-  // still fit the L2s) at load <= 1/16, which shortens the longest probe
-  // sequence a wave waits for.  Dimensions probed from LDS keep <=
-  // kLdsSlots slots.  Keys whose span is no larger than that table (dense
-  // keys, as primary keys usually are, or a filtered subset of them) get a
-  // direct table indexed by key - min instead: one load per probe, and keys
-  // outside the span miss without one.  A dense span (as many keys as slots)
-  // above the LDS size holds just the 4-byte values: every slot is a key,
-  // and half the bytes per slot put more of a table in each cache level.
-  const int lg = ilog2_ceil(n ? n : 1);
-  const int spread = (n > kLdsDim && n <= (1u << 20)) ? 4 : 1;
-  const uint64_t hash_slots = std::max<uint64_t>(64, 1ULL << (lg + spread));
+  int64_t kmin = 1, kmax = 0;
   if (n) {
-    int64_t kmin = 0, kmax = 0;
     int rc = rel::reduce_minmax(ctx, d_keys, rel::I32, n, &kmin, &kmax);
     if (rc != PJ_OK) {
       delete dm;
       return rc;
     }
-    const uint64_t span = (uint64_t)(kmax - kmin) + 1;
-    if (kmin > INT32_MIN && span <= std::max<uint64_t>(2 * n, hash_slots)) {
-      dm->lo = (int32_t)kmin;
-      dm->range = (uint32_t)span;
-    }
   }
-  dm->slots = dm->range ? dm->range : hash_slots;
-  const bool ids = row_ids && dm->range;
-  const bool dense = !row_ids && dm->range && dm->range == n && n > kLdsDim;
-  int rc = ids     ? dalloc(ctx, dm->slots * 4, (void **)&dm->ids)
-           : dense ? dalloc(ctx, dm->slots * 4, (void **)&dm->vals)
-                   : dalloc(ctx, dm->slots * 8, (void **)&dm->table);
+  dim_layout(dm, n, n, kmin, kmax);
+  const bool dense = dm->range && dm->range == n && n > kLdsDim;
+  int rc = dense ? dalloc(ctx, dm->slots * 4, (void **)&dm->vals)
+                 : dalloc(ctx, dm->slots * 8, (void **)&dm->table);
   if (rc != PJ_OK) {
     delete dm;
     return rc;
   }
   Scratch flags;
   auto fin = [&](int r) {
-    if (r != PJ_OK) {
-      if (dm->table) dfree(ctx, dm->table);
-      if (dm->ids) dfree(ctx, dm->ids);
-      if (dm->vals) dfree(ctx, dm->vals);
-      delete dm;
-    }
+    if (r != PJ_OK) dim_free(ctx, dm);
     return r;
   };
   if ((rc = flags.alloc(ctx, 4)) != PJ_OK) return fin(rc);
   if ((rc = zero(ctx, flags.p, 4)) != PJ_OK) return fin(rc);
-  if (ids) {
-    if ((rc = fill_u32(ctx, dm->ids, kNoRow, dm->slots)) != PJ_OK) return fin(rc);
-    if (n)
-      hipLaunchKernelGGL(k_dim_build_ids,
-                         dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
-                         dim3(256), 0, ctx->stream, d_keys, n, dm->ids, dm->lo, dm->range,
-                         flags.as<unsigned int>());
-    if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
-    unsigned int f = 0;
-    if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
-    if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
-    if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
-    *out = dm;
-    return PJ_OK;
-  }
+  const dim3 bgrid((uint32_t)std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16));
+  Scratch bits;
   if (dense) {
-    Scratch bits;
     const uint64_t words = ceil_div(dm->slots, 32);
     if ((rc = bits.alloc(ctx, words * 4)) != PJ_OK) return fin(rc);
     if ((rc = zero(ctx, bits.p, words * 4)) != PJ_OK) return fin(rc);
-    hipLaunchKernelGGL(k_dim_build_dense,
-                       dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
-                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->vals,
-                       bits.as<uint32_t>(), dm->lo, dm->range, flags.as<unsigned int>());
-    if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
-    unsigned int f = 0;
-    if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
-    if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
-    if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
-    *out = dm;
-    return PJ_OK;
+    hipLaunchKernelGGL(k_dim_build_dense, bgrid, dim3(256), 0, ctx->stream, d_keys, d_vals, n,
+                       dm->vals, bits.as<uint32_t>(), dm->lo, dm->range, flags.as<unsigned int>());
+  } else {
+    const uint64_t gs = std::min<uint64_t>(ceil_div(dm->slots, 256), (uint64_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
+    if (n)
+      hipLaunchKernelGGL(k_dim_build, bgrid, dim3(256), 0, ctx->stream, d_keys, d_vals,
+                         (const uint8_t *)nullptr, n, dm->table, dm->range ? 0 : dm->slots - 1,
+                         dm->lo, dm->range, flags.as<unsigned int>(), false);
   }
-  const uint64_t gs = std::min<uint64_t>(ceil_div(dm->slots, 256), (uint64_t)ctx->num_cus * 16);
-  hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
-  if (n)
-    hipLaunchKernelGGL(k_dim_build, dim3(std::min<uint64_t>(ceil_div(n, 256), (uint64_t)ctx->num_cus * 16)),
-                       dim3(256), 0, ctx->stream, d_keys, d_vals, n, dm->table,
-                       dm->range ? 0 : dm->slots - 1, dm->lo, dm->range, flags.as<unsigned int>(),
-                       row_ids);
   if (hipGetLastError() != hipSuccess) return fin(fail(PJ_E_HIP, "dimension build launch failed"));
   unsigned int f = 0;
   if ((rc = d2h(ctx, &f, flags.p, 4)) != PJ_OK) return fin(rc);
-  if (f & 2) return fin(fail(PJ_E_ARG, "dimension key INT32_MIN is reserved"));
-  if (f & 1) return fin(fail(PJ_E_ARG, "dimension keys must be unique"));
-  if (f & 4) return fin(fail(PJ_E_HIP, "dimension key outside its direct table"));
+  if ((rc = dim_flags_error(f)) != PJ_OK) return fin(rc);
   *out = dm;
   return PJ_OK;
 }
 
 extern "C" int pj_dim_create(pj_ctx *ctx, const int32_t *d_keys, const int32_t *d_vals, uint64_t n,
                              pj_dim **out) {
-  return dim_create(ctx, d_keys, d_vals, n, false, out);
+  return dim_create(ctx, d_keys, d_vals, n, out);
 }
 
-int pj::rel::dim_create_rows(pj_ctx *ctx, const int32_t *d_keys, uint64_t n, pj_dim **out) {
-  return dim_create(ctx, d_keys, nullptr, n, true, out);
+namespace pj {
+// Per-workgroup min, max and count of the kept keys (keep may be null):
+// part[3 b] = min, part[3 b + 1] = max, part[3 b + 2] = count.
+__global__ __launch_bounds__(256) void k_dim_range(const int32_t *__restrict__ k,
+                                                   const uint8_t *__restrict__ keep, uint64_t n,
+                                                   long long *__restrict__ part) {
+  __shared__ int64_t smn[4], smx[4], scn[4];
+  int64_t mn = INT64_MAX, mx = INT64_MIN, c = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    if (keep && !keep[i]) continue;
+    const int64_t v = k[i];
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+    ++c;
+  }
+  mn = wave_min_i64(mn);
+  mx = wave_max_i64(mx);
+  c = wave_sum_i64(c);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smn[w] = mn;
+    smx[w] = mx;
+    scn[w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int x = 1; x < 4; ++x) {
+      mn = smn[x] < mn ? smn[x] : mn;
+      mx = smx[x] > mx ? smx[x] : mx;
+      c += scn[x];
+    }
+    part[3 * blockIdx.x] = mn;
+    part[3 * blockIdx.x + 1] = mx;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// Folds the partials of side i (workgroups first[i] .. first[i + 1] - 1)
+// into out[3 i .. 3 i + 2], one thread per side: the read-back stays a few
+// bytes per side.
+__global__ void k_dim_range_fold(const long long *__restrict__ part,
+                                 const uint32_t *__restrict__ first, int m,
+                                 long long *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  long long mn = INT64_MAX, mx = INT64_MIN, c = 0;
+  for (uint32_t b = first[i]; b < first[i + 1]; ++b) {
+    mn = part[3 * b] < mn ? part[3 * b] : mn;
+    mx = part[3 * b + 1] > mx ? part[3 * b + 1] : mx;
+    c += part[3 * b + 2];
+  }
+  out[3 * i] = mn;
+  out[3 * i + 1] = mx;
+  out[3 * i + 2] = c;
+}
+}  // namespace pj
+
+int pj::rel::dims_create_rows(pj_ctx *ctx, int m, const int32_t *const *d_keys,
+                              const uint8_t *const *d_keep, const uint64_t *n, pj_dim **out) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  for (int i = 0; i < m; ++i) out[i] = nullptr;
+  for (int i = 0; i < m; ++i) {
+    if (n[i] && !d_keys[i]) return fail(PJ_E_ARG, "dimension without keys");
+    if (n[i] >= (1ULL << 31)) return fail(PJ_E_CAPACITY, "dimension above 2^31 rows");
+  }
+  // 1. min, max and count of every side's kept keys, one read-back for all
+  std::vector<uint32_t> first(m + 1, 0);
+  for (int i = 0; i < m; ++i)
+    first[i + 1] = first[i] + (uint32_t)(n[i] ? std::min<uint64_t>(ceil_div(n[i], 1024), 256) : 0);
+  std::vector<long long> range(3 * (uint64_t)m);
+  for (int i = 0; i < m; ++i) {  // sides of no rows
+    range[3 * i] = INT64_MAX;
+    range[3 * i + 1] = INT64_MIN;
+    range[3 * i + 2] = 0;
+  }
+  if (first[m]) {
+    Scratch dpart, dfirst, dout;
+    PJ_TRY(dpart.alloc(ctx, (uint64_t)first[m] * 3 * 8));
+    PJ_TRY(dfirst.alloc(ctx, (uint64_t)(m + 1) * 4));
+    PJ_TRY(dout.alloc(ctx, (uint64_t)m * 3 * 8));
+    PJ_HIP(hipMemcpyAsync(dfirst.p, first.data(), (uint64_t)(m + 1) * 4, hipMemcpyHostToDevice,
+                          ctx->stream));
+    for (int i = 0; i < m; ++i)
+      if (n[i])
+        PJ_LAUNCH(ctx, "dim_range", k_dim_range, first[i + 1] - first[i], 256, 0, d_keys[i],
+                  d_keep[i], n[i], dpart.as<long long>() + 3 * (uint64_t)first[i]);
+    PJ_LAUNCH(ctx, "dim_range_fold", k_dim_range_fold, (uint32_t)ceil_div(m, 64), 64, 0,
+              dpart.as<const long long>(), dfirst.as<const uint32_t>(), m, dout.as<long long>());
+    PJ_TRY(d2h(ctx, range.data(), dout.p, (uint64_t)m * 3 * 8));
+  }
+  // 2. every table, then one read-back of every side's flags
+  std::vector<pj_dim *> dms(m, nullptr);
+  auto drop = [&](int r) {
+    for (pj_dim *d : dms) dim_free(ctx, d);
+    return r;
+  };
+  Scratch flags;
+  int rc = flags.alloc(ctx, (uint64_t)m * 4);
+  if (rc == PJ_OK) rc = zero(ctx, flags.p, (uint64_t)m * 4);
+  if (rc != PJ_OK) return rc;
+  for (int i = 0; i < m; ++i) {
+    const int64_t kmin = range[3 * i], kmax = range[3 * i + 1], cnt = range[3 * i + 2];
+    pj_dim *dm = dms[i] = new pj_dim();
+    dm->n = n[i];
+    dim_layout(dm, (uint64_t)cnt, n[i], kmin, kmax);
+    unsigned int *fl = flags.as<unsigned int>() + i;
+    const dim3 bgrid((uint32_t)std::min<uint64_t>(ceil_div(n[i], 256), (uint64_t)ctx->num_cus * 16));
+    if (dm->range) {  // 4-byte build rows, kNoRow where the span has no key
+      if ((rc = dalloc(ctx, dm->slots * 4, (void **)&dm->ids)) != PJ_OK) return drop(rc);
+      if ((rc = fill_u32(ctx, dm->ids, kNoRow, dm->slots)) != PJ_OK) return drop(rc);
+      if (cnt)
+        hipLaunchKernelGGL(k_dim_build_ids, bgrid, dim3(256), 0, ctx->stream, d_keys[i],
+                           d_keep[i], n[i], dm->ids, dm->lo, dm->range, fl);
+    } else {
+      if ((rc = dalloc(ctx, dm->slots * 8, (void **)&dm->table)) != PJ_OK) return drop(rc);
+      const uint64_t gs = std::min<uint64_t>(ceil_div(dm->slots, 256), (uint64_t)ctx->num_cus * 16);
+      hipLaunchKernelGGL(k_dim_fill, dim3(gs), dim3(256), 0, ctx->stream, dm->table, dm->slots);
+      if (cnt)
+        hipLaunchKernelGGL(k_dim_build, bgrid, dim3(256), 0, ctx->stream, d_keys[i],
+                           (const int32_t *)nullptr, d_keep[i], n[i], dm->table, dm->slots - 1,
+                           0, 0u, fl, true);
+    }
+    if (hipGetLastError() != hipSuccess) return drop(fail(PJ_E_HIP, "dimension build launch failed"));
+  }
+  std::vector<unsigned int> f(m);
+  if ((rc = d2h(ctx, f.data(), flags.p, (uint64_t)m * 4)) != PJ_OK) return drop(rc);
+  for (int i = 0; i < m; ++i)
+    if ((rc = dim_flags_error(f[i])) != PJ_OK) return drop(rc);
+  for (int i = 0; i < m; ++i) out[i] = dms[i];
+  return PJ_OK;
 }
 
 extern "C" int pj_dim_destroy(pj_ctx *ctx, pj_dim *dim) {
   if (!dim) return PJ_OK;
   std::lock_guard<std::mutex> g(ctx->mu);
   PJ_TRY(sync(ctx));
-  if (dim->table) dfree(ctx, dim->table);
-  if (dim->ids) dfree(ctx, dim->ids);
-  if (dim->vals) dfree(ctx, dim->vals);
-  delete dim;
+  dim_free(ctx, dim);
   return PJ_OK;
 }
 

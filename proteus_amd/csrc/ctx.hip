@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 
 #include "pj_internal.hpp"
 
@@ -120,8 +121,18 @@ int h2d(pj_ctx *ctx, void *d, const void *h, uint64_t bytes) {
   return PJ_OK;
 }
 
+// Small read-backs (counts, ranges, flags) go through a pinned bounce
+// buffer: a copy into pageable memory depends on how the runtime stages it,
+// and has been seen to stall a plan for milliseconds.
 int d2h(pj_ctx *ctx, void *h, const void *d, uint64_t bytes) {
   if (!bytes) return PJ_OK;
+  if (bytes <= kBounceBytes && ctx->pinned_rb) {
+    std::lock_guard<std::mutex> g(ctx->rb_mu);
+    PJ_HIP(hipMemcpyAsync(ctx->pinned_rb, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PJ_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(h, ctx->pinned_rb, bytes);
+    return PJ_OK;
+  }
   PJ_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
   PJ_HIP(hipStreamSynchronize(ctx->stream));
   return PJ_OK;
@@ -471,7 +482,9 @@ int pj_ctx_create(int device, uint64_t hbm_budget_bytes, pj_ctx **out) {
     return fail(PJ_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
   e = hipHostMalloc(&ctx->pinned_small, 1 << 20, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->pinned_rb, kBounceBytes, hipHostMallocDefault);
   if (e != hipSuccess) {
+    if (ctx->pinned_small) (void)hipHostFree(ctx->pinned_small);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return fail(PJ_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
@@ -500,6 +513,7 @@ int pj_ctx_destroy(pj_ctx *ctx) {
   }
   for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
   if (ctx->pinned_small) (void)hipHostFree(ctx->pinned_small);
+  if (ctx->pinned_rb) (void)hipHostFree(ctx->pinned_rb);
   if (ctx->aux) {
     (void)hipStreamSynchronize(ctx->aux);
     (void)hipEventDestroy(ctx->aux_fork);

@@ -677,7 +677,21 @@ __global__ __launch_bounds__(256) void k_reduce_i64(const int64_t *__restrict__ 
   s = wave_sum_i64(s);
   mn = wave_min_i64(mn);
   mx = wave_max_i64(mx);
+  // one atomic triple per workgroup: same-address atomics serialise in L2
+  __shared__ int64_t rs[4], rmn[4], rmx[4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    rs[w] = s;
+    rmn[w] = mn;
+    rmx[w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int x = 1; x < 4; ++x) {
+      s += rs[x];
+      mn = rmn[x] < mn ? rmn[x] : mn;
+      mx = rmx[x] > mx ? rmx[x] : mx;
+    }
     atomicAdd(&acc[0], (unsigned long long)s);
     atomicMin((long long *)&acc[1], (long long)mn);
     atomicMax((long long *)&acc[2], (long long)mx);

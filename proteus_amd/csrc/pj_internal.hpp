@@ -146,6 +146,8 @@ struct TuneKey {
   }
 };
 
+constexpr uint64_t kBounceBytes = 1 << 20;  // d2h read-backs up to this size go through pinned memory
+
 }  // namespace pj
 
 struct pj_ctx {
@@ -163,6 +165,9 @@ struct pj_ctx {
   std::map<std::string, std::pair<double, uint64_t>> prof_acc;
   // pinned staging for small results
   void *pinned_small = nullptr;
+  // pinned bounce buffer of d2h (pj::kBounceBytes), and its lock
+  void *pinned_rb = nullptr;
+  std::mutex rb_mu;
   // autotuner
   bool autotune = false;
   std::map<pj::TuneKey, pj_radix_cfg> tuned;

@@ -98,7 +98,7 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
 // rows.
 struct PipeTable {
   const unsigned long long *table;  // 64-bit slots key << 32 | build row
-  const uint32_t *ids;  // or (direct tables of dim_create_rows): the build row of
+  const uint32_t *ids;  // or (direct tables of dims_create_rows): the build row of
                         // key lo + i at i, 0xffffffff where no key
   uint64_t mask;   // hash tables: slots - 1
   int32_t lo;      // direct tables: the smallest key
@@ -115,11 +115,17 @@ struct PipeProg {
   int32_t depth = 0;
 };
 int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint64_t *n_out);
-// The table of a dimension built with pj_dim_create or dim_create_rows.
+// The table of a dimension built with dims_create_rows.
 int dim_table(const pj_dim *dim, PipeTable *t);
-// pj_dim_create with the build row ids 0 .. n-1 as values; a direct table
-// then holds 4-byte ids (half the bytes a probe's cache lines cover).
-int dim_create_rows(pj_ctx *ctx, const int32_t *d_keys, uint64_t n, pj_dim **out);
+// Tables of build rows for m build sides at once: side i's keys d_keys[i]
+// (n[i] rows) with the build row ids 0 .. n[i]-1 as values; rows whose
+// d_keep[i] byte is 0 (d_keep[i] may be null) are left out, a semijoin
+// filter folded into the build (their keys are neither checked nor found).
+// Direct tables hold 4-byte ids (half the bytes a probe's cache lines cover).
+// Two synchronisations in all: one for every side's key range, one for
+// every side's error flags.  On error no table is returned.
+int dims_create_rows(pj_ctx *ctx, int m, const int32_t *const *d_keys,
+                     const uint8_t *const *d_keep, const uint64_t *n, pj_dim **out);
 
 // Stable sort permutation of n rows by nk keys (the first most
 // significant): dir 1 ascending, -1 descending, 0 ignored; rank (int32 keys

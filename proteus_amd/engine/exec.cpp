@@ -1701,10 +1701,32 @@ class Exec {
     // Every build side runs before the source, as the reference runs build
     // pipelines before the probe pipeline: a bloom filter built on a build
     // side can be probed below.
+    // A build side select(p, x) is not compacted: x's rows are kept with p as
+    // a byte mask, and the dimension table leaves out the rows p rejects
+    // (the probe finds no row for their keys, as after the select).  This
+    // saves the select's compaction, its count read-back and the gathers of
+    // x's columns; the pipeline reads them at x's row ids.
     std::map<const Json *, Rel> built;
+    std::map<const Json *, Col> keep;
     for (auto it = stages.rbegin(); it != stages.rend(); ++it) {
       const std::string k = (**it)["operator"].as_string();
-      if (is_join_op(k)) built.emplace(*it, exec(*join_sides(**it).build_in, na));
+      if (!is_join_op(k)) continue;
+      const Json *bi = join_sides(**it).build_in;
+      while ((*bi)["operator"].is_string() && is_pass_through((*bi)["operator"].as_string()))
+        bi = &(*bi)["input"];
+      if ((*bi)["operator"].is_string() && (*bi)["operator"].as_string() == "select") {
+        Rel x = exec((*bi)["input"], need_below(*bi, na));
+        keep.emplace(*it, eval_as((*bi)["p"], x, R::BOOL));
+        if (!na.all) {  // the columns the select would have passed on
+          std::vector<std::pair<Attr, Col>> cols;
+          for (auto &c : x.cols)
+            if (na.has(c.first.name)) cols.push_back(c);
+          x.cols = std::move(cols);
+        }
+        built.emplace(*it, std::move(x));
+      } else {
+        built.emplace(*it, exec(*join_sides(**it).build_in, na));
+      }
     }
     Rel src = exec(*node, na);
     std::vector<Rel> builds;
@@ -1723,6 +1745,9 @@ class Exec {
     PipeCompiler pc{*this, scopes, R::PipeProg(), 0, {}};
     int cur = 0;
     bool ok = true;
+    std::vector<Col> bkeys;  // per join: build keys, keep mask, build rows
+    std::vector<const uint8_t *> bkeep;
+    std::vector<uint64_t> brows;
     for (auto it = stages.rbegin(); it != stages.rend() && ok; ++it) {
       const Json &st = **it;
       const std::string k = st["operator"].as_string();
@@ -1770,19 +1795,13 @@ class Exec {
           ok = false;
           break;
         }
-        Col bk = eval_as(*js.build_k, b, R::I32);
-        pj_dim *dim = nullptr;  // the values are the build rows
-        int rc = R::dim_create_rows(ctx_, (const int32_t *)bk.dev, b.n, &dim);
-        if (rc == PJ_E_ARG) {  // duplicate or reserved build keys
-          ok = false;
-          break;
-        }
-        ck(rc);
-        dims.push_back(dim);
-        R::PipeTable t;
-        ck(R::dim_table(dim, &t));
+        // the tables are built together after the loop
+        bkeys.push_back(eval_as(*js.build_k, b, R::I32));
+        auto km = keep.find(*it);
+        bkeep.push_back(km == keep.end() ? nullptr : (const uint8_t *)km->second.dev);
+        brows.push_back(b.n);
         const int tj = (int)pc.p.tables.size();
-        pc.p.tables.push_back(t);
+        pc.p.tables.push_back(R::PipeTable{});
         pc.emit(*js.probe_k, cur, R::I32);
         pc.push(R::OP_PROBE, R::I32, tj, 0);
         // the join's output = probe attributes + build attributes
@@ -1796,6 +1815,20 @@ class Exec {
         scopes.push_back(ns);
         cur = (int)scopes.size() - 1;
         builds.push_back(std::move(b));
+      }
+    }
+    if (ok) {  // the dimension tables (values = build rows), built together
+      const int m = (int)bkeys.size();
+      std::vector<const int32_t *> kp(m);
+      for (int i = 0; i < m; ++i) kp[i] = (const int32_t *)bkeys[i].dev;
+      dims.assign(m, nullptr);
+      int rc = R::dims_create_rows(ctx_, m, kp.data(), bkeep.data(), brows.data(), dims.data());
+      if (rc == PJ_E_ARG) {  // duplicate or reserved build keys
+        dims.clear();
+        ok = false;
+      } else {
+        ck(rc);
+        for (int i = 0; i < m; ++i) ck(R::dim_table(dims[i], &pc.p.tables[i]));
       }
     }
     if (!ok) {
