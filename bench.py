@@ -347,13 +347,17 @@ def out_of_gpu(ctx, steps):
         d.free()
     budget = 8 << 30
     got = ctx.join_out_of_gpu_reduce(*host, hbm_budget=budget)
+    m0 = ctx.mem_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.join_out_of_gpu_reduce(*host, hbm_budget=budget)
     sec = (time.perf_counter() - t0) / steps
+    m1 = ctx.mem_stats()
     return {"build_rows": nr, "probe_rows": ns, "hbm_budget_bytes": budget, "ms": sec * 1e3,
             "input_gtuples_per_s": (nr + ns) / sec / 1e9,
-            "pcie_gbs": 24.0 * (nr + ns) / sec / 1e9, "identical": got == ref}
+            "pcie_gbs": 24.0 * (nr + ns) / sec / 1e9, "identical": got == ref,
+            "pool": {"mallocs": m1["mallocs"] - m0["mallocs"],
+                     "evictions": m1["evictions"] - m0["evictions"], "cached": m1["cached"]}}
 
 
 def staging(ctx, steps):
