@@ -964,22 +964,26 @@ __global__ __launch_bounds__(256) void k_dim_range(const int32_t *__restrict__ k
 }
 
 // Folds the partials of side i (workgroups first[i] .. first[i + 1] - 1)
-// into out[3 i .. 3 i + 2], one thread per side: the read-back stays a few
+// into out[3 i .. 3 i + 2], one wave per side: the read-back stays a few
 // bytes per side.
-__global__ void k_dim_range_fold(const long long *__restrict__ part,
-                                 const uint32_t *__restrict__ first, int m,
-                                 long long *__restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  long long mn = INT64_MAX, mx = INT64_MIN, c = 0;
-  for (uint32_t b = first[i]; b < first[i + 1]; ++b) {
+__global__ __launch_bounds__(64) void k_dim_range_fold(const long long *__restrict__ part,
+                                                       const uint32_t *__restrict__ first,
+                                                       long long *__restrict__ out) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  int64_t mn = INT64_MAX, mx = INT64_MIN, c = 0;
+  for (uint32_t b = first[i] + lane; b < first[i + 1]; b += 64) {
     mn = part[3 * b] < mn ? part[3 * b] : mn;
     mx = part[3 * b + 1] > mx ? part[3 * b + 1] : mx;
     c += part[3 * b + 2];
   }
-  out[3 * i] = mn;
-  out[3 * i + 1] = mx;
-  out[3 * i + 2] = c;
+  mn = wave_min_i64(mn);
+  mx = wave_max_i64(mx);
+  c = wave_sum_i64(c);
+  if (lane == 0) {
+    out[3 * i] = mn;
+    out[3 * i + 1] = mx;
+    out[3 * i + 2] = c;
+  }
 }
 }  // namespace pj
 
@@ -1012,8 +1016,8 @@ int pj::rel::dims_create_rows(pj_ctx *ctx, int m, const int32_t *const *d_keys,
       if (n[i])
         PJ_LAUNCH(ctx, "dim_range", k_dim_range, first[i + 1] - first[i], 256, 0, d_keys[i],
                   d_keep[i], n[i], dpart.as<long long>() + 3 * (uint64_t)first[i]);
-    PJ_LAUNCH(ctx, "dim_range_fold", k_dim_range_fold, (uint32_t)ceil_div(m, 64), 64, 0,
-              dpart.as<const long long>(), dfirst.as<const uint32_t>(), m, dout.as<long long>());
+    PJ_LAUNCH(ctx, "dim_range_fold", k_dim_range_fold, (uint32_t)m, 64, 0,
+              dpart.as<const long long>(), dfirst.as<const uint32_t>(), dout.as<long long>());
     PJ_TRY(d2h(ctx, range.data(), dout.p, (uint64_t)m * 3 * 8));
   }
   // 2. every table, then one read-back of every side's flags

@@ -781,6 +781,9 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 // per lane (k_chain_q).
 constexpr int kChainRowsA = 4;
 constexpr int kChainRowsB = 1;
+// Per wave, the queue of stage-0 survivors: one tile's survivors (64 RA) on
+// top of fewer than 64 RB left over from the tiles before.
+constexpr int kChainQueue = 64 * kChainRowsA + 64 * kChainRowsB;
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
@@ -1063,16 +1066,37 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
   const int ntab = a.ntab;
   uint32_t *ridl = lds;                                   // ntab * RA * 256
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * 64 * RA);  // per wave: rows, then ids
-  uint32_t *qrid = qrow + 64 * RA;
+  constexpr uint32_t QC = 64 * RA + 64 * RB;  // queue capacity (kChainQueue)
+  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * QC);  // per wave: rows, then ids
+  uint32_t *qrid = qrow + QC;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   const ChainStage s0 = a.st[0];
   const int t0 = s0.kind == 0 ? s0.tab : -1;  // the table stage 0 probes, if any
+  // phase B: the other stages and the outputs over queue entries [e0, e1)
+  auto phase_b = [&](uint32_t e0, uint32_t e1) {
+    ChainRows<RB> cb;
+    cb.ridl = ridl;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const uint32_t e = e0 + k * 64u + lane;
+      cb.alive[k] = e < e1;
+      const uint32_t ec = cb.alive[k] ? e : e0;
+      cb.row[k] = qrow[ec];
+      if (t0 >= 0) ridl[(t0 * RB + k) * 256 + threadIdx.x] = cb.alive[k] ? qrid[ec] : 0u;
+    }
+    bool live = true;
+    int32_t pre[RB];
+    chain_prefetch<1, K, RB>(a, cb, pre);
+    chain_stages<1, K, RB>(a, cb, live, pre);
+    if (live) chain_emit<RB>(a, cb, &s_cnt, lane);
+    __builtin_amdgcn_wave_barrier();
+  };
+  uint32_t qn = 0;  // rows in the wave's queue
   const uint64_t stride = (uint64_t)gridDim.x * 256u * RA;  // 64-bit: no wrap near n = 2^32
   for (uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA; base < a.n;
        base += stride) {
-    // ---- phase A: stage 0 on RA rows per lane ----------------------------
+    // ---- phase A: stage 0 on RA rows per lane --------------------------------
     ChainRows<RA> ca;
     ca.ridl = ridl;
 #pragma unroll
@@ -1087,41 +1111,34 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
       chain_prefetch<0, 1, RA>(a, ca, pre);
       chain_stages<0, 1, RA>(a, ca, live, pre);
     }
-    // ---- compaction of the survivors into the wave's queue ------------------
+    // ---- compaction of the survivors onto the wave's queue ------------------
     const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
-    uint32_t q = 0;
 #pragma unroll
     for (int k = 0; k < RA; ++k) {
       const unsigned long long m = __ballot(ca.alive[k]);
       if (ca.alive[k]) {
-        const uint32_t at = q + __popcll(m & lt);
+        const uint32_t at = qn + __popcll(m & lt);
         qrow[at] = ca.row[k];
         if (t0 >= 0) qrid[at] = ridl[(t0 * RA + k) * 256 + threadIdx.x];
       }
-      q += __popcll(m);
+      qn += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- phase B: the other stages and the outputs over the queue ------------
-    for (uint32_t q0 = 0; q0 < q; q0 += 64u * RB) {
-      ChainRows<RB> cb;
-      cb.ridl = ridl;
-#pragma unroll
-      for (int k = 0; k < RB; ++k) {
-        const uint32_t e = q0 + k * 64u + lane;
-        cb.alive[k] = e < q;
-        const uint32_t ec = cb.alive[k] ? e : 0u;
-        cb.row[k] = qrow[ec];
-        if (t0 >= 0) ridl[(t0 * RB + k) * 256 + threadIdx.x] = cb.alive[k] ? qrid[ec] : 0u;
+    // ---- phase B on whole groups of 64 RB queued rows; the rest waits -------
+    uint32_t q0 = 0;
+    for (; q0 + 64u * RB <= qn; q0 += 64u * RB) phase_b(q0, q0 + 64u * RB);
+    // the rest (< 64 RB <= q0 entries, so source and target do not overlap)
+    // moves to the front
+    const uint32_t rem = qn - q0;
+    if (q0)
+      for (uint32_t j = lane; j < rem; j += 64) {
+        qrow[j] = qrow[q0 + j];
+        qrid[j] = qrid[q0 + j];
       }
-      bool live = true;
-      int32_t pre[RB];
-      chain_prefetch<1, K, RB>(a, cb, pre);
-      chain_stages<1, K, RB>(a, cb, live, pre);
-      if (live) chain_emit<RB>(a, cb, &s_cnt, lane);
-      __builtin_amdgcn_wave_barrier();
-    }
+    qn = rem;
     __builtin_amdgcn_wave_barrier();
   }
+  if (qn) phase_b(0, qn);  // the wave's last survivors
   __syncthreads();
   if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
 }
@@ -1499,7 +1516,7 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       // ridl slots of every table for phase A's rows, and per wave a queue
       // of (row, stage-0 build row) pairs
       constexpr int RA = kChainRowsA, RB = kChainRowsB;
-      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * 2 * 64 * RA * 4;
+      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * 2 * kChainQueue * 4;
       switch (ca.nst) {
         case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<1, RA, RB>), grid, 256, lds, da); break;
         case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<2, RA, RB>), grid, 256, lds, da); break;
