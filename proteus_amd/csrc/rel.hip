@@ -782,8 +782,11 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 constexpr int kChainRowsA = 4;
 constexpr int kChainRowsB = 1;
 // Per wave, the queue of stage-0 survivors: one tile's survivors (64 RA) on
-// top of fewer than 64 RB left over from the tiles before.
+// top of fewer than 64 RB left over from the tiles before; and (plans of
+// three or more stages) the queue of stage-1 survivors: 64 on top of fewer
+// than 64 left over.
 constexpr int kChainQueue = 64 * kChainRowsA + 64 * kChainRowsB;
+constexpr int kChainQueue2 = 128;
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
@@ -1046,53 +1049,102 @@ __device__ __forceinline__ void chain_emit(const ChainArgs &a, const ChainRows<R
   }
 }
 
-// The probe chain with the survivors of stage 0 compacted.  Each wave takes
-// tiles of 64 * RA consecutive rows: stage 0 runs on RA rows per lane, then
-// the rows still alive (with the build row stage 0 found, when it probes a
-// table) go to a wave-private LDS queue, and the remaining stages and the
-// outputs run over the queue with RB rows per lane.  A selective first
-// stage (the star plan's date join keeps 1/7 of the rows) would otherwise
-// leave every later stage computing RA rows per lane, most of them dead:
-// the kernel is bound by instruction issue, not by its bytes.  The waves of
-// a workgroup share only the output counter; a wave's LDS queue and the
-// ridl slots of its own lanes are written and read by that wave alone, in
-// program order.
+// The probe chain with the survivors of stages 0 and 1 compacted.  Each wave
+// takes tiles of 64 * RA consecutive rows: stage 0 runs on RA rows per lane
+// (phase A), and the rows still alive (with the build row stage 0 found,
+// when it probes a table) go onto a wave-private LDS queue.  Whenever 64 rows
+// wait there, stage 1 runs on them, one per lane (phase B), and its
+// survivors (with the build rows of stages 0 and 1) go onto a second queue;
+// whenever 64 wait there, the remaining stages and the outputs run on them
+// (phase C).  Rows left in the queues at the end are flushed with fewer
+// lanes.  Selective stages (the star plan's date join keeps 1/7 of the rows,
+// the supplier join 1/5 of those) would otherwise run every later stage, a
+// chain of dependent memory round trips, with most lanes dead.  Plans of
+// one or two stages run stage 1 and the outputs in phase B.  The waves of a
+// workgroup share only the output counter; a wave's queues and the ridl
+// slots of its own lanes are written and read by that wave alone, in program
+// order (phase B and C slots alias phase A's, whose stage-0 rows are copied
+// onto the queue first).
 template <int K, int RA, int RB>
 __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ ap) {
+  static_assert(RB == 1, "phases B and C run one row per lane");
   static_assert(RB <= RA, "phase B reuses phase A's ridl slots");
+  constexpr bool Q2 = K > 2;  // stages 2.. run from the second queue
+  constexpr uint32_t QC = 64 * RA + 64 * RB, QC2 = kChainQueue2;
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_cnt;
   const ChainArgs &a = *ap;
   const int ntab = a.ntab;
   uint32_t *ridl = lds;                                   // ntab * RA * 256
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  constexpr uint32_t QC = 64 * RA + 64 * RB;  // queue capacity (kChainQueue)
-  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * QC);  // per wave: rows, then ids
+  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * QC + 3 * QC2);  // per wave
   uint32_t *qrid = qrow + QC;
+  uint32_t *q2row = qrid + QC, *q2r0 = q2row + QC2, *q2r1 = q2r0 + QC2;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  const ChainStage s0 = a.st[0];
-  const int t0 = s0.kind == 0 ? s0.tab : -1;  // the table stage 0 probes, if any
-  // phase B: the other stages and the outputs over queue entries [e0, e1)
-  auto phase_b = [&](uint32_t e0, uint32_t e1) {
-    ChainRows<RB> cb;
-    cb.ridl = ridl;
-#pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const uint32_t e = e0 + k * 64u + lane;
-      cb.alive[k] = e < e1;
-      const uint32_t ec = cb.alive[k] ? e : e0;
-      cb.row[k] = qrow[ec];
-      if (t0 >= 0) ridl[(t0 * RB + k) * 256 + threadIdx.x] = cb.alive[k] ? qrid[ec] : 0u;
-    }
+  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+  const int t0 = a.st[0].kind == 0 ? a.st[0].tab : -1;  // the tables stages 0 and 1 probe, if any
+  const int t1 = (K > 1 && a.st[1].kind == 0) ? a.st[1].tab : -1;
+  // phase C: stages 2.. and the outputs over second-queue entries [e0, e1)
+  auto phase_c = [&](uint32_t e0, uint32_t e1) {
+    ChainRows<1> cc;
+    cc.ridl = ridl;
+    const uint32_t e = e0 + lane;
+    cc.alive[0] = e < e1;
+    const uint32_t ec = cc.alive[0] ? e : e0;
+    cc.row[0] = q2row[ec];
+    if (t0 >= 0) ridl[t0 * 256 + threadIdx.x] = cc.alive[0] ? q2r0[ec] : 0u;
+    if (t1 >= 0) ridl[t1 * 256 + threadIdx.x] = cc.alive[0] ? q2r1[ec] : 0u;
     bool live = true;
-    int32_t pre[RB];
-    chain_prefetch<1, K, RB>(a, cb, pre);
-    chain_stages<1, K, RB>(a, cb, live, pre);
-    if (live) chain_emit<RB>(a, cb, &s_cnt, lane);
+    int32_t pre[1];
+    chain_prefetch<2, K, 1>(a, cc, pre);
+    chain_stages<2, K, 1>(a, cc, live, pre);
+    if (live) chain_emit<1>(a, cc, &s_cnt, lane);
     __builtin_amdgcn_wave_barrier();
   };
-  uint32_t qn = 0;  // rows in the wave's queue
+  uint32_t q2n = 0;  // rows in the second queue
+  // phase B: stage 1 over first-queue entries [e0, e1); then, in plans of
+  // two stages, the outputs, else its survivors onto the second queue
+  auto phase_b = [&](uint32_t e0, uint32_t e1) {
+    ChainRows<1> cb;
+    cb.ridl = ridl;
+    const uint32_t e = e0 + lane;
+    cb.alive[0] = e < e1;
+    const uint32_t ec = cb.alive[0] ? e : e0;
+    cb.row[0] = qrow[ec];
+    if (t0 >= 0) ridl[t0 * 256 + threadIdx.x] = cb.alive[0] ? qrid[ec] : 0u;
+    bool live = true;
+    int32_t pre[1];
+    if constexpr (Q2) {
+      chain_prefetch<1, 2, 1>(a, cb, pre);
+      chain_stages<1, 2, 1>(a, cb, live, pre);
+      const unsigned long long m = __ballot(cb.alive[0]);
+      if (cb.alive[0]) {
+        const uint32_t at = q2n + __popcll(m & lt);
+        q2row[at] = cb.row[0];
+        if (t0 >= 0) q2r0[at] = ridl[t0 * 256 + threadIdx.x];
+        if (t1 >= 0) q2r1[at] = ridl[t1 * 256 + threadIdx.x];
+      }
+      q2n += __popcll(m);
+      __builtin_amdgcn_wave_barrier();
+      if (q2n >= 64) {
+        phase_c(0, 64);
+        // the rest (< 64 entries) moves to the front: no overlap
+        for (uint32_t j = lane; j < q2n - 64; j += 64) {
+          q2row[j] = q2row[64 + j];
+          q2r0[j] = q2r0[64 + j];
+          q2r1[j] = q2r1[64 + j];
+        }
+        q2n -= 64;
+      }
+    } else {
+      chain_prefetch<1, K, 1>(a, cb, pre);
+      chain_stages<1, K, 1>(a, cb, live, pre);
+      if (live) chain_emit<1>(a, cb, &s_cnt, lane);
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+  uint32_t qn = 0;  // rows in the first queue
   const uint64_t stride = (uint64_t)gridDim.x * 256u * RA;  // 64-bit: no wrap near n = 2^32
   for (uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA; base < a.n;
        base += stride) {
@@ -1111,8 +1163,7 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
       chain_prefetch<0, 1, RA>(a, ca, pre);
       chain_stages<0, 1, RA>(a, ca, live, pre);
     }
-    // ---- compaction of the survivors onto the wave's queue ------------------
-    const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+    // ---- compaction of the survivors onto the first queue -------------------
 #pragma unroll
     for (int k = 0; k < RA; ++k) {
       const unsigned long long m = __ballot(ca.alive[k]);
@@ -1124,10 +1175,10 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
       qn += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- phase B on whole groups of 64 RB queued rows; the rest waits -------
+    // ---- phase B on whole groups of 64 queued rows; the rest waits ----------
     uint32_t q0 = 0;
-    for (; q0 + 64u * RB <= qn; q0 += 64u * RB) phase_b(q0, q0 + 64u * RB);
-    // the rest (< 64 RB <= q0 entries, so source and target do not overlap)
+    for (; q0 + 64u <= qn; q0 += 64u) phase_b(q0, q0 + 64u);
+    // the rest (< 64 <= q0 entries, so source and target do not overlap)
     // moves to the front
     const uint32_t rem = qn - q0;
     if (q0)
@@ -1138,7 +1189,10 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
     qn = rem;
     __builtin_amdgcn_wave_barrier();
   }
-  if (qn) phase_b(0, qn);  // the wave's last survivors
+  if (qn) phase_b(0, qn);  // the wave's last stage-0 survivors
+  if constexpr (Q2) {
+    if (q2n) phase_c(0, q2n);  // and its last stage-1 survivors
+  }
   __syncthreads();
   if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
 }
@@ -1516,7 +1570,7 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       // ridl slots of every table for phase A's rows, and per wave a queue
       // of (row, stage-0 build row) pairs
       constexpr int RA = kChainRowsA, RB = kChainRowsB;
-      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * 2 * kChainQueue * 4;
+      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * (2 * kChainQueue + 3 * kChainQueue2) * 4;
       switch (ca.nst) {
         case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<1, RA, RB>), grid, 256, lds, da); break;
         case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<2, RA, RB>), grid, 256, lds, da); break;

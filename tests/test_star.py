@@ -114,13 +114,19 @@ def _chain_tables(ctx, n, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["all_survive_first", "filter_first", "single_join"])
+@pytest.mark.parametrize("case", ["all_survive_first", "filter_first", "single_join",
+                                  "join_filter_join", "join_join_filter"])
 def test_probe_chain_stage0_compaction(ctx, case):
     """The fused probe chain (the rel_pipeline_chain kernel) when stage 0
     keeps every row (each wave's queue holds its whole tile and the later
     stages run in several rounds), when stage 0 is a filter on a fact column
-    (no build row to carry), and with a single join; 300,001 fact rows (a
-    partial last tile).  Grouped results equal numpy's."""
+    (no build row to carry), with a single join, and with three stages whose
+    stage 0 is a join: a filter at stage 1 (no build row for the second
+    queue), or a join at stage 1 and a filter last, grouped by a column of
+    the stage-0 table and summing one of the stage-1 table (both build rows
+    come back from the second queue).  300,001 fact rows (a partial last
+    tile, and rows left in both queues at the end).  Grouped results equal
+    numpy's."""
     ctx.clear_catalog()
     n = 300_001
     c, devs = _chain_tables(ctx, n, 21)
@@ -147,11 +153,25 @@ def test_probe_chain_stage0_compaction(ctx, case):
         keep = (fv > 50) & (bw[fb] >= 0)
         g, x = aw[fa][keep], (fv.astype(np.int64) + bw[fb])[keep]
         gkey, xval = col("A", "w"), rb.add(col("F", "v"), col("B", "w"))
-    else:
+    elif case == "single_join":
         j = F.join(B, col("B", "k"), col("F", "b"), 13, 2000)
         keep = bw[fb] >= 0
         g, x = bw[fb][keep], fv[keep].astype(np.int64)
         gkey, xval = col("B", "w"), col("F", "v")
+    elif case == "join_filter_join":
+        j = F.join(B, col("B", "k"), col("F", "b"), 13, 2000) \
+             .filter(rb.gt(col("F", "v"), rb.lit(-20))) \
+             .join(A, col("A", "k"), col("F", "a"), 11, 1000)
+        keep = (bw[fb] >= 0) & (fv > -20)
+        g, x = bw[fb][keep], (fv.astype(np.int64) + aw[fa])[keep]
+        gkey, xval = col("B", "w"), rb.add(col("F", "v"), col("A", "w"))
+    else:
+        j = F.join(B, col("B", "k"), col("F", "b"), 13, 2000) \
+             .join(A, col("A", "k"), col("F", "a"), 11, 1000) \
+             .filter(rb.lt(col("F", "v"), rb.lit(60)))
+        keep = (bw[fb] >= 0) & (fv < 60)
+        g, x = bw[fb][keep], aw[fa][keep]
+        gkey, xval = col("B", "w"), col("A", "w")
     q = j.groupby([gkey], [(rb.as_(rb.cast(xval, "int64"), "o", "s"), "sum"),
                            (rb.as_(rb.int64(1), "o", "c"), "sum")], 6, 64) \
          .sort([gkey, col("o", "s"), col("o", "c")], ["ASC", "NONE", "NONE"])
