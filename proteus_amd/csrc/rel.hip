@@ -781,12 +781,24 @@ __global__ __launch_bounds__(kPipeWave) void k_pipe(PipeArgs a) {
 // per lane (k_chain_q).
 constexpr int kChainRowsA = 4;
 constexpr int kChainRowsB = 1;
-// Per wave, the queue of stage-0 survivors: one tile's survivors (64 RA) on
-// top of fewer than 64 RB left over from the tiles before; and (plans of
-// three or more stages) the queue of stage-1 survivors: 64 on top of fewer
-// than 64 left over.
+// Per wave, queue L (1 <= L <= D) holds rows that passed stages 0 .. L-1,
+// with those stages' build rows.  Queue 1 takes one tile's stage-0
+// survivors (64 RA) on top of fewer than 64 left over from the tiles
+// before; queue L >= 2 takes 64 on top of fewer than 64.  D = the number of
+// queues: stages 1 .. D-1 each feed the next queue, and level D runs the
+// remaining stages and the outputs.
 constexpr int kChainQueue = 64 * kChainRowsA + 64 * kChainRowsB;
-constexpr int kChainQueue2 = 128;
+constexpr int kChainQueueN = 128;
+constexpr int kChainMaxQueues = 3;
+__host__ __device__ constexpr int chain_queues(int nst) {
+  return nst - 1 < 1 ? 1 : (nst - 1 > kChainMaxQueues ? kChainMaxQueues : nst - 1);
+}
+// LDS words of one wave's queues: row + L build rows per entry of queue L
+__host__ __device__ constexpr int chain_queue_words(int nst) {
+  int w = 2 * kChainQueue;
+  for (int L = 2; L <= chain_queues(nst); ++L) w += (1 + L) * kChainQueueN;
+  return w;
+}
 constexpr int kChainMaxStages = 8;
 constexpr int kChainMaxCols = 32;
 struct ChainStage {
@@ -1049,108 +1061,129 @@ __device__ __forceinline__ void chain_emit(const ChainArgs &a, const ChainRows<R
   }
 }
 
-// The probe chain with the survivors of stages 0 and 1 compacted.  Each wave
-// takes tiles of 64 * RA consecutive rows: stage 0 runs on RA rows per lane
-// (phase A), and the rows still alive (with the build row stage 0 found,
-// when it probes a table) go onto a wave-private LDS queue.  Whenever 64 rows
-// wait there, stage 1 runs on them, one per lane (phase B), and its
-// survivors (with the build rows of stages 0 and 1) go onto a second queue;
-// whenever 64 wait there, the remaining stages and the outputs run on them
-// (phase C).  Rows left in the queues at the end are flushed with fewer
-// lanes.  Selective stages (the star plan's date join keeps 1/7 of the rows,
-// the supplier join 1/5 of those) would otherwise run every later stage, a
-// chain of dependent memory round trips, with most lanes dead.  Plans of
-// one or two stages run stage 1 and the outputs in phase B.  The waves of a
-// workgroup share only the output counter; a wave's queues and the ridl
-// slots of its own lanes are written and read by that wave alone, in program
-// order (phase B and C slots alias phase A's, whose stage-0 rows are copied
-// onto the queue first).
+// One wave's queues and the state its levels share.
+struct ChainWave {
+  const ChainArgs *a;
+  uint32_t *ridl;
+  uint32_t *qrow[kChainMaxQueues + 1];  // index L = queue L
+  uint32_t *qrid[kChainMaxQueues + 1];  // queue L: build row of stage s at [s * cap + e]
+  uint32_t cap[kChainMaxQueues + 1];
+  uint32_t n[kChainMaxQueues + 1];      // entries waiting (wave-uniform)
+  int tab[kChainMaxQueues];             // table stage s probes, or -1
+  uint32_t *s_cnt;
+  int lane;
+  unsigned long long lt;
+};
+
+// Level L over queue-L entries [e0, e1), one row per lane: the build rows of
+// stages 0 .. L-1 come back into the lanes' ridl slots, then stage L runs
+// and its survivors go onto queue L + 1 (which is worked off whenever 64
+// rows wait there), or, at the last level D, the remaining stages and the
+// outputs run.
+template <int L, int D, int K>
+__device__ __forceinline__ void chain_level(ChainWave &w, uint32_t e0, uint32_t e1) {
+  const ChainArgs &a = *w.a;
+  ChainRows<1> cr;
+  cr.ridl = w.ridl;
+  const uint32_t e = e0 + w.lane;
+  cr.alive[0] = e < e1;
+  const uint32_t ec = cr.alive[0] ? e : e0;
+  cr.row[0] = w.qrow[L][ec];
+#pragma unroll
+  for (int s = 0; s < L; ++s)
+    if (w.tab[s] >= 0)
+      w.ridl[w.tab[s] * 256 + threadIdx.x] = cr.alive[0] ? w.qrid[L][s * w.cap[L] + ec] : 0u;
+  bool live = true;
+  int32_t pre[1];
+  if constexpr (L < D) {
+    constexpr int M = L + 1;
+    chain_prefetch<L, M, 1>(a, cr, pre);
+    chain_stages<L, M, 1>(a, cr, live, pre);
+    const unsigned long long m = __ballot(cr.alive[0]);
+    if (cr.alive[0]) {
+      const uint32_t at = w.n[M] + __popcll(m & w.lt);
+      w.qrow[M][at] = cr.row[0];
+#pragma unroll
+      for (int s = 0; s < M; ++s)
+        if (w.tab[s] >= 0) w.qrid[M][s * w.cap[M] + at] = w.ridl[w.tab[s] * 256 + threadIdx.x];
+    }
+    w.n[M] += __popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    if (w.n[M] >= 64) {
+      chain_level<M, D, K>(w, 0, 64);
+      // the rest (< 64 entries) moves to the front: no overlap
+      for (uint32_t j = w.lane; j < w.n[M] - 64; j += 64) {
+        w.qrow[M][j] = w.qrow[M][64 + j];
+#pragma unroll
+        for (int s = 0; s < M; ++s) w.qrid[M][s * w.cap[M] + j] = w.qrid[M][s * w.cap[M] + 64 + j];
+      }
+      w.n[M] -= 64;
+    }
+  } else {
+    chain_prefetch<L, K, 1>(a, cr, pre);
+    chain_stages<L, K, 1>(a, cr, live, pre);
+    if (live) chain_emit<1>(a, cr, w.s_cnt, w.lane);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// The probe chain with the survivors of the first stages compacted.  Each
+// wave takes tiles of 64 * RA consecutive rows: stage 0 runs on RA rows per
+// lane, and the rows still alive (with the build row stage 0 found, when it
+// probes a table) go onto the wave's first LDS queue.  Whenever 64 rows wait
+// in a queue, the next stage runs on them, one per lane, and its survivors
+// go onto the next queue; the last queue's level runs the remaining stages
+// and the outputs.  Rows left in the queues at the end are flushed, level by
+// level, with fewer lanes.  Selective stages (the star plan's date join
+// keeps 1/7 of the rows, the supplier join 1/5 of those, the part join 1/25
+// of those) would otherwise run every later stage, a chain of dependent
+// memory round trips, with most lanes dead.  The waves of a workgroup share
+// only the output counter; a wave's queues and the ridl slots of its own
+// lanes are written and read by that wave alone, in program order (the
+// one-row slots of the levels alias phase A's, whose stage-0 rows are copied
+// onto the first queue before a level runs).
 template <int K, int RA, int RB>
 __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ ap) {
-  static_assert(RB == 1, "phases B and C run one row per lane");
-  static_assert(RB <= RA, "phase B reuses phase A's ridl slots");
-  constexpr bool Q2 = K > 2;  // stages 2.. run from the second queue
-  constexpr uint32_t QC = 64 * RA + 64 * RB, QC2 = kChainQueue2;
+  static_assert(RB == 1, "the levels run one row per lane");
+  static_assert(RB <= RA, "the levels reuse phase A's ridl slots");
+  constexpr int D = chain_queues(K);
+  constexpr uint32_t QW = chain_queue_words(K);
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_cnt;
   const ChainArgs &a = *ap;
   const int ntab = a.ntab;
-  uint32_t *ridl = lds;                                   // ntab * RA * 256
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t *qrow = lds + ntab * RA * 256 + wid * (2 * QC + 3 * QC2);  // per wave
-  uint32_t *qrid = qrow + QC;
-  uint32_t *q2row = qrid + QC, *q2r0 = q2row + QC2, *q2r1 = q2r0 + QC2;
+  ChainWave w;
+  w.a = ap;
+  w.ridl = lds;  // ntab * RA * 256
+  w.s_cnt = &s_cnt;
+  w.lane = lane;
+  w.lt = lane ? ((1ULL << lane) - 1) : 0ULL;
+  {
+    uint32_t *q = lds + ntab * RA * 256 + wid * QW;
+    w.cap[0] = 0;
+    w.qrow[0] = w.qrid[0] = nullptr;
+#pragma unroll
+    for (int L = 1; L <= kChainMaxQueues; ++L) {
+      w.cap[L] = L == 1 ? kChainQueue : kChainQueueN;
+      w.qrow[L] = q;
+      w.qrid[L] = q + w.cap[L];
+      w.n[L] = 0;
+      if (L <= D) q += (1 + L) * w.cap[L];
+    }
+#pragma unroll
+    for (int st = 0; st < kChainMaxQueues; ++st)
+      w.tab[st] = (st < K && a.st[st].kind == 0) ? a.st[st].tab : -1;
+  }
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  const unsigned long long lt = lane ? ((1ULL << lane) - 1) : 0ULL;
-  const int t0 = a.st[0].kind == 0 ? a.st[0].tab : -1;  // the tables stages 0 and 1 probe, if any
-  const int t1 = (K > 1 && a.st[1].kind == 0) ? a.st[1].tab : -1;
-  // phase C: stages 2.. and the outputs over second-queue entries [e0, e1)
-  auto phase_c = [&](uint32_t e0, uint32_t e1) {
-    ChainRows<1> cc;
-    cc.ridl = ridl;
-    const uint32_t e = e0 + lane;
-    cc.alive[0] = e < e1;
-    const uint32_t ec = cc.alive[0] ? e : e0;
-    cc.row[0] = q2row[ec];
-    if (t0 >= 0) ridl[t0 * 256 + threadIdx.x] = cc.alive[0] ? q2r0[ec] : 0u;
-    if (t1 >= 0) ridl[t1 * 256 + threadIdx.x] = cc.alive[0] ? q2r1[ec] : 0u;
-    bool live = true;
-    int32_t pre[1];
-    chain_prefetch<2, K, 1>(a, cc, pre);
-    chain_stages<2, K, 1>(a, cc, live, pre);
-    if (live) chain_emit<1>(a, cc, &s_cnt, lane);
-    __builtin_amdgcn_wave_barrier();
-  };
-  uint32_t q2n = 0;  // rows in the second queue
-  // phase B: stage 1 over first-queue entries [e0, e1); then, in plans of
-  // two stages, the outputs, else its survivors onto the second queue
-  auto phase_b = [&](uint32_t e0, uint32_t e1) {
-    ChainRows<1> cb;
-    cb.ridl = ridl;
-    const uint32_t e = e0 + lane;
-    cb.alive[0] = e < e1;
-    const uint32_t ec = cb.alive[0] ? e : e0;
-    cb.row[0] = qrow[ec];
-    if (t0 >= 0) ridl[t0 * 256 + threadIdx.x] = cb.alive[0] ? qrid[ec] : 0u;
-    bool live = true;
-    int32_t pre[1];
-    if constexpr (Q2) {
-      chain_prefetch<1, 2, 1>(a, cb, pre);
-      chain_stages<1, 2, 1>(a, cb, live, pre);
-      const unsigned long long m = __ballot(cb.alive[0]);
-      if (cb.alive[0]) {
-        const uint32_t at = q2n + __popcll(m & lt);
-        q2row[at] = cb.row[0];
-        if (t0 >= 0) q2r0[at] = ridl[t0 * 256 + threadIdx.x];
-        if (t1 >= 0) q2r1[at] = ridl[t1 * 256 + threadIdx.x];
-      }
-      q2n += __popcll(m);
-      __builtin_amdgcn_wave_barrier();
-      if (q2n >= 64) {
-        phase_c(0, 64);
-        // the rest (< 64 entries) moves to the front: no overlap
-        for (uint32_t j = lane; j < q2n - 64; j += 64) {
-          q2row[j] = q2row[64 + j];
-          q2r0[j] = q2r0[64 + j];
-          q2r1[j] = q2r1[64 + j];
-        }
-        q2n -= 64;
-      }
-    } else {
-      chain_prefetch<1, K, 1>(a, cb, pre);
-      chain_stages<1, K, 1>(a, cb, live, pre);
-      if (live) chain_emit<1>(a, cb, &s_cnt, lane);
-    }
-    __builtin_amdgcn_wave_barrier();
-  };
-  uint32_t qn = 0;  // rows in the first queue
+  const int t0 = w.tab[0];
   const uint64_t stride = (uint64_t)gridDim.x * 256u * RA;  // 64-bit: no wrap near n = 2^32
   for (uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA; base < a.n;
        base += stride) {
     // ---- phase A: stage 0 on RA rows per lane --------------------------------
     ChainRows<RA> ca;
-    ca.ridl = ridl;
+    ca.ridl = w.ridl;
 #pragma unroll
     for (int k = 0; k < RA; ++k) {
       const uint64_t r = base + k * 64u + lane;
@@ -1164,20 +1197,22 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
       chain_stages<0, 1, RA>(a, ca, live, pre);
     }
     // ---- compaction of the survivors onto the first queue -------------------
+    uint32_t *qrow = w.qrow[1], *qrid = w.qrid[1];
+    uint32_t qn = w.n[1];
 #pragma unroll
     for (int k = 0; k < RA; ++k) {
       const unsigned long long m = __ballot(ca.alive[k]);
       if (ca.alive[k]) {
-        const uint32_t at = qn + __popcll(m & lt);
+        const uint32_t at = qn + __popcll(m & w.lt);
         qrow[at] = ca.row[k];
-        if (t0 >= 0) qrid[at] = ridl[(t0 * RA + k) * 256 + threadIdx.x];
+        if (t0 >= 0) qrid[at] = w.ridl[(t0 * RA + k) * 256 + threadIdx.x];
       }
       qn += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- phase B on whole groups of 64 queued rows; the rest waits ----------
+    // ---- level 1 on whole groups of 64 queued rows; the rest waits ----------
     uint32_t q0 = 0;
-    for (; q0 + 64u <= qn; q0 += 64u) phase_b(q0, q0 + 64u);
+    for (; q0 + 64u <= qn; q0 += 64u) chain_level<1, D, K>(w, q0, q0 + 64u);
     // the rest (< 64 <= q0 entries, so source and target do not overlap)
     // moves to the front
     const uint32_t rem = qn - q0;
@@ -1186,12 +1221,16 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
         qrow[j] = qrow[q0 + j];
         qrid[j] = qrid[q0 + j];
       }
-    qn = rem;
+    w.n[1] = rem;
     __builtin_amdgcn_wave_barrier();
   }
-  if (qn) phase_b(0, qn);  // the wave's last stage-0 survivors
-  if constexpr (Q2) {
-    if (q2n) phase_c(0, q2n);  // and its last stage-1 survivors
+  // the rows left in the queues, level by level
+  if (w.n[1]) chain_level<1, D, K>(w, 0, w.n[1]);
+  if constexpr (D >= 2) {
+    if (w.n[2]) chain_level<2, D, K>(w, 0, w.n[2]);
+  }
+  if constexpr (D >= 3) {
+    if (w.n[3]) chain_level<3, D, K>(w, 0, w.n[3]);
   }
   __syncthreads();
   if (threadIdx.x == 0) a.counts[blockIdx.x] = s_cnt;
@@ -1570,7 +1609,7 @@ int pipeline(pj_ctx *ctx, const PipeProg &p, uint64_t n, void *const *outs, uint
       // ridl slots of every table for phase A's rows, and per wave a queue
       // of (row, stage-0 build row) pairs
       constexpr int RA = kChainRowsA, RB = kChainRowsB;
-      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * (2 * kChainQueue + 3 * kChainQueue2) * 4;
+      const size_t lds = (size_t)ca.ntab * RA * 256 * 4 + 4 * (size_t)chain_queue_words(ca.nst) * 4;
       switch (ca.nst) {
         case 1: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<1, RA, RB>), grid, 256, lds, da); break;
         case 2: PJ_LAUNCH(ctx, "rel_pipeline_chain", (k_chain_q<2, RA, RB>), grid, 256, lds, da); break;
