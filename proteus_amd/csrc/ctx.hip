@@ -271,6 +271,10 @@ int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
                   pj_join_agg *out) {
   int64_t h[kJoinAccWords];
   PJ_TRY(d2h(ctx, h, d_acc, sizeof(h)));
+  return join_acc_fold(h, out);
+}
+
+int join_acc_fold(const int64_t *h, pj_join_agg *out) {
   out->count = 0;
   out->sum_b = 0;
   out->sum_p = 0;

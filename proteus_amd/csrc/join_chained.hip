@@ -340,6 +340,10 @@ extern "C" int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
                                       uint64_t max_build, pj_join_agg *out) {
   std::lock_guard<std::mutex> g(ctx->mu);
   PJ_TRY(chained_check(hash_bits, nb, max_build));
+  // dense build keys (a primary key 0..n-1): a direct table of the values
+  bool done = false;
+  PJ_TRY(join_direct(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, out, &done));
+  if (done) return PJ_OK;
   if (chained_goes_partitioned(nb, np))
     return join_partitioned_auto(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, out);
   Scratch tab, acc;

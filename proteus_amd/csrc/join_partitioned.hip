@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "radix.hpp"
+#include "rel_api.hpp"
 
 namespace pj {
 
@@ -995,11 +996,18 @@ static void derive_bits(pj_radix_cfg &c, uint64_t nb) {
   }
 }
 
-int validate_cfg(pj_radix_cfg &c, uint64_t nb) {
+int validate_cfg(pj_radix_cfg &c, uint64_t nb, uint64_t np) {
   // Large builds take the 8192-row tables (fan-out 2^7 in both passes and the
   // wide bitmap join); smaller ones the 4096-row tables.
   if (c.lds_log == 0) c.lds_log = nb > (1ULL << 26) ? 14 : 13;
-  if (c.probe_unit == 0) c.probe_unit = 65536;
+  // about 1024 join units (four per CU of a 256-CU device), 4096..65536 probe
+  // tuples each: a small probe side split into 65536-tuple units leaves most
+  // workgroups without one (2^24 probe tuples over 256 partitions made 256)
+  if (c.probe_unit == 0) {
+    uint64_t pu = 4096;
+    while (pu < 65536 && pu * 2 * 1024 <= np) pu *= 2;
+    c.probe_unit = (uint32_t)pu;
+  }
   if (c.lds_log < 11 || c.lds_log > 14) return fail(PJ_E_ARG, "lds_log %d not in [11, 14]", c.lds_log);
   if (c.bits1 == 0 && c.bits2 == 0) derive_bits(c, nb);
   if (c.bits1 < 0 || c.bits1 > 8 || c.bits2 < 0 || c.bits2 > 8 || (c.bits1 == 0 && c.bits2 > 0))
@@ -1047,12 +1055,132 @@ using namespace pj;
 
 extern "C" int pj_join_partitioned_config(pj_ctx *ctx, uint64_t nb, uint64_t np,
                                           pj_radix_cfg *cfg) {
-  (void)np;
   pj_radix_cfg c{};
   auto it = ctx->tuned.find(TuneKey{ilog2_ceil(nb ? nb : 1), ilog2_ceil(np ? np : 1)});
   if (it != ctx->tuned.end()) c = it->second;
-  PJ_TRY(validate_cfg(c, nb));
+  PJ_TRY(validate_cfg(c, nb, np));
   *cfg = c;
+  return PJ_OK;
+}
+
+namespace pj {
+// The direct table of dense build keys: lo = range[0], span = range[1] -
+// range[0] + 1 (the keys' MIN and MAX, computed on the device).  When span is
+// the row count nb, vals[key - lo] = value (0 without values) and
+// seen[key - lo] = 1 (seen zeroed); plain stores: a repeated key leaves some
+// slot of seen at 0, which the probe kernel finds.  Otherwise the keys are
+// not dense: flag bit 1, and nothing else happens.
+__device__ __forceinline__ bool direct_span(const int64_t *range, uint64_t nb, int32_t *lo,
+                                            uint32_t *span) {
+  const int64_t mn = range[0], mx = range[1];
+  *lo = (int32_t)mn;
+  *span = (uint32_t)(mx - mn + 1);
+  return (uint64_t)(mx - mn + 1) == nb;
+}
+__global__ __launch_bounds__(256) void k_direct_build(const int32_t *__restrict__ bk,
+                                                      const int32_t *__restrict__ bv, uint64_t nb,
+                                                      const int64_t *__restrict__ range,
+                                                      int32_t *__restrict__ vals,
+                                                      uint8_t *__restrict__ seen,
+                                                      uint32_t *__restrict__ flag) {
+  int32_t lo;
+  uint32_t span;
+  if (!direct_span(range, nb, &lo, &span)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flag, 2u);
+    return;
+  }
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nb; i += stride) {
+    const uint32_t off = (uint32_t)bk[i] - (uint32_t)lo;  // < span: lo and span cover every key
+    vals[off] = bv ? bv[i] : 0;
+    seen[off] = 1;
+  }
+}
+
+// Probe of the direct table with the fused reduce: four probe rows per lane
+// per step (16-byte loads, three steps in flight), one table read each.  Each
+// workgroup first checks its slice of seen (a slot at 0: flag bit 0).
+__global__ __launch_bounds__(256) void k_direct_probe(const int32_t *__restrict__ pk,
+                                                      const int32_t *__restrict__ pv, uint64_t np,
+                                                      uint64_t nb, const int64_t *__restrict__ range,
+                                                      const int32_t *__restrict__ vals,
+                                                      const uint8_t *__restrict__ seen,
+                                                      uint32_t *__restrict__ flag,
+                                                      unsigned long long *__restrict__ acc_out) {
+  int32_t lo;
+  uint32_t span;
+  if (!direct_span(range, nb, &lo, &span)) return;  // uniform: no flush, no result
+  {
+    const uint64_t per = ceil_div((uint64_t)span, (uint64_t)gridDim.x);
+    const uint64_t a0 = (uint64_t)blockIdx.x * per, a1 = a0 + per < span ? a0 + per : span;
+    bool hole = false;
+    for (uint64_t i = a0 + threadIdx.x; i < a1; i += 256) hole |= seen[i] == 0;
+    if (hole) atomicOr(flag, 1u);
+  }
+  JoinAcc acc;
+  acc.init();
+  auto probe1 = [&](int32_t k, int32_t p) {
+    const uint32_t off = (uint32_t)k - (uint32_t)lo;
+    if (off < span) acc.add1(vals[off], p);
+  };
+  auto probe4 = [&](const int4 k4, const int4 v4) {
+    const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+    const int32_t vv[4] = {pv ? v4.x : 0, pv ? v4.y : 0, pv ? v4.z : 0, pv ? v4.w : 0};
+    bool hit[4];
+    int32_t b[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t off = (uint32_t)kk[c] - (uint32_t)lo;
+      hit[c] = off < span;
+      b[c] = vals[hit[c] ? off : 0u];
+    }
+    acc.add4_1(hit, b, vv);
+  };
+  // every workgroup takes one contiguous slice of the probe rows
+  const uint64_t per = (ceil_div(np, (uint64_t)gridDim.x) + 3) & ~3ULL;
+  const uint64_t s0 = (uint64_t)blockIdx.x * per;
+  const uint64_t s1 = s0 + per < np ? s0 + per : np;
+  if ((((uintptr_t)pk | (uintptr_t)pv) & 15) == 0) {
+    if (s0 < s1) probe_range<256>(pk, pv, s0, s1, probe4, probe1);
+  } else {  // columns not 16-byte aligned: one row at a time
+    for (uint64_t x = s0 + threadIdx.x; x < s1; x += 256) probe1(pk[x], pv ? pv[x] : 0);
+  }
+  join_acc_flush(acc, acc_out);
+}
+}  // namespace pj
+
+int pj::join_direct(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv, uint64_t nb,
+                    const int32_t *d_pk, const int32_t *d_pv, uint64_t np, pj_join_agg *out,
+                    bool *done) {
+  *done = false;
+  if (nb < 2 || nb > kDirectMaxSpan || !np) return PJ_OK;
+  // the key range stays on the device: the table is sized for nb keys, and
+  // the kernels do nothing when the span is not nb; one read-back at the end
+  // brings the result and the flags together
+  Scratch part, range, vals, seen, acc;
+  PJ_TRY(range.alloc(ctx, 16));
+  PJ_TRY(rel::minmax_i32_device(ctx, d_bk, nb, part, range.as<int64_t>()));
+  PJ_TRY(vals.alloc(ctx, nb * 4));
+  PJ_TRY(seen.alloc(ctx, nb));
+  PJ_TRY(zero(ctx, seen.p, nb));
+  PJ_TRY(acc.alloc(ctx, kJoinAccWords * 8));
+  PJ_TRY(join_acc_init(ctx, acc.as<unsigned long long>()));  // clears the flag word too
+  uint32_t *flag = reinterpret_cast<uint32_t *>(acc.as<unsigned long long>() + kJoinAccFlagWord);
+  const uint64_t bgrid = std::min<uint64_t>(ceil_div(nb, 256), (uint64_t)ctx->num_cus * 16);
+  PJ_LAUNCH(ctx, "join_direct_build", k_direct_build, bgrid, 256, 0, d_bk, d_bv, nb,
+            range.as<const int64_t>(), vals.as<int32_t>(), seen.as<uint8_t>(), flag);
+  // slices of >= 16 Ki rows, at most 8 workgroups per CU
+  const uint64_t pgrid = std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(np, 16384), (uint64_t)ctx->num_cus * 8));
+  PJ_LAUNCH(ctx, "join_direct_probe", k_direct_probe, pgrid, 256, 0, d_pk, d_pv, np, nb,
+            range.as<const int64_t>(), vals.as<const int32_t>(), seen.as<const uint8_t>(), flag,
+            acc.as<unsigned long long>());
+  // the flag word sits in the accumulator's first copy
+  int64_t h[kJoinAccWords];
+  PJ_TRY(d2h(ctx, h, acc.p, sizeof(h)));
+  if (((const uint32_t *)h)[2 * kJoinAccFlagWord]) return PJ_OK;  // not dense, or a repeat
+  PJ_TRY(join_acc_fold(h, out));
+  *done = true;
   return PJ_OK;
 }
 
@@ -1062,7 +1190,7 @@ int pj::join_partitioned_auto(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d
   pj_radix_cfg c{};
   auto it = ctx->tuned.find(TuneKey{ilog2_ceil(nb ? nb : 1), ilog2_ceil(np ? np : 1)});
   if (it != ctx->tuned.end()) c = it->second;
-  PJ_TRY(validate_cfg(c, nb));
+  PJ_TRY(validate_cfg(c, nb, np));
   return join_partitioned(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, c, out);
 }
 
@@ -1101,7 +1229,7 @@ extern "C" int pj_join_partitioned_reduce(pj_ctx *ctx, const int32_t *d_bk,
     }
     if (it != ctx->tuned.end()) c = it->second;
   }
-  PJ_TRY(validate_cfg(c, nb));
+  PJ_TRY(validate_cfg(c, nb, np));
   return join_partitioned(ctx, d_bk, d_bv, nb, d_pk, d_pv, np, c, out);
 }
 
@@ -1113,7 +1241,7 @@ int pj_autotune_join(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv, uint
   for (int lds : {12, 13, 14}) {
     pj_radix_cfg base{};
     base.lds_log = lds;
-    if (validate_cfg(base, nb) != PJ_OK) continue;
+    if (validate_cfg(base, nb, np) != PJ_OK) continue;
     int B = base.bits1 + base.bits2;
     for (int b1 = std::max(1, B - 8); b1 <= std::min(8, B); ++b1) {
       pj_radix_cfg c = base;

@@ -436,6 +436,24 @@ int join_acc_init(pj_ctx *ctx, unsigned long long *d_acc);
 int noalloc_scratch_init(int device);
 int join_acc_read(pj_ctx *ctx, const unsigned long long *d_acc,
                   pj_join_agg *out);
+// Combines a host copy (kJoinAccWords words) of the accumulator copies.
+int join_acc_fold(const int64_t *h, pj_join_agg *out);
+namespace rel {
+// MIN and MAX of n >= 1 int32 rows into d_res[0], d_res[1] (device), queued
+// on the context stream with no read-back; part receives the per-workgroup
+// partials and must outlive the queued work.
+int minmax_i32_device(pj_ctx *ctx, const int32_t *col, uint64_t n, Scratch &part, int64_t *d_res);
+}  // namespace rel
+
+// Join -> reduce through a direct table when the build keys are dense (as
+// many distinct keys as the span min..max holds, at most kDirectMaxSpan):
+// the build values indexed by key - min, 4 bytes a key, and one table read
+// per probe row.  *done is false (and nothing is written to out) when the
+// keys are not dense or repeat one; the caller then takes another path.
+constexpr uint64_t kDirectMaxSpan = 1ULL << 22;  // 16 MiB of values: in the Infinity Cache
+int join_direct(pj_ctx *ctx, const int32_t *d_bk, const int32_t *d_bv, uint64_t nb,
+                const int32_t *d_pk, const int32_t *d_pv, uint64_t np, pj_join_agg *out,
+                bool *done);
 // The radix-partitioned join -> reduce with the configuration chosen for
 // (nb, np) (the tuned one when a tuning exists), for a caller that holds the
 // context lock and has checked the sizes.

@@ -1409,9 +1409,88 @@ int reduce(pj_ctx *ctx, const void *col, int t, uint64_t n, int monoid, int64_t 
   return PJ_OK;
 }
 
+// MIN and MAX of an int32 column in one pass: per-workgroup partials
+// part[2 b] = min, part[2 b + 1] = max (four loads in flight per thread),
+// then one workgroup folds them into res[0], res[1].
+__global__ __launch_bounds__(256) void k_minmax_i32(const int32_t *__restrict__ v, uint64_t n,
+                                                    int64_t *__restrict__ part) {
+  __shared__ int64_t smn[4], smx[4];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  const uint64_t stride = (uint64_t)gridDim.x * 1024;
+  for (uint64_t b = (uint64_t)blockIdx.x * 1024 + threadIdx.x; b < n; b += stride) {
+    int32_t x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = v[b + j * 256 < n ? b + j * 256 : b];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      mn = x[j] < mn ? x[j] : mn;
+      mx = x[j] > mx ? x[j] : mx;
+    }
+  }
+  mn = wave_min_i64(mn);
+  mx = wave_max_i64(mx);
+  if ((threadIdx.x & 63) == 0) {
+    smn[threadIdx.x >> 6] = mn;
+    smx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      mn = smn[w] < mn ? smn[w] : mn;
+      mx = smx[w] > mx ? smx[w] : mx;
+    }
+    part[2 * blockIdx.x] = mn;
+    part[2 * blockIdx.x + 1] = mx;
+  }
+}
+__global__ __launch_bounds__(256) void k_minmax_fold(const int64_t *__restrict__ part, int np,
+                                                     int64_t *__restrict__ res) {
+  __shared__ int64_t smn[4], smx[4];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    mn = part[2 * i] < mn ? part[2 * i] : mn;
+    mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+  }
+  mn = wave_min_i64(mn);
+  mx = wave_max_i64(mx);
+  if ((threadIdx.x & 63) == 0) {
+    smn[threadIdx.x >> 6] = mn;
+    smx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      mn = smn[w] < mn ? smn[w] : mn;
+      mx = smx[w] > mx ? smx[w] : mx;
+    }
+    res[0] = mn;
+    res[1] = mx;
+  }
+}
+
+int minmax_i32_device(pj_ctx *ctx, const int32_t *col, uint64_t n, Scratch &part, int64_t *d_res) {
+  if (!n) return fail(PJ_E_ARG, "minmax of no rows");
+  const int np = (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, ceil_div(n, 8192)));
+  PJ_TRY(part.alloc(ctx, (uint64_t)np * 16));
+  PJ_LAUNCH(ctx, "rel_minmax", k_minmax_i32, np, 256, 0, col, n, part.as<int64_t>());
+  PJ_LAUNCH(ctx, "rel_minmax_fold", k_minmax_fold, 1, 256, 0, part.as<const int64_t>(), np,
+            d_res);
+  return PJ_OK;
+}
+
 int reduce_minmax(pj_ctx *ctx, const void *col, int t, uint64_t n, int64_t *mn, int64_t *mx) {
   if (t == F64 || t == F32) return fail(PJ_E_ARG, "reduce_minmax over an integer column only");
   if (!n) return fail(PJ_E_ARG, "reduce_minmax of no rows");
+  if (t == I32) {  // one pass for both, one fold, one read-back
+    Scratch part, res;
+    PJ_TRY(res.alloc(ctx, 16));
+    PJ_TRY(minmax_i32_device(ctx, (const int32_t *)col, n, part, res.as<int64_t>()));
+    int64_t v[2] = {0, 0};
+    PJ_TRY(d2h(ctx, v, res.p, 16));
+    *mn = v[0];
+    *mx = v[1];
+    return PJ_OK;
+  }
   Scratch part, res;
   const int np = (int)std::min<uint64_t>(1024, grid_for(ctx, n, 256));
   PJ_TRY(part.alloc(ctx, (uint64_t)np * 16));

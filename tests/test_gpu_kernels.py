@@ -97,25 +97,79 @@ def test_join_chained_c1_full_size(ctx):
 
 
 def test_join_chained_dispatch(ctx):
-    """pj_join_chained_reduce keeps the chained table for small probe sides and
-    runs the radix-partitioned join for large ones (C1: 2^20 x 2^24); both
-    give the chained table's result, itself checked against the oracle."""
-    for nr, ns, path in ((1 << 12, 1 << 16, "join_chained_probe"),
-                         (1 << 20, 1 << 24, "pjoin_probe")):
-        rk = ctx.gen_perm_i32(nr, 31)
-        rv = ctx.gen_uniform_i32(nr, 31, 1, 0, 1 << 31)
-        sk = ctx.gen_uniform_i32(ns, 31, 2, 0, int(1.33 * nr))
-        sv = ctx.gen_uniform_i32(ns, 31, 3, 0, 1 << 31)
+    """pj_join_chained_reduce tries the direct table first (its kernels run on
+    every call and do nothing when the keys are not dense) and keeps its
+    result for dense build keys (a permutation of 0..n-1); other keys take the
+    chained table at small probe sides and the radix-partitioned join for
+    large ones (the C1 size, 2^20 x 2^24, keys spread 3 apart); a dense span
+    with a repeated key is found by the direct table and goes to the chained
+    table.  Every result equals the chained oracle's."""
+    fallbacks = ("join_chained_probe", "pjoin_probe")
+    for nr, ns, step, repeat, want_fb in ((1 << 12, 1 << 16, 1, False, set()),
+                                          (1 << 12, 1 << 16, 3, False, {"join_chained_probe"}),
+                                          (1 << 20, 1 << 24, 3, False, {"pjoin_probe"}),
+                                          (1 << 12, 1 << 16, 1, True, {"join_chained_probe"})):
+        rs = np.random.default_rng(nr + step + repeat)
+        rk = (rs.permutation(nr) * step).astype(np.int32)
+        if repeat:
+            rk[7] = rk[8]  # the span still holds nr keys' worth of slots, one of them twice
+        rv = rs.integers(-2**31, 2**31, nr).astype(np.int32)
+        sk = rs.integers(-5, int(1.33 * nr * step), ns).astype(np.int32)
+        sv = rs.integers(-2**31, 2**31, ns).astype(np.int32)
+        d = _dev(ctx, rk, rv, sk, sv)
+        hb = (nr - 1).bit_length() + 1
         ctx.prof_enable(True)
         ctx.prof_reset()
-        got = ctx.join_chained_reduce(rk, rv, sk, sv, (nr - 1).bit_length() + 1)
-        ran = {n: ctx.prof_get(n)[1] for n in ("join_chained_probe", "pjoin_probe")}
+        got = ctx.join_chained_reduce(d[0], d[1], d[2], d[3], hb)
+        ran = {n for n in fallbacks if ctx.prof_get(n)[1] > 0}
+        direct = ctx.prof_get("join_direct_probe")[1]
         ctx.prof_enable(False)
-        assert ran[path] > 0 and sum(ran.values()) == ran[path], ran
-        table = ctx.join_chained_reduce_noalloc(rk, rv, sk, sv, (nr - 1).bit_length() + 1)
-        rc, want = O.join_dense(rk.numpy(), rv.numpy(), sk.numpy(), sv.numpy())
-        assert rc == O.OR_OK and got == want and table == want
-        _free(rk, rv, sk, sv)
+        assert ran == want_fb and direct == 1, (nr, step, repeat, ran, direct)
+        rc, want = O.join_chained(rk, rv, sk, sv, hb, threads=8)
+        assert rc == O.OR_OK and got == want, (nr, step, repeat)
+        if not repeat:
+            assert ctx.join_chained_reduce_noalloc(d[0], d[1], d[2], d[3], hb) == want
+        _free(*d)
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=5, nr=3000, ns=50_001, lo=-1500, off=0),
+    dict(seed=6, nr=1 << 16, ns=1 << 18, lo=I32MAX - (1 << 16) + 1, off=0),
+    dict(seed=7, nr=5000, ns=40_003, lo=10, off=1),
+])
+def test_join_direct_dense_keys(ctx, case):
+    """The direct table of pj_join_chained_reduce over dense build keys (a
+    permutation of lo .. lo + nr - 1, negative keys, a span ending at
+    INT32_MAX): probe keys inside and outside the span, INT32_MIN and
+    INT32_MAX among them, a probe side that is not a multiple of 4 rows, and
+    columns 4-byte but not 16-byte aligned (one row at a time); with and
+    without values.  COUNT, SUMs, MIN and MAX equal the chained oracle's, and
+    only the direct join's kernels run."""
+    rs = np.random.default_rng(case["seed"])
+    nr, ns, lo, off = case["nr"], case["ns"], case["lo"], case["off"]
+    rk = (lo + rs.permutation(nr).astype(np.int64)).astype(np.int32)
+    rv = rs.integers(-2**31, 2**31, nr).astype(np.int32)
+    sk = np.clip(lo - nr // 2 + rs.integers(0, 2 * nr, ns), I32MIN, I32MAX).astype(np.int32)
+    sk[:4] = [I32MIN, I32MAX, lo, lo + nr - 1]
+    sv = rs.integers(-2**31, 2**31, ns).astype(np.int32)
+    hb = (nr - 1).bit_length() + 1
+    rc, want = O.join_chained(rk, rv, sk, sv, hb, threads=8)
+    assert rc == O.OR_OK
+    # each column off elements into its allocation
+    pad = [np.concatenate([np.zeros(off, np.int32), a]) for a in (rk, rv, sk, sv)]
+    d = _dev(ctx, *pad)
+    v = [pa.DeviceArray(ctx, x.ptr + 4 * off, len(a), np.int32) for x, a in zip(d, (rk, rv, sk, sv))]
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    got = ctx.join_chained_reduce(v[0], v[1], v[2], v[3], hb)
+    got_nv = ctx.join_chained_reduce(v[0], None, v[2], None, hb)
+    ran = {n: ctx.prof_get(n)[1] for n in ("join_direct_probe", "join_chained_probe", "pjoin_probe")}
+    ctx.prof_enable(False)
+    assert ran["join_direct_probe"] == 2 and ran["join_chained_probe"] == 0 and ran["pjoin_probe"] == 0
+    assert got == want
+    assert got_nv["count"] == want["count"] and got_nv["sum_b"] == 0
+    assert got_nv["sum_p"] == 0 and got_nv["min_b"] == 0 == got_nv["max_b"]
+    _free(*d)
 
 
 def test_join_chained_single_duplicate(ctx):
