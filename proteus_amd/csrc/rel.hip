@@ -491,6 +491,116 @@ __global__ __launch_bounds__(256) void k_gb_generic(GbArgs a, uint64_t n, uint32
   }
 }
 
+// The global table's insert of k_gb_generic for the row `row`: slot index,
+// or -1 when all S slots are taken by other keys.
+__device__ __forceinline__ int64_t gb_global_slot(const GbArgs &a, uint64_t row, uint32_t *slots,
+                                                  uint64_t mask) {
+  const uint64_t S = mask + 1;
+  uint64_t s = key_hash(a, row) & mask;
+  for (uint64_t probes = 0; probes < S; ++probes) {
+    uint32_t cur = __hip_atomic_load(&slots[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == 0) {
+      const uint32_t prev = atomicCAS(&slots[s], 0u, (uint32_t)(row + 1));
+      if (prev == 0) return (int64_t)s;
+      cur = prev;
+    }
+    if (key_eq(a, row, cur - 1)) return (int64_t)s;
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
+// One integer aggregate value (or a count) into an accumulator with the
+// monoid's atomic; works on LDS and global addresses alike.
+__device__ __forceinline__ void gb_apply_int(int64_t *slot, int m, int64_t v) {
+  switch (m) {
+    case M_CNT:
+    case M_SUM: atomicAdd((unsigned long long *)slot, (unsigned long long)v); break;
+    case M_MIN: atomicMin((long long *)slot, (long long)v); break;
+    case M_MAX: atomicMax((long long *)slot, (long long)v); break;
+    case M_AND: atomicAnd((unsigned long long *)slot, (unsigned long long)(v != 0)); break;
+    default: atomicOr((unsigned long long *)slot, (unsigned long long)(v != 0)); break;
+  }
+}
+
+// Group-by of integer aggregates (and counts) with few groups: every
+// workgroup folds its rows into a private LDS table of LS slots (the
+// representative row + 1, 0 = empty, and one accumulator per aggregate),
+// with LDS atomics and no global traffic per row, then merges each occupied
+// slot into the global table once: the global slot of its representative
+// row, and the monoid's atomic on each accumulator (a count adds its local
+// count).  Keys are compared through the representative rows, as in
+// k_gb_generic.  A row whose key finds neither its slot nor a free one
+// within kGbLdsProbes local probes (a workgroup that sees more groups than
+// its table holds) goes to the global table directly.  LDS: na * LS int64
+// accumulators, then LS * 4 bytes of rows.
+constexpr uint32_t kGbLdsProbes = 32;
+__global__ __launch_bounds__(256) void k_gb_lds(GbArgs a, uint64_t n, uint32_t *slots,
+                                                uint64_t mask, int64_t *acc,
+                                                unsigned int *overflow, uint32_t LS) {
+  extern __shared__ int64_t lacc[];  // na * LS accumulators, then LS rows
+  uint32_t *lrow = reinterpret_cast<uint32_t *>(lacc + (uint64_t)a.na * LS);
+  const uint64_t S = mask + 1;
+  for (uint32_t i = threadIdx.x; i < LS; i += 256) {
+    lrow[i] = 0;
+    for (int j = 0; j < a.na; ++j) {
+      const int m = a.am[j];
+      lacc[(uint64_t)j * LS + i] = m == M_MIN ? INT64_MAX : m == M_MAX ? INT64_MIN : m == M_AND ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const uint32_t lmask = LS - 1;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t row = (uint64_t)blockIdx.x * 256 + threadIdx.x; row < n; row += stride) {
+    uint32_t s = (uint32_t)key_hash(a, row) & lmask;
+    int64_t *base = nullptr;
+    uint64_t step = 0;
+    for (uint32_t probes = 0; probes < kGbLdsProbes; ++probes) {
+      uint32_t cur = __hip_atomic_load(&lrow[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (cur == 0) {
+        const uint32_t prev = atomicCAS(&lrow[s], 0u, (uint32_t)(row + 1));
+        if (prev == 0) {
+          base = lacc + s;
+          break;
+        }
+        cur = prev;
+      }
+      if (key_eq(a, row, cur - 1)) {
+        base = lacc + s;
+        break;
+      }
+      s = (s + 1) & lmask;
+    }
+    if (base) {
+      step = LS;
+    } else {  // the local table is full of other keys: straight to the global one
+      const int64_t gs = gb_global_slot(a, row, slots, mask);
+      if (gs < 0) {
+        atomicOr(overflow, 1u);
+        continue;
+      }
+      base = acc + gs;
+      step = S;
+    }
+    for (int j = 0; j < a.na; ++j) {
+      const int m = a.am[j];
+      gb_apply_int(base + (uint64_t)j * step, m, m == M_CNT ? 1 : ld_slot(a.ac[j], a.at[j], row));
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < LS; i += 256) {
+    const uint32_t r = lrow[i];
+    if (!r) continue;
+    const int64_t gs = gb_global_slot(a, r - 1, slots, mask);
+    if (gs < 0) {
+      atomicOr(overflow, 1u);
+      continue;
+    }
+    for (int j = 0; j < a.na; ++j)
+      gb_apply_int(acc + (uint64_t)j * S + gs, a.am[j], lacc[(uint64_t)j * LS + i]);
+  }
+}
+
 __global__ void k_fill_i64(int64_t *d, int64_t v, uint64_t n) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = v;
@@ -1551,8 +1661,22 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
     PJ_LAUNCH(ctx, "rel_fill", k_fill_i64, grid_for(ctx, S, 256), 256, 0, acc.as<int64_t>() + (uint64_t)j * S,
               ident, S);
   }
-  PJ_LAUNCH(ctx, "rel_groupby", k_gb_generic, grid_for(ctx, n, 256), 256, 0, a, n,
-            slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>());
+  // integer aggregates over few groups: workgroup-private LDS tables
+  bool ints = true;
+  for (int j = 0; j < na; ++j)
+    ints &= aggs[j].monoid == M_CNT || !(aggs[j].t == F64 || aggs[j].t == F32);
+  uint32_t LS = 0;  // local slots: a power of two, <= S, rows + accumulators in 64 KiB
+  if (ints && n >= (1u << 16))
+    for (uint32_t l = 4096; l >= 256 && !LS; l >>= 1)
+      if (l <= S && (uint64_t)l * (4 + 8 * (uint64_t)na) <= 65536) LS = l;
+  if (LS) {
+    const uint64_t grid = std::min<uint64_t>(ceil_div(n, 4096), (uint64_t)ctx->num_cus * 2);
+    PJ_LAUNCH(ctx, "rel_groupby_lds", k_gb_lds, grid, 256, (size_t)LS * (4 + 8 * na), a, n,
+              slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>(), LS);
+  } else {
+    PJ_LAUNCH(ctx, "rel_groupby", k_gb_generic, grid_for(ctx, n, 256), 256, 0, a, n,
+              slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>());
+  }
   unsigned int of = 0;
   PJ_TRY(d2h(ctx, &of, flag.p, 4));
   if (of) return fail(PJ_E_CAPACITY, "group-by: more groups than the table of %llu slots", (unsigned long long)S);

@@ -388,6 +388,56 @@ def test_groupby_plan_runs_lds_kernel(ctx, groups, dist):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("na,nb", [(12, 10), (50, 100)])
+def test_groupby_two_keys_integer_aggregates_lds_tables(ctx, na, nb):
+    """A two-key groupby with integer COUNT / SUM / MIN / MAX (signed values)
+    runs the workgroup-private LDS tables of the general group-by
+    (rel_groupby_lds): 120 groups fit every workgroup's table; 5000 groups
+    overflow the local tables, whose rows go to the global table.  Equals
+    numpy."""
+    n = 300_003
+    rs = np.random.default_rng(na * nb)
+    a = rs.integers(0, na, n).astype(np.int32)
+    b_ = rs.integers(-nb // 2, nb - nb // 2, n).astype(np.int32)
+    v = rs.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    dev = [ctx.to_device(x) for x in (a, b_, v)]
+    ctx.clear_catalog()
+    ctx.register_column("G", "a", "int", dev[0])
+    ctx.register_column("G", "b", "int", dev[1])
+    ctx.register_column("G", "v", "int64", dev[2])
+    q = rb.RelBuilder(ctx).scan("G", ["a", "b", "v"]).to_gpu().unpack() \
+        .groupby([rb.col("G", "a"), rb.col("G", "b")],
+                 [(rb.as_(rb.int64(1), "g", "cnt"), "sum"),
+                  (rb.as_(rb.col("G", "v"), "g", "s"), "sum"),
+                  (rb.as_(rb.col("G", "v"), "g", "mn"), "min"),
+                  (rb.as_(rb.col("G", "v"), "g", "mx"), "max")],
+                 13, 8192)
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    out = ctx.execute_plan(json.dumps(q.plan()))
+    lds = ctx.prof_get("rel_groupby_lds")[1]
+    ctx.prof_enable(False)
+    assert lds == 1
+    got = sorted(tuple(int(x) for x in ln.split(",")) for ln in out.splitlines() if ln)
+    uniq, inv = np.unique(np.stack([a, b_], axis=1), axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    g = len(uniq)
+    cnt = np.bincount(inv, minlength=g)
+    sm = np.zeros(g, np.int64)
+    np.add.at(sm, inv, v)
+    mn = np.full(g, np.iinfo(np.int64).max)
+    np.minimum.at(mn, inv, v)
+    mx = np.full(g, np.iinfo(np.int64).min)
+    np.maximum.at(mx, inv, v)
+    want = sorted((int(uniq[i, 0]), int(uniq[i, 1]), int(cnt[i]), int(sm[i]), int(mn[i]), int(mx[i]))
+                  for i in range(g))
+    assert got == want
+    for x in dev:
+        x.free()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
 def test_groupby_plan_falls_back_to_general_kernel(ctx):
     """Shapes the LDS group-by does not compute (two integer value columns,
     an FP64 value, two keys) run the general kernel with the same results as
