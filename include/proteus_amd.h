@@ -112,7 +112,11 @@ typedef struct {
  * "hashjoin-chained" (core/olap/lib/plan/plan-parser.cpp:977-1006).
  * hash_bits and max_build are the operator's fields; nb > max_build returns
  * PJ_E_CAPACITY (the reference asserts, hash-join-chained.cpp:599-608).
- * d_bv / d_pv may be NULL (value 0). */
+ * d_bv / d_pv may be NULL (value 0).  The device algorithm depends on the
+ * inputs, not the result: dense build keys (as many distinct keys as
+ * min..max holds, up to 2^22) join through a direct table of the build
+ * values; others through the open-addressing table (small probe sides) or
+ * the radix-partitioned join (large ones). */
 int pj_join_chained_reduce(pj_ctx *ctx, const int32_t *d_bk,
                            const int32_t *d_bv, uint64_t nb,
                            const int32_t *d_pk, const int32_t *d_pv,
