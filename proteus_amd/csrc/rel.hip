@@ -1289,21 +1289,33 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
   __syncthreads();
   const int t0 = w.tab[0];
   const uint64_t stride = (uint64_t)gridDim.x * 256u * RA;  // 64-bit: no wrap near n = 2^32
-  for (uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA; base < a.n;
-       base += stride) {
-    // ---- phase A: stage 0 on RA rows per lane --------------------------------
-    ChainRows<RA> ca;
-    ca.ridl = w.ridl;
+  auto tile_rows = [&](uint64_t base, ChainRows<RA> &c) {
+    c.ridl = w.ridl;
 #pragma unroll
     for (int k = 0; k < RA; ++k) {
       const uint64_t r = base + k * 64u + lane;
-      ca.alive[k] = r < a.n;
-      ca.row[k] = ca.alive[k] ? (uint32_t)r : 0u;
+      c.alive[k] = r < a.n;
+      c.row[k] = c.alive[k] ? (uint32_t)r : 0u;
     }
+  };
+  // stage 0's keys of the wave's next tile, loaded while the levels work
+  // on the current one
+  int32_t pre_next[RA];
+  uint64_t base = (uint64_t)blockIdx.x * 256u * RA + wid * 64u * RA;
+  {
+    ChainRows<RA> cn;
+    tile_rows(base, cn);
+    chain_prefetch<0, 1, RA>(a, cn, pre_next);
+  }
+  for (; base < a.n; base += stride) {
+    // ---- phase A: stage 0 on RA rows per lane --------------------------------
+    ChainRows<RA> ca;
+    tile_rows(base, ca);
     {
       bool live = true;
       int32_t pre[RA];
-      chain_prefetch<0, 1, RA>(a, ca, pre);
+#pragma unroll
+      for (int k = 0; k < RA; ++k) pre[k] = pre_next[k];
       chain_stages<0, 1, RA>(a, ca, live, pre);
     }
     // ---- compaction of the survivors onto the first queue -------------------
@@ -1320,6 +1332,11 @@ __global__ __launch_bounds__(256) void k_chain_q(const ChainArgs *__restrict__ a
       qn += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
+    {
+      ChainRows<RA> cn;
+      tile_rows(base + stride, cn);
+      chain_prefetch<0, 1, RA>(a, cn, pre_next);
+    }
     // ---- level 1 on whole groups of 64 queued rows; the rest waits ----------
     uint32_t q0 = 0;
     for (; q0 + 64u <= qn; q0 += 64u) chain_level<1, D, K>(w, q0, q0 + 64u);
