@@ -266,17 +266,26 @@ __global__ __launch_bounds__(kVmBlock) void k_nz_mask(const uint32_t *v, uint64_
   }
 }
 
-// Exclusive scan of m u32 counts into u64 offsets (off[m] = total), one
-// workgroup: each thread scans a contiguous run.
+// Exclusive scan of m u32 counts into u64 offsets, one workgroup (each
+// thread scans a contiguous run): off[i] = sum of cnt[0 .. i), off[m] = the
+// total; *mx (mx may be null) = the largest count.
 __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *cnt, uint64_t m,
-                                                       unsigned long long *off) {
+                                                       unsigned long long *off,
+                                                       unsigned long long *mx) {
   __shared__ unsigned long long part[1024];
+  __shared__ uint32_t s_mx;
+  if (threadIdx.x == 0) s_mx = 0;
   const uint64_t per = (m + 1023) / 1024;
   const uint64_t a = threadIdx.x * per, b = a + per < m ? a + per : m;
   unsigned long long s = 0;
-  for (uint64_t i = a; i < b; ++i) s += cnt[i];
+  uint32_t lm = 0;
+  for (uint64_t i = a; i < b; ++i) {
+    s += cnt[i];
+    lm = cnt[i] > lm ? cnt[i] : lm;
+  }
   part[threadIdx.x] = s;
   __syncthreads();
+  if (mx && lm) atomicMax(&s_mx, lm);
   for (int o = 1; o < 1024; o <<= 1) {
     unsigned long long x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
     __syncthreads();
@@ -289,6 +298,8 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *cnt, uint6
     run += cnt[i];
   }
   if (threadIdx.x == 1023) off[m] = part[1023];
+  // every thread's atomicMax came before the scan's barriers
+  if (mx && threadIdx.x == 0) *mx = s_mx;
 }
 
 __global__ __launch_bounds__(kVmBlock) void k_sel_write(const uint8_t *mask, uint64_t n,
@@ -1469,7 +1480,7 @@ static int compact_mask(pj_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t t
   Scratch off;
   PJ_TRY(off.alloc(ctx, (tiles + 1) * 8));
   PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, tile_cnt, tiles,
-            off.as<unsigned long long>());
+            off.as<unsigned long long>(), (unsigned long long *)nullptr);
   PJ_LAUNCH(ctx, "rel_compact", k_sel_write, tiles, kVmBlock, 0, mask, n,
             off.as<const unsigned long long>(), out_idx);
   unsigned long long total = 0;
@@ -1605,6 +1616,30 @@ int minmax_i32_device(pj_ctx *ctx, const int32_t *col, uint64_t n, Scratch &part
   return PJ_OK;
 }
 
+int d2h_many(pj_ctx *ctx, int n, void *const *dst, const void *const *src, const uint64_t *bytes) {
+  uint64_t tot = 0;
+  for (int i = 0; i < n; ++i) tot += (bytes[i] + 15) & ~15ULL;
+  if (tot > kBounceBytes || !ctx->pinned_rb) {
+    for (int i = 0; i < n; ++i) PJ_TRY(d2h(ctx, dst[i], src[i], bytes[i]));
+    return PJ_OK;
+  }
+  std::lock_guard<std::mutex> g(ctx->rb_mu);
+  char *rb = (char *)ctx->pinned_rb;
+  uint64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bytes[i])
+      PJ_HIP(hipMemcpyAsync(rb + off, src[i], bytes[i], hipMemcpyDeviceToHost, ctx->stream));
+    off += (bytes[i] + 15) & ~15ULL;
+  }
+  PJ_HIP(hipStreamSynchronize(ctx->stream));
+  off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bytes[i]) memcpy(dst[i], rb + off, bytes[i]);
+    off += (bytes[i] + 15) & ~15ULL;
+  }
+  return PJ_OK;
+}
+
 int reduce_minmax(pj_ctx *ctx, const void *col, int t, uint64_t n, int64_t *mn, int64_t *mx) {
   if (t == F64 || t == F32) return fail(PJ_E_ARG, "reduce_minmax over an integer column only");
   if (!n) return fail(PJ_E_ARG, "reduce_minmax of no rows");
@@ -1694,18 +1729,29 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
     PJ_LAUNCH(ctx, "rel_groupby", k_gb_generic, grid_for(ctx, n, 256), 256, 0, a, n,
               slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>());
   }
-  unsigned int of = 0;
-  PJ_TRY(d2h(ctx, &of, flag.p, 4));
-  if (of) return fail(PJ_E_CAPACITY, "group-by: more groups than the table of %llu slots", (unsigned long long)S);
+  // the occupied slots, then the overflow flag and their count in one read-back
   const uint64_t tiles = ceil_div(S, kSelTile);
-  Scratch mask, cnt, occ;
+  Scratch mask, cnt, occ, off;
   PJ_TRY(mask.alloc(ctx, S));
   PJ_TRY(cnt.alloc(ctx, tiles * 4));
   PJ_TRY(occ.alloc(ctx, S * 4));
+  PJ_TRY(off.alloc(ctx, (tiles + 1) * 8));
   PJ_LAUNCH(ctx, "rel_groupby_occ", k_nz_mask, tiles, kVmBlock, 0, slots.as<const uint32_t>(), S,
             mask.as<uint8_t>(), cnt.as<uint32_t>());
-  uint64_t ng = 0;
-  PJ_TRY(compact_mask(ctx, mask.as<uint8_t>(), S, tiles, cnt.as<uint32_t>(), occ.as<uint32_t>(), &ng));
+  PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, cnt.as<const uint32_t>(), tiles,
+            off.as<unsigned long long>(), (unsigned long long *)nullptr);
+  PJ_LAUNCH(ctx, "rel_compact", k_sel_write, tiles, kVmBlock, 0, mask.as<const uint8_t>(), S,
+            off.as<const unsigned long long>(), occ.as<uint32_t>());
+  unsigned int of = 0;
+  unsigned long long ngl = 0;
+  {
+    void *dst[2] = {&of, &ngl};
+    const void *src[2] = {flag.p, off.as<unsigned long long>() + tiles};
+    const uint64_t by[2] = {4, 8};
+    PJ_TRY(d2h_many(ctx, 2, dst, src, by));
+  }
+  if (of) return fail(PJ_E_CAPACITY, "group-by: more groups than the table of %llu slots", (unsigned long long)S);
+  const uint64_t ng = ngl;
   if (ng > max_groups)
     return fail(PJ_E_CAPACITY, "group-by produced %llu groups, maxInputSize %llu",
                 (unsigned long long)ng, (unsigned long long)max_groups);
@@ -1731,19 +1777,18 @@ struct Segments {
       PJ_TRY(bufs.back()->alloc(ctx, g * r * col_size(t)));
     }
     PJ_TRY(counts.alloc(ctx, g * 4));
-    PJ_TRY(offs.alloc(ctx, (g + 1) * 8));
+    PJ_TRY(offs.alloc(ctx, (g + 2) * 8));  // offsets, the total, the largest count
     return PJ_OK;
   }
   int compact(pj_ctx *ctx, const std::vector<int32_t> &types, void *const *outs, uint64_t *n_out) {
     PJ_LAUNCH(ctx, "rel_scan", k_scan_counts, 1, 1024, 0, counts.as<const uint32_t>(), grid,
-              offs.as<unsigned long long>());
-    // the segments' sizes and the total, read back together: the largest
+              offs.as<unsigned long long>(), offs.as<unsigned long long>() + grid + 1);
+    // the total and the largest segment, read back together: the largest
     // segment sizes the copy grid (most segments hold a few hundred rows)
-    std::vector<uint32_t> cnt(grid);
-    unsigned long long total = 0;
-    PJ_TRY(d2h(ctx, cnt.data(), counts.p, grid * 4));
-    PJ_TRY(d2h(ctx, &total, offs.as<unsigned long long>() + grid, 8));
-    const uint32_t mx = cnt.empty() ? 0u : *std::max_element(cnt.begin(), cnt.end());
+    unsigned long long tm[2] = {0, 0};
+    PJ_TRY(d2h(ctx, tm, offs.as<unsigned long long>() + grid, 16));
+    const unsigned long long total = tm[0];
+    const uint32_t mx = (uint32_t)tm[1];
     if (total && !types.empty()) {
       if (types.size() > (size_t)kPipeMaxOut)
         return fail(PJ_E_UNSUPPORTED, "pipeline with more than %d outputs", kPipeMaxOut);

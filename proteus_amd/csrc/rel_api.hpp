@@ -70,6 +70,9 @@ int select(pj_ctx *ctx, const Prog &pred, uint64_t n, uint32_t *out_idx, uint64_
 // dst[i] = src[idx[i]] for elements of esize bytes (1, 4 or 8).
 int gather(pj_ctx *ctx, const void *src, int esize, const uint32_t *idx, uint64_t n, void *dst);
 int iota(pj_ctx *ctx, uint32_t *d, uint64_t n);
+// n device-to-host copies with one synchronisation (through pinned memory
+// when they fit it together): dst[i] <- src[i], bytes[i] bytes.
+int d2h_many(pj_ctx *ctx, int n, void *const *dst, const void *const *src, const uint64_t *bytes);
 
 enum Monoid : int32_t { M_SUM = 0, M_MIN = 1, M_MAX = 2, M_AND = 3, M_OR = 4, M_CNT = 5 };
 // Reduction of a column: integer types and bool accumulate in int64, F64/F32

@@ -1998,34 +1998,40 @@ class Exec {
     return out;
   }
 
-  void load_host(const Col &c, uint64_t n, std::vector<int64_t> *iv, std::vector<double> *fv) {
-    iv->assign(n, 0);
-    fv->assign(n, 0.0);
-    switch (c.t) {
-      case R::I32: {
-        auto v = download<int32_t>(c, n);
-        for (uint64_t i = 0; i < n; ++i) (*iv)[i] = v[i];
-        break;
+  // Every column's n rows to the host with one synchronisation: integers
+  // and bools widened into iv[k], floats into fv[k].
+  void load_host(const std::vector<Col> &cols, uint64_t n, std::vector<std::vector<int64_t>> *iv,
+                 std::vector<std::vector<double>> *fv) {
+    const size_t m = cols.size();
+    std::vector<std::vector<uint8_t>> raw(m);
+    std::vector<void *> dst;
+    std::vector<const void *> src;
+    std::vector<uint64_t> by;
+    for (size_t k = 0; k < m; ++k) {
+      raw[k].resize(n * R::col_size(cols[k].t));
+      if (n) {
+        dst.push_back(raw[k].data());
+        src.push_back(cols[k].dev);
+        by.push_back(raw[k].size());
       }
-      case R::I64: {
-        auto v = download<int64_t>(c, n);
-        *iv = v;
-        break;
-      }
-      case R::F64: {
-        auto v = download<double>(c, n);
-        *fv = v;
-        break;
-      }
-      case R::F32: {
-        auto v = download<float>(c, n);
-        for (uint64_t i = 0; i < n; ++i) (*fv)[i] = v[i];
-        break;
-      }
-      default: {
-        auto v = download<uint8_t>(c, n);
-        for (uint64_t i = 0; i < n; ++i) (*iv)[i] = v[i];
-        break;
+    }
+    if (!dst.empty()) ck(R::d2h_many(ctx_, (int)dst.size(), dst.data(), src.data(), by.data()));
+    iv->assign(m, {});
+    fv->assign(m, {});
+    for (size_t k = 0; k < m; ++k) {
+      auto &I = (*iv)[k];
+      auto &F = (*fv)[k];
+      I.assign(n, 0);
+      F.assign(n, 0.0);
+      const uint8_t *p = raw[k].data();
+      for (uint64_t i = 0; i < n; ++i) {
+        switch (cols[k].t) {
+          case R::I32: { int32_t x; memcpy(&x, p + 4 * i, 4); I[i] = x; break; }
+          case R::I64: { int64_t x; memcpy(&x, p + 8 * i, 8); I[i] = x; break; }
+          case R::F64: { double x; memcpy(&x, p + 8 * i, 8); F[i] = x; break; }
+          case R::F32: { float x; memcpy(&x, p + 4 * i, 4); F[i] = x; break; }
+          default: I[i] = p[i]; break;
+        }
       }
     }
   }
@@ -2061,9 +2067,9 @@ class Exec {
         names.push_back(c.first.name);
       }
     }
-    std::vector<std::vector<int64_t>> iv(cols.size());
-    std::vector<std::vector<double>> fv(cols.size());
-    for (size_t k = 0; k < cols.size(); ++k) load_host(cols[k], in.n, &iv[k], &fv[k]);
+    std::vector<std::vector<int64_t>> iv;
+    std::vector<std::vector<double>> fv;
+    load_host(cols, in.n, &iv, &fv);
     auto field = [&](size_t k, uint64_t r, bool json) -> std::string {
       const Col &c = cols[k];
       if (c.dict) {
