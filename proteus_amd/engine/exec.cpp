@@ -214,12 +214,6 @@ class Exec {
     c.dev = c.owner->p;
     return c;
   }
-  template <class T>
-  std::vector<T> download(const Col &c, uint64_t n) {
-    std::vector<T> v(n);
-    if (n) ck(pj_memcpy_d2h(ctx_, v.data(), c.dev, n * sizeof(T)));
-    return v;
-  }
   std::string path(const std::string &p) const {
     if (!p.empty() && p[0] == '/') return p;
     return base_.empty() ? p : base_ + "/" + p;
