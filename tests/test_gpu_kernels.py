@@ -132,6 +132,23 @@ def test_join_chained_dispatch(ctx):
         _free(*d)
 
 
+def test_join_direct_smallest_sizes(ctx):
+    """The direct table at its smallest: two dense build keys (INT32_MIN and
+    INT32_MIN + 1, then 5 and 6) and probe sides of 1 to 7 rows (below one
+    four-row vector); each result equals the chained oracle's."""
+    for lo in (I32MIN, 5):
+        rk = np.array([lo + 1, lo], np.int32)
+        rv = np.array([-7, 11], np.int32)
+        for ns in range(1, 8):
+            sk = np.array([lo, lo + 1, lo + 2, I32MAX, lo, 0, lo + 1][:ns], np.int32)
+            sv = np.arange(ns, dtype=np.int32) * 3 - 4
+            rc, want = O.join_chained(rk, rv, sk, sv, 2)
+            assert rc == O.OR_OK
+            d = _dev(ctx, rk, rv, sk, sv)
+            assert ctx.join_chained_reduce(d[0], d[1], d[2], d[3], 2) == want, (lo, ns)
+            _free(*d)
+
+
 @pytest.mark.parametrize("case", [
     dict(seed=5, nr=3000, ns=50_001, lo=-1500, off=0),
     dict(seed=6, nr=1 << 16, ns=1 << 18, lo=I32MAX - (1 << 16) + 1, off=0),

@@ -114,6 +114,28 @@ def _chain_tables(ctx, n, seed):
 
 
 @pytest.mark.gpu
+def test_probe_chain_dimension_filter_keeps_nothing(ctx):
+    """A fused chain whose filtered dimension keeps no row (the filter's
+    mask rejects every build row, so its table holds no key): every fact
+    row misses and the grouped result is empty, as numpy says."""
+    ctx.clear_catalog()
+    n = 100_003
+    c, devs = _chain_tables(ctx, n, 5)
+    b = rb.RelBuilder(ctx)
+    col = rb.col
+    F = b.scan("F", ["a", "b", "v"]).to_gpu().unpack()
+    A = b.scan("A", ["k", "w"]).to_gpu().unpack().filter(rb.gt(col("A", "w"), rb.lit(100)))
+    B = b.scan("B", ["k", "w"]).to_gpu().unpack()
+    q = F.join(B, col("B", "k"), col("F", "b"), 13, 2000) \
+         .join(A, col("A", "k"), col("F", "a"), 11, 1000) \
+         .groupby([col("A", "w")], [(rb.as_(rb.int64(1), "o", "c"), "sum")], 6, 64)
+    assert _rows(str(q.prepare().execute())) == []
+    for d in devs:
+        d.free()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", ["all_survive_first", "filter_first", "single_join",
                                   "join_filter_join", "join_join_filter"])
 def test_probe_chain_stage0_compaction(ctx, case):
