@@ -114,6 +114,47 @@ def _chain_tables(ctx, n, seed):
 
 
 @pytest.mark.gpu
+def test_probe_chain_duplicates_only_among_rejected_rows(ctx):
+    """A filtered dimension whose key repeats only in rows the filter
+    rejects: the table leaves those rows out, so the fused chain runs (no
+    duplicate-key fallback) and joins the kept rows, as numpy says."""
+    ctx.clear_catalog()
+    n = 120_001
+    c, devs = _chain_tables(ctx, n, 9)
+    rs = np.random.default_rng(9)
+    keys = np.concatenate([np.arange(1000), rs.integers(0, 1000, 500)]).astype(np.int32)
+    flag = np.concatenate([np.ones(1000), np.zeros(500)]).astype(np.int32)  # 1 = kept
+    wv = (keys % 13).astype(np.int32)
+    perm = rs.permutation(len(keys))
+    keys, flag, wv = keys[perm], flag[perm], wv[perm]
+    for name, v in (("k", keys), ("f", flag), ("w", wv)):
+        d = ctx.to_device(v)
+        devs.append(d)
+        ctx.register_column("D", name, "int", d)
+    b = rb.RelBuilder(ctx)
+    col = rb.col
+    F = b.scan("F", ["a", "b", "v"]).to_gpu().unpack()
+    D = b.scan("D", ["k", "f", "w"]).to_gpu().unpack().filter(rb.eq(col("D", "f"), rb.lit(1)))
+    q = F.join(D, col("D", "k"), col("F", "a"), 11, 2000) \
+         .groupby([col("D", "w")], [(rb.as_(rb.cast(col("F", "v"), "int64"), "o", "s"), "sum"),
+                                    (rb.as_(rb.int64(1), "o", "c"), "sum")], 6, 64)
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    got = sorted(_rows(str(q.prepare().execute())))
+    chain = ctx.prof_get("rel_pipeline_chain")[1]
+    ctx.prof_enable(False)
+    assert chain >= 1
+    fa, fv = c[("F", "a")], c[("F", "v")]
+    g = fa % 13  # every fact key 0..999 meets its one kept row, whose w is key % 13
+    want = sorted([int(k), int(fv[g == k].astype(np.int64).sum()), int((g == k).sum())]
+                  for k in np.unique(g))
+    assert got == want
+    for d in devs:
+        d.free()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
 def test_probe_chain_dimension_filter_keeps_nothing(ctx):
     """A fused chain whose filtered dimension keeps no row (the filter's
     mask rejects every build row, so its table holds no key): every fact
