@@ -134,7 +134,10 @@ int dims_create_rows(pj_ctx *ctx, int m, const int32_t *const *d_keys,
 // significant): dir 1 ascending, -1 descending, 0 ignored; rank (int32 keys
 // only, may be null) maps a dictionary code in [0, nrank) to its position in
 // string order (codes outside sort after all others).  perm (n entries)
-// receives the row ids in sorted order.
+// receives the row ids in sorted order.  key0_out (may be null, n entries):
+// when key 0 is a plain int32 key with a direction and the radix passes ran,
+// its values in sorted order, and *key0_done is set (so no gather of that
+// column through perm is needed).
 struct SortKey {
   const void *col;
   int32_t t;
@@ -142,7 +145,8 @@ struct SortKey {
   const int32_t *rank;
   uint32_t nrank;
 };
-int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm);
+int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm,
+              int32_t *key0_out = nullptr, bool *key0_done = nullptr);
 
 // The words and log2 size of the context's bloom filter `id` (built with
 // pj_bloom_build_i32), for OP_BLOOM.

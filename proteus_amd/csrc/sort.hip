@@ -417,10 +417,25 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const unsigned long long *__
   }
 }
 
+// int32 values back from their ascending (desc = 0) or descending 32-bit
+// sort keys: the inverse of k_sort_key's I32 case without ranks.
+__global__ __launch_bounds__(256) void k_sort_unkey_i32(const uint32_t *__restrict__ key,
+                                                        uint64_t n, int desc,
+                                                        int32_t *__restrict__ out) {
+  const uint64_t gs = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += gs) {
+    const uint32_t u = desc ? ~key[i] : key[i];
+    out[i] = (int32_t)(u ^ 0x80000000u);
+  }
+}
+
 // Sorts the row permutation perm (n entries, initialised here) by the keys;
-// strides[k] (bytes) defaults to the column's element size.
+// strides[k] (bytes) defaults to the column's element size.  key0_out (may
+// be null): when key 0 is a plain int32 key with a direction and the radix
+// passes ran, its values in sorted order (n entries) and *key0_done = true.
 static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *strides, int nk,
-                             uint64_t n, uint32_t *perm) {
+                             uint64_t n, uint32_t *perm, int32_t *key0_out = nullptr,
+                             bool *key0_done = nullptr) {
   if (nk < 0 || (nk && !keys) || (n && !perm)) return fail(PJ_E_ARG, "sort with invalid arguments");
   if (!n) return PJ_OK;
   if (n > (uint64_t)INT32_MAX) return fail(PJ_E_CAPACITY, "sort of more than 2^31-1 rows");
@@ -463,6 +478,7 @@ static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *s
     return sync(ctx);
   }
   Scratch p1, k0, k1;
+  const uint32_t *k0_sorted = nullptr;  // key 0's 32-bit sort keys after its passes
   PJ_TRY(p1.alloc(ctx, n * 4));
   PJ_TRY(k0.alloc(ctx, n * (wide ? 8 : 4)));
   PJ_TRY(k1.alloc(ctx, n * (wide ? 8 : 4)));
@@ -477,6 +493,7 @@ static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *s
       PJ_LAUNCH(ctx, "sort_key", k_sort_key<uint32_t>, grid, 256, 0, (const char *)keys[k].col,
                 stride, keys[k].t, desc, keys[k].rank, keys[k].nrank, va, n, ka);
       PJ_TRY(radix_sort_pairs<uint32_t>(ctx, &ka, &kb, &va, &vb, n));
+      if (k == 0) k0_sorted = ka;
     } else {
       unsigned long long *ka = k0.as<unsigned long long>(), *kb = k1.as<unsigned long long>();
       PJ_LAUNCH(ctx, "sort_key", k_sort_key<unsigned long long>, grid, 256, 0,
@@ -486,11 +503,19 @@ static int sort_perm_strided(pj_ctx *ctx, const SortKey *keys, const uint64_t *s
     }
   }
   if (va != perm) PJ_HIP(hipMemcpyAsync(perm, va, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  // key 0 was sorted last: its sort keys are in the final order
+  if (key0_out && key0_done && nk >= 1 && keys[0].dir && keys[0].t == I32 && !keys[0].rank) {
+    PJ_LAUNCH(ctx, "sort_unkey", k_sort_unkey_i32, grid, 256, 0, k0_sorted, n,
+              keys[0].dir < 0 ? 1 : 0, key0_out);
+    *key0_done = true;
+  }
   return sync(ctx);
 }
 
-int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm) {
-  return sort_perm_strided(ctx, keys, nullptr, nk, n, perm);
+int sort_perm(pj_ctx *ctx, const SortKey *keys, int nk, uint64_t n, uint32_t *perm,
+              int32_t *key0_out, bool *key0_done) {
+  if (key0_done) *key0_done = false;
+  return sort_perm_strided(ctx, keys, nullptr, nk, n, perm, key0_out, key0_done);
 }
 
 // out row i = in row perm[i], rows of `words` 4-byte words

@@ -1973,7 +1973,16 @@ class Exec {
     idx.t = R::I32;
     idx.owner = alloc(std::max<uint64_t>(in.n, 1) * 4);
     idx.dev = idx.owner->p;
-    ck(R::sort_perm(ctx_, keys.data(), (int)keys.size(), in.n, (uint32_t *)idx.owner->p));
+    // a plain int32 first key comes back sorted from the passes themselves
+    Col key0;
+    bool key0_done = false;
+    if (!cols.empty() && cols[0].t == R::I32 && !cols[0].dict) {
+      key0.t = R::I32;
+      key0.owner = alloc(std::max<uint64_t>(in.n, 1) * 4);
+      key0.dev = key0.owner->p;
+    }
+    ck(R::sort_perm(ctx_, keys.data(), (int)keys.size(), in.n, (uint32_t *)idx.owner->p,
+                    (int32_t *)key0.dev, &key0_done));
     Rel out;
     out.n = in.n;
     const Json &rt = op["rowType"];
@@ -1981,9 +1990,15 @@ class Exec {
       Col g;
       g.t = cols[k].t;
       g.dict = cols[k].dict;
-      g.owner = alloc(in.n * R::col_size(g.t));
-      g.dev = g.owner->p;
-      ck(R::gather(ctx_, cols[k].dev, R::col_size(g.t), (const uint32_t *)idx.dev, in.n, g.owner->p));
+      if (k == 0 && key0_done) {
+        g.owner = key0.owner;
+        g.dev = key0.dev;
+      } else {
+        g.owner = alloc(in.n * R::col_size(g.t));
+        g.dev = g.owner->p;
+        ck(R::gather(ctx_, cols[k].dev, R::col_size(g.t), (const uint32_t *)idx.dev, in.n,
+                     g.owner->p));
+      }
       Attr a = reg_attr(es[k]["expression"], "sort", "key" + std::to_string(k));
       if (rt.is_array() && k < rt.size())
         a = Attr{rt[k]["relName"].as_string(), rt[k]["attrName"].as_string()};
