@@ -933,13 +933,27 @@ __global__ __launch_bounds__(256) void k_dim_range(const int32_t *__restrict__ k
                                                    long long *__restrict__ part) {
   __shared__ int64_t smn[4], smx[4], scn[4];
   int64_t mn = INT64_MAX, mx = INT64_MIN, c = 0;
+  // four rows per lane per step, every load issued before any is used
+  // (indices clamped to row 0, lanes past the end masked out)
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    if (keep && !keep[i]) continue;
-    const int64_t v = k[i];
-    mn = v < mn ? v : mn;
-    mx = v > mx ? v : mx;
-    ++c;
+  for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += 4 * stride) {
+    int32_t v[4];
+    bool on[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t i = b + j * stride;
+      on[j] = i < n;
+      v[j] = k[on[j] ? i : 0];
+      const bool kp = keep ? keep[on[j] ? i : 0] != 0 : true;
+      on[j] = on[j] && kp;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (on[j]) {
+        mn = v[j] < mn ? v[j] : mn;
+        mx = v[j] > mx ? v[j] : mx;
+        ++c;
+      }
   }
   mn = wave_min_i64(mn);
   mx = wave_max_i64(mx);
@@ -998,7 +1012,7 @@ int pj::rel::dims_create_rows(pj_ctx *ctx, int m, const int32_t *const *d_keys,
   // 1. min, max and count of every side's kept keys, one read-back for all
   std::vector<uint32_t> first(m + 1, 0);
   for (int i = 0; i < m; ++i)
-    first[i + 1] = first[i] + (uint32_t)(n[i] ? std::min<uint64_t>(ceil_div(n[i], 1024), 256) : 0);
+    first[i + 1] = first[i] + (uint32_t)(n[i] ? std::min<uint64_t>(ceil_div(n[i], 2048), (uint64_t)ctx->num_cus * 4) : 0);
   std::vector<long long> range(3 * (uint64_t)m);
   for (int i = 0; i < m; ++i) {  // sides of no rows
     range[3 * i] = INT64_MAX;

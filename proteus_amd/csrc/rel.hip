@@ -543,11 +543,18 @@ __device__ __forceinline__ void gb_apply_int(int64_t *slot, int m, int64_t v) {
 // count).  Keys are compared through the representative rows, as in
 // k_gb_generic.  A row whose key finds neither its slot nor a free one
 // within kGbLdsProbes local probes (a workgroup that sees more groups than
-// its table holds) goes to the global table directly.  LDS: na * LS int64
-// accumulators, then LS * 4 bytes of rows.
+// its table holds) goes to the global table directly.  The merge first lists
+// the occupied slots (a ballot per wave), so that each thread merges its
+// share of the workgroup's groups rather than scanning its share of the
+// slots.  LDS: na * LS int64 accumulators, LS * 4 bytes of rows, then the
+// list of LS 16-bit slot indices (LS <= 4096).
 constexpr uint32_t kGbLdsProbes = 32;
+constexpr uint32_t kGbLdsSlotBytes = 4 + 2;  // per slot beside the accumulators
+constexpr uint64_t kGbLdsRows = 2048;          // rows per workgroup (grid <= 8 per CU)
+constexpr uint64_t kGbMaxCopies = 16;          // global accumulator copies
+constexpr uint64_t kGbCopyBytes = 16ULL << 20;
 __global__ __launch_bounds__(256) void k_gb_lds(GbArgs a, uint64_t n, uint32_t *slots,
-                                                uint64_t mask, int64_t *acc,
+                                                uint64_t mask, int64_t *acc, uint32_t R,
                                                 unsigned int *overflow, uint32_t LS) {
   extern __shared__ int64_t lacc[];  // na * LS accumulators, then LS rows
   uint32_t *lrow = reinterpret_cast<uint32_t *>(lacc + (uint64_t)a.na * LS);
@@ -590,25 +597,40 @@ __global__ __launch_bounds__(256) void k_gb_lds(GbArgs a, uint64_t n, uint32_t *
         atomicOr(overflow, 1u);
         continue;
       }
-      base = acc + gs;
-      step = S;
+      base = acc + gs;  // copy 0
+      step = (uint64_t)R * S;
     }
     for (int j = 0; j < a.na; ++j) {
       const int m = a.am[j];
       gb_apply_int(base + (uint64_t)j * step, m, m == M_CNT ? 1 : ld_slot(a.ac[j], a.at[j], row));
     }
   }
+  __shared__ uint32_t nocc;
+  if (threadIdx.x == 0) nocc = 0;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < LS; i += 256) {
+  uint16_t *list = reinterpret_cast<uint16_t *>(lrow + LS);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = threadIdx.x; i < LS; i += 256) {  // LS is a multiple of 256
+    const bool occ = lrow[i] != 0;
+    const uint64_t bal = __ballot(occ);
+    uint32_t base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&nocc, (uint32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (occ) list[base + __popcll(bal & ((1ULL << lane) - 1))] = (uint16_t)i;
+  }
+  __syncthreads();
+  const uint32_t nl = nocc;
+  const uint64_t copy = blockIdx.x % R;
+  for (uint32_t x = threadIdx.x; x < nl; x += 256) {
+    const uint32_t i = list[x];
     const uint32_t r = lrow[i];
-    if (!r) continue;
     const int64_t gs = gb_global_slot(a, r - 1, slots, mask);
     if (gs < 0) {
       atomicOr(overflow, 1u);
       continue;
     }
     for (int j = 0; j < a.na; ++j)
-      gb_apply_int(acc + (uint64_t)j * S + gs, a.am[j], lacc[(uint64_t)j * LS + i]);
+      gb_apply_int(acc + ((uint64_t)j * R + copy) * S + gs, a.am[j], lacc[(uint64_t)j * LS + i]);
   }
 }
 
@@ -618,14 +640,30 @@ __global__ void k_fill_i64(int64_t *d, int64_t v, uint64_t n) {
 }
 
 // Group g of the output = the g-th occupied slot (slot order).
+// R > 1 (integer aggregates only): aggregate j of slot s is the monoid over
+// copies acc[(j R + r) S + s], r < R.
 __global__ void k_gb_emit(GbArgs a, const uint32_t *slots, const uint32_t *occ, uint64_t ng,
-                          const int64_t *acc, uint64_t S, GbOut out) {
+                          const int64_t *acc, uint64_t S, uint32_t R, GbOut out) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += stride) {
     const uint32_t s = occ[g];
     const uint64_t rep = slots[s] - 1;
     for (int k = 0; k < a.nk; ++k) st_slot(out.keys[k], a.kt[k], g, ld_slot(a.kc[k], a.kt[k], rep));
-    for (int j = 0; j < a.na; ++j) ((int64_t *)out.aggs[j])[g] = acc[(uint64_t)j * S + s];
+    for (int j = 0; j < a.na; ++j) {
+      const int64_t *c = acc + (uint64_t)j * R * S + s;
+      int64_t v = c[0];
+      for (uint32_t r = 1; r < R; ++r) {
+        const int64_t x = c[(uint64_t)r * S];
+        switch (a.am[j]) {
+          case M_MIN: v = x < v ? x : v; break;
+          case M_MAX: v = x > v ? x : v; break;
+          case M_AND: v &= x; break;
+          case M_OR: v |= x; break;
+          default: v += x; break;  // sums and counts
+        }
+      }
+      ((int64_t *)out.aggs[j])[g] = v;
+    }
   }
 }
 
@@ -1695,13 +1733,33 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
   }
   uint64_t S = 2ULL << ilog2_ceil(std::min<uint64_t>(max_groups, n));
   if (S < 64) S = 64;
+  // integer aggregates over few groups: workgroup-private LDS tables
+  bool ints = true;
+  for (int j = 0; j < na; ++j)
+    ints &= aggs[j].monoid == M_CNT || !(aggs[j].t == F64 || aggs[j].t == F32);
+  uint32_t LS = 0;  // local slots: a power of two, <= S, rows + list + accumulators in 64 KiB
+  if (ints && n >= (1u << 16))
+    for (uint32_t l = 4096; l >= 256 && !LS; l >>= 1)
+      if (l <= S && (uint64_t)l * (kGbLdsSlotBytes + 8 * (uint64_t)na) <= 65536) LS = l;
+  // With LDS tables, every workgroup merges each of its groups into the
+  // global accumulators once: R copies of them (workgroup b merges into copy
+  // b % R, the emit folds the copies) keep the same-address atomics of a
+  // small group count to about 32 per address.  Copies are bounded to
+  // kGbCopyBytes in all.
+  const uint64_t grid = LS ? std::min<uint64_t>(ceil_div(n, kGbLdsRows), (uint64_t)ctx->num_cus * 8) : 0;
+  uint32_t R = 1;
+  if (LS) {
+    const uint64_t per = std::max<uint64_t>(1, (uint64_t)na) * S * 8;
+    R = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({ceil_div(grid, 32), kGbMaxCopies, kGbCopyBytes / per}));
+  }
   Scratch slots, acc, flag;
   PJ_TRY(slots.alloc(ctx, S * 4));
   PJ_TRY(zero(ctx, slots.p, S * 4));
-  PJ_TRY(acc.alloc(ctx, std::max<uint64_t>(1, (uint64_t)na) * S * 8));
+  PJ_TRY(acc.alloc(ctx, std::max<uint64_t>(1, (uint64_t)na) * R * S * 8));
   PJ_TRY(flag.alloc(ctx, 4));
   PJ_TRY(zero(ctx, flag.p, 4));
-  for (int j = 0; j < na; ++j) {
+  for (int j = 0; j < na; ++j) {  // aggregate j: copies j R .. j R + R - 1 of S slots
     const bool f = aggs[j].monoid != M_CNT && (aggs[j].t == F64 || aggs[j].t == F32);
     int64_t ident;
     switch (aggs[j].monoid) {
@@ -1710,21 +1768,12 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
       case M_AND: ident = 1; break;
       default: ident = 0; break;
     }
-    PJ_LAUNCH(ctx, "rel_fill", k_fill_i64, grid_for(ctx, S, 256), 256, 0, acc.as<int64_t>() + (uint64_t)j * S,
-              ident, S);
+    PJ_LAUNCH(ctx, "rel_fill", k_fill_i64, grid_for(ctx, R * S, 256), 256, 0,
+              acc.as<int64_t>() + (uint64_t)j * R * S, ident, R * S);
   }
-  // integer aggregates over few groups: workgroup-private LDS tables
-  bool ints = true;
-  for (int j = 0; j < na; ++j)
-    ints &= aggs[j].monoid == M_CNT || !(aggs[j].t == F64 || aggs[j].t == F32);
-  uint32_t LS = 0;  // local slots: a power of two, <= S, rows + accumulators in 64 KiB
-  if (ints && n >= (1u << 16))
-    for (uint32_t l = 4096; l >= 256 && !LS; l >>= 1)
-      if (l <= S && (uint64_t)l * (4 + 8 * (uint64_t)na) <= 65536) LS = l;
   if (LS) {
-    const uint64_t grid = std::min<uint64_t>(ceil_div(n, 4096), (uint64_t)ctx->num_cus * 2);
-    PJ_LAUNCH(ctx, "rel_groupby_lds", k_gb_lds, grid, 256, (size_t)LS * (4 + 8 * na), a, n,
-              slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>(), LS);
+    PJ_LAUNCH(ctx, "rel_groupby_lds", k_gb_lds, grid, 256, (size_t)LS * (kGbLdsSlotBytes + 8 * na), a, n,
+              slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), R, flag.as<unsigned int>(), LS);
   } else {
     PJ_LAUNCH(ctx, "rel_groupby", k_gb_generic, grid_for(ctx, n, 256), 256, 0, a, n,
               slots.as<uint32_t>(), S - 1, acc.as<int64_t>(), flag.as<unsigned int>());
@@ -1757,7 +1806,8 @@ int groupby(pj_ctx *ctx, const GbKey *keys, int nk, const GbAgg *aggs, int na, u
                 (unsigned long long)ng, (unsigned long long)max_groups);
   if (ng)
     PJ_LAUNCH(ctx, "rel_groupby_emit", k_gb_emit, grid_for(ctx, ng, 256), 256, 0, a,
-              slots.as<const uint32_t>(), occ.as<const uint32_t>(), ng, acc.as<const int64_t>(), S, out);
+              slots.as<const uint32_t>(), occ.as<const uint32_t>(), ng, acc.as<const int64_t>(), S,
+              R, out);
   PJ_TRY(sync(ctx));
   *n_groups = ng;
   return PJ_OK;

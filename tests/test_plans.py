@@ -438,6 +438,47 @@ def test_groupby_two_keys_integer_aggregates_lds_tables(ctx, na, nb):
 
 
 @pytest.mark.gpu
+def test_groupby_boolean_aggregates_lds_tables(ctx):
+    """AND / OR aggregates with a count through the LDS group-by at 700001
+    rows: 342 workgroups, whose groups merge into 11 copies of the global
+    accumulators that the emit folds with each aggregate's monoid.
+    Even groups hold no v = 0 (AND(v > 0) true) and odd groups no v = 999
+    (OR(v > 998) false).  Equals numpy."""
+    n, ng = 700_001, 40
+    rs = np.random.default_rng(7)
+    a = rs.integers(0, ng, n).astype(np.int32)
+    v = rs.integers(0, 1000, n).astype(np.int32)
+    v[(a % 2 == 0) & (v == 0)] = 1
+    v[(a % 2 == 1) & (v == 999)] = 998
+    dev = [ctx.to_device(x) for x in (a, v)]
+    ctx.clear_catalog()
+    ctx.register_column("B", "a", "int", dev[0])
+    ctx.register_column("B", "v", "int", dev[1])
+    va = rb.col("B", "v")
+    q = rb.RelBuilder(ctx).scan("B", ["a", "v"]).to_gpu().unpack() \
+        .groupby([rb.col("B", "a")],
+                 [(rb.as_(rb.int64(1), "g", "cnt"), "sum"),
+                  (rb.as_(rb.gt(va, rb.lit(0)), "g", "all_pos"), "and"),
+                  (rb.as_(rb.gt(va, rb.lit(998)), "g", "any_top"), "or")],
+                 10, 1024)
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    out = ctx.execute_plan(json.dumps(q.plan()))
+    lds = ctx.prof_get("rel_groupby_lds")[1]
+    ctx.prof_enable(False)
+    assert lds == 1
+    got = sorted(tuple(ln.split(",")) for ln in out.splitlines() if ln)
+    want = sorted((str(g), str(int((a == g).sum())),
+                   "true" if (v[a == g] > 0).all() else "false",
+                   "true" if (v[a == g] > 998).any() else "false") for g in range(ng))
+    assert {w[2] for w in want} == {"true", "false"} and {w[3] for w in want} == {"true", "false"}
+    assert got == want
+    for x in dev:
+        x.free()
+    ctx.clear_catalog()
+
+
+@pytest.mark.gpu
 def test_groupby_plan_falls_back_to_general_kernel(ctx):
     """Shapes the LDS group-by does not compute (two integer value columns,
     an FP64 value, two keys) run the general kernel with the same results as
