@@ -53,7 +53,7 @@ template <>
 struct RsCfg<unsigned long long> {
   static constexpr int BLOCK = 512, IPT = 16, PER_CU = 1;
 };
-constexpr int kRsHistWaves = 4;  // k_rs_chunk_hist: 256 threads
+constexpr int kRsHistWaves = 16;  // k_rs_chunk_hist: 1024 threads
 
 template <class K>
 __device__ __forceinline__ uint32_t rs_digit(K k, int shift) {
@@ -99,30 +99,42 @@ __global__ __launch_bounds__(256) void k_rs_digits(const K *__restrict__ keys, u
 // Digit counts of chunk c = [c * chunk, min(n, (c + 1) * chunk)) into row c
 // of H[G][256].  One LDS copy per wave spreads same-digit atomics.
 template <class K>
-__global__ __launch_bounds__(256) void k_rs_chunk_hist(const K *__restrict__ keys, uint64_t n,
-                                                       uint64_t chunk, int shift,
-                                                       uint32_t *__restrict__ H) {
+__global__ __launch_bounds__(kRsHistWaves * 64) void k_rs_chunk_hist(
+    const K *__restrict__ keys, uint64_t n, uint64_t chunk, int shift, uint32_t *__restrict__ H) {
+  constexpr int B = kRsHistWaves * 64;
   __shared__ uint32_t h[kRsHistWaves][256];
   const int tid = threadIdx.x, w = tid >> 6;
-  for (int i = tid; i < kRsHistWaves * 256; i += 256) (&h[0][0])[i] = 0;
+  for (int i = tid; i < kRsHistWaves * 256; i += B) (&h[0][0])[i] = 0;
   __syncthreads();
   const uint64_t lo = (uint64_t)blockIdx.x * chunk;
   const uint64_t hi = lo + chunk < n ? lo + chunk : n;
-  // four keys per thread in flight before their atomics
+  // eight keys per thread in flight before their atomics; a wave whose 64
+  // keys share one digit (the skewed high bytes of floating-point keys)
+  // adds 64 once instead of 64 atomics on one LDS address
   uint64_t j = lo + tid;
-  for (; j + 3 * 256 < hi; j += 4 * 256) {
-    const K a = keys[j], b = keys[j + 256], c = keys[j + 512], d = keys[j + 768];
-    atomicAdd(&h[w][rs_digit(a, shift)], 1u);
-    atomicAdd(&h[w][rs_digit(b, shift)], 1u);
-    atomicAdd(&h[w][rs_digit(c, shift)], 1u);
-    atomicAdd(&h[w][rs_digit(d, shift)], 1u);
-  }
-  for (; j < hi; j += 256) atomicAdd(&h[w][rs_digit(keys[j], shift)], 1u);
-  __syncthreads();
-  uint32_t s = 0;
+  for (; j + 7 * B < hi; j += 8 * B) {
+    K x[8];
 #pragma unroll
-  for (int x = 0; x < kRsHistWaves; ++x) s += h[x][tid];
-  H[(uint64_t)blockIdx.x * 256 + tid] = s;
+    for (int u = 0; u < 8; ++u) x[u] = keys[j + u * B];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t d = rs_digit(x[u], shift);
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(d == d0) == ~0ULL) {
+        if ((tid & 63) == 0) atomicAdd(&h[w][d0], 64u);
+      } else {
+        atomicAdd(&h[w][d], 1u);
+      }
+    }
+  }
+  for (; j < hi; j += B) atomicAdd(&h[w][rs_digit(keys[j], shift)], 1u);
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int x = 0; x < kRsHistWaves; ++x) s += h[x][tid];
+    H[(uint64_t)blockIdx.x * 256 + tid] = s;
+  }
 }
 
 // One stable pass: chunk c of (ik, iv) to its (digit, chunk) ranges of
@@ -277,7 +289,7 @@ int radix_sort_pairs(pj_ctx *ctx, K **keys, K **keys_alt, uint32_t **vals, uint3
   PJ_TRY(ctr.alloc(ctx, 256));
   for (int p : passes) {
     const int shift = 8 * p;
-    PJ_LAUNCH(ctx, "sort_hist", k_rs_chunk_hist<K>, G, 256, 0, *keys, n, chunk, shift,
+    PJ_LAUNCH(ctx, "sort_hist", k_rs_chunk_hist<K>, G, kRsHistWaves * 64, 0, *keys, n, chunk, shift,
               H.as<uint32_t>());
     const dim3 cgrid(1, (uint32_t)nslab);
     PJ_LAUNCH(ctx, "sort_colsum", k_colsum, cgrid, 256, 0, H.as<const uint32_t>(), (int)G, 256,
